@@ -1,0 +1,208 @@
+/*
+ * pgd_hip.h -- C ABI of the MI355X implementation of PGD's exact edge-centric
+ * 2/3/4-node graphlet decomposition.
+ *
+ * Plain C types only: pointers, sizes, status codes.  Host buffers belong to
+ * the caller; device memory belongs to the context.  Calls on one context are
+ * serialised on that context's HIP stream; different contexts are
+ * independent.  Errors are returned as pgd_status codes and never cross the
+ * ABI as exceptions.
+ *
+ * Each entry point names the reference member function it replaces
+ * (graphlet_core.{h,cpp} of the reference, file:line).
+ */
+#ifndef PGD_HIP_H
+#define PGD_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t pgd_status;
+#define PGD_OK               0
+#define PGD_ERR_INVALID     -1  /* bad argument: null pointer, out-of-range id, unknown name */
+#define PGD_ERR_NO_DEVICE   -2  /* no HIP device, or the device index is out of range */
+#define PGD_ERR_HIP         -3  /* a HIP runtime call failed */
+#define PGD_ERR_OOM         -4  /* device allocation failed, or the context's memory cap was hit */
+#define PGD_ERR_UNSUPPORTED -5  /* input outside what the device path supports (e.g. n >= 2^31) */
+#define PGD_ERR_STATE       -6  /* call order: e.g. counting before the edge order exists */
+#define PGD_ERR_BUSY        -7  /* serving queue full / request not finished */
+
+typedef struct pgd_ctx pgd_ctx;
+typedef struct pgd_graph pgd_graph;
+
+/* Number of per-edge sums the decomposition reduces, in this order:
+ *   0 sum T  1 sum S  2 sum D  3 sum K4  4 sum C4  5..14 N1..N10
+ * (graphlet_core.cpp:1325-1364 and the reduction at 1429-1448). */
+#define PGD_NSUMS   15
+/* The 17 graphlet totals, in the reference's printed order
+ * (graphlet_core.cpp:2061-2084): 2_1edge 2_indep 3_tris 2_star 3_1edge
+ * 3_indep 4_clique 4_chordcycle 4_tailed_tris 4_cycle 3_star 4_path 4_1edge
+ * 4_2edge 4_2star 4_tri 4_indep. */
+#define PGD_NTOTALS 17
+/* Per-edge (micro) columns, stored at E_ordered position e
+ * (graphlet_core.cpp:1498-1504, 1559-1575). */
+#define PGD_MICRO_TRI       0  /* tri */
+#define PGD_MICRO_CLIQUE    1  /* local_4_clique */
+#define PGD_MICRO_CHORDAL   2  /* local_4_chordal_cycle */
+#define PGD_MICRO_TAILED    3  /* local_4_tailed_tris (never written: 0) */
+#define PGD_MICRO_CYCLE     4  /* local_4_cycle */
+#define PGD_MICRO_STAR3     5  /* local_3_star */
+#define PGD_MICRO_PATH      6  /* local_4_path */
+#define PGD_MICRO_STAR2     7  /* get_2_star (graphlet_core.h:353-355) */
+#define PGD_NMICRO          8
+
+/* graph construction flags */
+#define PGD_DEDUP   1u  /* drop repeated undirected pairs, keeping the first
+                           occurrence (remove_multiple_edges, graphlet_core.cpp:530-556) */
+
+/* decomposition flags */
+#define PGD_MICRO        1u   /* also produce the per-edge columns */
+#define PGD_ENGINE_EDGE  2u   /* force the edge-centric engine */
+#define PGD_ENGINE_ORIENT 4u  /* force the degree-oriented engine */
+#define PGD_ENGINE_DENSE 8u   /* force the dense (int8 MFMA) engine */
+
+/* ---- context -------------------------------------------------------- */
+pgd_status  pgd_open(int32_t device, pgd_ctx **out);
+pgd_status  pgd_close(pgd_ctx *ctx);
+const char *pgd_status_str(pgd_status s);
+/* Cap on device bytes this context may allocate (0 = no cap). */
+pgd_status  pgd_set_mem_cap(pgd_ctx *ctx, int64_t bytes);
+pgd_status  pgd_mem_in_use(pgd_ctx *ctx, int64_t *bytes);
+/* Last error detail for this context (static storage owned by the context). */
+const char *pgd_last_error(pgd_ctx *ctx);
+
+/* ---- graph construction -----------------------------------------------
+ * Replaces the CSR assembly of read_mtx (graphlet_core.cpp:509-517) and
+ * read_edge_list (446-452), and remove_multiple_edges (530-556) with
+ * PGD_DEDUP.  Input: `nlines` undirected pairs (src[i], dst[i]) with 0-based
+ * ids in [0, n), in the order the file lists them; self loops are dropped.
+ * Each vertex's adjacency list is built in line order, as the reference's
+ * reader appends.  n must be the reader's vertex count (1 + largest id for
+ * .mtx; number of distinct endpoints for edge lists). */
+pgd_status pgd_graph_from_coo(pgd_ctx *ctx, int64_t n, int64_t nlines,
+                              const int32_t *src, const int32_t *dst,
+                              uint32_t flags, pgd_graph **out);
+pgd_status pgd_graph_free(pgd_graph *g);
+/* n = num_vertices(), m = num_edges(), max degree, avg degree
+ * (graphlet_core.h:220-221, 262, 276; vertex_degrees graphlet_core.cpp:877-891) */
+pgd_status pgd_graph_info(pgd_graph *g, int64_t *n, int64_t *m, int32_t *max_deg, int32_t *avg_deg);
+/* Download the CSR: row holds n+1, col holds 2m (get_vertices/get_edges). */
+pgd_status pgd_graph_csr(pgd_graph *g, int64_t *row, int32_t *col);
+
+/* ---- preprocessing ----------------------------------------------------
+ * order_vertex_neighbors + create_edge_list_arrays (preprocess,
+ * graphlet_core.cpp:77-93, 929-949, 1846-1953).  method: "deg"/"degree",
+ * "kcore", "degree_kcore", "degree_vol", "kcore_vol", "degree_kcore_vol";
+ * any other name keeps the input order. */
+pgd_status pgd_preprocess(pgd_graph *g, const char *neigh_method, int32_t small_to_large);
+/* e_v, e_u: m ints each, indexed by edge id. */
+pgd_status pgd_edge_arrays(pgd_graph *g, int32_t *e_v, int32_t *e_u);
+
+/* compute_cores (graphlet_core.cpp:958-1018): core numbers (n ints, may be
+ * NULL) and the printed K (maximum core over vertices 0..n-2). */
+pgd_status pgd_compute_cores(pgd_graph *g, int32_t *core, int32_t *max_core);
+
+/* sort_edges (graphlet_core.cpp:854-867): builds E_ordered on the device.
+ * ordering: "natural", "deg"/"degree", "kcore", "kcore_deg", "deg_vol",
+ * "kcore_vol", "kcore_deg_vol", or any other name (key = degree of e_u, as
+ * get_ordering_value's default).  "rand"/"var" are PGD_ERR_UNSUPPORTED
+ * (their reference keys are time-seeded / undefined). */
+pgd_status pgd_sort_edges(pgd_graph *g, const char *ordering, int32_t small_to_large);
+/* E_ordered edge ids (m ints). */
+pgd_status pgd_e_ordered(pgd_graph *g, int32_t *ids);
+
+/* ---- decomposition ----------------------------------------------------
+ * graphlet_decomposition / graphlet_decomposition_micro
+ * (graphlet_core.cpp:1366-1474, 1483-1625): the per-edge counting and the
+ * reduction of the 15 sums.  Counting is exact uint64 integer arithmetic.
+ * sums receives PGD_NSUMS values.  With PGD_MICRO the per-edge columns are
+ * kept on the device (fetch with pgd_micro). */
+pgd_status pgd_decompose(pgd_graph *g, uint32_t flags, uint64_t *sums);
+/* Download one micro column (m uint64 values in E_ordered order). */
+pgd_status pgd_micro(pgd_graph *g, int32_t column, uint64_t *out);
+/* Raw per-edge counts by edge id: T, K4 (4-cliques on the edge) and C4
+ * (induced 4-cycles on the edge); any pointer may be NULL. */
+pgd_status pgd_edge_counts(pgd_graph *g, uint64_t *tri, uint64_t *k4, uint64_t *c4);
+/* Which engine the last pgd_decompose used: 1 edge-centric, 2 oriented,
+ * 3 dense. */
+int32_t    pgd_last_engine(pgd_graph *g);
+/* Device time (ms) of the last pgd_decompose's counting kernels, and of its
+ * dominant kernel alone, with its launch count (HIP events on the context's
+ * stream). */
+pgd_status pgd_last_timing(pgd_graph *g, double *total_ms, double *dominant_ms, int32_t *dominant_launches);
+
+/* ---- derivation -------------------------------------------------------
+ * The closed-form derivation of the 17 totals (graphlet_core.cpp:1449-1472,
+ * graphlet_core.h:548-557) in integer arithmetic that reproduces the
+ * reference build's binary64 steps bit for bit (round-to-nearest-even
+ * conversions, multiplication by fl(1/3), fl(1/6), truncation, and the
+ * signed/unsigned wrap of the n^3 and n^4 terms).  Host only. */
+pgd_status pgd_derive(int64_t n, int64_t m, const uint64_t *sums, uint64_t *totals);
+
+/* ---- batched form -----------------------------------------------------
+ * Thousands of small graphs counted in one persistent launch.  The caller
+ * packs the graphs' COO lists back to back (ids local to each graph);
+ * graph g owns lines [line_off[g], line_off[g+1]) and has n[g] vertices.
+ * The batch builds each graph's CSR, preprocessing and E_ordered as the
+ * plain path does (deg/deg defaults), then counts every graph with one
+ * persistent kernel that pulls (graph, edge-range) tiles from a device
+ * queue; the launch is captured in a HIP graph and replayed by
+ * pgd_batch_run.  sums receives PGD_NSUMS values per graph and totals
+ * PGD_NTOTALS per graph (either may be NULL). */
+typedef struct pgd_batch pgd_batch;
+pgd_status pgd_batch_create(pgd_ctx *ctx, int32_t ngraphs, const int32_t *n,
+                            const int64_t *line_off, const int32_t *src, const int32_t *dst,
+                            uint32_t flags, pgd_batch **out);
+pgd_status pgd_batch_run(pgd_batch *b, uint64_t *sums, uint64_t *totals);
+/* Per-edge micro column of graph gi (m_gi values, E_ordered order). */
+pgd_status pgd_batch_micro(pgd_batch *b, int32_t gi, int32_t column, uint64_t *out);
+pgd_status pgd_batch_info(pgd_batch *b, int64_t *total_edges, int32_t *tile_edges, int32_t *bin_threshold);
+pgd_status pgd_batch_free(pgd_batch *b);
+/* Tile size / hub-bin threshold used by later batch runs (0 = autotuned). */
+pgd_status pgd_batch_set_tuning(pgd_batch *b, int32_t tile_edges, int32_t bin_threshold);
+pgd_status pgd_batch_autotune(pgd_batch *b, int32_t *tile_edges, int32_t *bin_threshold);
+
+/* ---- serving ----------------------------------------------------------
+ * A request loop with continuous batching: requests (one graph each) are
+ * admitted when their estimated HBM footprint fits the server's budget and a
+ * slot is free; admitted requests are counted by the persistent kernel while
+ * later ones queue.  pgd_server_poll returns finished request ids. */
+typedef struct pgd_server pgd_server;
+pgd_status pgd_server_create(pgd_ctx *ctx, int64_t hbm_budget_bytes, int32_t max_inflight, pgd_server **out);
+/* Submit a graph; returns its request id.  PGD_ERR_BUSY if it can never
+ * fit the budget. */
+pgd_status pgd_server_submit(pgd_server *s, int32_t n, int64_t nlines, const int32_t *src,
+                             const int32_t *dst, uint32_t flags, int64_t *req_id);
+/* Drive the loop: admit queued requests, launch, collect finished ones.
+ * Writes up to cap finished request ids into done (count in *ndone). */
+pgd_status pgd_server_poll(pgd_server *s, int64_t *done, int32_t cap, int32_t *ndone);
+/* Result of a finished request: sums (PGD_NSUMS), totals (PGD_NTOTALS). */
+pgd_status pgd_server_result(pgd_server *s, int64_t req_id, uint64_t *sums, uint64_t *totals, int64_t *m);
+pgd_status pgd_server_micro(pgd_server *s, int64_t req_id, int32_t column, uint64_t *out);
+pgd_status pgd_server_release(pgd_server *s, int64_t req_id);
+pgd_status pgd_server_free(pgd_server *s);
+
+/* ---- synthetic inputs (host) ------------------------------------------ */
+int64_t pgd_gen_gnp(int32_t n, double p, uint64_t seed, int32_t *src, int32_t *dst, int64_t cap);
+int64_t pgd_gen_ba(int32_t n, int32_t m, uint64_t seed, int32_t *src, int32_t *dst, int64_t cap);
+int64_t pgd_gen_rmat(int32_t scale, int32_t ef, double a, double b, double c, uint64_t seed,
+                     int32_t *src, int32_t *dst, int64_t cap);
+int32_t pgd_write_mtx(const char *path, int32_t n, int64_t m, const int32_t *src, const int32_t *dst);
+
+/* ---- file reader (host) -----------------------------------------------
+ * read_graph (graphlet_core.cpp:259-283): parses a .mtx or edge-list file
+ * with the reference reader's rules and returns the COO in line order plus
+ * the reader's vertex count; *src/*dst are malloc'd (free with pgd_free).
+ * Returns PGD_ERR_INVALID for a missing file or an id the reference reader
+ * cannot handle.  *dedup is set when the reference would remove duplicates
+ * (edge lists). */
+pgd_status pgd_read_graph(const char *path, int64_t *n, int64_t *nlines, int32_t **src, int32_t **dst,
+                          int32_t *dedup, int64_t *self_loops);
+void       pgd_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PGD_HIP_H */

@@ -1,0 +1,267 @@
+"""pgd_amd -- MI355X implementation of PGD's exact 2/3/4-node graphlet decomposition.
+
+Python mirror of the reference's library interface (graphlet::graphlet_core,
+graphlet_core.h:54-703) over the C ABI in include/pgd_hip.h.  Every count is
+computed by the HIP kernels in pgd_amd/csrc; there is no CPU fallback: if the
+library or the GPU is missing, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (DEDUP, ENGINE_DENSE, ENGINE_EDGE, ENGINE_ORIENT, MICRO, NMICRO, NSUMS, NTOTALS,
+                   TOTAL_NAMES, PGDError, check, lib, ptr)
+
+__all__ = [
+    "Context", "GraphletCore", "derive", "gen_gnp", "gen_ba", "gen_rmat", "write_mtx", "read_graph",
+    "TOTAL_NAMES", "PGDError", "MICRO_COLUMNS",
+]
+
+MICRO_COLUMNS = ["tri", "local_4_clique", "local_4_chordal_cycle", "local_4_tailed_tris", "local_4_cycle",
+                 "local_3_star", "local_4_path", "2_star"]
+_ENGINE = {None: 0, "edge": ENGINE_EDGE, "orient": ENGINE_ORIENT, "dense": ENGINE_DENSE}
+
+
+class Context:
+    """A device context (pgd_open): one HIP stream and the device memory it owns."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().pgd_open(device, C.byref(h)), what="pgd_open")
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().pgd_close(self.h)
+            self.h = None
+
+    def set_mem_cap(self, nbytes: int) -> None:
+        check(lib().pgd_set_mem_cap(self.h, nbytes), self.h, "pgd_set_mem_cap")
+
+    def mem_in_use(self) -> int:
+        v = C.c_int64()
+        check(lib().pgd_mem_in_use(self.h, C.byref(v)), self.h)
+        return v.value
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+_default_ctx: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+# ---------------------------------------------------------------------------
+# host utilities (no GPU needed)
+# ---------------------------------------------------------------------------
+def derive(n: int, m: int, sums) -> list[int]:
+    """The 17 totals from the 15 sums, bit-identical to the reference build."""
+    s = np.ascontiguousarray(np.asarray(sums, dtype=np.uint64))
+    t = np.zeros(NTOTALS, dtype=np.uint64)
+    check(lib().pgd_derive(n, m, ptr(s, C.c_uint64), ptr(t, C.c_uint64)), what="pgd_derive")
+    return [int(x) for x in t]
+
+
+def _gen(fn, *args):
+    m = fn(*args, None, None, 0)
+    if m < 0:
+        raise PGDError(-1, "generator arguments")
+    src = np.empty(max(m, 1), dtype=np.int32)
+    dst = np.empty(max(m, 1), dtype=np.int32)
+    fn(*args, ptr(src, C.c_int32), ptr(dst, C.c_int32), m)
+    return src[:m], dst[:m]
+
+
+def gen_gnp(n: int, p: float, seed: int):
+    """Erdos-Renyi G(n,p) edge list (src<dst, row-major pair order)."""
+    return _gen(lib().pgd_gen_gnp, n, p, seed)
+
+
+def gen_ba(n: int, m: int, seed: int):
+    """Barabasi-Albert graph with m attachments per new vertex."""
+    return _gen(lib().pgd_gen_ba, n, m, seed)
+
+
+def gen_rmat(scale: int, ef: int = 16, a=0.57, b=0.19, c=0.19, seed: int = 1):
+    """R-MAT graph, symmetrised, self loops and duplicates removed, sorted."""
+    return _gen(lib().pgd_gen_rmat, scale, ef, a, b, c, seed)
+
+
+def write_mtx(path: str, n: int, src, dst) -> None:
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    dst = np.ascontiguousarray(dst, dtype=np.int32)
+    check(lib().pgd_write_mtx(path.encode(), n, len(src), ptr(src, C.c_int32), ptr(dst, C.c_int32)),
+          what="pgd_write_mtx")
+
+
+def read_graph(path: str):
+    """Parse a file with the reference reader's rules (read_graph).
+
+    Returns (n, src, dst, dedup, self_loops): the COO in line order."""
+    n, nl, loops, dd = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+    ps, pd = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)()
+    check(lib().pgd_read_graph(path.encode(), C.byref(n), C.byref(nl), C.byref(ps), C.byref(pd), C.byref(dd),
+                               C.byref(loops)), what=f"read_graph({path})")
+    k = nl.value
+    src = np.ctypeslib.as_array(ps, shape=(max(k, 1),))[:k].copy()
+    dst = np.ctypeslib.as_array(pd, shape=(max(k, 1),))[:k].copy()
+    lib().pgd_free(C.cast(ps, C.c_void_p))
+    lib().pgd_free(C.cast(pd, C.c_void_p))
+    return n.value, src, dst, bool(dd.value), loops.value
+
+
+# ---------------------------------------------------------------------------
+class GraphletCore:
+    """Mirror of graphlet::graphlet_core for one graph on one device.
+
+    Construction follows the reference constructor (graphlet_core.cpp:59-67):
+    read (or take) the edge list, build the CSR, then preprocess (neighbour
+    order + edge arrays).  graphlet_decomposition() runs sort_edges first when
+    no edge order exists (graphlet_core.cpp:1376-1378)."""
+
+    def __init__(self, src, dst, n: int | None = None, dedup: bool = False, ctx: Context | None = None,
+                 neigh_ordering: str = "deg", neigh_small_to_large: bool = False, preprocess: bool = True):
+        self.ctx = ctx or default_context()
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        dst = np.ascontiguousarray(dst, dtype=np.int32)
+        if n is None:
+            keep = src != dst
+            n = int(max(src[keep].max(initial=-1), dst[keep].max(initial=-1))) + 1
+        h = C.c_void_p()
+        check(lib().pgd_graph_from_coo(self.ctx.h, n, len(src), ptr(src, C.c_int32), ptr(dst, C.c_int32),
+                                       DEDUP if dedup else 0, C.byref(h)), self.ctx.h, "pgd_graph_from_coo")
+        self.h = h
+        nn, mm, md, ad = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        lib().pgd_graph_info(h, C.byref(nn), C.byref(mm), C.byref(md), C.byref(ad))
+        self.n, self.m, self.max_degree, self.avg_degree = nn.value, mm.value, md.value, ad.value
+        self.sums: list[int] | None = None
+        self.totals: dict[str, int] | None = None
+        self._ordered = False
+        if preprocess:
+            self.preprocess(neigh_ordering, neigh_small_to_large)
+
+    @classmethod
+    def from_file(cls, path: str, **kw) -> "GraphletCore":
+        n, src, dst, dedup, _ = read_graph(path)
+        return cls(src, dst, n=n, dedup=dedup, **kw)
+
+    def free(self) -> None:
+        if getattr(self, "h", None):
+            lib().pgd_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # -- preprocessing ---------------------------------------------------
+    def preprocess(self, neigh_ordering: str = "deg", small_to_large: bool = False) -> None:
+        check(lib().pgd_preprocess(self.h, neigh_ordering.encode(), int(small_to_large)), self.ctx.h,
+              "pgd_preprocess")
+        self._ordered = False
+
+    def compute_cores(self, want_cores: bool = False):
+        K = C.c_int32()
+        core = np.zeros(max(self.n, 1), dtype=np.int32) if want_cores else None
+        check(lib().pgd_compute_cores(self.h, ptr(core, C.c_int32) if want_cores else None, C.byref(K)),
+              self.ctx.h, "pgd_compute_cores")
+        return (K.value, core[: self.n]) if want_cores else K.value
+
+    def sort_edges(self, ordering: str = "deg", small_to_large: bool = False) -> None:
+        check(lib().pgd_sort_edges(self.h, ordering.encode(), int(small_to_large)), self.ctx.h, "pgd_sort_edges")
+        self._ordered = True
+
+    # -- state getters ----------------------------------------------------
+    def csr(self):
+        row = np.zeros(self.n + 1, dtype=np.int64)
+        col = np.zeros(max(2 * self.m, 1), dtype=np.int32)
+        check(lib().pgd_graph_csr(self.h, ptr(row, C.c_int64), ptr(col, C.c_int32)), self.ctx.h)
+        return row, col[: 2 * self.m]
+
+    def edge_arrays(self):
+        ev = np.zeros(max(self.m, 1), dtype=np.int32)
+        eu = np.zeros(max(self.m, 1), dtype=np.int32)
+        check(lib().pgd_edge_arrays(self.h, ptr(ev, C.c_int32), ptr(eu, C.c_int32)), self.ctx.h)
+        return ev[: self.m], eu[: self.m]
+
+    def e_ordered(self):
+        ids = np.zeros(max(self.m, 1), dtype=np.int32)
+        check(lib().pgd_e_ordered(self.h, ptr(ids, C.c_int32)), self.ctx.h)
+        return ids[: self.m]
+
+    # -- decomposition ----------------------------------------------------
+    def decompose(self, micro: bool = False, engine: str | None = None) -> list[int]:
+        """Device counting + reduction; returns the 15 sums."""
+        if not self._ordered:
+            self.sort_edges()
+        s = np.zeros(NSUMS, dtype=np.uint64)
+        flags = (MICRO if micro else 0) | _ENGINE[engine]
+        check(lib().pgd_decompose(self.h, flags, ptr(s, C.c_uint64)), self.ctx.h, "pgd_decompose")
+        self.sums = [int(x) for x in s]
+        return self.sums
+
+    def graphlet_decomposition(self, engine: str | None = None) -> dict[str, int]:
+        """graphlet_decomposition (graphlet_core.cpp:1366-1474)."""
+        self.decompose(False, engine)
+        self.totals = dict(zip(TOTAL_NAMES, derive(self.n, self.m, self.sums)))
+        return self.totals
+
+    def graphlet_decomposition_micro(self, engine: str | None = None) -> dict[str, int]:
+        """graphlet_decomposition_micro (graphlet_core.cpp:1483-1625)."""
+        self.decompose(True, engine)
+        self.totals = dict(zip(TOTAL_NAMES, derive(self.n, self.m, self.sums)))
+        return self.totals
+
+    def micro(self, column: int | str) -> np.ndarray:
+        if isinstance(column, str):
+            column = MICRO_COLUMNS.index(column)
+        out = np.zeros(max(self.m, 1), dtype=np.uint64)
+        check(lib().pgd_micro(self.h, column, ptr(out, C.c_uint64)), self.ctx.h, "pgd_micro")
+        return out[: self.m]
+
+    def edge_counts(self):
+        """(T, K4, C4) per edge id."""
+        t = np.zeros(max(self.m, 1), dtype=np.uint64)
+        k = np.zeros(max(self.m, 1), dtype=np.uint64)
+        c = np.zeros(max(self.m, 1), dtype=np.uint64)
+        check(lib().pgd_edge_counts(self.h, ptr(t, C.c_uint64), ptr(k, C.c_uint64), ptr(c, C.c_uint64)),
+              self.ctx.h)
+        return t[: self.m], k[: self.m], c[: self.m]
+
+    def last_engine(self) -> str:
+        return {1: "edge", 2: "orient", 3: "dense"}.get(lib().pgd_last_engine(self.h), "none")
+
+    def last_timing(self):
+        a, b, k = C.c_double(), C.c_double(), C.c_int32()
+        lib().pgd_last_timing(self.h, C.byref(a), C.byref(b), C.byref(k))
+        return a.value, b.value, k.value
+
+    # -- output formats (graphlet_core.cpp:2061-2084, 2138-2162, 1668-1761) --
+    def macro_text(self) -> str:
+        return "".join(f"{k} = {v}\n" for k, v in self.totals.items())
+
+    def micro_text(self) -> str:
+        ev, eu = self.edge_arrays()
+        order = self.e_ordered()
+        cols = [self.micro(c) for c in ("tri", "2_star", "local_4_clique", "local_4_chordal_cycle",
+                                        "local_4_tailed_tris", "local_4_cycle", "local_3_star", "local_4_path")]
+        v1 = ev[order].astype(np.uint64) + np.uint64(1)
+        u1 = eu[order].astype(np.uint64) + np.uint64(1)
+        head = "% src,dst,triangle,2-star,4-clique,4-chordal-cycle,4-tailed-triangle,4-cycle,3-star,4-path\n"
+        rows = np.stack([v1, u1] + [c.astype(np.uint64) for c in cols], axis=1) if self.m else None
+        if rows is None:
+            return head
+        return head + "\n".join(",".join(str(int(x)) for x in r) for r in rows) + "\n"

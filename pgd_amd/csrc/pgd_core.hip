@@ -1,0 +1,948 @@
+// pgd_core.hip -- context, device CSR construction, preprocessing (neighbour
+// order, orientation, k-core, edge order), the counting epilogue and the
+// C-ABI glue for the plain (one graph) path.
+//
+// Device data layout (HBM), per graph:
+//   row   int64[n+1]   CSR offsets            (graphlet_core.h:84-86 `vertices`)
+//   col   int32[2m]    neighbour ids           (`edges`, reference order)
+//   src   int32[2m]    owning vertex of each CSR slot
+//   deg   int32[n]
+//   ev/eu int32[m]     edge endpoints by edge id (graphlet_core.cpp:929-949)
+//   order int32[m]     E_ordered               (graphlet_core.cpp:854-867)
+//   T u32[m], K4 u64[m], C4 u64[m]             per-edge counts by edge id
+//   micro u64[8][m]    per-edge columns at E_ordered position (SoA)
+// Sorting uses rocPRIM's stable LSD radix sort; scans/reductions use rocPRIM.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <new>
+
+#include "pgd_internal.hpp"
+
+using namespace pgd;
+
+// ===========================================================================
+// memory
+// ===========================================================================
+namespace pgd {
+
+void* dmalloc(pgd_ctx* ctx, size_t bytes) {
+    if (ctx->mem_cap > 0 && ctx->mem_used + (int64_t)bytes > ctx->mem_cap)
+        throw Error{PGD_ERR_OOM, "device memory cap exceeded (" + std::to_string(ctx->mem_used + bytes) +
+                                     " > " + std::to_string(ctx->mem_cap) + " bytes)"};
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{PGD_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e)};
+    }
+    ctx->mem_used += (int64_t)bytes;
+    return p;
+}
+
+void dfree(pgd_ctx* ctx, void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    ctx->mem_used -= (int64_t)bytes;
+}
+
+void* tmp_storage(pgd_ctx* ctx, size_t bytes) {
+    if (bytes > ctx->tmp_bytes) {
+        if (ctx->tmp) dfree(ctx, ctx->tmp, ctx->tmp_bytes);
+        ctx->tmp = nullptr;
+        ctx->tmp_bytes = 0;
+        size_t b = std::max(bytes, (size_t)1 << 20);
+        ctx->tmp = dmalloc(ctx, b);
+        ctx->tmp_bytes = b;
+    }
+    return ctx->tmp;
+}
+
+}  // namespace pgd
+
+namespace {
+
+int bits_for(int64_t maxval) {  // bits needed to represent values in [0, maxval]
+    int b = 0;
+    while (b < 63 && ((int64_t)1 << b) <= maxval) b++;
+    return std::max(b, 1);
+}
+
+template <class K, class V>
+void sort_pairs(pgd_ctx* ctx, K* kin, K* kout, V* vin, V* vout, size_t n, unsigned b0, unsigned b1) {
+    if (n == 0) return;
+    size_t bytes = 0;
+    PGD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, b0, b1, ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, b0, b1, ctx->stream));
+}
+
+template <class K>
+void sort_keys(pgd_ctx* ctx, K* kin, K* kout, size_t n, unsigned b0, unsigned b1) {
+    if (n == 0) return;
+    size_t bytes = 0;
+    PGD_HIP(rocprim::radix_sort_keys(nullptr, bytes, kin, kout, n, b0, b1, ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::radix_sort_keys(t, bytes, kin, kout, n, b0, b1, ctx->stream));
+}
+
+template <class T, class U>
+void exclusive_sum(pgd_ctx* ctx, const T* in, U* out, size_t n) {
+    if (n == 0) return;
+    size_t bytes = 0;
+    PGD_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, U(0), n, rocprim::plus<U>(), ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::exclusive_scan(t, bytes, in, out, U(0), n, rocprim::plus<U>(), ctx->stream));
+}
+
+template <class T>
+void reduce_max(pgd_ctx* ctx, const T* in, T* out, size_t n) {
+    size_t bytes = 0;
+    PGD_HIP(rocprim::reduce(nullptr, bytes, in, out, T(0), n, rocprim::maximum<T>(), ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::reduce(t, bytes, in, out, T(0), n, rocprim::maximum<T>(), ctx->stream));
+}
+
+constexpr int kBlock = 256;
+
+inline int grid_for(int64_t n, int block = kBlock) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 1 << 20) g = 1 << 20;
+    return (int)g;
+}
+
+// ===========================================================================
+// CSR construction kernels
+// ===========================================================================
+__global__ void k_line_keep(const int32_t* __restrict__ s, const int32_t* __restrict__ d, int64_t L, int64_t n,
+                            int32_t* __restrict__ keep, uint32_t* __restrict__ bad) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t a = s[i], b = d[i];
+        if (a < 0 || b < 0 || a >= n || b >= n) { atomicOr(bad, 1u); keep[i] = 0; continue; }
+        keep[i] = a != b;
+    }
+}
+
+__global__ void k_pair_keys(const int32_t* __restrict__ s, const int32_t* __restrict__ d, const int32_t* __restrict__ keep,
+                            int64_t L, uint64_t* __restrict__ key, int32_t* __restrict__ idx) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t a = (uint32_t)s[i], b = (uint32_t)d[i];
+        uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        key[i] = keep[i] ? (((uint64_t)lo << 32) | hi) : ~0ULL;
+        idx[i] = (int32_t)i;
+    }
+}
+
+// sorted by pair key, stable by line: every later copy of a pair is dropped
+__global__ void k_dedup_mark(const uint64_t* __restrict__ key, const int32_t* __restrict__ idx, int64_t L,
+                             int32_t* __restrict__ keep) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < L; j += (int64_t)gridDim.x * blockDim.x) {
+        if (j > 0 && key[j] != ~0ULL && key[j] == key[j - 1]) keep[idx[j]] = 0;
+    }
+}
+
+// line i (kept, compacted index o) -> slots 2o (src->dst) and 2o+1 (dst->src)
+__global__ void k_make_slots(const int32_t* __restrict__ s, const int32_t* __restrict__ d,
+                             const int32_t* __restrict__ keep, const int64_t* __restrict__ off, int64_t L,
+                             uint32_t* __restrict__ owner, int32_t* __restrict__ nb, int32_t* __restrict__ deg) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L; i += (int64_t)gridDim.x * blockDim.x) {
+        if (!keep[i]) continue;
+        int64_t o = off[i];
+        int32_t a = s[i], b = d[i];
+        owner[2 * o] = (uint32_t)a;
+        nb[2 * o] = b;
+        owner[2 * o + 1] = (uint32_t)b;
+        nb[2 * o + 1] = a;
+        atomicAdd(&deg[a], 1);
+        atomicAdd(&deg[b], 1);
+    }
+}
+
+__global__ void k_fill_row_tail(int64_t* row, int64_t n, int64_t nnz) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) row[n] = nnz;
+}
+
+// ===========================================================================
+// preprocessing kernels
+// ===========================================================================
+// neighbour-order key of vertex x (order_vertex_neighbors, graphlet_core.cpp:1910-1953)
+enum NeighKey { NK_NONE = 0, NK_DEG, NK_KCORE, NK_DEG_KCORE, NK_DEG_VOL, NK_KCORE_VOL, NK_DEG_KCORE_VOL };
+
+// the reference's kcore[] is 1-based: kcore[x] holds the core of vertex x-1
+__device__ __forceinline__ int32_t kc1(const int32_t* core, int64_t x) { return x >= 1 ? core[x - 1] : 0; }
+
+__global__ void k_vertex_feature(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                                 const int32_t* __restrict__ deg, const int32_t* __restrict__ core, int64_t n,
+                                 int kind, int32_t* __restrict__ feat) {
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t f = 0;
+        switch (kind) {
+            case NK_DEG: f = (uint32_t)deg[x]; break;
+            case NK_KCORE: f = (uint32_t)kc1(core, x); break;
+            case NK_DEG_KCORE: f = (uint32_t)deg[x] + (uint32_t)kc1(core, x); break;
+            default:
+                for (int64_t j = row[x]; j < row[x + 1]; j++) {
+                    int32_t y = col[j];
+                    if (kind == NK_DEG_VOL) f += (uint32_t)deg[y];
+                    else if (kind == NK_KCORE_VOL) f += (uint32_t)kc1(core, y);
+                    else f += (uint32_t)deg[y] + (uint32_t)kc1(core, y);
+                }
+        }
+        feat[x] = (int32_t)f;
+    }
+}
+
+// composite key (owner, key) for the stable per-vertex sort; the large->small
+// order reverses the input first so equal keys end in reverse input order
+// (bucket_sort_neighbors_parallel, graphlet_core.cpp:1846-1877)
+__global__ void k_neigh_keys(const int32_t* __restrict__ src, const int32_t* __restrict__ col,
+                             const int32_t* __restrict__ feat, int64_t nnz, int s2l,
+                             uint64_t* __restrict__ key, int32_t* __restrict__ val) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t p = s2l ? i : nnz - 1 - i;
+        int32_t x = col[p];
+        uint32_t k = (uint32_t)feat[x] ^ 0x80000000u;
+        if (!s2l) k = ~k;
+        key[i] = ((uint64_t)(uint32_t)src[p] << 32) | k;
+        val[i] = x;
+    }
+}
+
+__global__ void k_edge_flags(const int32_t* __restrict__ src, const int32_t* __restrict__ col, int64_t nnz,
+                             int32_t* __restrict__ f) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x)
+        f[p] = src[p] < col[p];
+}
+
+// create_edge_list_arrays, graphlet_core.cpp:929-949: edge id = scan order of
+// (v<u) slots; e_v is the strictly lower-degree endpoint, ties go to u.
+__global__ void k_edge_arrays(const int32_t* __restrict__ src, const int32_t* __restrict__ col,
+                              const int32_t* __restrict__ deg, const int64_t* __restrict__ eid, int64_t nnz,
+                              int32_t* __restrict__ ev, int32_t* __restrict__ eu) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        int32_t v = src[p], u = col[p];
+        if (v < u) {
+            int64_t e = eid[p];
+            if (deg[v] < deg[u]) { ev[e] = v; eu[e] = u; }
+            else { ev[e] = u; eu[e] = v; }
+        }
+    }
+}
+
+// ---- k-core: level-synchronous peel -------------------------------------
+__global__ void k_core_init(const int32_t* __restrict__ deg, int64_t n, int32_t* __restrict__ cur,
+                            uint8_t* __restrict__ alive) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        cur[v] = deg[v];
+        alive[v] = 1;
+    }
+}
+
+__global__ void k_core_select(int32_t k, int64_t n, const int32_t* __restrict__ cur, uint8_t* __restrict__ alive,
+                              int32_t* __restrict__ core, int32_t* __restrict__ frontier, int32_t* __restrict__ stat) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        if (!alive[v]) continue;
+        int32_t c = cur[v];
+        if (c <= k) {
+            alive[v] = 0;
+            core[v] = k;
+            frontier[atomicAdd(&stat[0], 1)] = (int32_t)v;
+        } else {
+            atomicMin(&stat[1], c);
+        }
+    }
+}
+
+// one wavefront per removed vertex
+__global__ void k_core_update(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                              const int32_t* __restrict__ frontier, int32_t cnt, const uint8_t* __restrict__ alive,
+                              int32_t* __restrict__ cur) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t i = wave; i < cnt; i += nw) {
+        int32_t v = frontier[i];
+        for (int64_t j = row[v] + lane; j < row[v + 1]; j += 64) {
+            int32_t u = col[j];
+            if (alive[u]) atomicSub(&cur[u], 1);
+        }
+    }
+}
+
+// ---- edge order keys (get_ordering_value, graphlet_core.cpp:773-834) -------
+enum OrderKind { OK_NATURAL = 0, OK_DEG, OK_KCORE, OK_KCORE_DEG, OK_DEG_VOL, OK_KCORE_VOL, OK_DEG_KCORE_VOL, OK_DEFAULT };
+
+__global__ void k_order_keys(const int32_t* __restrict__ ev, const int32_t* __restrict__ eu,
+                             const int32_t* __restrict__ deg, const int32_t* __restrict__ core,
+                             const int64_t* __restrict__ row, const int32_t* __restrict__ col, int64_t m,
+                             int kind, int s2l, uint64_t* __restrict__ key) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+        int32_t v = ev[e], u = eu[e];
+        long long val = e;
+        switch (kind) {
+            case OK_NATURAL: break;
+            case OK_DEG: val = (long long)deg[u] + deg[v]; break;
+            case OK_KCORE: val = (long long)kc1(core, u) + kc1(core, v); break;
+            case OK_KCORE_DEG:
+                val = (int32_t)((uint32_t)(deg[u] + deg[v]) * (uint32_t)(kc1(core, u) + kc1(core, v)));
+                break;
+            case OK_DEG_VOL:
+            case OK_KCORE_VOL:
+            case OK_DEG_KCORE_VOL: {
+                val = 0;
+                int32_t ends[2] = {u, v};
+                for (int t = 0; t < 2; t++)
+                    for (int64_t j = row[ends[t]]; j < row[ends[t] + 1]; j++) {
+                        int32_t y = col[j];
+                        if (kind == OK_DEG_VOL) val += deg[y];
+                        else if (kind == OK_KCORE_VOL) val += kc1(core, y);
+                        else val += (long long)deg[y] * kc1(core, y);
+                    }
+                break;
+            }
+            default: val = deg[u]; break;
+        }
+        uint32_t k = (uint32_t)(int32_t)val ^ 0x80000000u;  // Vertex keeps an int bound
+        if (!s2l) k = ~k;                                   // decr_bound: larger bound first
+        key[e] = ((uint64_t)k << 32) | (uint32_t)~(uint32_t)e;  // ties: larger id first
+    }
+}
+
+__global__ void k_order_from_keys(const uint64_t* __restrict__ key, int64_t m, int32_t* __restrict__ order) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x)
+        order[e] = (int32_t)~(uint32_t)(key[e] & 0xffffffffULL);
+}
+
+// ===========================================================================
+// epilogue: per-edge equations (solve_graphlet_equations,
+// graphlet_core.cpp:1325-1364), micro columns (1559-1575) and the reduction
+// of the 15 sums (1429-1448) in uint64 wrap-around arithmetic.
+// ===========================================================================
+// (double)p * 0.5 converted back to uint64, for even p: exact below 2^53,
+// otherwise p rounded to 53 significant bits (nearest, ties to even) halved.
+__device__ __forceinline__ uint64_t half_rne(uint64_t p) {
+    if (p < (1ULL << 53)) return p >> 1;
+    int L = 64 - __clzll((long long)p);
+    int sh = L - 53;
+    uint64_t q = p >> sh;
+    uint64_t rem = p & ((1ULL << sh) - 1);
+    uint64_t hf = 1ULL << (sh - 1);
+    if (rem > hf || (rem == hf && (q & 1))) q++;
+    return q << (sh - 1);
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+constexpr uint64_t kExact = 1ULL << 53;
+
+__global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ order, const int32_t* __restrict__ ev,
+                                                  const int32_t* __restrict__ eu, const int64_t* __restrict__ row,
+                                                  const uint32_t* __restrict__ Tc, const uint64_t* __restrict__ K4c,
+                                                  const uint64_t* __restrict__ C4c, int64_t n, int64_t m,
+                                                  uint64_t* __restrict__ sums, uint64_t* __restrict__ micro,
+                                                  uint32_t* __restrict__ flags) {
+    uint64_t acc[PGD_NSUMS];
+#pragma unroll
+    for (int k = 0; k < PGD_NSUMS; k++) acc[k] = 0;
+    uint32_t inexact = 0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+        int32_t id = order[e];
+        int32_t v = ev[id], u = eu[id];
+        int64_t dv = row[v + 1] - row[v], du = row[u + 1] - row[u];
+        uint64_t T = Tc[id], K = K4c[id], C = C4c[id];
+        uint64_t sv = (uint64_t)dv - T - 1, su = (uint64_t)du - T - 1;
+        uint64_t S = sv + su;
+        uint64_t D = (uint64_t)n - (S + T) - 2;
+        acc[0] += T;
+        acc[1] += S;
+        acc[2] += D;
+        acc[3] += K;
+        acc[4] += C;
+        acc[5] += half_rne((T + S) * (T + S - 1));
+        acc[6] += half_rne(S * (S - 1));
+        acc[7] += half_rne(T * (T - 1));
+        acc[8] += T * S;
+        acc[9] += sv * su;
+        acc[10] += sv * (sv - 1) / 2 + su * (su - 1) / 2;
+        acc[11] += (uint64_t)(m - dv - du + 1);
+        acc[12] += half_rne(D * (D - 1));
+        acc[13] += S * D;
+        acc[14] += T * D;
+        if (micro) {
+            uint64_t tt = T * (T - 1);
+            int64_t svs = (int64_t)sv, sus = (int64_t)su;
+            uint64_t a3 = (uint64_t)(svs * (svs - 1)), b3 = (uint64_t)(sus * (sus - 1));
+            // exact below 2^53; larger values are recomputed on the host
+            if (tt >= kExact || K >= kExact || a3 >= kExact || b3 >= kExact || (a3 >> 1) + (b3 >> 1) >= kExact)
+                inexact++;
+            micro[(size_t)PGD_MICRO_TRI * m + e] = T;
+            micro[(size_t)PGD_MICRO_CLIQUE * m + e] = K;
+            micro[(size_t)PGD_MICRO_CHORDAL * m + e] = (tt >> 1) - K;
+            micro[(size_t)PGD_MICRO_TAILED * m + e] = 0;
+            micro[(size_t)PGD_MICRO_CYCLE * m + e] = C;
+            micro[(size_t)PGD_MICRO_STAR3 * m + e] = (a3 >> 1) + (b3 >> 1);
+            micro[(size_t)PGD_MICRO_PATH * m + e] = (uint64_t)(svs * sus) - C;
+            micro[(size_t)PGD_MICRO_STAR2 * m + e] = S;
+        }
+    }
+    __shared__ uint64_t red[PGD_NSUMS][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < PGD_NSUMS; k++) {
+        uint64_t s = wave_sum_u64(acc[k]);
+        if (lane == 0) red[k][w] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < PGD_NSUMS) {
+        uint64_t s = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[threadIdx.x][i];
+        atomicAdd((unsigned long long*)&sums[threadIdx.x], (unsigned long long)s);
+    }
+    if (inexact) atomicAdd(flags, inexact);
+}
+
+}  // namespace
+
+namespace pgd {
+void epilogue(pgd_ctx* ctx, const GraphView& g, EdgeCounts c, uint64_t* d_sums, uint64_t* d_micro, uint32_t* d_flags) {
+    PGD_HIP(hipMemsetAsync(d_sums, 0, PGD_NSUMS * sizeof(uint64_t), ctx->stream));
+    PGD_HIP(hipMemsetAsync(d_flags, 0, sizeof(uint32_t), ctx->stream));
+    if (g.m == 0) return;
+    int grid = std::min(grid_for(g.m), ctx->num_cus * 8);
+    hipLaunchKernelGGL(k_epilogue, dim3(grid), dim3(256), 0, ctx->stream, g.order, g.ev, g.eu, g.row, c.T, c.K4,
+                       c.C4, g.n, g.m, d_sums, d_micro, d_flags);
+    PGD_HIP(hipGetLastError());
+}
+}  // namespace pgd
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+namespace {
+
+template <class F>
+pgd_status guarded(pgd_ctx* ctx, F&& f) {
+    try {
+        f();
+        return PGD_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->last_err = e.what;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->last_err = "host allocation failed";
+        return PGD_ERR_OOM;
+    } catch (...) {
+        if (ctx) ctx->last_err = "unexpected exception";
+        return PGD_ERR_INVALID;
+    }
+}
+
+void sync(pgd_ctx* ctx) { PGD_HIP(hipStreamSynchronize(ctx->stream)); }
+
+int neigh_kind(const char* m) {
+    std::string s(m ? m : "");
+    if (s == "deg" || s == "degree") return NK_DEG;
+    if (s == "kcore") return NK_KCORE;
+    if (s == "degree_kcore" || s == "deg_kcore") return NK_DEG_KCORE;
+    if (s == "degree_vol" || s == "deg_vol") return NK_DEG_VOL;
+    if (s == "kcore_vol") return NK_KCORE_VOL;
+    if (s == "degree_kcore_vol" || s == "deg_kcore_vol") return NK_DEG_KCORE_VOL;
+    return NK_NONE;
+}
+
+// get_ordering_enum (graphlet_core.cpp:713-763); -1: rand/var (unsupported)
+int order_kind(const char* o) {
+    std::string s(o ? o : "");
+    if (s == "natural") return OK_NATURAL;
+    if (s == "rand" || s == "random" || s == "var") return -1;
+    if (s == "deg" || s == "degree") return OK_DEG;
+    if (s == "kcore" || s == "kcores") return OK_KCORE;
+    if (s == "kcore_deg" || s == "kcore_degree") return OK_KCORE_DEG;
+    if (s == "deg_vol" || s == "degree_vol") return OK_DEG_VOL;
+    if (s == "kcore_vol" || s == "kcore_volume") return OK_KCORE_VOL;
+    if (s == "kcore_deg_vol" || s == "kcore_degree_vol") return OK_DEG_KCORE_VOL;
+    static const char* rest[] = {
+        "triangles", "tri", "triangle", "wedges", "tcores", "triangle_cores", "triangle_core", "tcore", "coloring",
+        "greedy_coloring", "tri_vol", "triangle_vol", "tcore_vol", "tcore_volume", "triangle_core_vol",
+        "triangles_only", "triangle_only", "dynamic_largest_first", "lfo", "LFO", "triangle_core_max", "tcore_max",
+        "tcores_max", "degree_triangles", "degree_tri", "deg_tri", "deg_triangles", "kcore_tri", "kcore_triangles",
+        "kcore_triangle", "kcore_deg_tri", "kcore_degree_triangle", "kcore_degree_tri", "kcore_tri_vol",
+        "kcore_triangle_vol", "kcore_deg_tri_vol", "kcore_degree_tri_vol", "dist_two_lfo", "dist_two_largest_first",
+        "dist_two_dlfo", "dist_two_dynamic_lfo", "dist_two_slo", "dist_two_smallest_last", "ido", "incidence",
+        "incidence_deg", "dist_two_ido", "dist_two_incidence", "dist_two_tcore", "dist_two_triangle_core",
+        "tcore_min", "triangle_core_min", "tri_core_min", "common", "common_neighbors"};
+    for (const char* r : rest)
+        if (s == r) return OK_DEFAULT;
+    return OK_NATURAL;  // unknown names fall back to natural order
+}
+
+void compute_cores(pgd_graph* g) {
+    if (g->have_core) return;
+    pgd_ctx* ctx = g->ctx;
+    const int64_t n = g->n;
+    g->core.alloc(ctx, (size_t)std::max<int64_t>(n, 1));
+    g->K = 0;
+    if (n == 0) { g->have_core = true; return; }
+    DBuf<int32_t> cur, frontier, stat;
+    DBuf<uint8_t> alive;
+    cur.alloc(ctx, n);
+    frontier.alloc(ctx, n);
+    stat.alloc(ctx, 2);
+    alive.alloc(ctx, n);
+    int grid = std::min(grid_for(n), ctx->num_cus * 16);
+    hipLaunchKernelGGL(k_core_init, dim3(grid), dim3(kBlock), 0, ctx->stream, g->deg.get(), n, cur.get(),
+                       alive.get());
+    PGD_HIP(hipGetLastError());
+    int64_t remaining = n;
+    int32_t k = 0;
+    int32_t h[2];
+    while (remaining > 0) {
+        h[0] = 0;
+        h[1] = INT_MAX;
+        PGD_HIP(hipMemcpyAsync(stat.get(), h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_core_select, dim3(grid), dim3(kBlock), 0, ctx->stream, k, n, cur.get(), alive.get(),
+                           g->core.get(), frontier.get(), stat.get());
+        PGD_HIP(hipGetLastError());
+        PGD_HIP(hipMemcpyAsync(h, stat.get(), sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        sync(ctx);
+        if (h[0] == 0) {
+            k = std::max(k + 1, h[1]);
+            continue;
+        }
+        int ugrid = std::min(grid_for((int64_t)h[0] * 64), ctx->num_cus * 16);
+        hipLaunchKernelGGL(k_core_update, dim3(ugrid), dim3(kBlock), 0, ctx->stream, g->row.get(), g->col.get(),
+                           frontier.get(), h[0], alive.get(), cur.get());
+        PGD_HIP(hipGetLastError());
+        remaining -= h[0];
+    }
+    // printed K: maximum core over vertices 0..n-2 (graphlet_core.cpp:1011-1015)
+    if (n >= 2) {
+        DBuf<int32_t> mx;
+        mx.alloc(ctx, 1);
+        reduce_max(ctx, g->core.get(), mx.get(), (size_t)(n - 1));
+        PGD_HIP(hipMemcpyAsync(&g->K, mx.get(), sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        sync(ctx);
+    }
+    g->have_core = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pgd_status_str(pgd_status s) {
+    switch (s) {
+        case PGD_OK: return "ok";
+        case PGD_ERR_INVALID: return "invalid argument";
+        case PGD_ERR_NO_DEVICE: return "no HIP device";
+        case PGD_ERR_HIP: return "HIP runtime error";
+        case PGD_ERR_OOM: return "out of device memory";
+        case PGD_ERR_UNSUPPORTED: return "unsupported input";
+        case PGD_ERR_STATE: return "call out of order";
+        case PGD_ERR_BUSY: return "busy";
+        default: return "unknown status";
+    }
+}
+
+pgd_status pgd_open(int32_t device, pgd_ctx** out) {
+    if (!out) return PGD_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return PGD_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= count) return PGD_ERR_NO_DEVICE;
+    pgd_ctx* ctx = new (std::nothrow) pgd_ctx();
+    if (!ctx) return PGD_ERR_OOM;
+    ctx->device = device;
+    pgd_status st = guarded(ctx, [&] {
+        PGD_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        PGD_HIP(hipGetDeviceProperties(&prop, device));
+        ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        PGD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    });
+    if (st != PGD_OK) {
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return PGD_OK;
+}
+
+pgd_status pgd_close(pgd_ctx* ctx) {
+    if (!ctx) return PGD_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->tmp) dfree(ctx, ctx->tmp, ctx->tmp_bytes);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PGD_OK;
+}
+
+pgd_status pgd_set_mem_cap(pgd_ctx* ctx, int64_t bytes) {
+    if (!ctx || bytes < 0) return PGD_ERR_INVALID;
+    ctx->mem_cap = bytes;
+    return PGD_OK;
+}
+
+pgd_status pgd_mem_in_use(pgd_ctx* ctx, int64_t* bytes) {
+    if (!ctx || !bytes) return PGD_ERR_INVALID;
+    *bytes = ctx->mem_used;
+    return PGD_OK;
+}
+
+const char* pgd_last_error(pgd_ctx* ctx) { return ctx ? ctx->last_err.c_str() : "no context"; }
+
+pgd_status pgd_graph_from_coo(pgd_ctx* ctx, int64_t n, int64_t L, const int32_t* src, const int32_t* dst,
+                              uint32_t flags, pgd_graph** out) {
+    if (!ctx || !out || n < 0 || L < 0 || (L > 0 && (!src || !dst))) return PGD_ERR_INVALID;
+    if (n >= INT32_MAX || L >= ((int64_t)1 << 31)) return PGD_ERR_UNSUPPORTED;
+    *out = nullptr;
+    pgd_graph* g = new (std::nothrow) pgd_graph();
+    if (!g) return PGD_ERR_OOM;
+    g->ctx = ctx;
+    pgd_status st = guarded(ctx, [&] {
+        PGD_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        DBuf<int32_t> ds, dd, keep;
+        DBuf<int64_t> off;
+        DBuf<uint32_t> bad;
+        ds.alloc(ctx, L);
+        dd.alloc(ctx, L);
+        keep.alloc(ctx, L);
+        off.alloc(ctx, L + 1);
+        bad.alloc(ctx, 1);
+        if (L) {
+            PGD_HIP(hipMemcpyAsync(ds.get(), src, L * sizeof(int32_t), hipMemcpyHostToDevice, s));
+            PGD_HIP(hipMemcpyAsync(dd.get(), dst, L * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        }
+        PGD_HIP(hipMemsetAsync(bad.get(), 0, sizeof(uint32_t), s));
+        int grid = std::min(grid_for(L), ctx->num_cus * 16);
+        if (L) {
+            hipLaunchKernelGGL(k_line_keep, dim3(grid), dim3(kBlock), 0, s, ds.get(), dd.get(), L, n, keep.get(),
+                               bad.get());
+            PGD_HIP(hipGetLastError());
+        }
+        uint32_t hbad = 0;
+        PGD_HIP(hipMemcpyAsync(&hbad, bad.get(), sizeof(hbad), hipMemcpyDeviceToHost, s));
+        sync(ctx);
+        PGD_CHECK(hbad == 0, PGD_ERR_INVALID, "vertex id outside [0, n)");
+        if ((flags & PGD_DEDUP) && L) {
+            DBuf<uint64_t> k0, k1;
+            DBuf<int32_t> i0, i1;
+            k0.alloc(ctx, L); k1.alloc(ctx, L); i0.alloc(ctx, L); i1.alloc(ctx, L);
+            hipLaunchKernelGGL(k_pair_keys, dim3(grid), dim3(kBlock), 0, s, ds.get(), dd.get(), keep.get(), L,
+                               k0.get(), i0.get());
+            PGD_HIP(hipGetLastError());
+            sort_pairs(ctx, k0.get(), k1.get(), i0.get(), i1.get(), (size_t)L, 0, 64);
+            hipLaunchKernelGGL(k_dedup_mark, dim3(grid), dim3(kBlock), 0, s, k1.get(), i1.get(), L, keep.get());
+            PGD_HIP(hipGetLastError());
+        }
+        exclusive_sum(ctx, keep.get(), off.get(), (size_t)L);
+        int64_t kept = 0;
+        if (L) {
+            int64_t last_off = 0;
+            int32_t last_keep = 0;
+            PGD_HIP(hipMemcpyAsync(&last_off, off.get() + (L - 1), sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            PGD_HIP(hipMemcpyAsync(&last_keep, keep.get() + (L - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            sync(ctx);
+            kept = last_off + last_keep;
+        }
+        const int64_t nnz = 2 * kept;
+        g->n = n;
+        g->nnz = nnz;
+        g->m = nnz / 2;
+        g->row.alloc(ctx, n + 1);
+        g->col.alloc(ctx, nnz);
+        g->src.alloc(ctx, nnz);
+        g->deg.alloc(ctx, std::max<int64_t>(n, 1));
+        PGD_HIP(hipMemsetAsync(g->deg.get(), 0, std::max<int64_t>(n, 1) * sizeof(int32_t), s));
+        if (nnz) {
+            DBuf<uint32_t> own;
+            DBuf<int32_t> nb;
+            own.alloc(ctx, nnz);
+            nb.alloc(ctx, nnz);
+            hipLaunchKernelGGL(k_make_slots, dim3(grid), dim3(kBlock), 0, s, ds.get(), dd.get(), keep.get(),
+                               off.get(), L, own.get(), nb.get(), g->deg.get());
+            PGD_HIP(hipGetLastError());
+            // stable by owner: every adjacency list stays in line order
+            sort_pairs(ctx, own.get(), (uint32_t*)g->src.get(), nb.get(), g->col.get(), (size_t)nnz, 0,
+                       (unsigned)bits_for(n - 1));
+        }
+        if (n) exclusive_sum(ctx, g->deg.get(), g->row.get(), (size_t)n);
+        hipLaunchKernelGGL(k_fill_row_tail, dim3(1), dim3(1), 0, s, g->row.get(), n, nnz);
+        PGD_HIP(hipGetLastError());
+        int32_t md = 0;
+        if (n) {
+            DBuf<int32_t> mx;
+            mx.alloc(ctx, 1);
+            reduce_max(ctx, g->deg.get(), mx.get(), (size_t)n);
+            PGD_HIP(hipMemcpyAsync(&md, mx.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        }
+        sync(ctx);
+        // vertex_degrees: max_degree starts from vertex 0's degree; avg is an int
+        g->max_deg = md;
+        g->avg_deg = n > 0 ? (int32_t)((double)nnz / (double)n) : 0;
+    });
+    if (st != PGD_OK) {
+        delete g;
+        return st;
+    }
+    *out = g;
+    return PGD_OK;
+}
+
+pgd_status pgd_graph_free(pgd_graph* g) {
+    if (!g) return PGD_ERR_INVALID;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->orient) orient_state_free(g->ctx, g->orient);
+    delete g;
+    return PGD_OK;
+}
+
+pgd_status pgd_graph_info(pgd_graph* g, int64_t* n, int64_t* m, int32_t* max_deg, int32_t* avg_deg) {
+    if (!g) return PGD_ERR_INVALID;
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (max_deg) *max_deg = g->max_deg;
+    if (avg_deg) *avg_deg = g->avg_deg;
+    return PGD_OK;
+}
+
+pgd_status pgd_graph_csr(pgd_graph* g, int64_t* row, int32_t* col) {
+    if (!g || !row || (!col && g->nnz)) return PGD_ERR_INVALID;
+    return guarded(g->ctx, [&] {
+        PGD_HIP(hipMemcpyAsync(row, g->row.get(), (g->n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                               g->ctx->stream));
+        if (g->nnz)
+            PGD_HIP(hipMemcpyAsync(col, g->col.get(), g->nnz * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                   g->ctx->stream));
+        sync(g->ctx);
+    });
+}
+
+pgd_status pgd_preprocess(pgd_graph* g, const char* method, int32_t s2l) {
+    if (!g) return PGD_ERR_INVALID;
+    return guarded(g->ctx, [&] {
+        pgd_ctx* ctx = g->ctx;
+        hipStream_t s = ctx->stream;
+        PGD_HIP(hipSetDevice(ctx->device));
+        const int64_t n = g->n, nnz = g->nnz;
+        int kind = neigh_kind(method);
+        int grid = std::min(grid_for(std::max(n, nnz)), ctx->num_cus * 16);
+        if (kind != NK_NONE && nnz) {
+            if (kind == NK_KCORE || kind == NK_DEG_KCORE || kind == NK_KCORE_VOL || kind == NK_DEG_KCORE_VOL)
+                compute_cores(g);
+            DBuf<int32_t> feat, v0;
+            DBuf<uint64_t> k0, k1;
+            feat.alloc(ctx, std::max<int64_t>(n, 1));
+            hipLaunchKernelGGL(k_vertex_feature, dim3(grid), dim3(kBlock), 0, s, g->row.get(), g->col.get(),
+                               g->deg.get(), g->have_core ? g->core.get() : nullptr, n, kind, feat.get());
+            PGD_HIP(hipGetLastError());
+            k0.alloc(ctx, nnz); k1.alloc(ctx, nnz); v0.alloc(ctx, nnz);
+            hipLaunchKernelGGL(k_neigh_keys, dim3(grid), dim3(kBlock), 0, s, g->src.get(), g->col.get(), feat.get(),
+                               nnz, s2l ? 1 : 0, k0.get(), v0.get());
+            PGD_HIP(hipGetLastError());
+            sort_pairs(ctx, k0.get(), k1.get(), v0.get(), g->col.get(), (size_t)nnz, 0,
+                       32u + (unsigned)bits_for(n - 1));
+        }
+        // orientation and edge ids
+        g->ev.alloc(ctx, std::max<int64_t>(g->m, 1));
+        g->eu.alloc(ctx, std::max<int64_t>(g->m, 1));
+        if (nnz) {
+            DBuf<int32_t> f;
+            DBuf<int64_t> eid;
+            f.alloc(ctx, nnz);
+            eid.alloc(ctx, nnz);
+            hipLaunchKernelGGL(k_edge_flags, dim3(grid), dim3(kBlock), 0, s, g->src.get(), g->col.get(), nnz, f.get());
+            PGD_HIP(hipGetLastError());
+            exclusive_sum(ctx, f.get(), eid.get(), (size_t)nnz);
+            hipLaunchKernelGGL(k_edge_arrays, dim3(grid), dim3(kBlock), 0, s, g->src.get(), g->col.get(),
+                               g->deg.get(), eid.get(), nnz, g->ev.get(), g->eu.get());
+            PGD_HIP(hipGetLastError());
+        }
+        sync(ctx);
+        g->prepped = true;
+        g->have_order = false;
+        g->have_counts = false;
+        if (g->orient) { orient_state_free(ctx, g->orient); g->orient = nullptr; }
+    });
+}
+
+pgd_status pgd_edge_arrays(pgd_graph* g, int32_t* e_v, int32_t* e_u) {
+    if (!g || (!e_v && !e_u)) return PGD_ERR_INVALID;
+    if (!g->prepped) return PGD_ERR_STATE;
+    return guarded(g->ctx, [&] {
+        if (g->m && e_v)
+            PGD_HIP(hipMemcpyAsync(e_v, g->ev.get(), g->m * sizeof(int32_t), hipMemcpyDeviceToHost, g->ctx->stream));
+        if (g->m && e_u)
+            PGD_HIP(hipMemcpyAsync(e_u, g->eu.get(), g->m * sizeof(int32_t), hipMemcpyDeviceToHost, g->ctx->stream));
+        sync(g->ctx);
+    });
+}
+
+pgd_status pgd_compute_cores(pgd_graph* g, int32_t* core, int32_t* max_core) {
+    if (!g) return PGD_ERR_INVALID;
+    return guarded(g->ctx, [&] {
+        PGD_HIP(hipSetDevice(g->ctx->device));
+        compute_cores(g);
+        if (core && g->n)
+            PGD_HIP(hipMemcpyAsync(core, g->core.get(), g->n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                   g->ctx->stream));
+        sync(g->ctx);
+        if (max_core) *max_core = g->K;
+    });
+}
+
+pgd_status pgd_sort_edges(pgd_graph* g, const char* ordering, int32_t s2l) {
+    if (!g) return PGD_ERR_INVALID;
+    if (!g->prepped) return PGD_ERR_STATE;
+    int kind = order_kind(ordering);
+    if (kind < 0) {
+        g->ctx->last_err = "ordering has no reproducible key in the reference (rand/var)";
+        return PGD_ERR_UNSUPPORTED;
+    }
+    return guarded(g->ctx, [&] {
+        pgd_ctx* ctx = g->ctx;
+        PGD_HIP(hipSetDevice(ctx->device));
+        const int64_t m = g->m;
+        if (kind == OK_KCORE || kind == OK_KCORE_DEG || kind == OK_KCORE_VOL || kind == OK_DEG_KCORE_VOL)
+            compute_cores(g);
+        g->order.alloc(ctx, std::max<int64_t>(m, 1));
+        if (m) {
+            DBuf<uint64_t> k0, k1;
+            k0.alloc(ctx, m);
+            k1.alloc(ctx, m);
+            int grid = std::min(grid_for(m), ctx->num_cus * 16);
+            hipLaunchKernelGGL(k_order_keys, dim3(grid), dim3(kBlock), 0, ctx->stream, g->ev.get(), g->eu.get(),
+                               g->deg.get(), g->have_core ? g->core.get() : nullptr, g->row.get(), g->col.get(), m,
+                               kind, s2l ? 1 : 0, k0.get());
+            PGD_HIP(hipGetLastError());
+            sort_keys(ctx, k0.get(), k1.get(), (size_t)m, 0, 64);
+            hipLaunchKernelGGL(k_order_from_keys, dim3(grid), dim3(kBlock), 0, ctx->stream, k1.get(), m,
+                               g->order.get());
+            PGD_HIP(hipGetLastError());
+        }
+        sync(ctx);
+        g->have_order = true;
+    });
+}
+
+pgd_status pgd_e_ordered(pgd_graph* g, int32_t* ids) {
+    if (!g || !ids) return PGD_ERR_INVALID;
+    if (!g->have_order) return PGD_ERR_STATE;
+    return guarded(g->ctx, [&] {
+        if (g->m)
+            PGD_HIP(hipMemcpyAsync(ids, g->order.get(), g->m * sizeof(int32_t), hipMemcpyDeviceToHost, g->ctx->stream));
+        sync(g->ctx);
+    });
+}
+
+pgd_status pgd_decompose(pgd_graph* g, uint32_t flags, uint64_t* sums) {
+    if (!g) return PGD_ERR_INVALID;
+    if (!g->prepped || !g->have_order) return PGD_ERR_STATE;
+    return guarded(g->ctx, [&] {
+        pgd_ctx* ctx = g->ctx;
+        PGD_HIP(hipSetDevice(ctx->device));
+        const int64_t m = std::max<int64_t>(g->m, 1);
+        if (g->T.n != (size_t)m) {
+            g->T.alloc(ctx, m);
+            g->K4.alloc(ctx, m);
+            g->C4.alloc(ctx, m);
+        }
+        if (!g->sums.get()) {
+            g->sums.alloc(ctx, PGD_NSUMS);
+            g->flags.alloc(ctx, 1);
+        }
+        const bool micro = (flags & PGD_MICRO) != 0;
+        if (micro && g->micro.n != (size_t)PGD_NMICRO * m) g->micro.alloc(ctx, (size_t)PGD_NMICRO * m);
+        GraphView v = g->view();
+        EdgeCounts c{g->T.get(), g->K4.get(), g->C4.get()};
+        hipEvent_t e0, e1;
+        PGD_HIP(hipEventCreate(&e0));
+        PGD_HIP(hipEventCreate(&e1));
+        PGD_HIP(hipEventRecord(e0, ctx->stream));
+        Timing t;
+        int engine;
+        if (flags & PGD_ENGINE_EDGE) engine = 1;
+        else if (flags & PGD_ENGINE_ORIENT) engine = 2;
+        else if (flags & PGD_ENGINE_DENSE) engine = 3;
+        else engine = engine_dense_ok(v) ? 3 : (g->n <= 65536 && g->max_deg <= 4096 ? 1 : 2);
+        if (g->m > 0) {
+            if (engine == 1) engine_edge(ctx, v, c, &t);
+            else if (engine == 2) engine_orient(ctx, v, c, &t, &g->orient);
+            else {
+                PGD_CHECK(engine_dense_ok(v), PGD_ERR_UNSUPPORTED, "graph too large for the dense engine");
+                engine_dense(ctx, v, c, &t);
+            }
+        }
+        epilogue(ctx, v, c, g->sums.get(), micro ? g->micro.get() : nullptr, g->flags.get());
+        PGD_HIP(hipEventRecord(e1, ctx->stream));
+        uint32_t inexact = 0;
+        PGD_HIP(hipMemcpyAsync(sums, g->sums.get(), PGD_NSUMS * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        PGD_HIP(hipMemcpyAsync(&inexact, g->flags.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        sync(ctx);
+        float ms = 0;
+        PGD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        t.total_ms = ms;
+        g->timing = t;
+        g->last_engine = engine;
+        g->have_counts = true;
+        g->have_micro = micro;
+        PGD_CHECK(!micro || inexact == 0, PGD_ERR_UNSUPPORTED,
+                  "a per-edge micro value exceeds 2^53; the exact double emulation for it is not implemented");
+    });
+}
+
+pgd_status pgd_micro(pgd_graph* g, int32_t column, uint64_t* out) {
+    if (!g || !out || column < 0 || column >= PGD_NMICRO) return PGD_ERR_INVALID;
+    if (!g->have_micro) return PGD_ERR_STATE;
+    return guarded(g->ctx, [&] {
+        if (g->m)
+            PGD_HIP(hipMemcpyAsync(out, g->micro.get() + (size_t)column * g->m, g->m * sizeof(uint64_t),
+                                   hipMemcpyDeviceToHost, g->ctx->stream));
+        sync(g->ctx);
+    });
+}
+
+pgd_status pgd_edge_counts(pgd_graph* g, uint64_t* tri, uint64_t* k4, uint64_t* c4) {
+    if (!g) return PGD_ERR_INVALID;
+    if (!g->have_counts) return PGD_ERR_STATE;
+    return guarded(g->ctx, [&] {
+        const int64_t m = g->m;
+        if (m && tri) {
+            std::vector<uint32_t> t(m);
+            PGD_HIP(hipMemcpyAsync(t.data(), g->T.get(), m * sizeof(uint32_t), hipMemcpyDeviceToHost, g->ctx->stream));
+            sync(g->ctx);
+            for (int64_t i = 0; i < m; i++) tri[i] = t[i];
+        }
+        if (m && k4)
+            PGD_HIP(hipMemcpyAsync(k4, g->K4.get(), m * sizeof(uint64_t), hipMemcpyDeviceToHost, g->ctx->stream));
+        if (m && c4)
+            PGD_HIP(hipMemcpyAsync(c4, g->C4.get(), m * sizeof(uint64_t), hipMemcpyDeviceToHost, g->ctx->stream));
+        sync(g->ctx);
+    });
+}
+
+int32_t pgd_last_engine(pgd_graph* g) { return g ? g->last_engine : 0; }
+
+pgd_status pgd_last_timing(pgd_graph* g, double* total_ms, double* dominant_ms, int32_t* dominant_launches) {
+    if (!g) return PGD_ERR_INVALID;
+    if (total_ms) *total_ms = g->timing.total_ms;
+    if (dominant_ms) *dominant_ms = g->timing.dominant_ms;
+    if (dominant_launches) *dominant_launches = g->timing.dominant_launches;
+    return PGD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+// pgd_internal.hpp -- shared host/device declarations of the MI355X
+// implementation (not part of the public ABI; see include/pgd_hip.h).
+#pragma once
+#include <cstring>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pgd_hip.h"
+
+namespace pgd {
+
+// ---------------------------------------------------------------------------
+// error plumbing: every HIP call goes through PGD_HIP, which records the
+// failing call in the context and returns PGD_ERR_HIP from the entry point.
+// ---------------------------------------------------------------------------
+struct Error {
+    pgd_status code;
+    std::string what;
+};
+
+#define PGD_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e__ = (call);                                                          \
+        if (e__ != hipSuccess)                                                            \
+            throw ::pgd::Error{PGD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e__)}; \
+    } while (0)
+
+#define PGD_CHECK(cond, code, msg)                                  \
+    do {                                                            \
+        if (!(cond)) throw ::pgd::Error{(code), std::string(msg)}; \
+    } while (0)
+
+}  // namespace pgd
+
+struct pgd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 256;
+    int64_t mem_cap = 0;   // 0: no cap
+    int64_t mem_used = 0;
+    std::string last_err;
+    // rocPRIM temporary storage, grown on demand
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+};
+
+namespace pgd {
+
+void* dmalloc(pgd_ctx* ctx, size_t bytes);
+void dfree(pgd_ctx* ctx, void* p, size_t bytes);
+void* tmp_storage(pgd_ctx* ctx, size_t bytes);
+
+// RAII device buffer charged against the context's cap
+template <typename T>
+struct DBuf {
+    pgd_ctx* ctx = nullptr;
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { reset(); }
+    void alloc(pgd_ctx* c, size_t count) {
+        reset();
+        ctx = c;
+        n = count;
+        p = (T*)dmalloc(c, (count ? count : 1) * sizeof(T));
+    }
+    void reset() {
+        if (p) dfree(ctx, p, (n ? n : 1) * sizeof(T));
+        p = nullptr;
+        n = 0;
+    }
+    T* get() const { return p; }
+};
+
+// Timing of the counting stage of the last decomposition
+struct Timing {
+    double total_ms = 0, dominant_ms = 0;
+    int dominant_launches = 0;
+};
+
+// Per-edge counts shared by every engine: indexed by edge id.
+struct EdgeCounts {
+    uint32_t* T;   // triangles on the edge
+    uint64_t* K4;  // 4-cliques on the edge
+    uint64_t* C4;  // induced 4-cycles on the edge
+};
+
+// Read-only view of a preprocessed graph as the engines see it.
+struct GraphView {
+    int64_t n, m, nnz;
+    int32_t max_deg;
+    const int64_t* row;   // n+1
+    const int32_t* col;   // nnz, reference neighbour order
+    const int32_t* src;   // nnz, vertex owning each CSR position
+    const int32_t* deg;   // n
+    const int32_t* ev;    // m
+    const int32_t* eu;    // m
+    const int32_t* order; // m, E_ordered
+};
+
+// engines (each writes EdgeCounts for every edge id)
+struct OrientState;
+void engine_edge(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t);
+void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t, OrientState** cache);
+void orient_state_free(pgd_ctx* ctx, OrientState* s);
+bool engine_dense_ok(const GraphView& g);
+void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t);
+
+// epilogue: the 15 sums (and the micro columns when micro != nullptr),
+// processed in E_ordered order.
+void epilogue(pgd_ctx* ctx, const GraphView& g, EdgeCounts c, uint64_t* d_sums, uint64_t* d_micro,
+              uint32_t* d_flags);
+
+// launch-geometry helpers
+inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+}  // namespace pgd
+
+struct pgd_graph {
+    pgd_ctx* ctx = nullptr;
+    int64_t n = 0, m = 0, nnz = 0;
+    int32_t max_deg = 0, avg_deg = 0;
+    pgd::DBuf<int64_t> row;
+    pgd::DBuf<int32_t> col, src, deg;
+    bool prepped = false;
+    pgd::DBuf<int32_t> ev, eu;
+    bool have_core = false;
+    int32_t K = 0;
+    pgd::DBuf<int32_t> core;
+    bool have_order = false;
+    pgd::DBuf<int32_t> order;
+    bool have_counts = false, have_micro = false;
+    pgd::DBuf<uint32_t> T;
+    pgd::DBuf<uint64_t> K4, C4, micro, sums;
+    pgd::DBuf<uint32_t> flags;
+    int last_engine = 0;
+    pgd::Timing timing;
+    pgd::OrientState* orient = nullptr;
+    pgd::GraphView view() const {
+        return pgd::GraphView{n, m, nnz, max_deg, row.get(), col.get(), src.get(), deg.get(),
+                              ev.get(), eu.get(), order.get()};
+    }
+};
