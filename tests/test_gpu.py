@@ -1,0 +1,164 @@
+"""Parity of the HIP path against the reference goldens and the oracle (MI355X).
+
+Every case runs the full device pipeline: CSR construction from the reader's
+COO, neighbour ordering, orientation, k-core, E_ordered, counting, the
+reduction and (host, integer) derivation, then compares bit for bit.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import pgd_amd
+from tests.oracle_lib import OracleGraph, run_pipeline
+from tests.test_oracle import GOLDENS, GOLD, flags_to_args, golden_graph_path
+
+pytestmark = pytest.mark.gpu
+
+ENGINES = ["edge"]
+
+
+@pytest.fixture(scope="module")
+def tmp():
+    d = tempfile.mkdtemp(prefix="pgd_gpu_")
+    yield d
+    shutil.rmtree(d)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def product_run(path, flags, ctx, engine, micro=True):
+    kw = flags_to_args(flags)
+    g = pgd_amd.GraphletCore.from_file(path, ctx=ctx, neigh_ordering=kw["neigh"],
+                                       neigh_small_to_large=kw["s2l_neigh"])
+    K = g.compute_cores()
+    g.sort_edges(kw["ordering"], kw["s2l"])
+    totals = g.graphlet_decomposition_micro(engine) if micro else g.graphlet_decomposition(engine)
+    return g, [totals[k] for k in pgd_amd.TOTAL_NAMES], K
+
+
+def micro_matrix(g) -> np.ndarray:
+    ev, eu = g.edge_arrays()
+    order = g.e_ordered()
+    cols = [ev[order].astype(np.uint64) + np.uint64(1), eu[order].astype(np.uint64) + np.uint64(1)]
+    cols += [g.micro(c) for c in ("tri", "2_star", "local_4_clique", "local_4_chordal_cycle", "local_4_tailed_tris",
+                                  "local_4_cycle", "local_3_star", "local_4_path")]
+    return np.stack(cols, axis=1) if g.m else np.zeros((0, 10), np.uint64)
+
+
+def micro_sha(mat: np.ndarray) -> str:
+    head = "% src,dst,triangle,2-star,4-clique,4-chordal-cycle,4-tailed-triangle,4-cycle,3-star,4-path\n"
+    body = "".join(",".join(map(str, row)) + "\n" for row in mat.tolist())
+    return hashlib.sha256((head + body).encode()).hexdigest()
+
+
+CASES = sorted(k for k, e in GOLDENS.items() if e.get("m", 0) <= 400000)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("name", CASES)
+def test_matches_reference_golden(name, engine, ctx, tmp):
+    ent = GOLDENS[name]
+    say(f"[gpu] golden {name} engine={engine} m={ent.get('m')}")
+    path = golden_graph_path(name, ent, tmp)
+    g, totals, K = product_run(path, ent["flags"], ctx, engine)
+    assert g.last_engine() == engine
+    assert totals == ent["totals"]
+    assert K == ent["K"]
+    mat = micro_matrix(g)
+    if "micro_file" in ent:
+        want = np.load(os.path.join(GOLD, ent["micro_file"]))["micro"]
+        assert np.array_equal(mat, want)
+    assert micro_sha(mat) == ent["micro_sha256"]
+    g.free()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("scale,seed", [(15, 5), (16, 6)])
+def test_rmat_per_edge_against_oracle(scale, seed, engine, ctx):
+    """Per-edge T/K4/C4 of every edge id against the oracle (R-MAT, n up to 65536,
+    which takes the HBM-bitmap path of the edge engine)."""
+    say(f"[gpu] rmat scale={scale} seed={seed} engine={engine}")
+    s, d = pgd_amd.gen_rmat(scale, 16, seed=seed)
+    n = int(max(s.max(), d.max())) + 1
+    g = pgd_amd.GraphletCore(s, d, n=n, ctx=ctx)
+    g.sort_edges("deg")
+    sums = g.decompose(micro=False, engine=engine)
+    T, K4, C4 = g.edge_counts()
+    og = OracleGraph.from_edges(s, d, n)
+    og.order_neighbors("deg")
+    ev, eu = og.edge_arrays()
+    ids = np.arange(len(ev), dtype=np.int32)
+    rng = np.random.default_rng(seed)
+    sample = np.sort(rng.choice(ids, size=min(len(ids), 4000), replace=False)).astype(np.int32)
+    res = og.decompose(ev, eu, sample, micro=True)
+    assert np.array_equal(T[sample], res["tri"])
+    assert np.array_equal(K4[sample], res["k4"])
+    assert np.array_equal(C4[sample], res["c4"])
+    assert sums[0] == int(T.sum()) and sums[3] == int(K4.sum()) and sums[4] == int(C4.sum())
+    g.free()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_deterministic_repeat(engine, ctx):
+    s, d = pgd_amd.gen_ba(4000, 6, 21)
+    g = pgd_amd.GraphletCore(s, d, ctx=ctx)
+    a = g.graphlet_decomposition_micro(engine)
+    ma = micro_matrix(g)
+    b = g.graphlet_decomposition_micro(engine)
+    mb = micro_matrix(g)
+    assert a == b
+    assert np.array_equal(ma, mb)
+    g.free()
+
+
+def test_engines_agree_per_edge(ctx):
+    if len(ENGINES) < 2:
+        pytest.skip("one engine")
+    s, d = pgd_amd.gen_rmat(13, 16, seed=9)
+    g = pgd_amd.GraphletCore(s, d, ctx=ctx)
+    g.sort_edges("deg")
+    out = {}
+    for e in ENGINES:
+        g.decompose(micro=False, engine=e)
+        out[e] = g.edge_counts()
+    for e in ENGINES[1:]:
+        for a, b in zip(out[ENGINES[0]], out[e]):
+            assert np.array_equal(a, b)
+    g.free()
+
+
+def test_invalid_ids_rejected(ctx):
+    s = np.array([0, 1, 5], dtype=np.int32)
+    d = np.array([1, 2, 1], dtype=np.int32)
+    with pytest.raises(pgd_amd.PGDError) as e:
+        pgd_amd.GraphletCore(s, d, n=3, ctx=ctx)
+    assert e.value.code == -1
+
+
+def test_memory_cap_enforced():
+    with pgd_amd.Context(0) as c:
+        c.set_mem_cap(1 << 16)
+        s, d = pgd_amd.gen_gnp(3000, 0.01, 2)
+        with pytest.raises(pgd_amd.PGDError) as e:
+            pgd_amd.GraphletCore(s, d, ctx=c)
+        assert e.value.code == -4
+
+
+def test_empty_and_tiny_graphs(ctx):
+    for s, d, n in [([], [], 0), ([0], [0], 1), ([0], [1], 2), ([0, 1], [1, 2], 5)]:
+        g = pgd_amd.GraphletCore(np.array(s, np.int32), np.array(d, np.int32), n=n, ctx=ctx)
+        tot = g.graphlet_decomposition_micro()
+        og = OracleGraph.from_edges(s, d, n)
+        want = run_pipeline(og, micro=False)["totals"]
+        assert [tot[k] for k in pgd_amd.TOTAL_NAMES] == want
+        g.free()

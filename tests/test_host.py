@@ -1,0 +1,133 @@
+"""Host-side checks of the product library (CPU only, no device calls)."""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import re
+import shutil
+import tempfile
+
+import numpy as np
+import pytest
+
+import pgd_amd
+from pgd_amd import _lib
+from tests.oracle_lib import OracleGraph
+from tests.oracle_lib import derive as oracle_derive
+from tests.test_oracle import GOLDENS, golden_graph_path
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols() -> list[str]:
+    text = open(os.path.join(ROOT, "include", "pgd_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pgd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    syms = header_symbols()
+    assert len(syms) >= 40
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTS) == syms
+
+
+def test_status_strings():
+    L = _lib.lib()
+    for code in range(0, -8, -1):
+        assert L.pgd_status_str(code)
+
+
+@pytest.fixture(scope="module")
+def tmp():
+    d = tempfile.mkdtemp(prefix="pgd_host_")
+    yield d
+    shutil.rmtree(d)
+
+
+def coo_to_csr(n, src, dst, dedup):
+    """Adjacency lists in line order; dedup keeps each pair's first line."""
+    lists = [[] for _ in range(n)]
+    seen = set()
+    for a, b in zip(src.tolist(), dst.tolist()):
+        if a == b:
+            continue
+        if dedup:
+            k = (min(a, b), max(a, b))
+            if k in seen:
+                continue
+            seen.add(k)
+        lists[a].append(b)
+        lists[b].append(a)
+    row = np.zeros(n + 1, dtype=np.int64)
+    row[1:] = np.cumsum([len(x) for x in lists])
+    col = np.array([y for x in lists for y in x], dtype=np.int32)
+    return row, col
+
+
+@pytest.mark.parametrize("name", sorted(k for k in GOLDENS if "@" not in k))
+def test_reader_matches_oracle_reader(name, tmp):
+    ent = GOLDENS[name]
+    if ent.get("m", 0) > 200000:
+        pytest.skip("large input")
+    path = golden_graph_path(name, ent, tmp)
+    n, src, dst, dedup, _ = pgd_amd.read_graph(path)
+    row, col = coo_to_csr(n, src, dst, dedup)
+    orow, ocol = OracleGraph.from_file(path).csr()
+    assert np.array_equal(row, orow)
+    assert np.array_equal(col, ocol)
+
+
+def test_reader_rejects_missing_file():
+    with pytest.raises(pgd_amd.PGDError):
+        pgd_amd.read_graph("/nonexistent/graph.mtx")
+
+
+def _rand_sums(rng, big: bool):
+    if big:
+        return [int(x) for x in rng.integers(0, 2**64, size=15, dtype=np.uint64)]
+    scale = 2 ** int(rng.integers(1, 63))
+    return [int(x) for x in rng.integers(0, scale, size=15, dtype=np.uint64)]
+
+
+def test_derive_emulation_matches_binary64():
+    """Integer derivation == the oracle's native binary64 derivation."""
+    rng = np.random.default_rng(7)
+    for i in range(20000):
+        n = int(rng.choice([int(rng.integers(0, 5000)), int(rng.integers(0, 2**22)), int(rng.integers(0, 2**31))]))
+        m = int(rng.integers(0, max(1, n * 20)))
+        s = _rand_sums(rng, big=i % 3 == 0)
+        assert pgd_amd.derive(n, m, s) == oracle_derive(n, m, s), (n, m, s)
+
+
+def test_derive_rounding_edge_values():
+    """Values around 2^53 and 2^64 where round-to-even and fl(1/3) matter."""
+    vals = []
+    for b in (52, 53, 54, 55, 60, 62, 63, 64):
+        for d in (-3, -2, -1, 0, 1, 2, 3, 5, 7):
+            vals.append(((1 << b) + d) % (1 << 64))
+    vals += [3 * (1 << 53) + 1, 3 * (1 << 53) + 2, 6 * (1 << 55) - 1, (1 << 64) - 1, (1 << 64) - 2048]
+    for a in vals:
+        for n in (5, 2**21 + 7, 2**24, 2**31 - 1):
+            s = [a] * 15
+            assert pgd_amd.derive(n, 1000, s) == oracle_derive(n, 1000, s)
+            s2 = [a if k % 2 else (a >> 3) for k in range(15)]
+            assert pgd_amd.derive(n, 3, s2) == oracle_derive(n, 3, s2)
+
+
+def test_generators_deterministic():
+    a = pgd_amd.gen_gnp(500, 0.05, 9)
+    b = pgd_amd.gen_gnp(500, 0.05, 9)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    s, d = pgd_amd.gen_rmat(12, 16, seed=4)
+    assert np.all(s < d)
+    key = s.astype(np.int64) * (1 << 32) + d
+    assert np.all(np.diff(key) > 0)  # sorted, no duplicates
+    s, d = pgd_amd.gen_ba(2000, 5, 3)
+    assert np.all(s != d)
+    assert len(set(zip(np.minimum(s, d).tolist(), np.maximum(s, d).tolist()))) == len(s)
+    m = len(pgd_amd.gen_gnp(2000, 0.01, 1)[0])
+    assert 18000 < m < 22000
