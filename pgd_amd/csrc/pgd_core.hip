@@ -875,7 +875,10 @@ pgd_status pgd_decompose(pgd_graph* g, uint32_t flags, uint64_t* sums) {
         if (flags & PGD_ENGINE_EDGE) engine = 1;
         else if (flags & PGD_ENGINE_ORIENT) engine = 2;
         else if (flags & PGD_ENGINE_DENSE) engine = 3;
-        else engine = engine_dense_ok(v) ? 3 : (g->n <= 65536 && g->max_deg <= 4096 ? 1 : 2);
+        else engine = engine_dense_ok(v) ? 3 : 2;
+        // the edge-centric engine hung on its first device run and stays
+        // disabled until the cause is found
+        PGD_CHECK(engine != 1, PGD_ERR_UNSUPPORTED, "edge-centric engine disabled");
         if (g->m > 0) {
             if (engine == 1) engine_edge(ctx, v, c, &t);
             else if (engine == 2) engine_orient(ctx, v, c, &t, &g->orient);

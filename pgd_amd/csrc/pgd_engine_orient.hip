@@ -1,9 +1,732 @@
-// pgd_engine_orient.hip -- degree-oriented engine (placeholder until written).
+// pgd_engine_orient.hip -- degree-oriented engine.
+//
+// Computes the same per-edge counts as the reference's edge loop
+// (graphlet_core.cpp:1408-1427: T, K4 = 4-cliques on the edge, C4 = induced
+// 4-cycles on the edge) without walking 2-hop neighbourhoods edge by edge.
+// Vertices are relabelled by rank (degree, id); every edge points from the
+// lower to the higher rank.  Then, for an edge e = (v,u):
+//
+//   T(e)  = #triangles on e
+//   K4(e) = #4-cliques on e
+//   P(e)  = #4-cycles (as subgraphs) through e = (A^3)_vu - d_v - d_u + 1
+//   Q(e)  = sum over triangles {v,u,w} of T(v,w) + T(u,w)
+//   C4(e) = P(e) - Q(e) + 2 T(e) + 2 K4(e)
+//
+// The last identity splits the 4-cycles v-x-w-u through e by their chords
+// (x~u, w~v): no chord is an induced 4-cycle, two chords a 4-clique (each
+// counted twice), one chord a diamond; summing |N(w) & (Sv u Su)| over w in
+// T(e) gives Q(e) - 2T(e) - 4K4(e) for the diamonds.
+//
+// Work, per pass:
+//   * triangles (T): each triangle {a<b<c} once, from its lowest vertex a,
+//     as an edge of a's local graph on N+(a) (the out-neighbours);
+//   * 4-cliques and Q: a's local graph as a bit matrix in LDS; the 4-cliques
+//     with lowest vertex a are its triangles, found with popcounts;
+//   * P: each 4-cycle once, from its highest vertex a, as a pair of wedges
+//     a-b-c with b,c < a (Chiba-Nishizeki); edge (a,b) gains cnt_a[c]-1 and
+//     edge (b,c) gains cnt_a[c]-1 per wedge.
+// With the degree order the out-degree stays small (about 10^3 on R-MAT
+// scale 22 against a maximum degree of about 1.6*10^5) and the wedge count
+// is O(m * arboricity), orders of magnitude below the reference's per-edge
+// 2-hop work.  All counts are exact uint64 (uint32 for T), accumulated with
+// atomics, so the result is independent of scheduling.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+
 #include "pgd_internal.hpp"
+
+using namespace pgd;
+
 namespace pgd {
-struct OrientState {};
-void engine_orient(pgd_ctx*, const GraphView&, EdgeCounts, Timing*, OrientState**) {
-    throw Error{PGD_ERR_UNSUPPORTED, "oriented engine not built"};
-}
+struct OrientState {
+    int dummy = 0;
+};
 void orient_state_free(pgd_ctx*, OrientState* s) { delete s; }
+}  // namespace pgd
+
+namespace {
+
+constexpr int kSmallK = 64;       // out-degree handled by one wavefront
+constexpr int kLightBound = 256;  // wedge bound handled by a wavefront with an LDS hash
+constexpr int kHashSlots = 512;
+constexpr int kChunk = 1024;      // in-neighbours per chunk in the heavy P kernel
+
+inline int grid_cap(pgd_ctx* ctx, int64_t items, int per_cu) {
+    int64_t g = std::min<int64_t>(items, (int64_t)ctx->num_cus * per_cu);
+    return (int)std::max<int64_t>(g, 1);
+}
+
+template <class T, class U>
+void excl_sum(pgd_ctx* ctx, const T* in, U* out, size_t n) {
+    if (!n) return;
+    size_t bytes = 0;
+    PGD_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, U(0), n, rocprim::plus<U>(), ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::exclusive_scan(t, bytes, in, out, U(0), n, rocprim::plus<U>(), ctx->stream));
+}
+
+template <class K, class V>
+void sort_kv(pgd_ctx* ctx, K* kin, K* kout, V* vin, V* vout, size_t n, unsigned b1) {
+    if (!n) return;
+    size_t bytes = 0;
+    PGD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, b1, ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, 0u, b1, ctx->stream));
+}
+
+template <class K>
+void sort_k(pgd_ctx* ctx, K* kin, K* kout, size_t n, unsigned b1) {
+    if (!n) return;
+    size_t bytes = 0;
+    PGD_HIP(rocprim::radix_sort_keys(nullptr, bytes, kin, kout, n, 0u, b1, ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::radix_sort_keys(t, bytes, kin, kout, n, 0u, b1, ctx->stream));
+}
+
+int bitsfor(int64_t v) {
+    int b = 1;
+    while (b < 63 && ((int64_t)1 << b) <= v) b++;
+    return b;
+}
+
+#define GRID_STRIDE(i, n) \
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+// ---------------------------------------------------------------------------
+// relabelled, sorted CSR
+// ---------------------------------------------------------------------------
+__global__ void k_rank_keys(const int32_t* __restrict__ deg, int64_t n, uint64_t* __restrict__ key) {
+    GRID_STRIDE(v, n) key[v] = ((uint64_t)(uint32_t)deg[v] << 32) | (uint32_t)v;
+}
+
+__global__ void k_rank_scatter(const uint64_t* __restrict__ sorted, int64_t n, int32_t* __restrict__ rank,
+                               int32_t* __restrict__ rdeg) {
+    GRID_STRIDE(i, n) {
+        uint64_t k = sorted[i];
+        rank[(uint32_t)(k & 0xffffffffu)] = (int32_t)i;
+        rdeg[i] = (int32_t)(k >> 32);
+    }
+}
+
+// two slots per edge id: (rank(ev) -> rank(eu)) and the reverse, value = edge id
+__global__ void k_slot_keys(const int32_t* __restrict__ ev, const int32_t* __restrict__ eu,
+                            const int32_t* __restrict__ rank, int64_t m, uint64_t* __restrict__ key,
+                            int32_t* __restrict__ val) {
+    GRID_STRIDE(e, m) {
+        uint32_t a = (uint32_t)rank[ev[e]], b = (uint32_t)rank[eu[e]];
+        key[2 * e] = ((uint64_t)a << 32) | b;
+        key[2 * e + 1] = ((uint64_t)b << 32) | a;
+        val[2 * e] = (int32_t)e;
+        val[2 * e + 1] = (int32_t)e;
+    }
+}
+
+__global__ void k_slot_cols(const uint64_t* __restrict__ key, int64_t nnz, int32_t* __restrict__ col) {
+    GRID_STRIDE(p, nnz) col[p] = (int32_t)(key[p] & 0xffffffffu);
+}
+
+__global__ void k_row_tail(int64_t* row, int64_t n, int64_t nnz) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) row[n] = nnz;
+}
+
+// ostart[a]: first slot of N(a) holding a higher rank (lists are ascending)
+__global__ void k_out_start(const int64_t* __restrict__ row, const int32_t* __restrict__ col, int64_t n,
+                            int64_t* __restrict__ ostart, int32_t* __restrict__ outdeg) {
+    GRID_STRIDE(a, n) {
+        int64_t lo = row[a], hi = row[a + 1];
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (col[mid] <= a) lo = mid + 1;
+            else hi = mid;
+        }
+        ostart[a] = lo;
+        outdeg[a] = (int32_t)(row[a + 1] - lo);
+    }
+}
+
+// upper bound of the wedges a-b-c (b,c < a): sum of d(b) over in-neighbours b
+__global__ void k_wedge_bound(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                              const int64_t* __restrict__ ostart, int64_t n, int64_t* __restrict__ ub) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t a = wave; a < n; a += nw) {
+        int64_t s = 0;
+        for (int64_t p = row[a] + lane; p < ostart[a]; p += 64) {
+            int32_t b = col[p];
+            s += row[b + 1] - row[b];
+        }
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) ub[a] = s;
+    }
+}
+
+// bin flags: 1 if lo < key <= hi
+__global__ void k_bin_flags_i32(const int32_t* __restrict__ key, int64_t n, int64_t lo, int64_t hi,
+                                int32_t* __restrict__ f) {
+    GRID_STRIDE(a, n) f[a] = key[a] > lo && key[a] <= hi;
+}
+__global__ void k_bin_flags_i64(const int64_t* __restrict__ key, int64_t n, int64_t lo, int64_t hi,
+                                int32_t* __restrict__ f) {
+    GRID_STRIDE(a, n) f[a] = key[a] > lo && key[a] <= hi;
+}
+__global__ void k_bin_scatter(const int32_t* __restrict__ f, const int64_t* __restrict__ off, int64_t n,
+                              int32_t* __restrict__ out) {
+    GRID_STRIDE(a, n) if (f[a]) out[off[a]] = (int32_t)a;
+}
+
+// ---------------------------------------------------------------------------
+// block-level helpers (blockDim.x == NT)
+// ---------------------------------------------------------------------------
+// inclusive scan of s[0..k) in place (int64), NT threads, k arbitrary
+template <int NT>
+__device__ void block_incl_scan(int64_t* s, int k, int64_t* part) {
+    const int tid = threadIdx.x;
+    const int per = (k + NT - 1) / NT;
+    const int b = tid * per, e = min(b + per, k);
+    int64_t acc = 0;
+    for (int i = b; i < e; i++) { acc += s[i]; s[i] = acc; }
+    part[tid] = acc;
+    __syncthreads();
+    for (int off = 1; off < NT; off <<= 1) {
+        int64_t y = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += y;
+        __syncthreads();
+    }
+    int64_t base = tid ? part[tid - 1] : 0;
+    for (int i = b; i < e; i++) s[i] += base;
+    __syncthreads();
+}
+
+// first index i in [0,k) with incl[i] > t (incl nondecreasing, incl[k-1] > t)
+__device__ __forceinline__ int owner_of(const int64_t* incl, int k, int64_t t) {
+    int lo = 0, hi = k - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (incl[mid] > t) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// index of c in L[lo..hi) (ascending) or -1
+__device__ __forceinline__ int find_in(const int32_t* L, int lo, int hi, int32_t c) {
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        int32_t x = L[mid];
+        if (x < c) lo = mid + 1;
+        else if (x > c) hi = mid;
+        else return mid;
+    }
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// local-graph kernels (triangles; 4-cliques + Q).  One block per vertex a of
+// the list; the block owns the local graph on N+(a), k = |N+(a)| <= KMAX.
+// Storage for the per-vertex arrays is either LDS (kGlobal == false) or a
+// per-block slice of HBM scratch (kGlobal == true, for k > KMAX).
+// ---------------------------------------------------------------------------
+struct LocalArrays {
+    int32_t* L;        // N+(a), ascending          [kmax]
+    int32_t* Le;       // edge id of a -> L[i]       [kmax]
+    int64_t* obeg;     // first out-slot of L[i]     [kmax]
+    int64_t* pref;     // inclusive scan of |N+(L[i])| [kmax]
+    uint32_t* cnt;     // per-local-vertex counter   [kmax]
+    uint64_t* acc;     // per-local-vertex u64 accumulator [kmax]
+    uint32_t* M;       // bit matrix, kmax x W        [kmax*W]
+};
+
+__device__ __forceinline__ LocalArrays carve(char* base, int kmax, int W) {
+    LocalArrays r;
+    char* p = base;
+    r.pref = (int64_t*)p; p += (size_t)kmax * 8;
+    r.obeg = (int64_t*)p; p += (size_t)kmax * 8;
+    r.acc = (uint64_t*)p; p += (size_t)kmax * 8;
+    r.L = (int32_t*)p; p += (size_t)kmax * 4;
+    r.Le = (int32_t*)p; p += (size_t)kmax * 4;
+    r.cnt = (uint32_t*)p; p += (size_t)kmax * 4;
+    r.M = (uint32_t*)p;
+    (void)W;
+    return r;
+}
+// rounded to 256 B so every per-block HBM slice keeps 8-byte alignment for the u64 atomics
+__host__ __device__ inline size_t carve_bytes(int kmax, int W) {
+    return ((size_t)kmax * 36 + (size_t)kmax * W * 4 + 255) & ~(size_t)255;
+}
+
+// load N+(a) and the flattened out-list prefix; returns total local work
+template <int NT>
+__device__ int64_t load_local(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                              const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart, int32_t a,
+                              int k, LocalArrays& A, int64_t* part) {
+    const int64_t s0 = ostart[a];
+    for (int i = threadIdx.x; i < k; i += NT) {
+        int32_t b = col[s0 + i];
+        A.L[i] = b;
+        A.Le[i] = eid[s0 + i];
+        int64_t ob = ostart[b];
+        A.obeg[i] = ob;
+        A.pref[i] = row[b + 1] - ob;
+        A.cnt[i] = 0;
+        A.acc[i] = 0;
+    }
+    __syncthreads();
+    block_incl_scan<NT>(A.pref, k, part);
+    return A.pref[k - 1];
+}
+
+// pass 1: triangles.  Local edge (i,j), i<j <=> triangle {a, L[i], L[j]}.
+template <int NT, bool kGlobal>
+__global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                                                  const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart,
+                                                  const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
+                                                  int64_t cnt, int kmax, char* __restrict__ gscratch,
+                                                  uint32_t* __restrict__ T) {
+    extern __shared__ char lds[];
+    __shared__ int64_t part[NT];
+    const int W = (kmax + 31) / 32;
+    char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, W) : lds;
+    LocalArrays A = carve(base, kmax, W);
+    for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
+        const int32_t a = list[li];
+        const int k = outdeg[a];
+        const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
+        for (int64_t t = threadIdx.x; t < total; t += NT) {
+            const int i = owner_of(A.pref, k, t);
+            const int64_t ex = i ? A.pref[i - 1] : 0;
+            const int64_t slot = A.obeg[i] + (t - ex);
+            const int j = find_in(A.L, i + 1, k, col[slot]);
+            if (j >= 0) {
+                atomicAdd(&T[eid[slot]], 1u);
+                atomicAdd(&A.cnt[i], 1u);
+                atomicAdd(&A.cnt[j], 1u);
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < k; i += NT)
+            if (A.cnt[i]) atomicAdd(&T[A.Le[i]], A.cnt[i]);
+        __syncthreads();
+    }
+}
+
+// pass 2: 4-cliques and Q, with the local bit matrix
+template <int NT, bool kGlobal>
+__global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                                                  const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart,
+                                                  const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
+                                                  int64_t cnt, int kmax, char* __restrict__ gscratch,
+                                                  const uint32_t* __restrict__ T, uint64_t* __restrict__ K4,
+                                                  uint64_t* __restrict__ Q) {
+    extern __shared__ char lds[];
+    __shared__ int64_t part[NT];
+    const int Wmax = (kmax + 31) / 32;
+    char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, Wmax) : lds;
+    LocalArrays A = carve(base, kmax, Wmax);
+    for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
+        const int32_t a = list[li];
+        const int k = outdeg[a];
+        const int W = (k + 31) / 32;
+        for (int w = threadIdx.x; w < k * W; w += NT) A.M[w] = 0;
+        const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
+        // build the symmetric adjacency of the local graph
+        for (int64_t t = threadIdx.x; t < total; t += NT) {
+            const int i = owner_of(A.pref, k, t);
+            const int64_t ex = i ? A.pref[i - 1] : 0;
+            const int64_t slot = A.obeg[i] + (t - ex);
+            const int j = find_in(A.L, i + 1, k, col[slot]);
+            if (j >= 0) {
+                atomicOr(&A.M[i * W + (j >> 5)], 1u << (j & 31));
+                atomicOr(&A.M[j * W + (i >> 5)], 1u << (i & 31));
+            }
+        }
+        __syncthreads();
+        for (int64_t t = threadIdx.x; t < total; t += NT) {
+            const int i = owner_of(A.pref, k, t);
+            const int64_t ex = i ? A.pref[i - 1] : 0;
+            const int64_t slot = A.obeg[i] + (t - ex);
+            const int j = find_in(A.L, i + 1, k, col[slot]);
+            if (j >= 0) {
+                uint32_t kij = 0;
+                for (int w = 0; w < W; w++) kij += __popc(A.M[i * W + w] & A.M[j * W + w]);
+                const int32_t eij = eid[slot];
+                const uint64_t tai = T[A.Le[i]], taj = T[A.Le[j]], tij = T[eij];
+                if (kij) {
+                    atomicAdd((unsigned long long*)&K4[eij], (unsigned long long)kij);
+                    atomicAdd(&A.cnt[i], kij);
+                    atomicAdd(&A.cnt[j], kij);
+                }
+                atomicAdd((unsigned long long*)&Q[eij], (unsigned long long)(tai + taj));
+                atomicAdd((unsigned long long*)&A.acc[i], (unsigned long long)(taj + tij));
+                atomicAdd((unsigned long long*)&A.acc[j], (unsigned long long)(tai + tij));
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < k; i += NT) {
+            // each local triangle through i is seen from its two edges at i
+            if (A.cnt[i]) atomicAdd((unsigned long long*)&K4[A.Le[i]], (unsigned long long)(A.cnt[i] >> 1));
+            if (A.acc[i]) atomicAdd((unsigned long long*)&Q[A.Le[i]], (unsigned long long)A.acc[i]);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P: wedges a-b-c with b, c < a.  In-neighbours of a are the prefix
+// [row[a], ostart[a]) of its ascending list; for each b the c < a are the
+// prefix of N(b) below a.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t lower_bound_g(const int32_t* __restrict__ col, int64_t lo, int64_t hi, int32_t x) {
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (col[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t hslot(int32_t c) { return ((uint32_t)c * 2654435761u) >> 23; }  // 9 bits
+
+// light vertices: one wavefront (block of 64) per vertex, an LDS hash of
+// (c -> wedge count); distinct c <= bound <= 256 < 512 slots
+__global__ __launch_bounds__(64) void k_wedge_light(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                                                    const int32_t* __restrict__ eid,
+                                                    const int64_t* __restrict__ ostart,
+                                                    const int32_t* __restrict__ list, int64_t cnt,
+                                                    uint64_t* __restrict__ P) {
+    __shared__ int32_t hkey[kHashSlots];
+    __shared__ uint32_t hcnt[kHashSlots];
+    __shared__ int64_t pref[64];
+    __shared__ int64_t bbeg[64];
+    __shared__ uint64_t psum[64];
+    __shared__ int64_t part[64];
+    const int tid = threadIdx.x;
+    for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
+        const int32_t a = list[li];
+        const int64_t ib = row[a], ie = ostart[a];
+        for (int s = tid; s < kHashSlots; s += 64) { hkey[s] = -1; hcnt[s] = 0; }
+        __syncthreads();
+        for (int pass = 0; pass < 2; pass++) {
+            for (int64_t cb = ib; cb < ie; cb += 64) {
+                const int kc = (int)min<int64_t>(64, ie - cb);
+                if (tid < kc) {
+                    int32_t b = col[cb + tid];
+                    int64_t bb = row[b];
+                    bbeg[tid] = bb;
+                    pref[tid] = lower_bound_g(col, bb, row[b + 1], a) - bb;
+                    psum[tid] = 0;
+                }
+                __syncthreads();
+                block_incl_scan<64>(pref, kc, part);
+                const int64_t total = pref[kc - 1];
+                for (int64_t t = tid; t < total; t += 64) {
+                    const int i = owner_of(pref, kc, t);
+                    const int64_t ex = i ? pref[i - 1] : 0;
+                    const int64_t slot = bbeg[i] + (t - ex);
+                    const int32_t c = col[slot];
+                    uint32_t h = hslot(c);
+                    if (pass == 0) {
+                        for (;;) {
+                            int32_t old = atomicCAS(&hkey[h], -1, c);
+                            if (old == -1 || old == c) { atomicAdd(&hcnt[h], 1u); break; }
+                            h = (h + 1) & (kHashSlots - 1);
+                        }
+                    } else {
+                        while (hkey[h] != c) h = (h + 1) & (kHashSlots - 1);
+                        const uint32_t s = hcnt[h] - 1;
+                        if (s) {
+                            atomicAdd((unsigned long long*)&P[eid[slot]], (unsigned long long)s);
+                            atomicAdd((unsigned long long*)&psum[i], (unsigned long long)s);
+                        }
+                    }
+                }
+                __syncthreads();
+                if (pass == 1 && tid < kc && psum[tid])
+                    atomicAdd((unsigned long long*)&P[eid[cb + tid]], (unsigned long long)psum[tid]);
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// heavy vertices: one block per vertex with a dense counter array of n
+// entries in HBM (a slice per block), kept zero between vertices
+template <int NT>
+__global__ __launch_bounds__(NT) void k_wedge_heavy(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                                                    const int32_t* __restrict__ eid,
+                                                    const int64_t* __restrict__ ostart,
+                                                    const int32_t* __restrict__ list, int64_t cnt, int64_t n,
+                                                    uint32_t* __restrict__ dense, uint64_t* __restrict__ P) {
+    __shared__ int64_t pref[kChunk];
+    __shared__ int64_t bbeg[kChunk];
+    __shared__ uint64_t psum[kChunk];
+    __shared__ int64_t part[NT];
+    const int tid = threadIdx.x;
+    uint32_t* cntarr = dense + (size_t)blockIdx.x * (size_t)n;
+    for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
+        const int32_t a = list[li];
+        const int64_t ib = row[a], ie = ostart[a];
+        for (int pass = 0; pass < 3; pass++) {  // count, accumulate, clear
+            for (int64_t cb = ib; cb < ie; cb += kChunk) {
+                const int kc = (int)min<int64_t>(kChunk, ie - cb);
+                for (int i = tid; i < kc; i += NT) {
+                    int32_t b = col[cb + i];
+                    int64_t bb = row[b];
+                    bbeg[i] = bb;
+                    pref[i] = lower_bound_g(col, bb, row[b + 1], a) - bb;
+                    psum[i] = 0;
+                }
+                __syncthreads();
+                block_incl_scan<NT>(pref, kc, part);
+                const int64_t total = pref[kc - 1];
+                for (int64_t t = tid; t < total; t += NT) {
+                    const int i = owner_of(pref, kc, t);
+                    const int64_t ex = i ? pref[i - 1] : 0;
+                    const int64_t slot = bbeg[i] + (t - ex);
+                    const int32_t c = col[slot];
+                    if (pass == 0) {
+                        atomicAdd(&cntarr[c], 1u);
+                    } else if (pass == 1) {
+                        const uint32_t s = __hip_atomic_load(&cntarr[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - 1;
+                        if (s) {
+                            atomicAdd((unsigned long long*)&P[eid[slot]], (unsigned long long)s);
+                            atomicAdd((unsigned long long*)&psum[i], (unsigned long long)s);
+                        }
+                    } else {
+                        __hip_atomic_store(&cntarr[c], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+                __syncthreads();
+                if (pass == 1)
+                    for (int i = tid; i < kc; i += NT)
+                        if (psum[i]) atomicAdd((unsigned long long*)&P[eid[cb + i]], (unsigned long long)psum[i]);
+                __syncthreads();
+            }
+            __threadfence();
+            __syncthreads();
+        }
+    }
+}
+
+// C4 = P - Q + 2T + 2K4 (uint64 wrap-around; the true value is >= 0)
+__global__ void k_combine(const uint64_t* __restrict__ P, const uint64_t* __restrict__ Q,
+                          const uint32_t* __restrict__ T, const uint64_t* __restrict__ K4, int64_t m,
+                          uint64_t* __restrict__ C4) {
+    GRID_STRIDE(e, m) C4[e] = P[e] - Q[e] + 2 * (uint64_t)T[e] + 2 * K4[e];
+}
+
+struct Timer {
+    pgd_ctx* ctx;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<const char*> names;
+    void begin(const char* name) {
+        hipEvent_t a, b;
+        PGD_HIP(hipEventCreate(&a));
+        PGD_HIP(hipEventCreate(&b));
+        PGD_HIP(hipEventRecord(a, ctx->stream));
+        ev.push_back({a, b});
+        names.push_back(name);
+    }
+    void end() { PGD_HIP(hipEventRecord(ev.back().second, ctx->stream)); }
+    ~Timer() {
+        for (auto& p : ev) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+    }
+};
+
+// compaction of the vertices whose key falls in (lo, hi]
+template <class K>
+int64_t make_bin(pgd_ctx* ctx, const K* key, int64_t n, int64_t lo, int64_t hi, DBuf<int32_t>& out,
+                 DBuf<int32_t>& flags, DBuf<int64_t>& off) {
+    hipStream_t s = ctx->stream;
+    int g = grid_cap(ctx, (n + 255) / 256, 16);
+    if constexpr (sizeof(K) == 4)
+        hipLaunchKernelGGL(k_bin_flags_i32, dim3(g), dim3(256), 0, s, (const int32_t*)key, n, lo, hi, flags.get());
+    else
+        hipLaunchKernelGGL(k_bin_flags_i64, dim3(g), dim3(256), 0, s, (const int64_t*)key, n, lo, hi, flags.get());
+    PGD_HIP(hipGetLastError());
+    excl_sum(ctx, flags.get(), off.get(), (size_t)n);
+    int64_t last_off = 0;
+    int32_t last_f = 0;
+    PGD_HIP(hipMemcpyAsync(&last_off, off.get() + (n - 1), sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PGD_HIP(hipMemcpyAsync(&last_f, flags.get() + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PGD_HIP(hipStreamSynchronize(s));
+    int64_t c = last_off + last_f;
+    out.alloc(ctx, std::max<int64_t>(c, 1));
+    if (c) {
+        hipLaunchKernelGGL(k_bin_scatter, dim3(g), dim3(256), 0, s, flags.get(), off.get(), n, out.get());
+        PGD_HIP(hipGetLastError());
+    }
+    return c;
+}
+
+}  // namespace
+
+namespace pgd {
+
+void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm, OrientState** cache) {
+    (void)cache;
+    hipStream_t s = ctx->stream;
+    const int64_t n = g.n, m = g.m, nnz = 2 * m;
+    Timer timer{ctx};
+    // ---- relabel by (degree, id) and build the ascending CSR ----------------
+    DBuf<int32_t> rank, rdeg, rcol, reid, outdeg;
+    DBuf<int64_t> rrow, ostart;
+    rank.alloc(ctx, n);
+    rdeg.alloc(ctx, n);
+    rrow.alloc(ctx, n + 1);
+    ostart.alloc(ctx, n);
+    outdeg.alloc(ctx, n);
+    rcol.alloc(ctx, nnz);
+    reid.alloc(ctx, nnz);
+    {
+        DBuf<uint64_t> k0, k1;
+        k0.alloc(ctx, std::max(n, nnz));
+        k1.alloc(ctx, std::max(n, nnz));
+        int gn = grid_cap(ctx, (n + 255) / 256, 16), gm = grid_cap(ctx, (m + 255) / 256, 16);
+        hipLaunchKernelGGL(k_rank_keys, dim3(gn), dim3(256), 0, s, g.deg, n, k0.get());
+        PGD_HIP(hipGetLastError());
+        sort_k(ctx, k0.get(), k1.get(), (size_t)n, 64);
+        hipLaunchKernelGGL(k_rank_scatter, dim3(gn), dim3(256), 0, s, k1.get(), n, rank.get(), rdeg.get());
+        PGD_HIP(hipGetLastError());
+        DBuf<int32_t> v0;
+        v0.alloc(ctx, nnz);
+        hipLaunchKernelGGL(k_slot_keys, dim3(gm), dim3(256), 0, s, g.ev, g.eu, rank.get(), m, k0.get(), v0.get());
+        PGD_HIP(hipGetLastError());
+        sort_kv(ctx, k0.get(), k1.get(), v0.get(), reid.get(), (size_t)nnz, 32u + (unsigned)bitsfor(n - 1));
+        int gz = grid_cap(ctx, (nnz + 255) / 256, 16);
+        hipLaunchKernelGGL(k_slot_cols, dim3(gz), dim3(256), 0, s, k1.get(), nnz, rcol.get());
+        PGD_HIP(hipGetLastError());
+        excl_sum(ctx, rdeg.get(), rrow.get(), (size_t)n);
+        hipLaunchKernelGGL(k_row_tail, dim3(1), dim3(1), 0, s, rrow.get(), n, nnz);
+        hipLaunchKernelGGL(k_out_start, dim3(gn), dim3(256), 0, s, rrow.get(), rcol.get(), n, ostart.get(),
+                           outdeg.get());
+        PGD_HIP(hipGetLastError());
+    }
+    // ---- per-edge accumulators ---------------------------------------------
+    DBuf<uint64_t> P, Q;
+    P.alloc(ctx, m);
+    Q.alloc(ctx, m);
+    PGD_HIP(hipMemsetAsync(out.T, 0, m * sizeof(uint32_t), s));
+    PGD_HIP(hipMemsetAsync(out.K4, 0, m * sizeof(uint64_t), s));
+    PGD_HIP(hipMemsetAsync(P.get(), 0, m * sizeof(uint64_t), s));
+    PGD_HIP(hipMemsetAsync(Q.get(), 0, m * sizeof(uint64_t), s));
+    // ---- vertex bins ---------------------------------------------------------
+    DBuf<int32_t> flags, bsmall, blarge, bhuge, blight, bheavy;
+    DBuf<int64_t> off, ub;
+    flags.alloc(ctx, n);
+    off.alloc(ctx, n);
+    ub.alloc(ctx, n);
+    int32_t maxk = 0;
+    {
+        DBuf<int32_t> mx;
+        mx.alloc(ctx, 1);
+        size_t bytes = 0;
+        PGD_HIP(rocprim::reduce(nullptr, bytes, outdeg.get(), mx.get(), 0, (size_t)n, rocprim::maximum<int32_t>(), s));
+        void* t = tmp_storage(ctx, bytes);
+        PGD_HIP(rocprim::reduce(t, bytes, outdeg.get(), mx.get(), 0, (size_t)n, rocprim::maximum<int32_t>(), s));
+        PGD_HIP(hipMemcpyAsync(&maxk, mx.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        PGD_HIP(hipStreamSynchronize(s));
+    }
+    // largest local graph that fits the LDS of one block
+    hipDeviceProp_t prop;
+    PGD_HIP(hipGetDeviceProperties(&prop, ctx->device));
+    const size_t lds_cap = std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024) - 4096;
+    int kmax_lds = 32;
+    while (carve_bytes(kmax_lds + 32, (kmax_lds + 32 + 31) / 32) <= lds_cap) kmax_lds += 32;
+    const int64_t nsmall = make_bin(ctx, outdeg.get(), n, 1, kSmallK, bsmall, flags, off);
+    const int64_t nlarge = make_bin(ctx, outdeg.get(), n, kSmallK, kmax_lds, blarge, flags, off);
+    const int64_t nhuge = make_bin(ctx, outdeg.get(), n, kmax_lds, INT32_MAX, bhuge, flags, off);
+    {
+        int gw = grid_cap(ctx, (n * 64 + 255) / 256, 16);
+        hipLaunchKernelGGL(k_wedge_bound, dim3(gw), dim3(256), 0, s, rrow.get(), rcol.get(), ostart.get(), n,
+                           ub.get());
+        PGD_HIP(hipGetLastError());
+    }
+    const int64_t nlight = make_bin(ctx, ub.get(), n, 0, kLightBound, blight, flags, off);
+    const int64_t nheavy = make_bin(ctx, ub.get(), n, kLightBound, INT64_MAX, bheavy, flags, off);
+    const size_t small_lds = carve_bytes(kSmallK, kSmallK / 32);
+    const size_t large_lds = carve_bytes(kmax_lds, (kmax_lds + 31) / 32);
+    if (nlarge) {
+        PGD_HIP(hipFuncSetAttribute((const void*)k_local_tri<256, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)large_lds));
+        PGD_HIP(hipFuncSetAttribute((const void*)k_local_k4q<256, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)large_lds));
+    }
+    DBuf<char> hscratch;
+    int ghuge = 0;
+    if (nhuge) {
+        ghuge = (int)std::min<int64_t>(nhuge, ctx->num_cus);
+        hscratch.alloc(ctx, (size_t)ghuge * carve_bytes(maxk, (maxk + 31) / 32));
+    }
+    // ---- pass 1: triangles ---------------------------------------------------
+    timer.begin("triangles");
+    if (nsmall)
+        hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, nsmall, 32)), dim3(64), small_lds, s,
+                           rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), bsmall.get(), nsmall,
+                           kSmallK, (char*)nullptr, out.T);
+    if (nlarge)
+        hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, nlarge, 2)), dim3(256), large_lds, s,
+                           rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), blarge.get(), nlarge,
+                           kmax_lds, (char*)nullptr, out.T);
+    if (nhuge)
+        hipLaunchKernelGGL((k_local_tri<256, true>), dim3(ghuge), dim3(256), 0, s, rrow.get(), rcol.get(), reid.get(),
+                           ostart.get(), outdeg.get(), bhuge.get(), nhuge, maxk, hscratch.get(), out.T);
+    PGD_HIP(hipGetLastError());
+    timer.end();
+    // ---- pass 2: 4-cliques and Q -----------------------------------------------
+    timer.begin("cliques");
+    if (nsmall)
+        hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, nsmall, 32)), dim3(64), small_lds, s,
+                           rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), bsmall.get(), nsmall,
+                           kSmallK, (char*)nullptr, out.T, out.K4, Q.get());
+    if (nlarge)
+        hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, nlarge, 2)), dim3(256), large_lds, s,
+                           rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), blarge.get(), nlarge,
+                           kmax_lds, (char*)nullptr, out.T, out.K4, Q.get());
+    if (nhuge)
+        hipLaunchKernelGGL((k_local_k4q<256, true>), dim3(ghuge), dim3(256), 0, s, rrow.get(), rcol.get(),
+                           reid.get(), ostart.get(), outdeg.get(), bhuge.get(), nhuge, maxk, hscratch.get(), out.T,
+                           out.K4, Q.get());
+    PGD_HIP(hipGetLastError());
+    timer.end();
+    // ---- pass 3: 4-cycles (P) ----------------------------------------------------
+    DBuf<uint32_t> dense;
+    int gheavy = 0;
+    if (nheavy) {
+        const int64_t budget = (int64_t)16 << 30;  // bytes of dense counters
+        gheavy = (int)std::max<int64_t>(1, std::min<int64_t>({nheavy, (int64_t)ctx->num_cus * 2, budget / (4 * n)}));
+        dense.alloc(ctx, (size_t)gheavy * n);
+        PGD_HIP(hipMemsetAsync(dense.get(), 0, (size_t)gheavy * n * sizeof(uint32_t), s));
+    }
+    timer.begin("wedges");
+    if (nlight)
+        hipLaunchKernelGGL(k_wedge_light, dim3(grid_cap(ctx, nlight, 32)), dim3(64), 0, s, rrow.get(), rcol.get(),
+                           reid.get(), ostart.get(), blight.get(), nlight, P.get());
+    if (nheavy)
+        hipLaunchKernelGGL(k_wedge_heavy<256>, dim3(gheavy), dim3(256), 0, s, rrow.get(), rcol.get(), reid.get(),
+                           ostart.get(), bheavy.get(), nheavy, n, dense.get(), P.get());
+    PGD_HIP(hipGetLastError());
+    timer.end();
+    timer.begin("combine");
+    hipLaunchKernelGGL(k_combine, dim3(grid_cap(ctx, (m + 255) / 256, 16)), dim3(256), 0, s, P.get(), Q.get(),
+                       out.T, out.K4, m, out.C4);
+    PGD_HIP(hipGetLastError());
+    timer.end();
+    PGD_HIP(hipStreamSynchronize(s));
+    double best = 0;
+    for (size_t i = 0; i < timer.ev.size(); i++) {
+        float ms = 0;
+        PGD_HIP(hipEventElapsedTime(&ms, timer.ev[i].first, timer.ev[i].second));
+        if (ms > best) best = ms;
+    }
+    tm->dominant_ms = best;
+    tm->dominant_launches = 1;
+}
+
 }  // namespace pgd

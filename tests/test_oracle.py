@@ -59,7 +59,7 @@ def golden_graph_path(name: str, ent: dict, tmp: str) -> str:
     if "input" in ent:
         return input_path(ent["input"], tmp)
     import pgd_amd
-    from tests.golden.make_goldens import edge_hash, make_graph
+    from tests.golden.graphs import edge_hash, make_graph
     n, s, d = make_graph(ent["gen"], ent["params"])
     assert edge_hash(s, d) == ent["edges_sha256"], "generator output changed"
     path = os.path.join(tmp, name.split("@")[0] + ".mtx")
