@@ -7,7 +7,9 @@ library or the GPU is missing, the calls raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -31,9 +33,14 @@ class Context:
         h = C.c_void_p()
         check(lib().pgd_open(device, C.byref(h)), what="pgd_open")
         self.h = h
+        self._graphs = weakref.WeakSet()   # graphs whose device memory this context owns
+        _live_contexts.add(self)
 
     def close(self) -> None:
+        """Free every graph of this context, then the context itself."""
         if self.h:
+            for g in list(self._graphs):
+                g.free()
             lib().pgd_close(self.h)
             self.h = None
 
@@ -53,6 +60,14 @@ class Context:
 
 
 _default_ctx: Context | None = None
+_live_contexts: "weakref.WeakSet[Context]" = weakref.WeakSet()
+
+
+@atexit.register
+def _release_device_objects() -> None:
+    """Free device objects before the HIP runtime is torn down at exit."""
+    for c in list(_live_contexts):
+        c.close()
 
 
 def default_context() -> Context:
@@ -142,6 +157,7 @@ class GraphletCore:
         check(lib().pgd_graph_from_coo(self.ctx.h, n, len(src), ptr(src, C.c_int32), ptr(dst, C.c_int32),
                                        DEDUP if dedup else 0, C.byref(h)), self.ctx.h, "pgd_graph_from_coo")
         self.h = h
+        self.ctx._graphs.add(self)
         nn, mm, md, ad = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
         lib().pgd_graph_info(h, C.byref(nn), C.byref(mm), C.byref(md), C.byref(ad))
         self.n, self.m, self.max_degree, self.avg_degree = nn.value, mm.value, md.value, ad.value
@@ -157,9 +173,9 @@ class GraphletCore:
         return cls(src, dst, n=n, dedup=dedup, **kw)
 
     def free(self) -> None:
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             lib().pgd_graph_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

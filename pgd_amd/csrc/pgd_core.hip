@@ -144,6 +144,12 @@ __global__ void k_dedup_mark(const uint64_t* __restrict__ key, const int32_t* __
     }
 }
 
+// number of repeated pairs in the sorted pair keys
+__global__ void k_count_repeats(const uint64_t* __restrict__ key, int64_t L, uint32_t* __restrict__ reps) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < L; j += (int64_t)gridDim.x * blockDim.x)
+        if (j > 0 && key[j] != ~0ULL && key[j] == key[j - 1]) atomicAdd(reps, 1u);
+}
+
 // line i (kept, compacted index o) -> slots 2o (src->dst) and 2o+1 (dst->src)
 __global__ void k_make_slots(const int32_t* __restrict__ s, const int32_t* __restrict__ d,
                              const int32_t* __restrict__ keep, const int64_t* __restrict__ off, int64_t L,
@@ -634,7 +640,10 @@ pgd_status pgd_graph_from_coo(pgd_ctx* ctx, int64_t n, int64_t L, const int32_t*
         PGD_HIP(hipMemcpyAsync(&hbad, bad.get(), sizeof(hbad), hipMemcpyDeviceToHost, s));
         sync(ctx);
         PGD_CHECK(hbad == 0, PGD_ERR_INVALID, "vertex id outside [0, n)");
-        if ((flags & PGD_DEDUP) && L) {
+        if (L) {
+            // sort the undirected pairs (stable by line): with PGD_DEDUP later copies
+            // are dropped; without it (.mtx) a repeated pair is a multi-edge, which
+            // the reference counts with multiplicity and this build rejects
             DBuf<uint64_t> k0, k1;
             DBuf<int32_t> i0, i1;
             k0.alloc(ctx, L); k1.alloc(ctx, L); i0.alloc(ctx, L); i1.alloc(ctx, L);
@@ -642,8 +651,17 @@ pgd_status pgd_graph_from_coo(pgd_ctx* ctx, int64_t n, int64_t L, const int32_t*
                                k0.get(), i0.get());
             PGD_HIP(hipGetLastError());
             sort_pairs(ctx, k0.get(), k1.get(), i0.get(), i1.get(), (size_t)L, 0, 64);
-            hipLaunchKernelGGL(k_dedup_mark, dim3(grid), dim3(kBlock), 0, s, k1.get(), i1.get(), L, keep.get());
+            if (flags & PGD_DEDUP) {
+                hipLaunchKernelGGL(k_dedup_mark, dim3(grid), dim3(kBlock), 0, s, k1.get(), i1.get(), L, keep.get());
+            } else {
+                hipLaunchKernelGGL(k_count_repeats, dim3(grid), dim3(kBlock), 0, s, k1.get(), L, bad.get());
+            }
             PGD_HIP(hipGetLastError());
+            uint32_t reps = 0;
+            PGD_HIP(hipMemcpyAsync(&reps, bad.get(), sizeof(reps), hipMemcpyDeviceToHost, s));
+            sync(ctx);
+            PGD_CHECK(reps == 0, PGD_ERR_UNSUPPORTED,
+                      "the edge list repeats an undirected pair (a multi-edge); only simple graphs are supported");
         }
         exclusive_sum(ctx, keep.get(), off.get(), (size_t)L);
         int64_t kept = 0;
