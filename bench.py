@@ -60,6 +60,38 @@ def survey_bytes(row: np.ndarray, col: np.ndarray, ev: np.ndarray, eu: np.ndarra
     return 4.0 * per.sum() + 8.0 * (n + 1) + 8.0 * m + (64.0 * m if micro else 0.0) + 136.0
 
 
+def measured_traffic(kernel: str, workload: str) -> float | None:
+    """HBM bytes per launch of `kernel` on `workload`, from the rocprofv3
+    FETCH_SIZE/WRITE_SIZE summaries committed under profiles/ (calibrated on
+    a plain copy; see the summary files)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 4 and p[0] == "traffic_bytes" and p[1] == kernel and p[2] == workload:
+                    return float(p[3])
+    return None
+
+
+def roofline(phases: list[list[dict]], workload: str = "") -> dict | None:
+    """Roofline of the dominant counting kernel: its algorithmic HBM bytes
+    (the engine's work model for that pass, per launch) over its average
+    device time measured with HIP events on the engine's stream."""
+    if not phases or not phases[0]:
+        return None
+    names = [p["name"] for p in phases[0]]
+    avg = {nm: float(np.mean([ph[i]["ms"] for ph in phases])) for i, nm in enumerate(names)}
+    dom = max(avg, key=avg.get)
+    i = names.index(dom)
+    by = phases[0][i]["bytes"]
+    ach = by / (avg[dom] / 1e3) / 1e9
+    tr = measured_traffic(dom, workload)
+    return {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": tr, "algorithmic_GB": by / 1e9, "ms": avg[dom],
+            "units": phases[0][i]["units"]}
+
+
 def cpu_baseline(g, target_s: float, threads: int) -> dict:
     """The oracle (the reference's OpenMP edge loop restated in C) on a
     deterministic, work-stratified sample of E_ordered, extrapolated to the
@@ -139,11 +171,13 @@ def main() -> int:
     t0 = time.perf_counter()
     dom = []
     tot = []
+    phases = []
     for i in range(args.steps):
         totals = step()
         a, b, _ = g.last_timing()
         tot.append(a)
         dom.append(b)
+        phases.append(g.last_phases())
         log(f"[bench] step {i}: counting {a:.1f} ms (dominant kernel {b:.1f} ms)")
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
@@ -159,7 +193,9 @@ def main() -> int:
         "config": {"workload": f"rmat_scale{args.scale}_ef{args.ef}_a0.57_b0.19_c0.19_seed{args.seed}",
                    "n": g.n, "m": m, "max_degree": g.max_degree, "micro": True, "engine": g.last_engine()},
         "counting_ms": float(np.mean(tot)),
-        "roofline": None,
+        "roofline": roofline(phases, f"rmat_scale{args.scale}_ef{args.ef}_a0.57_b0.19_c0.19_seed{args.seed}"),
+        "phases": [{"name": p["name"], "ms": round(float(np.mean([ph[j]["ms"] for ph in phases])), 3),
+                    "GB": round(p["bytes"] / 1e9, 3), "units": p["units"]} for j, p in enumerate(phases[0])],
         "survey_model": {"bytes": B, "achieved_GBs": B / dt / 1e9, "frac_of_hbm_peak": B / dt / 1e9 / HBM_PEAK_GBS},
         "totals": dict(zip(pgd_amd.TOTAL_NAMES, totals)),
     }

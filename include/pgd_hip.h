@@ -84,6 +84,13 @@ const char *pgd_last_error(pgd_ctx *ctx);
 pgd_status pgd_graph_from_coo(pgd_ctx *ctx, int64_t n, int64_t nlines,
                               const int32_t *src, const int32_t *dst,
                               uint32_t flags, pgd_graph **out);
+/* Upload a CSR as-is: `row` (n+1 offsets) and `col` (row[n] ids), the
+ * symmetric simple adjacency held by graphlet_core's `vertices`/`edges`
+ * (graphlet_core.h:84-86) after its reader and, if already run, its neighbour
+ * ordering.  Neighbour order is kept, so a later pgd_preprocess with a method
+ * that orders nothing ("natural") reproduces the object's edge ids. */
+pgd_status pgd_graph_from_csr(pgd_ctx *ctx, int64_t n, const int64_t *row, const int32_t *col,
+                              pgd_graph **out);
 pgd_status pgd_graph_free(pgd_graph *g);
 /* n = num_vertices(), m = num_edges(), max degree, avg degree
  * (graphlet_core.h:220-221, 262, 276; vertex_degrees graphlet_core.cpp:877-891) */
@@ -132,6 +139,13 @@ int32_t    pgd_last_engine(pgd_graph *g);
  * dominant kernel alone, with its launch count (HIP events on the context's
  * stream). */
 pgd_status pgd_last_timing(pgd_graph *g, double *total_ms, double *dominant_ms, int32_t *dominant_launches);
+/* Per-pass profile of the last pgd_decompose: pass `idx` (0-based) gives its
+ * kernel name (NUL-terminated, truncated to cap bytes), device time, the
+ * algorithmic HBM bytes of its work model and the work units it processed.
+ * PGD_ERR_INVALID once idx is past the last pass.  No reference
+ * counterpart: the reference only prints one wall-clock time. */
+pgd_status pgd_last_phase(pgd_graph *g, int32_t idx, char *name, int32_t cap, double *ms, double *bytes,
+                          double *units);
 
 /* ---- derivation -------------------------------------------------------
  * The closed-form derivation of the 17 totals (graphlet_core.cpp:1449-1472,

@@ -168,6 +168,27 @@ class GraphletCore:
             self.preprocess(neigh_ordering, neigh_small_to_large)
 
     @classmethod
+    def from_csr(cls, row, col, ctx: Context | None = None) -> "GraphletCore":
+        """Adopt a CSR as-is (graphlet_core's vertices/edges), keeping its
+        neighbour order; preprocess with "natural" to keep the edge ids."""
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        row = np.ascontiguousarray(row, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().pgd_graph_from_csr(self.ctx.h, len(row) - 1, ptr(row, C.c_int64), ptr(col, C.c_int32),
+                                       C.byref(h)), self.ctx.h, "pgd_graph_from_csr")
+        self.h = h
+        self.ctx._graphs.add(self)
+        nn, mm, md, ad = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        lib().pgd_graph_info(h, C.byref(nn), C.byref(mm), C.byref(md), C.byref(ad))
+        self.n, self.m, self.max_degree, self.avg_degree = nn.value, mm.value, md.value, ad.value
+        self.sums = None
+        self.totals = None
+        self._ordered = False
+        return self
+
+    @classmethod
     def from_file(cls, path: str, **kw) -> "GraphletCore":
         n, src, dst, dedup, _ = read_graph(path)
         return cls(src, dst, n=n, dedup=dedup, **kw)
@@ -259,6 +280,17 @@ class GraphletCore:
 
     def last_engine(self) -> str:
         return {1: "edge", 2: "orient", 3: "dense"}.get(lib().pgd_last_engine(self.h), "none")
+
+    def last_phases(self) -> list[dict]:
+        """Per-pass device time and work model of the last decomposition."""
+        out = []
+        buf = C.create_string_buffer(128)
+        ms, by, un = C.c_double(), C.c_double(), C.c_double()
+        i = 0
+        while lib().pgd_last_phase(self.h, i, buf, 128, C.byref(ms), C.byref(by), C.byref(un)) == 0:
+            out.append({"name": buf.value.decode(), "ms": ms.value, "bytes": by.value, "units": un.value})
+            i += 1
+        return out
 
     def last_timing(self):
         a, b, k = C.c_double(), C.c_double(), C.c_int32()

@@ -32,9 +32,9 @@ TOTAL_NAMES = [
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = [
     "pgd_open", "pgd_close", "pgd_status_str", "pgd_set_mem_cap", "pgd_mem_in_use", "pgd_last_error",
-    "pgd_graph_from_coo", "pgd_graph_free", "pgd_graph_info", "pgd_graph_csr", "pgd_preprocess",
+    "pgd_graph_from_coo", "pgd_graph_from_csr", "pgd_graph_free", "pgd_graph_info", "pgd_graph_csr", "pgd_preprocess",
     "pgd_edge_arrays", "pgd_compute_cores", "pgd_sort_edges", "pgd_e_ordered", "pgd_decompose", "pgd_micro",
-    "pgd_edge_counts", "pgd_last_engine", "pgd_last_timing", "pgd_derive", "pgd_batch_create", "pgd_batch_run",
+    "pgd_edge_counts", "pgd_last_engine", "pgd_last_timing", "pgd_last_phase", "pgd_derive", "pgd_batch_create", "pgd_batch_run",
     "pgd_batch_micro", "pgd_batch_info", "pgd_batch_free", "pgd_batch_set_tuning", "pgd_batch_autotune",
     "pgd_server_create", "pgd_server_submit", "pgd_server_poll", "pgd_server_result", "pgd_server_micro",
     "pgd_server_release", "pgd_server_free", "pgd_gen_gnp", "pgd_gen_ba", "pgd_gen_rmat", "pgd_write_mtx",
@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
         "pgd_mem_in_use": (I32, [P, PI64]),
         "pgd_last_error": (C.c_char_p, [P]),
         "pgd_graph_from_coo": (I32, [P, I64, I64, PI32, PI32, U32, C.POINTER(P)]),
+        "pgd_graph_from_csr": (I32, [P, I64, PI64, PI32, C.POINTER(P)]),
         "pgd_graph_free": (I32, [P]),
         "pgd_graph_info": (I32, [P, PI64, PI64, PI32, PI32]),
         "pgd_graph_csr": (I32, [P, PI64, PI32]),
@@ -83,6 +84,7 @@ def lib() -> C.CDLL:
         "pgd_edge_counts": (I32, [P, PU64, PU64, PU64]),
         "pgd_last_engine": (I32, [P]),
         "pgd_last_timing": (I32, [P, C.POINTER(D), C.POINTER(D), PI32]),
+        "pgd_last_phase": (I32, [P, I32, C.c_char_p, I32, C.POINTER(D), C.POINTER(D), C.POINTER(D)]),
         "pgd_derive": (I32, [I64, I64, PU64, PU64]),
         "pgd_batch_create": (I32, [P, I32, PI32, PI64, PI32, PI32, U32, C.POINTER(P)]),
         "pgd_batch_run": (I32, [P, PU64, PU64]),

@@ -167,6 +167,26 @@ __global__ void k_make_slots(const int32_t* __restrict__ s, const int32_t* __res
     }
 }
 
+// owner of every CSR slot and the degrees, from the offsets (one wavefront per vertex)
+__global__ void k_csr_owner(const int64_t* __restrict__ row, const int32_t* __restrict__ col, int64_t n,
+                            int32_t* __restrict__ src, int32_t* __restrict__ deg, uint32_t* __restrict__ bad) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t v = wave; v < n; v += nw) {
+        const int64_t b = row[v], e = row[v + 1];
+        if (lane == 0) {
+            deg[v] = (int32_t)(e - b);
+            if (e < b) atomicOr(bad, 1u);
+        }
+        for (int64_t j = b + lane; j < e; j += 64) {
+            src[j] = (int32_t)v;
+            int32_t u = col[j];
+            if (u < 0 || u >= n || u == v) atomicOr(bad, 1u);
+        }
+    }
+}
+
 __global__ void k_fill_row_tail(int64_t* row, int64_t n, int64_t nnz) {
     if (blockIdx.x == 0 && threadIdx.x == 0) row[n] = nnz;
 }
@@ -717,6 +737,56 @@ pgd_status pgd_graph_from_coo(pgd_ctx* ctx, int64_t n, int64_t L, const int32_t*
     return PGD_OK;
 }
 
+pgd_status pgd_graph_from_csr(pgd_ctx* ctx, int64_t n, const int64_t* row, const int32_t* col, pgd_graph** out) {
+    if (!ctx || !out || n < 0 || !row) return PGD_ERR_INVALID;
+    if (n >= INT32_MAX) return PGD_ERR_UNSUPPORTED;
+    const int64_t nnz = row[n];
+    if (row[0] != 0 || nnz < 0 || (nnz & 1) || (nnz > 0 && !col)) return PGD_ERR_INVALID;
+    *out = nullptr;
+    pgd_graph* g = new (std::nothrow) pgd_graph();
+    if (!g) return PGD_ERR_OOM;
+    g->ctx = ctx;
+    pgd_status st = guarded(ctx, [&] {
+        PGD_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        g->n = n;
+        g->nnz = nnz;
+        g->m = nnz / 2;
+        g->row.alloc(ctx, n + 1);
+        g->col.alloc(ctx, nnz);
+        g->src.alloc(ctx, nnz);
+        g->deg.alloc(ctx, std::max<int64_t>(n, 1));
+        DBuf<uint32_t> bad;
+        bad.alloc(ctx, 1);
+        PGD_HIP(hipMemsetAsync(bad.get(), 0, sizeof(uint32_t), s));
+        PGD_HIP(hipMemcpyAsync(g->row.get(), row, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        if (nnz) PGD_HIP(hipMemcpyAsync(g->col.get(), col, nnz * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        int32_t md = 0;
+        if (n) {
+            int grid = std::min(grid_for(n * 64), ctx->num_cus * 16);
+            hipLaunchKernelGGL(k_csr_owner, dim3(grid), dim3(kBlock), 0, s, g->row.get(), g->col.get(), n,
+                               g->src.get(), g->deg.get(), bad.get());
+            PGD_HIP(hipGetLastError());
+            DBuf<int32_t> mx;
+            mx.alloc(ctx, 1);
+            reduce_max(ctx, g->deg.get(), mx.get(), (size_t)n);
+            PGD_HIP(hipMemcpyAsync(&md, mx.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        }
+        uint32_t hbad = 0;
+        PGD_HIP(hipMemcpyAsync(&hbad, bad.get(), sizeof(hbad), hipMemcpyDeviceToHost, s));
+        sync(ctx);
+        PGD_CHECK(hbad == 0, PGD_ERR_INVALID, "CSR with a decreasing offset, an id outside [0, n) or a self loop");
+        g->max_deg = md;
+        g->avg_deg = n > 0 ? (int32_t)((double)nnz / (double)n) : 0;
+    });
+    if (st != PGD_OK) {
+        delete g;
+        return st;
+    }
+    *out = g;
+    return PGD_OK;
+}
+
 pgd_status pgd_graph_free(pgd_graph* g) {
     if (!g) return PGD_ERR_INVALID;
     (void)hipSetDevice(g->ctx->device);
@@ -957,6 +1027,21 @@ pgd_status pgd_edge_counts(pgd_graph* g, uint64_t* tri, uint64_t* k4, uint64_t* 
 }
 
 int32_t pgd_last_engine(pgd_graph* g) { return g ? g->last_engine : 0; }
+
+pgd_status pgd_last_phase(pgd_graph* g, int32_t idx, char* name, int32_t cap, double* ms, double* bytes,
+                          double* units) {
+    if (!g || idx < 0 || idx >= (int32_t)g->timing.phases.size()) return PGD_ERR_INVALID;
+    const Phase& p = g->timing.phases[(size_t)idx];
+    if (name && cap > 0) {
+        size_t k = std::min<size_t>(p.name.size(), (size_t)cap - 1);
+        std::memcpy(name, p.name.data(), k);
+        name[k] = 0;
+    }
+    if (ms) *ms = p.ms;
+    if (bytes) *bytes = p.bytes;
+    if (units) *units = p.units;
+    return PGD_OK;
+}
 
 pgd_status pgd_last_timing(pgd_graph* g, double* total_ms, double* dominant_ms, int32_t* dominant_launches) {
     if (!g) return PGD_ERR_INVALID;

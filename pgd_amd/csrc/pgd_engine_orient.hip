@@ -285,7 +285,7 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
                                                   const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart,
                                                   const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
                                                   int64_t cnt, int kmax, char* __restrict__ gscratch,
-                                                  uint32_t* __restrict__ T) {
+                                                  uint32_t* __restrict__ T, unsigned long long* __restrict__ work) {
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
     const int W = (kmax + 31) / 32;
@@ -295,6 +295,7 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
         const int32_t a = list[li];
         const int k = outdeg[a];
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
+        if (threadIdx.x == 0) atomicAdd(work, (unsigned long long)total);
         for (int64_t t = threadIdx.x; t < total; t += NT) {
             const int i = owner_of(A.pref, k, t);
             const int64_t ex = i ? A.pref[i - 1] : 0;
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                                                   const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
                                                   int64_t cnt, int kmax, char* __restrict__ gscratch,
                                                   const uint32_t* __restrict__ T, uint64_t* __restrict__ K4,
-                                                  uint64_t* __restrict__ Q) {
+                                                  uint64_t* __restrict__ Q, unsigned long long* __restrict__ work) {
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
     const int Wmax = (kmax + 31) / 32;
@@ -332,6 +333,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
         const int W = (k + 31) / 32;
         for (int w = threadIdx.x; w < k * W; w += NT) A.M[w] = 0;
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
+        if (threadIdx.x == 0) atomicAdd(work, (unsigned long long)total);
         // build the symmetric adjacency of the local graph
         for (int64_t t = threadIdx.x; t < total; t += NT) {
             const int i = owner_of(A.pref, k, t);
@@ -396,7 +398,7 @@ __global__ __launch_bounds__(64) void k_wedge_light(const int64_t* __restrict__ 
                                                     const int32_t* __restrict__ eid,
                                                     const int64_t* __restrict__ ostart,
                                                     const int32_t* __restrict__ list, int64_t cnt,
-                                                    uint64_t* __restrict__ P) {
+                                                    uint64_t* __restrict__ P, unsigned long long* __restrict__ work) {
     __shared__ int32_t hkey[kHashSlots];
     __shared__ uint32_t hcnt[kHashSlots];
     __shared__ int64_t pref[64];
@@ -422,6 +424,7 @@ __global__ __launch_bounds__(64) void k_wedge_light(const int64_t* __restrict__ 
                 __syncthreads();
                 block_incl_scan<64>(pref, kc, part);
                 const int64_t total = pref[kc - 1];
+                if (pass == 0 && tid == 0) atomicAdd(work, (unsigned long long)total);
                 for (int64_t t = tid; t < total; t += 64) {
                     const int i = owner_of(pref, kc, t);
                     const int64_t ex = i ? pref[i - 1] : 0;
@@ -459,7 +462,8 @@ __global__ __launch_bounds__(NT) void k_wedge_heavy(const int64_t* __restrict__ 
                                                     const int32_t* __restrict__ eid,
                                                     const int64_t* __restrict__ ostart,
                                                     const int32_t* __restrict__ list, int64_t cnt, int64_t n,
-                                                    uint32_t* __restrict__ dense, uint64_t* __restrict__ P) {
+                                                    uint32_t* __restrict__ dense, uint64_t* __restrict__ P,
+                                                    unsigned long long* __restrict__ work) {
     __shared__ int64_t pref[kChunk];
     __shared__ int64_t bbeg[kChunk];
     __shared__ uint64_t psum[kChunk];
@@ -482,6 +486,7 @@ __global__ __launch_bounds__(NT) void k_wedge_heavy(const int64_t* __restrict__ 
                 __syncthreads();
                 block_incl_scan<NT>(pref, kc, part);
                 const int64_t total = pref[kc - 1];
+                if (pass == 0 && tid == 0) atomicAdd(work, (unsigned long long)total);
                 for (int64_t t = tid; t < total; t += NT) {
                     const int i = owner_of(pref, kc, t);
                     const int64_t ex = i ? pref[i - 1] : 0;
@@ -664,35 +669,43 @@ void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm,
         ghuge = (int)std::min<int64_t>(nhuge, ctx->num_cus);
         hscratch.alloc(ctx, (size_t)ghuge * carve_bytes(maxk, (maxk + 31) / 32));
     }
+    // work counters: local list entries (triangle pass, clique pass), wedges (light, heavy)
+    DBuf<unsigned long long> work;
+    work.alloc(ctx, 4);
+    PGD_HIP(hipMemsetAsync(work.get(), 0, 4 * sizeof(unsigned long long), s));
+    unsigned long long* w_tri = work.get();
+    unsigned long long* w_k4q = work.get() + 1;
+    unsigned long long* w_light = work.get() + 2;
+    unsigned long long* w_heavy = work.get() + 3;
     // ---- pass 1: triangles ---------------------------------------------------
-    timer.begin("triangles");
+    timer.begin("k_local_tri");
     if (nsmall)
         hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, nsmall, 32)), dim3(64), small_lds, s,
                            rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), bsmall.get(), nsmall,
-                           kSmallK, (char*)nullptr, out.T);
+                           kSmallK, (char*)nullptr, out.T, w_tri);
     if (nlarge)
         hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, nlarge, 2)), dim3(256), large_lds, s,
                            rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), blarge.get(), nlarge,
-                           kmax_lds, (char*)nullptr, out.T);
+                           kmax_lds, (char*)nullptr, out.T, w_tri);
     if (nhuge)
         hipLaunchKernelGGL((k_local_tri<256, true>), dim3(ghuge), dim3(256), 0, s, rrow.get(), rcol.get(), reid.get(),
-                           ostart.get(), outdeg.get(), bhuge.get(), nhuge, maxk, hscratch.get(), out.T);
+                           ostart.get(), outdeg.get(), bhuge.get(), nhuge, maxk, hscratch.get(), out.T, w_tri);
     PGD_HIP(hipGetLastError());
     timer.end();
     // ---- pass 2: 4-cliques and Q -----------------------------------------------
-    timer.begin("cliques");
+    timer.begin("k_local_k4q");
     if (nsmall)
         hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, nsmall, 32)), dim3(64), small_lds, s,
                            rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), bsmall.get(), nsmall,
-                           kSmallK, (char*)nullptr, out.T, out.K4, Q.get());
+                           kSmallK, (char*)nullptr, out.T, out.K4, Q.get(), w_k4q);
     if (nlarge)
         hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, nlarge, 2)), dim3(256), large_lds, s,
                            rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), blarge.get(), nlarge,
-                           kmax_lds, (char*)nullptr, out.T, out.K4, Q.get());
+                           kmax_lds, (char*)nullptr, out.T, out.K4, Q.get(), w_k4q);
     if (nhuge)
         hipLaunchKernelGGL((k_local_k4q<256, true>), dim3(ghuge), dim3(256), 0, s, rrow.get(), rcol.get(),
                            reid.get(), ostart.get(), outdeg.get(), bhuge.get(), nhuge, maxk, hscratch.get(), out.T,
-                           out.K4, Q.get());
+                           out.K4, Q.get(), w_k4q);
     PGD_HIP(hipGetLastError());
     timer.end();
     // ---- pass 3: 4-cycles (P) ----------------------------------------------------
@@ -704,25 +717,41 @@ void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm,
         dense.alloc(ctx, (size_t)gheavy * n);
         PGD_HIP(hipMemsetAsync(dense.get(), 0, (size_t)gheavy * n * sizeof(uint32_t), s));
     }
-    timer.begin("wedges");
+    timer.begin("k_wedge_light");
     if (nlight)
         hipLaunchKernelGGL(k_wedge_light, dim3(grid_cap(ctx, nlight, 32)), dim3(64), 0, s, rrow.get(), rcol.get(),
-                           reid.get(), ostart.get(), blight.get(), nlight, P.get());
-    if (nheavy)
-        hipLaunchKernelGGL(k_wedge_heavy<256>, dim3(gheavy), dim3(256), 0, s, rrow.get(), rcol.get(), reid.get(),
-                           ostart.get(), bheavy.get(), nheavy, n, dense.get(), P.get());
+                           reid.get(), ostart.get(), blight.get(), nlight, P.get(), w_light);
     PGD_HIP(hipGetLastError());
     timer.end();
-    timer.begin("combine");
+    timer.begin("k_wedge_heavy");
+    if (nheavy)
+        hipLaunchKernelGGL(k_wedge_heavy<256>, dim3(gheavy), dim3(256), 0, s, rrow.get(), rcol.get(), reid.get(),
+                           ostart.get(), bheavy.get(), nheavy, n, dense.get(), P.get(), w_heavy);
+    PGD_HIP(hipGetLastError());
+    timer.end();
+    timer.begin("k_combine");
     hipLaunchKernelGGL(k_combine, dim3(grid_cap(ctx, (m + 255) / 256, 16)), dim3(256), 0, s, P.get(), Q.get(),
                        out.T, out.K4, m, out.C4);
     PGD_HIP(hipGetLastError());
     timer.end();
+    unsigned long long wk[4] = {0, 0, 0, 0};
+    PGD_HIP(hipMemcpyAsync(wk, work.get(), sizeof(wk), hipMemcpyDeviceToHost, s));
     PGD_HIP(hipStreamSynchronize(s));
+    // algorithmic HBM bytes per pass (work models):
+    //   local passes: 4 B per out-list entry read per enumeration (1 for the
+    //     triangle pass, 2 for the clique pass: bit-matrix build + scan);
+    //   wedge passes: the 4 B endpoint id of every wedge, read once per pass
+    //     (light: count + accumulate; heavy: count + accumulate + clear), plus
+    //     the 4 B dense counter touched by each of the three heavy passes;
+    //   combine: 3 x 8 B + 4 B read and 8 B written per edge.
+    const double bytes_of[5] = {4.0 * wk[0], 8.0 * wk[1], 8.0 * wk[2], 24.0 * wk[3], 36.0 * m};
+    const double units_of[5] = {(double)wk[0], (double)wk[1], (double)wk[2], (double)wk[3], (double)m};
     double best = 0;
+    tm->phases.clear();
     for (size_t i = 0; i < timer.ev.size(); i++) {
         float ms = 0;
         PGD_HIP(hipEventElapsedTime(&ms, timer.ev[i].first, timer.ev[i].second));
+        tm->phases.push_back(Phase{timer.names[i], ms, bytes_of[i], units_of[i]});
         if (ms > best) best = ms;
     }
     tm->dominant_ms = best;

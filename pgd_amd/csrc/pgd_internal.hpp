@@ -78,9 +78,16 @@ struct DBuf {
 };
 
 // Timing of the counting stage of the last decomposition
+struct Phase {
+    std::string name;    // kernel (or kernel family) of one counting pass
+    double ms = 0;       // device time, HIP events on the context's stream
+    double bytes = 0;    // algorithmic HBM bytes of the pass (work model)
+    double units = 0;    // work units the pass processed (wedges, list entries)
+};
 struct Timing {
     double total_ms = 0, dominant_ms = 0;
     int dominant_launches = 0;
+    std::vector<Phase> phases;
 };
 
 // Per-edge counts shared by every engine: indexed by edge id.

@@ -172,3 +172,22 @@ def test_empty_and_tiny_graphs(ctx):
         want = run_pipeline(og, micro=False)["totals"]
         assert [tot[k] for k in pgd_amd.TOTAL_NAMES] == want
         g.free()
+
+
+def test_adopt_reference_csr(ctx, tmp):
+    """pgd_graph_from_csr keeps the neighbour order of an already ordered CSR
+    (the state graphlet_core holds), so edge ids, E_ordered and every micro
+    row equal the reference's."""
+    name = "DD21.mtx"
+    ent = GOLDENS[name]
+    path = golden_graph_path(name, ent, tmp)
+    og = OracleGraph.from_file(path)
+    og.order_neighbors("deg")
+    row, col = og.csr()
+    g = pgd_amd.GraphletCore.from_csr(row, col, ctx=ctx)
+    g.preprocess("natural")
+    g.sort_edges("deg")
+    totals = g.graphlet_decomposition_micro()
+    assert [totals[k] for k in pgd_amd.TOTAL_NAMES] == ent["totals"]
+    assert micro_sha(micro_matrix(g)) == ent["micro_sha256"]
+    g.free()
