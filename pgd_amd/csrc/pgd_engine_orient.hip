@@ -34,17 +34,12 @@
 #include <rocprim/rocprim.hpp>
 
 #include <algorithm>
+#include <memory>
 
 #include "pgd_internal.hpp"
 
 using namespace pgd;
 
-namespace pgd {
-struct OrientState {
-    int dummy = 0;
-};
-void orient_state_free(pgd_ctx*, OrientState* s) { delete s; }
-}  // namespace pgd
 
 namespace {
 
@@ -523,6 +518,9 @@ __global__ void k_combine(const uint64_t* __restrict__ P, const uint64_t* __rest
     GRID_STRIDE(e, m) C4[e] = P[e] - Q[e] + 2 * (uint64_t)T[e] + 2 * K4[e];
 }
 
+}  // namespace
+
+namespace pgd {
 struct Timer {
     pgd_ctx* ctx;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -543,8 +541,18 @@ struct Timer {
         }
     }
 };
+}  // namespace pgd
 
-// compaction of the vertices whose key falls in (lo, hi]
+namespace {
+
+// capacity-reusing allocation: keeps the buffer when it is already large enough
+template <class T>
+void ensure(DBuf<T>& b, pgd_ctx* ctx, size_t n) {
+    if (n == 0) n = 1;
+    if (b.get() == nullptr || b.n < n) b.alloc(ctx, n);
+}
+
+// compaction of the vertices whose key falls in (lo, hi] into `out` (capacity n)
 template <class K>
 int64_t make_bin(pgd_ctx* ctx, const K* key, int64_t n, int64_t lo, int64_t hi, DBuf<int32_t>& out,
                  DBuf<int32_t>& flags, DBuf<int64_t>& off) {
@@ -562,7 +570,7 @@ int64_t make_bin(pgd_ctx* ctx, const K* key, int64_t n, int64_t lo, int64_t hi, 
     PGD_HIP(hipMemcpyAsync(&last_f, flags.get() + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PGD_HIP(hipStreamSynchronize(s));
     int64_t c = last_off + last_f;
-    out.alloc(ctx, std::max<int64_t>(c, 1));
+    ensure(out, ctx, (size_t)n);
     if (c) {
         hipLaunchKernelGGL(k_bin_scatter, dim3(g), dim3(256), 0, s, flags.get(), off.get(), n, out.get());
         PGD_HIP(hipGetLastError());
@@ -570,173 +578,217 @@ int64_t make_bin(pgd_ctx* ctx, const K* key, int64_t n, int64_t lo, int64_t hi, 
     return c;
 }
 
+__global__ void k_fill_u64(uint64_t* __restrict__ p, int64_t n) {
+    GRID_STRIDE(i, n) p[i] = 0;
+}
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n) {
+    GRID_STRIDE(i, n) p[i] = 0;
+}
+
 }  // namespace
 
 namespace pgd {
 
-void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm, OrientState** cache) {
-    (void)cache;
+// Persistent state of the oriented engine for one graph: the relabelled CSR,
+// the vertex bins, scratch and accumulators.  Buffers are kept across calls
+// (their contents are rebuilt every call), so the counting launches can be
+// captured once in a HIP graph and replayed.
+struct OrientState {
+    OrientParams prm;
+    int64_t n = 0, m = 0;
+    DBuf<int32_t> rank, rdeg, rcol, reid, outdeg, v0, flags;
+    DBuf<int64_t> rrow, ostart, off, ub;
+    DBuf<uint64_t> k0, k1;
+    DBuf<int32_t> b_small, b_mid, b_large, b_huge, b_light, b_heavy;
+    int64_t n_small = 0, n_mid = 0, n_large = 0, n_huge = 0, n_light = 0, n_heavy = 0;
+    int kmax_lds = 0, maxk = 0, ghuge = 0, gheavy = 0;
+    size_t small_lds = 0, mid_lds = 0, large_lds = 0;
+    DBuf<char> hscratch;
+    DBuf<uint32_t> dense;
+    DBuf<uint64_t> P, Q;
+    DBuf<unsigned long long> work;
+};
+
+void orient_state_free(pgd_ctx*, OrientState* s) { delete s; }
+
+OrientParams& orient_params(OrientState* s) { return s->prm; }
+
+// relabel, ascending CSR, bins and scratch (host synchronisations; not capturable)
+void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
     hipStream_t s = ctx->stream;
     const int64_t n = g.n, m = g.m, nnz = 2 * m;
-    Timer timer{ctx};
-    // ---- relabel by (degree, id) and build the ascending CSR ----------------
-    DBuf<int32_t> rank, rdeg, rcol, reid, outdeg;
-    DBuf<int64_t> rrow, ostart;
-    rank.alloc(ctx, n);
-    rdeg.alloc(ctx, n);
-    rrow.alloc(ctx, n + 1);
-    ostart.alloc(ctx, n);
-    outdeg.alloc(ctx, n);
-    rcol.alloc(ctx, nnz);
-    reid.alloc(ctx, nnz);
-    {
-        DBuf<uint64_t> k0, k1;
-        k0.alloc(ctx, std::max(n, nnz));
-        k1.alloc(ctx, std::max(n, nnz));
-        int gn = grid_cap(ctx, (n + 255) / 256, 16), gm = grid_cap(ctx, (m + 255) / 256, 16);
-        hipLaunchKernelGGL(k_rank_keys, dim3(gn), dim3(256), 0, s, g.deg, n, k0.get());
-        PGD_HIP(hipGetLastError());
-        sort_k(ctx, k0.get(), k1.get(), (size_t)n, 64);
-        hipLaunchKernelGGL(k_rank_scatter, dim3(gn), dim3(256), 0, s, k1.get(), n, rank.get(), rdeg.get());
-        PGD_HIP(hipGetLastError());
-        DBuf<int32_t> v0;
-        v0.alloc(ctx, nnz);
-        hipLaunchKernelGGL(k_slot_keys, dim3(gm), dim3(256), 0, s, g.ev, g.eu, rank.get(), m, k0.get(), v0.get());
-        PGD_HIP(hipGetLastError());
-        sort_kv(ctx, k0.get(), k1.get(), v0.get(), reid.get(), (size_t)nnz, 32u + (unsigned)bitsfor(n - 1));
-        int gz = grid_cap(ctx, (nnz + 255) / 256, 16);
-        hipLaunchKernelGGL(k_slot_cols, dim3(gz), dim3(256), 0, s, k1.get(), nnz, rcol.get());
-        PGD_HIP(hipGetLastError());
-        excl_sum(ctx, rdeg.get(), rrow.get(), (size_t)n);
-        hipLaunchKernelGGL(k_row_tail, dim3(1), dim3(1), 0, s, rrow.get(), n, nnz);
-        hipLaunchKernelGGL(k_out_start, dim3(gn), dim3(256), 0, s, rrow.get(), rcol.get(), n, ostart.get(),
-                           outdeg.get());
-        PGD_HIP(hipGetLastError());
-    }
-    // ---- per-edge accumulators ---------------------------------------------
-    DBuf<uint64_t> P, Q;
-    P.alloc(ctx, m);
-    Q.alloc(ctx, m);
-    PGD_HIP(hipMemsetAsync(out.T, 0, m * sizeof(uint32_t), s));
-    PGD_HIP(hipMemsetAsync(out.K4, 0, m * sizeof(uint64_t), s));
-    PGD_HIP(hipMemsetAsync(P.get(), 0, m * sizeof(uint64_t), s));
-    PGD_HIP(hipMemsetAsync(Q.get(), 0, m * sizeof(uint64_t), s));
-    // ---- vertex bins ---------------------------------------------------------
-    DBuf<int32_t> flags, bsmall, blarge, bhuge, blight, bheavy;
-    DBuf<int64_t> off, ub;
-    flags.alloc(ctx, n);
-    off.alloc(ctx, n);
-    ub.alloc(ctx, n);
-    int32_t maxk = 0;
+    const OrientParams& prm = st.prm;
+    st.n = n;
+    st.m = m;
+    ensure(st.rank, ctx, n);
+    ensure(st.rdeg, ctx, n);
+    ensure(st.rrow, ctx, n + 1);
+    ensure(st.ostart, ctx, n);
+    ensure(st.outdeg, ctx, n);
+    ensure(st.rcol, ctx, nnz);
+    ensure(st.reid, ctx, nnz);
+    ensure(st.k0, ctx, std::max(n, nnz));
+    ensure(st.k1, ctx, std::max(n, nnz));
+    ensure(st.v0, ctx, nnz);
+    ensure(st.flags, ctx, n);
+    ensure(st.off, ctx, n);
+    ensure(st.ub, ctx, n);
+    ensure(st.P, ctx, m);
+    ensure(st.Q, ctx, m);
+    ensure(st.work, ctx, 4);
+    int gn = grid_cap(ctx, (n + 255) / 256, 16), gm = grid_cap(ctx, (m + 255) / 256, 16);
+    hipLaunchKernelGGL(k_rank_keys, dim3(gn), dim3(256), 0, s, g.deg, n, st.k0.get());
+    PGD_HIP(hipGetLastError());
+    sort_k(ctx, st.k0.get(), st.k1.get(), (size_t)n, 64);
+    hipLaunchKernelGGL(k_rank_scatter, dim3(gn), dim3(256), 0, s, st.k1.get(), n, st.rank.get(), st.rdeg.get());
+    PGD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_slot_keys, dim3(gm), dim3(256), 0, s, g.ev, g.eu, st.rank.get(), m, st.k0.get(),
+                       st.v0.get());
+    PGD_HIP(hipGetLastError());
+    sort_kv(ctx, st.k0.get(), st.k1.get(), st.v0.get(), st.reid.get(), (size_t)nnz, 32u + (unsigned)bitsfor(n - 1));
+    int gz = grid_cap(ctx, (nnz + 255) / 256, 16);
+    hipLaunchKernelGGL(k_slot_cols, dim3(gz), dim3(256), 0, s, st.k1.get(), nnz, st.rcol.get());
+    PGD_HIP(hipGetLastError());
+    excl_sum(ctx, st.rdeg.get(), st.rrow.get(), (size_t)n);
+    hipLaunchKernelGGL(k_row_tail, dim3(1), dim3(1), 0, s, st.rrow.get(), n, nnz);
+    hipLaunchKernelGGL(k_out_start, dim3(gn), dim3(256), 0, s, st.rrow.get(), st.rcol.get(), n, st.ostart.get(),
+                       st.outdeg.get());
+    PGD_HIP(hipGetLastError());
     {
         DBuf<int32_t> mx;
         mx.alloc(ctx, 1);
         size_t bytes = 0;
-        PGD_HIP(rocprim::reduce(nullptr, bytes, outdeg.get(), mx.get(), 0, (size_t)n, rocprim::maximum<int32_t>(), s));
+        PGD_HIP(rocprim::reduce(nullptr, bytes, st.outdeg.get(), mx.get(), 0, (size_t)n, rocprim::maximum<int32_t>(), s));
         void* t = tmp_storage(ctx, bytes);
-        PGD_HIP(rocprim::reduce(t, bytes, outdeg.get(), mx.get(), 0, (size_t)n, rocprim::maximum<int32_t>(), s));
-        PGD_HIP(hipMemcpyAsync(&maxk, mx.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        PGD_HIP(rocprim::reduce(t, bytes, st.outdeg.get(), mx.get(), 0, (size_t)n, rocprim::maximum<int32_t>(), s));
+        PGD_HIP(hipMemcpyAsync(&st.maxk, mx.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
         PGD_HIP(hipStreamSynchronize(s));
     }
-    // largest local graph that fits the LDS of one block
+    // the largest local graph whose arrays and bit matrix fit one block's LDS
     hipDeviceProp_t prop;
     PGD_HIP(hipGetDeviceProperties(&prop, ctx->device));
-    const size_t lds_cap = std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024) - 4096;
-    int kmax_lds = 32;
-    while (carve_bytes(kmax_lds + 32, (kmax_lds + 32 + 31) / 32) <= lds_cap) kmax_lds += 32;
-    const int64_t nsmall = make_bin(ctx, outdeg.get(), n, 1, kSmallK, bsmall, flags, off);
-    const int64_t nlarge = make_bin(ctx, outdeg.get(), n, kSmallK, kmax_lds, blarge, flags, off);
-    const int64_t nhuge = make_bin(ctx, outdeg.get(), n, kmax_lds, INT32_MAX, bhuge, flags, off);
+    const size_t lds_cap = std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024) - 8192;
+    int kl = 32;
+    while (carve_bytes(kl + 32, (kl + 32 + 31) / 32) <= lds_cap) kl += 32;
+    st.kmax_lds = std::max(kl, prm.mid_k);
+    st.n_small = make_bin(ctx, st.outdeg.get(), n, 1, prm.small_k, st.b_small, st.flags, st.off);
+    st.n_mid = make_bin(ctx, st.outdeg.get(), n, prm.small_k, prm.mid_k, st.b_mid, st.flags, st.off);
+    st.n_large = make_bin(ctx, st.outdeg.get(), n, prm.mid_k, st.kmax_lds, st.b_large, st.flags, st.off);
+    st.n_huge = make_bin(ctx, st.outdeg.get(), n, st.kmax_lds, INT32_MAX, st.b_huge, st.flags, st.off);
     {
         int gw = grid_cap(ctx, (n * 64 + 255) / 256, 16);
-        hipLaunchKernelGGL(k_wedge_bound, dim3(gw), dim3(256), 0, s, rrow.get(), rcol.get(), ostart.get(), n,
-                           ub.get());
+        hipLaunchKernelGGL(k_wedge_bound, dim3(gw), dim3(256), 0, s, st.rrow.get(), st.rcol.get(), st.ostart.get(),
+                           n, st.ub.get());
         PGD_HIP(hipGetLastError());
     }
-    const int64_t nlight = make_bin(ctx, ub.get(), n, 0, kLightBound, blight, flags, off);
-    const int64_t nheavy = make_bin(ctx, ub.get(), n, kLightBound, INT64_MAX, bheavy, flags, off);
-    const size_t small_lds = carve_bytes(kSmallK, kSmallK / 32);
-    const size_t large_lds = carve_bytes(kmax_lds, (kmax_lds + 31) / 32);
-    if (nlarge) {
-        PGD_HIP(hipFuncSetAttribute((const void*)k_local_tri<256, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)large_lds));
-        PGD_HIP(hipFuncSetAttribute((const void*)k_local_k4q<256, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)large_lds));
+    st.n_light = make_bin(ctx, st.ub.get(), n, 0, kLightBound, st.b_light, st.flags, st.off);
+    st.n_heavy = make_bin(ctx, st.ub.get(), n, kLightBound, INT64_MAX, st.b_heavy, st.flags, st.off);
+    st.small_lds = carve_bytes(prm.small_k, (prm.small_k + 31) / 32);
+    st.mid_lds = carve_bytes(prm.mid_k, (prm.mid_k + 31) / 32);
+    st.large_lds = carve_bytes(st.kmax_lds, (st.kmax_lds + 31) / 32);
+    if (st.n_large) {
+        PGD_HIP(hipFuncSetAttribute((const void*)k_local_tri<1024, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)st.large_lds));
+        PGD_HIP(hipFuncSetAttribute((const void*)k_local_k4q<1024, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)st.large_lds));
     }
-    DBuf<char> hscratch;
-    int ghuge = 0;
-    if (nhuge) {
-        ghuge = (int)std::min<int64_t>(nhuge, ctx->num_cus);
-        hscratch.alloc(ctx, (size_t)ghuge * carve_bytes(maxk, (maxk + 31) / 32));
+    if (st.n_huge) {
+        st.ghuge = (int)std::min<int64_t>(st.n_huge, ctx->num_cus);
+        ensure(st.hscratch, ctx, (size_t)st.ghuge * carve_bytes(st.maxk, (st.maxk + 31) / 32));
     }
-    // work counters: local list entries (triangle pass, clique pass), wedges (light, heavy)
-    DBuf<unsigned long long> work;
-    work.alloc(ctx, 4);
-    PGD_HIP(hipMemsetAsync(work.get(), 0, 4 * sizeof(unsigned long long), s));
-    unsigned long long* w_tri = work.get();
-    unsigned long long* w_k4q = work.get() + 1;
-    unsigned long long* w_light = work.get() + 2;
-    unsigned long long* w_heavy = work.get() + 3;
-    // ---- pass 1: triangles ---------------------------------------------------
-    timer.begin("k_local_tri");
-    if (nsmall)
-        hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, nsmall, 32)), dim3(64), small_lds, s,
-                           rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), bsmall.get(), nsmall,
-                           kSmallK, (char*)nullptr, out.T, w_tri);
-    if (nlarge)
-        hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, nlarge, 2)), dim3(256), large_lds, s,
-                           rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), blarge.get(), nlarge,
-                           kmax_lds, (char*)nullptr, out.T, w_tri);
-    if (nhuge)
-        hipLaunchKernelGGL((k_local_tri<256, true>), dim3(ghuge), dim3(256), 0, s, rrow.get(), rcol.get(), reid.get(),
-                           ostart.get(), outdeg.get(), bhuge.get(), nhuge, maxk, hscratch.get(), out.T, w_tri);
-    PGD_HIP(hipGetLastError());
-    timer.end();
-    // ---- pass 2: 4-cliques and Q -----------------------------------------------
-    timer.begin("k_local_k4q");
-    if (nsmall)
-        hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, nsmall, 32)), dim3(64), small_lds, s,
-                           rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), bsmall.get(), nsmall,
-                           kSmallK, (char*)nullptr, out.T, out.K4, Q.get(), w_k4q);
-    if (nlarge)
-        hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, nlarge, 2)), dim3(256), large_lds, s,
-                           rrow.get(), rcol.get(), reid.get(), ostart.get(), outdeg.get(), blarge.get(), nlarge,
-                           kmax_lds, (char*)nullptr, out.T, out.K4, Q.get(), w_k4q);
-    if (nhuge)
-        hipLaunchKernelGGL((k_local_k4q<256, true>), dim3(ghuge), dim3(256), 0, s, rrow.get(), rcol.get(),
-                           reid.get(), ostart.get(), outdeg.get(), bhuge.get(), nhuge, maxk, hscratch.get(), out.T,
-                           out.K4, Q.get(), w_k4q);
-    PGD_HIP(hipGetLastError());
-    timer.end();
-    // ---- pass 3: 4-cycles (P) ----------------------------------------------------
-    DBuf<uint32_t> dense;
-    int gheavy = 0;
-    if (nheavy) {
-        const int64_t budget = (int64_t)16 << 30;  // bytes of dense counters
-        gheavy = (int)std::max<int64_t>(1, std::min<int64_t>({nheavy, (int64_t)ctx->num_cus * 2, budget / (4 * n)}));
-        dense.alloc(ctx, (size_t)gheavy * n);
-        PGD_HIP(hipMemsetAsync(dense.get(), 0, (size_t)gheavy * n * sizeof(uint32_t), s));
+    if (st.n_heavy) {
+        const int64_t budget = (int64_t)16 << 30;  // bytes of dense wedge counters
+        st.gheavy = (int)std::max<int64_t>(
+            1, std::min<int64_t>({st.n_heavy, (int64_t)ctx->num_cus * prm.heavy_per_cu, budget / (4 * n)}));
+        size_t need = (size_t)st.gheavy * n;
+        if (st.dense.get() == nullptr || st.dense.n < need) {
+            st.dense.alloc(ctx, need);
+            PGD_HIP(hipMemsetAsync(st.dense.get(), 0, need * sizeof(uint32_t), s));  // kept zero by the kernel
+        }
     }
-    timer.begin("k_wedge_light");
-    if (nlight)
-        hipLaunchKernelGGL(k_wedge_light, dim3(grid_cap(ctx, nlight, 32)), dim3(64), 0, s, rrow.get(), rcol.get(),
-                           reid.get(), ostart.get(), blight.get(), nlight, P.get(), w_light);
+}
+
+// the counting launches (no host synchronisation, no allocation, no memset:
+// capturable in a HIP graph).  With a timer, each pass is bracketed by events.
+void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
+    hipStream_t s = ctx->stream;
+    const int64_t m = st.m, n = st.n;
+    const OrientParams& prm = st.prm;
+    int gm = grid_cap(ctx, (m + 255) / 256, 16);
+    hipLaunchKernelGGL(k_fill_u32, dim3(gm), dim3(256), 0, s, out.T, m);
+    hipLaunchKernelGGL(k_fill_u64, dim3(gm), dim3(256), 0, s, out.K4, m);
+    hipLaunchKernelGGL(k_fill_u64, dim3(gm), dim3(256), 0, s, st.P.get(), m);
+    hipLaunchKernelGGL(k_fill_u64, dim3(gm), dim3(256), 0, s, st.Q.get(), m);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)st.work.get(), (int64_t)4);
     PGD_HIP(hipGetLastError());
-    timer.end();
-    timer.begin("k_wedge_heavy");
-    if (nheavy)
-        hipLaunchKernelGGL(k_wedge_heavy<256>, dim3(gheavy), dim3(256), 0, s, rrow.get(), rcol.get(), reid.get(),
-                           ostart.get(), bheavy.get(), nheavy, n, dense.get(), P.get(), w_heavy);
+    unsigned long long* w = st.work.get();
+    const int32_t *rr = nullptr;
+    (void)rr;
+    if (timer) timer->begin("k_local_tri");
+    if (st.n_small)
+        hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
+                           st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (char*)nullptr, out.T, w);
+    if (st.n_mid)
+        hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
+                           st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (char*)nullptr, out.T, w);
+    if (st.n_large)
+        hipLaunchKernelGGL((k_local_tri<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
+                           st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kmax_lds, (char*)nullptr, out.T, w);
+    if (st.n_huge)
+        hipLaunchKernelGGL((k_local_tri<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
+                           st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
+                           st.hscratch.get(), out.T, w);
     PGD_HIP(hipGetLastError());
-    timer.end();
-    timer.begin("k_combine");
-    hipLaunchKernelGGL(k_combine, dim3(grid_cap(ctx, (m + 255) / 256, 16)), dim3(256), 0, s, P.get(), Q.get(),
-                       out.T, out.K4, m, out.C4);
+    if (timer) timer->end();
+    if (timer) timer->begin("k_local_k4q");
+    if (st.n_small)
+        hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
+                           st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (char*)nullptr, out.T, out.K4,
+                           st.Q.get(), w + 1);
+    if (st.n_mid)
+        hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
+                           st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (char*)nullptr, out.T, out.K4,
+                           st.Q.get(), w + 1);
+    if (st.n_large)
+        hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
+                           st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kmax_lds, (char*)nullptr, out.T, out.K4,
+                           st.Q.get(), w + 1);
+    if (st.n_huge)
+        hipLaunchKernelGGL((k_local_k4q<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
+                           st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
+                           st.hscratch.get(), out.T, out.K4, st.Q.get(), w + 1);
     PGD_HIP(hipGetLastError());
-    timer.end();
+    if (timer) timer->end();
+    if (timer) timer->begin("k_wedge_light");
+    if (st.n_light)
+        hipLaunchKernelGGL(k_wedge_light, dim3(grid_cap(ctx, st.n_light, prm.grid_small)), dim3(64), 0, s,
+                           st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(), st.b_light.get(), st.n_light,
+                           st.P.get(), w + 2);
+    PGD_HIP(hipGetLastError());
+    if (timer) timer->end();
+    if (timer) timer->begin("k_wedge_heavy");
+    if (st.n_heavy)
+        hipLaunchKernelGGL(k_wedge_heavy<256>, dim3(st.gheavy), dim3(256), 0, s, st.rrow.get(), st.rcol.get(),
+                           st.reid.get(), st.ostart.get(), st.b_heavy.get(), st.n_heavy, n, st.dense.get(),
+                           st.P.get(), w + 3);
+    PGD_HIP(hipGetLastError());
+    if (timer) timer->end();
+    if (timer) timer->begin("k_combine");
+    hipLaunchKernelGGL(k_combine, dim3(gm), dim3(256), 0, s, st.P.get(), st.Q.get(), out.T, out.K4, m, out.C4);
+    PGD_HIP(hipGetLastError());
+    if (timer) timer->end();
+}
+
+void orient_profile(pgd_ctx* ctx, OrientState& st, Timer& timer, Timing* tm) {
     unsigned long long wk[4] = {0, 0, 0, 0};
-    PGD_HIP(hipMemcpyAsync(wk, work.get(), sizeof(wk), hipMemcpyDeviceToHost, s));
-    PGD_HIP(hipStreamSynchronize(s));
+    PGD_HIP(hipMemcpyAsync(wk, st.work.get(), sizeof(wk), hipMemcpyDeviceToHost, ctx->stream));
+    PGD_HIP(hipStreamSynchronize(ctx->stream));
     // algorithmic HBM bytes per pass (work models):
     //   local passes: 4 B per out-list entry read per enumeration (1 for the
     //     triangle pass, 2 for the clique pass: bit-matrix build + scan);
@@ -744,18 +796,34 @@ void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm,
     //     (light: count + accumulate; heavy: count + accumulate + clear), plus
     //     the 4 B dense counter touched by each of the three heavy passes;
     //   combine: 3 x 8 B + 4 B read and 8 B written per edge.
-    const double bytes_of[5] = {4.0 * wk[0], 8.0 * wk[1], 8.0 * wk[2], 24.0 * wk[3], 36.0 * m};
-    const double units_of[5] = {(double)wk[0], (double)wk[1], (double)wk[2], (double)wk[3], (double)m};
+    const double bytes_of[5] = {4.0 * wk[0], 8.0 * wk[1], 8.0 * wk[2], 24.0 * wk[3], 36.0 * st.m};
+    const double units_of[5] = {(double)wk[0], (double)wk[1], (double)wk[2], (double)wk[3], (double)st.m};
     double best = 0;
     tm->phases.clear();
     for (size_t i = 0; i < timer.ev.size(); i++) {
         float ms = 0;
         PGD_HIP(hipEventElapsedTime(&ms, timer.ev[i].first, timer.ev[i].second));
-        tm->phases.push_back(Phase{timer.names[i], ms, bytes_of[i], units_of[i]});
+        tm->phases.push_back(Phase{timer.names[i], ms, i < 5 ? bytes_of[i] : 0, i < 5 ? units_of[i] : 0});
         if (ms > best) best = ms;
     }
     tm->dominant_ms = best;
     tm->dominant_launches = 1;
 }
+
+void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm, OrientState** cache) {
+    OrientState* st = cache ? *cache : nullptr;
+    std::unique_ptr<OrientState> local;
+    if (!st) {
+        st = new OrientState();
+        if (cache) *cache = st;
+        else local.reset(st);
+    }
+    orient_build(ctx, g, *st);
+    Timer timer{ctx};
+    orient_count(ctx, *st, out, &timer);
+    orient_profile(ctx, *st, timer, tm);
+}
+
+OrientState* orient_state_new() { return new OrientState(); }
 
 }  // namespace pgd

@@ -112,6 +112,19 @@ struct GraphView {
 
 // engines (each writes EdgeCounts for every edge id)
 struct OrientState;
+struct Timer;
+// tunables of the oriented engine (bin thresholds and per-CU grid factors)
+struct OrientParams {
+    int small_k = 64;       // out-degree up to which one wavefront owns a local graph
+    int mid_k = 256;        // up to which a 256-thread block does (1024 threads above)
+    int grid_small = 32;    // resident blocks per CU for the wavefront bins
+    int grid_mid = 8;       // resident blocks per CU for the 256-thread bin
+    int heavy_per_cu = 2;   // heavy-wedge blocks (each with a dense counter slice) per CU
+};
+OrientState* orient_state_new();
+OrientParams& orient_params(OrientState* s);
+void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st);
+void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer);
 void engine_edge(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t);
 void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t, OrientState** cache);
 void orient_state_free(pgd_ctx* ctx, OrientState* s);
