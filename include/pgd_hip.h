@@ -147,6 +147,24 @@ pgd_status pgd_last_timing(pgd_graph *g, double *total_ms, double *dominant_ms, 
 pgd_status pgd_last_phase(pgd_graph *g, int32_t idx, char *name, int32_t cap, double *ms, double *bytes,
                           double *units);
 
+/* ---- tuning -------------------------------------------------------------
+ * Launch parameters of the oriented engine.  No reference counterpart (the
+ * reference's -w/-b flags size its OpenMP schedule); results never depend on
+ * them, only the time does. */
+typedef struct pgd_tuning {
+    int32_t small_k;     /* out-degree up to which one wavefront owns a local graph (64) */
+    int32_t mid_k;       /* out-degree up to which a 256-thread block does, 1024 threads above (256) */
+    int32_t grid_small;  /* resident wavefront blocks per CU for the local passes (32) */
+    int32_t grid_mid;    /* resident 256-thread blocks per CU (8) */
+    int32_t wedge_log2r; /* endpoint tile of the heavy 4-cycle pass, 2^r LDS counters, r in 8..15 (15) */
+    int32_t wedge_nt;    /* threads per block of that pass: 256, 512 or 1024 (512) */
+    int32_t wedge_split; /* work items per block the heavy wedges are cut into (4) */
+} pgd_tuning;
+pgd_status pgd_tuning_default(pgd_tuning *t);
+/* Parameters of later pgd_decompose calls on g. */
+pgd_status pgd_graph_set_tuning(pgd_graph *g, const pgd_tuning *t);
+pgd_status pgd_graph_get_tuning(pgd_graph *g, pgd_tuning *t);
+
 /* ---- derivation -------------------------------------------------------
  * The closed-form derivation of the 17 totals (graphlet_core.cpp:1449-1472,
  * graphlet_core.h:548-557) in integer arithmetic that reproduces the
@@ -172,11 +190,20 @@ pgd_status pgd_batch_create(pgd_ctx *ctx, int32_t ngraphs, const int32_t *n,
 pgd_status pgd_batch_run(pgd_batch *b, uint64_t *sums, uint64_t *totals);
 /* Per-edge micro column of graph gi (m_gi values, E_ordered order). */
 pgd_status pgd_batch_micro(pgd_batch *b, int32_t gi, int32_t column, uint64_t *out);
-pgd_status pgd_batch_info(pgd_batch *b, int64_t *total_edges, int32_t *tile_edges, int32_t *bin_threshold);
+/* Sizes of the batch: graphs, total vertices and edges (after the reader's
+ * dedup rules). */
+pgd_status pgd_batch_info(pgd_batch *b, int32_t *ngraphs, int64_t *total_vertices, int64_t *total_edges);
 pgd_status pgd_batch_free(pgd_batch *b);
-/* Tile size / hub-bin threshold used by later batch runs (0 = autotuned). */
-pgd_status pgd_batch_set_tuning(pgd_batch *b, int32_t tile_edges, int32_t bin_threshold);
-pgd_status pgd_batch_autotune(pgd_batch *b, int32_t *tile_edges, int32_t *bin_threshold);
+/* Launch parameters for later runs (re-plans and re-captures the launch). */
+pgd_status pgd_batch_set_tuning(pgd_batch *b, const pgd_tuning *t);
+pgd_status pgd_batch_get_tuning(pgd_batch *b, pgd_tuning *t);
+/* Time each candidate of a built-in search space on this batch (replays of
+ * the captured launch), keep the fastest and return it.  Every candidate's
+ * sums are compared with the first one's: PGD_ERR_STATE if any differs.
+ * trials: replays per candidate (>= 1). */
+pgd_status pgd_batch_autotune(pgd_batch *b, int32_t trials, pgd_tuning *best, double *best_ms);
+/* Device time (ms) of the last pgd_batch_run's replay. */
+pgd_status pgd_batch_last_ms(pgd_batch *b, double *ms);
 
 /* ---- serving ----------------------------------------------------------
  * A request loop with continuous batching: requests (one graph each) are

@@ -14,16 +14,27 @@ import weakref
 import numpy as np
 
 from ._lib import (DEDUP, ENGINE_DENSE, ENGINE_EDGE, ENGINE_ORIENT, MICRO, NMICRO, NSUMS, NTOTALS,
-                   TOTAL_NAMES, PGDError, check, lib, ptr)
+                   TOTAL_NAMES, PGDError, Tuning, check, lib, ptr)
 
 __all__ = [
     "Context", "GraphletCore", "derive", "gen_gnp", "gen_ba", "gen_rmat", "write_mtx", "read_graph",
-    "TOTAL_NAMES", "PGDError", "MICRO_COLUMNS",
+    "TOTAL_NAMES", "PGDError", "MICRO_COLUMNS", "Tuning", "default_tuning",
 ]
 
 MICRO_COLUMNS = ["tri", "local_4_clique", "local_4_chordal_cycle", "local_4_tailed_tris", "local_4_cycle",
                  "local_3_star", "local_4_path", "2_star"]
 _ENGINE = {None: 0, "edge": ENGINE_EDGE, "orient": ENGINE_ORIENT, "dense": ENGINE_DENSE}
+
+
+def default_tuning(**overrides) -> Tuning:
+    """pgd_tuning_default, with fields replaced by keyword."""
+    t = Tuning()
+    check(lib().pgd_tuning_default(C.byref(t)), what="pgd_tuning_default")
+    for k, v in overrides.items():
+        if not hasattr(t, k):
+            raise KeyError(k)
+        setattr(t, k, int(v))
+    return t
 
 
 class Context:
@@ -277,6 +288,18 @@ class GraphletCore:
         check(lib().pgd_edge_counts(self.h, ptr(t, C.c_uint64), ptr(k, C.c_uint64), ptr(c, C.c_uint64)),
               self.ctx.h)
         return t[: self.m], k[: self.m], c[: self.m]
+
+    def set_tuning(self, tuning: Tuning | None = None, **overrides) -> None:
+        """Launch parameters of later decompositions (pgd_graph_set_tuning)."""
+        t = tuning if tuning is not None else default_tuning()
+        for k, v in overrides.items():
+            setattr(t, k, int(v))
+        check(lib().pgd_graph_set_tuning(self.h, C.byref(t)), self.ctx.h, "pgd_graph_set_tuning")
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        check(lib().pgd_graph_get_tuning(self.h, C.byref(t)), self.ctx.h)
+        return t
 
     def last_engine(self) -> str:
         return {1: "edge", 2: "orient", 3: "dense"}.get(lib().pgd_last_engine(self.h), "none")

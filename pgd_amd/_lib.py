@@ -35,11 +35,21 @@ EXPORTS = [
     "pgd_graph_from_coo", "pgd_graph_from_csr", "pgd_graph_free", "pgd_graph_info", "pgd_graph_csr", "pgd_preprocess",
     "pgd_edge_arrays", "pgd_compute_cores", "pgd_sort_edges", "pgd_e_ordered", "pgd_decompose", "pgd_micro",
     "pgd_edge_counts", "pgd_last_engine", "pgd_last_timing", "pgd_last_phase", "pgd_derive", "pgd_batch_create", "pgd_batch_run",
-    "pgd_batch_micro", "pgd_batch_info", "pgd_batch_free", "pgd_batch_set_tuning", "pgd_batch_autotune",
+    "pgd_batch_micro", "pgd_batch_info", "pgd_batch_free", "pgd_batch_set_tuning", "pgd_batch_get_tuning",
+    "pgd_batch_autotune", "pgd_batch_last_ms", "pgd_tuning_default", "pgd_graph_set_tuning", "pgd_graph_get_tuning",
     "pgd_server_create", "pgd_server_submit", "pgd_server_poll", "pgd_server_result", "pgd_server_micro",
     "pgd_server_release", "pgd_server_free", "pgd_gen_gnp", "pgd_gen_ba", "pgd_gen_rmat", "pgd_write_mtx",
     "pgd_read_graph", "pgd_free",
 ]
+
+
+class Tuning(C.Structure):
+    """pgd_tuning: launch parameters of the oriented engine."""
+    _fields_ = [(f, C.c_int32) for f in (
+        "small_k", "mid_k", "grid_small", "grid_mid", "wedge_log2r", "wedge_nt", "wedge_split")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class PGDError(RuntimeError):
@@ -89,10 +99,15 @@ def lib() -> C.CDLL:
         "pgd_batch_create": (I32, [P, I32, PI32, PI64, PI32, PI32, U32, C.POINTER(P)]),
         "pgd_batch_run": (I32, [P, PU64, PU64]),
         "pgd_batch_micro": (I32, [P, I32, I32, PU64]),
-        "pgd_batch_info": (I32, [P, PI64, PI32, PI32]),
+        "pgd_batch_info": (I32, [P, PI32, PI64, PI64]),
         "pgd_batch_free": (I32, [P]),
-        "pgd_batch_set_tuning": (I32, [P, I32, I32]),
-        "pgd_batch_autotune": (I32, [P, PI32, PI32]),
+        "pgd_batch_set_tuning": (I32, [P, C.POINTER(Tuning)]),
+        "pgd_batch_get_tuning": (I32, [P, C.POINTER(Tuning)]),
+        "pgd_batch_autotune": (I32, [P, I32, C.POINTER(Tuning), C.POINTER(D)]),
+        "pgd_batch_last_ms": (I32, [P, C.POINTER(D)]),
+        "pgd_tuning_default": (I32, [C.POINTER(Tuning)]),
+        "pgd_graph_set_tuning": (I32, [P, C.POINTER(Tuning)]),
+        "pgd_graph_get_tuning": (I32, [P, C.POINTER(Tuning)]),
         "pgd_server_create": (I32, [P, I64, I32, C.POINTER(P)]),
         "pgd_server_submit": (I32, [P, I32, I64, PI32, PI32, U32, PI64]),
         "pgd_server_poll": (I32, [P, PI64, I32, PI32]),

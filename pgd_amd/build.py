@@ -39,6 +39,7 @@ HIP_SOURCES = [
     "pgd_core.hip",
     "pgd_engine_edge.hip",
     "pgd_engine_orient.hip",
+    "pgd_engine_wedge.hip",
     "pgd_engine_dense.hip",
     "pgd_batch.hip",
 ]

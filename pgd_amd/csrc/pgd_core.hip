@@ -969,7 +969,7 @@ pgd_status pgd_decompose(pgd_graph* g, uint32_t flags, uint64_t* sums) {
         PGD_CHECK(engine != 1, PGD_ERR_UNSUPPORTED, "edge-centric engine disabled");
         if (g->m > 0) {
             if (engine == 1) engine_edge(ctx, v, c, &t);
-            else if (engine == 2) engine_orient(ctx, v, c, &t, &g->orient);
+            else if (engine == 2) engine_orient(ctx, v, c, &t, &g->orient, g->prm);
             else {
                 PGD_CHECK(engine_dense_ok(v), PGD_ERR_UNSUPPORTED, "graph too large for the dense engine");
                 engine_dense(ctx, v, c, &t);
@@ -1024,6 +1024,27 @@ pgd_status pgd_edge_counts(pgd_graph* g, uint64_t* tri, uint64_t* k4, uint64_t* 
             PGD_HIP(hipMemcpyAsync(c4, g->C4.get(), m * sizeof(uint64_t), hipMemcpyDeviceToHost, g->ctx->stream));
         sync(g->ctx);
     });
+}
+
+pgd_status pgd_tuning_default(pgd_tuning* t) {
+    if (!t) return PGD_ERR_INVALID;
+    *t = tuning_from(OrientParams{});
+    return PGD_OK;
+}
+
+pgd_status pgd_graph_set_tuning(pgd_graph* g, const pgd_tuning* t) {
+    if (!g || !t) return PGD_ERR_INVALID;
+    return guarded(g->ctx, [&] {
+        const char* why = tuning_problem(*t);
+        PGD_CHECK(why[0] == 0, PGD_ERR_INVALID, why);
+        g->prm = params_from(*t);
+    });
+}
+
+pgd_status pgd_graph_get_tuning(pgd_graph* g, pgd_tuning* t) {
+    if (!g || !t) return PGD_ERR_INVALID;
+    *t = tuning_from(g->prm);
+    return PGD_OK;
 }
 
 int32_t pgd_last_engine(pgd_graph* g) { return g ? g->last_engine : 0; }

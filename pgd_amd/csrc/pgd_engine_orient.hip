@@ -30,64 +30,18 @@
 // is O(m * arboricity), orders of magnitude below the reference's per-edge
 // 2-hop work.  All counts are exact uint64 (uint32 for T), accumulated with
 // atomics, so the result is independent of scheduling.
-#include <cstring>
-#include <rocprim/rocprim.hpp>
-
-#include <algorithm>
 #include <memory>
 
-#include "pgd_internal.hpp"
+#include "pgd_orient.hpp"
 
 using namespace pgd;
 
 
 namespace {
 
-constexpr int kSmallK = 64;       // out-degree handled by one wavefront
 constexpr int kLightBound = 256;  // wedge bound handled by a wavefront with an LDS hash
 constexpr int kHashSlots = 512;
-constexpr int kChunk = 1024;      // in-neighbours per chunk in the heavy P kernel
 
-inline int grid_cap(pgd_ctx* ctx, int64_t items, int per_cu) {
-    int64_t g = std::min<int64_t>(items, (int64_t)ctx->num_cus * per_cu);
-    return (int)std::max<int64_t>(g, 1);
-}
-
-template <class T, class U>
-void excl_sum(pgd_ctx* ctx, const T* in, U* out, size_t n) {
-    if (!n) return;
-    size_t bytes = 0;
-    PGD_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, U(0), n, rocprim::plus<U>(), ctx->stream));
-    void* t = tmp_storage(ctx, bytes);
-    PGD_HIP(rocprim::exclusive_scan(t, bytes, in, out, U(0), n, rocprim::plus<U>(), ctx->stream));
-}
-
-template <class K, class V>
-void sort_kv(pgd_ctx* ctx, K* kin, K* kout, V* vin, V* vout, size_t n, unsigned b1) {
-    if (!n) return;
-    size_t bytes = 0;
-    PGD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, b1, ctx->stream));
-    void* t = tmp_storage(ctx, bytes);
-    PGD_HIP(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, 0u, b1, ctx->stream));
-}
-
-template <class K>
-void sort_k(pgd_ctx* ctx, K* kin, K* kout, size_t n, unsigned b1) {
-    if (!n) return;
-    size_t bytes = 0;
-    PGD_HIP(rocprim::radix_sort_keys(nullptr, bytes, kin, kout, n, 0u, b1, ctx->stream));
-    void* t = tmp_storage(ctx, bytes);
-    PGD_HIP(rocprim::radix_sort_keys(t, bytes, kin, kout, n, 0u, b1, ctx->stream));
-}
-
-int bitsfor(int64_t v) {
-    int b = 1;
-    while (b < 63 && ((int64_t)1 << b) <= v) b++;
-    return b;
-}
-
-#define GRID_STRIDE(i, n) \
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
 // ---------------------------------------------------------------------------
 // relabelled, sorted CSR
@@ -172,52 +126,6 @@ __global__ void k_bin_scatter(const int32_t* __restrict__ f, const int64_t* __re
     GRID_STRIDE(a, n) if (f[a]) out[off[a]] = (int32_t)a;
 }
 
-// ---------------------------------------------------------------------------
-// block-level helpers (blockDim.x == NT)
-// ---------------------------------------------------------------------------
-// inclusive scan of s[0..k) in place (int64), NT threads, k arbitrary
-template <int NT>
-__device__ void block_incl_scan(int64_t* s, int k, int64_t* part) {
-    const int tid = threadIdx.x;
-    const int per = (k + NT - 1) / NT;
-    const int b = tid * per, e = min(b + per, k);
-    int64_t acc = 0;
-    for (int i = b; i < e; i++) { acc += s[i]; s[i] = acc; }
-    part[tid] = acc;
-    __syncthreads();
-    for (int off = 1; off < NT; off <<= 1) {
-        int64_t y = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += y;
-        __syncthreads();
-    }
-    int64_t base = tid ? part[tid - 1] : 0;
-    for (int i = b; i < e; i++) s[i] += base;
-    __syncthreads();
-}
-
-// first index i in [0,k) with incl[i] > t (incl nondecreasing, incl[k-1] > t)
-__device__ __forceinline__ int owner_of(const int64_t* incl, int k, int64_t t) {
-    int lo = 0, hi = k - 1;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (incl[mid] > t) hi = mid;
-        else lo = mid + 1;
-    }
-    return lo;
-}
-
-// index of c in L[lo..hi) (ascending) or -1
-__device__ __forceinline__ int find_in(const int32_t* L, int lo, int hi, int32_t c) {
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        int32_t x = L[mid];
-        if (x < c) lo = mid + 1;
-        else if (x > c) hi = mid;
-        else return mid;
-    }
-    return -1;
-}
 
 // ---------------------------------------------------------------------------
 // local-graph kernels (triangles; 4-cliques + Q).  One block per vertex a of
@@ -376,14 +284,6 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
 // [row[a], ostart[a]) of its ascending list; for each b the c < a are the
 // prefix of N(b) below a.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int64_t lower_bound_g(const int32_t* __restrict__ col, int64_t lo, int64_t hi, int32_t x) {
-    while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (col[mid] < x) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
 
 __device__ __forceinline__ uint32_t hslot(int32_t c) { return ((uint32_t)c * 2654435761u) >> 23; }  // 9 bits
 
@@ -450,67 +350,6 @@ __global__ __launch_bounds__(64) void k_wedge_light(const int64_t* __restrict__ 
     }
 }
 
-// heavy vertices: one block per vertex with a dense counter array of n
-// entries in HBM (a slice per block), kept zero between vertices
-template <int NT>
-__global__ __launch_bounds__(NT) void k_wedge_heavy(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
-                                                    const int32_t* __restrict__ eid,
-                                                    const int64_t* __restrict__ ostart,
-                                                    const int32_t* __restrict__ list, int64_t cnt, int64_t n,
-                                                    uint32_t* __restrict__ dense, uint64_t* __restrict__ P,
-                                                    unsigned long long* __restrict__ work) {
-    __shared__ int64_t pref[kChunk];
-    __shared__ int64_t bbeg[kChunk];
-    __shared__ uint64_t psum[kChunk];
-    __shared__ int64_t part[NT];
-    const int tid = threadIdx.x;
-    uint32_t* cntarr = dense + (size_t)blockIdx.x * (size_t)n;
-    for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
-        const int32_t a = list[li];
-        const int64_t ib = row[a], ie = ostart[a];
-        for (int pass = 0; pass < 3; pass++) {  // count, accumulate, clear
-            for (int64_t cb = ib; cb < ie; cb += kChunk) {
-                const int kc = (int)min<int64_t>(kChunk, ie - cb);
-                for (int i = tid; i < kc; i += NT) {
-                    int32_t b = col[cb + i];
-                    int64_t bb = row[b];
-                    bbeg[i] = bb;
-                    pref[i] = lower_bound_g(col, bb, row[b + 1], a) - bb;
-                    psum[i] = 0;
-                }
-                __syncthreads();
-                block_incl_scan<NT>(pref, kc, part);
-                const int64_t total = pref[kc - 1];
-                if (pass == 0 && tid == 0) atomicAdd(work, (unsigned long long)total);
-                for (int64_t t = tid; t < total; t += NT) {
-                    const int i = owner_of(pref, kc, t);
-                    const int64_t ex = i ? pref[i - 1] : 0;
-                    const int64_t slot = bbeg[i] + (t - ex);
-                    const int32_t c = col[slot];
-                    if (pass == 0) {
-                        atomicAdd(&cntarr[c], 1u);
-                    } else if (pass == 1) {
-                        const uint32_t s = __hip_atomic_load(&cntarr[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - 1;
-                        if (s) {
-                            atomicAdd((unsigned long long*)&P[eid[slot]], (unsigned long long)s);
-                            atomicAdd((unsigned long long*)&psum[i], (unsigned long long)s);
-                        }
-                    } else {
-                        __hip_atomic_store(&cntarr[c], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    }
-                }
-                __syncthreads();
-                if (pass == 1)
-                    for (int i = tid; i < kc; i += NT)
-                        if (psum[i]) atomicAdd((unsigned long long*)&P[eid[cb + i]], (unsigned long long)psum[i]);
-                __syncthreads();
-            }
-            __threadfence();
-            __syncthreads();
-        }
-    }
-}
-
 // C4 = P - Q + 2T + 2K4 (uint64 wrap-around; the true value is >= 0)
 __global__ void k_combine(const uint64_t* __restrict__ P, const uint64_t* __restrict__ Q,
                           const uint32_t* __restrict__ T, const uint64_t* __restrict__ K4, int64_t m,
@@ -520,37 +359,9 @@ __global__ void k_combine(const uint64_t* __restrict__ P, const uint64_t* __rest
 
 }  // namespace
 
-namespace pgd {
-struct Timer {
-    pgd_ctx* ctx;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    std::vector<const char*> names;
-    void begin(const char* name) {
-        hipEvent_t a, b;
-        PGD_HIP(hipEventCreate(&a));
-        PGD_HIP(hipEventCreate(&b));
-        PGD_HIP(hipEventRecord(a, ctx->stream));
-        ev.push_back({a, b});
-        names.push_back(name);
-    }
-    void end() { PGD_HIP(hipEventRecord(ev.back().second, ctx->stream)); }
-    ~Timer() {
-        for (auto& p : ev) {
-            (void)hipEventDestroy(p.first);
-            (void)hipEventDestroy(p.second);
-        }
-    }
-};
-}  // namespace pgd
 
 namespace {
 
-// capacity-reusing allocation: keeps the buffer when it is already large enough
-template <class T>
-void ensure(DBuf<T>& b, pgd_ctx* ctx, size_t n) {
-    if (n == 0) n = 1;
-    if (b.get() == nullptr || b.n < n) b.alloc(ctx, n);
-}
 
 // compaction of the vertices whose key falls in (lo, hi] into `out` (capacity n)
 template <class K>
@@ -589,25 +400,6 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n) {
 
 namespace pgd {
 
-// Persistent state of the oriented engine for one graph: the relabelled CSR,
-// the vertex bins, scratch and accumulators.  Buffers are kept across calls
-// (their contents are rebuilt every call), so the counting launches can be
-// captured once in a HIP graph and replayed.
-struct OrientState {
-    OrientParams prm;
-    int64_t n = 0, m = 0;
-    DBuf<int32_t> rank, rdeg, rcol, reid, outdeg, v0, flags;
-    DBuf<int64_t> rrow, ostart, off, ub;
-    DBuf<uint64_t> k0, k1;
-    DBuf<int32_t> b_small, b_mid, b_large, b_huge, b_light, b_heavy;
-    int64_t n_small = 0, n_mid = 0, n_large = 0, n_huge = 0, n_light = 0, n_heavy = 0;
-    int kmax_lds = 0, maxk = 0, ghuge = 0, gheavy = 0;
-    size_t small_lds = 0, mid_lds = 0, large_lds = 0;
-    DBuf<char> hscratch;
-    DBuf<uint32_t> dense;
-    DBuf<uint64_t> P, Q;
-    DBuf<unsigned long long> work;
-};
 
 void orient_state_free(pgd_ctx*, OrientState* s) { delete s; }
 
@@ -654,16 +446,7 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
     hipLaunchKernelGGL(k_out_start, dim3(gn), dim3(256), 0, s, st.rrow.get(), st.rcol.get(), n, st.ostart.get(),
                        st.outdeg.get());
     PGD_HIP(hipGetLastError());
-    {
-        DBuf<int32_t> mx;
-        mx.alloc(ctx, 1);
-        size_t bytes = 0;
-        PGD_HIP(rocprim::reduce(nullptr, bytes, st.outdeg.get(), mx.get(), 0, (size_t)n, rocprim::maximum<int32_t>(), s));
-        void* t = tmp_storage(ctx, bytes);
-        PGD_HIP(rocprim::reduce(t, bytes, st.outdeg.get(), mx.get(), 0, (size_t)n, rocprim::maximum<int32_t>(), s));
-        PGD_HIP(hipMemcpyAsync(&st.maxk, mx.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        PGD_HIP(hipStreamSynchronize(s));
-    }
+    st.maxk = device_max(ctx, st.outdeg.get(), (size_t)n);
     // the largest local graph whose arrays and bit matrix fit one block's LDS
     hipDeviceProp_t prop;
     PGD_HIP(hipGetDeviceProperties(&prop, ctx->device));
@@ -696,16 +479,7 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
         st.ghuge = (int)std::min<int64_t>(st.n_huge, ctx->num_cus);
         ensure(st.hscratch, ctx, (size_t)st.ghuge * carve_bytes(st.maxk, (st.maxk + 31) / 32));
     }
-    if (st.n_heavy) {
-        const int64_t budget = (int64_t)16 << 30;  // bytes of dense wedge counters
-        st.gheavy = (int)std::max<int64_t>(
-            1, std::min<int64_t>({st.n_heavy, (int64_t)ctx->num_cus * prm.heavy_per_cu, budget / (4 * n)}));
-        size_t need = (size_t)st.gheavy * n;
-        if (st.dense.get() == nullptr || st.dense.n < need) {
-            st.dense.alloc(ctx, need);
-            PGD_HIP(hipMemsetAsync(st.dense.get(), 0, need * sizeof(uint32_t), s));  // kept zero by the kernel
-        }
-    }
+    if (st.n_heavy) wedge_plan_build(ctx, st);
 }
 
 // the counting launches (no host synchronisation, no allocation, no memset:
@@ -772,12 +546,8 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
                            st.P.get(), w + 2);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
-    if (timer) timer->begin("k_wedge_heavy");
-    if (st.n_heavy)
-        hipLaunchKernelGGL(k_wedge_heavy<256>, dim3(st.gheavy), dim3(256), 0, s, st.rrow.get(), st.rcol.get(),
-                           st.reid.get(), st.ostart.get(), st.b_heavy.get(), st.n_heavy, n, st.dense.get(),
-                           st.P.get(), w + 3);
-    PGD_HIP(hipGetLastError());
+    if (timer) timer->begin("k_wedge_tiled");
+    if (st.n_heavy) wedge_plan_count(ctx, st, w + 3);
     if (timer) timer->end();
     if (timer) timer->begin("k_combine");
     hipLaunchKernelGGL(k_combine, dim3(gm), dim3(256), 0, s, st.P.get(), st.Q.get(), out.T, out.K4, m, out.C4);
@@ -792,11 +562,11 @@ void orient_profile(pgd_ctx* ctx, OrientState& st, Timer& timer, Timing* tm) {
     // algorithmic HBM bytes per pass (work models):
     //   local passes: 4 B per out-list entry read per enumeration (1 for the
     //     triangle pass, 2 for the clique pass: bit-matrix build + scan);
-    //   wedge passes: the 4 B endpoint id of every wedge, read once per pass
-    //     (light: count + accumulate; heavy: count + accumulate + clear), plus
-    //     the 4 B dense counter touched by each of the three heavy passes;
+    //   light wedges: the 4 B endpoint id of every wedge, read by the count
+    //     and the accumulate pass;
+    //   tiled wedges: the same two 4 B reads plus the 8 B per-slot update;
     //   combine: 3 x 8 B + 4 B read and 8 B written per edge.
-    const double bytes_of[5] = {4.0 * wk[0], 8.0 * wk[1], 8.0 * wk[2], 24.0 * wk[3], 36.0 * st.m};
+    const double bytes_of[5] = {4.0 * wk[0], 8.0 * wk[1], 8.0 * wk[2], 16.0 * wk[3], 36.0 * st.m};
     const double units_of[5] = {(double)wk[0], (double)wk[1], (double)wk[2], (double)wk[3], (double)st.m};
     double best = 0;
     tm->phases.clear();
@@ -810,7 +580,8 @@ void orient_profile(pgd_ctx* ctx, OrientState& st, Timer& timer, Timing* tm) {
     tm->dominant_launches = 1;
 }
 
-void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm, OrientState** cache) {
+void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm, OrientState** cache,
+                   const OrientParams& prm) {
     OrientState* st = cache ? *cache : nullptr;
     std::unique_ptr<OrientState> local;
     if (!st) {
@@ -818,6 +589,7 @@ void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm,
         if (cache) *cache = st;
         else local.reset(st);
     }
+    st->prm = prm;
     orient_build(ctx, g, *st);
     Timer timer{ctx};
     orient_count(ctx, *st, out, &timer);
@@ -825,5 +597,16 @@ void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm,
 }
 
 OrientState* orient_state_new() { return new OrientState(); }
+
+const char* tuning_problem(const pgd_tuning& t) {
+    if (t.small_k < 2 || t.small_k > 64) return "small_k must be in 2..64 (one wavefront)";
+    if (t.mid_k < t.small_k || t.mid_k > 256) return "mid_k must be in small_k..256";
+    if (t.grid_small < 1 || t.grid_small > 64 || t.grid_mid < 1 || t.grid_mid > 64)
+        return "grid factors must be in 1..64 blocks per CU";
+    if (t.wedge_log2r < 8 || t.wedge_log2r > 15) return "wedge_log2r must be in 8..15";
+    if (t.wedge_nt != 256 && t.wedge_nt != 512 && t.wedge_nt != 1024) return "wedge_nt must be 256, 512 or 1024";
+    if (t.wedge_split < 1 || t.wedge_split > 64) return "wedge_split must be in 1..64";
+    return "";
+}
 
 }  // namespace pgd

@@ -1,0 +1,471 @@
+// pgd_engine_wedge.hip -- the 4-cycle pass (P) of the oriented engine for
+// heavy vertices, with the wedge endpoints counted in LDS tiles.
+//
+// A 4-cycle is found once, from its highest-ranked vertex a, as two wedges
+// a-b-c and a-b'-c with b, b', c < a.  For the hub a, cnt_a[c] = number of
+// wedges a-.-c; the wedge a-b-c adds cnt_a[c]-1 to the edges (a,b) and (b,c).
+// cnt_a spans all ranks below a (millions for a hub), so it is cut into
+// endpoint tiles of 2^log2r ranks whose counters live in LDS.  For one hub,
+// the wedges of one tile are a set of segments: runs of the ascending lists
+// N(b) (b an in-neighbour of a) whose endpoints fall in the tile.  A per-vertex
+// tile table (one entry per run of equal tiles in N(b)) lets a block bucket a
+// hub's segments by tile with a counting sort, then count (LDS atomics) and
+// accumulate tile by tile.  The update of the edge (b,c) is added to a
+// per-slot accumulator PS (consecutive lanes hit consecutive slots of N(b)),
+// folded into P by edge id at the end.
+//
+// The work is cut into items (hub, tile range) bounded in segments (the
+// per-block buffer) and in wedges (load balance), and a persistent grid pulls
+// them largest first from a device queue.
+#include "pgd_orient.hpp"
+
+namespace pgd {
+namespace {
+
+constexpr int kTmax = 128;       // tiles per item (LDS bucket table)
+constexpr int kPlanLds = 2048;   // (hub, tile) cells counted in LDS by the planner
+
+__global__ void w_fill_u64(uint64_t* __restrict__ p, int64_t n) {
+    GRID_STRIDE(i, n) p[i] = 0;
+}
+__global__ void w_fill_u32(uint32_t* __restrict__ p, int64_t n) {
+    GRID_STRIDE(i, n) p[i] = 0;
+}
+
+// wcnt[s] = |N(b) below a| for every in-slot s = (a, b) of a heavy hub a;
+// one wavefront per hub
+__global__ void k_pair_wedges(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                              const int64_t* __restrict__ ostart, const int32_t* __restrict__ heavy, int64_t nheavy,
+                              int32_t* __restrict__ wcnt) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t h = wave; h < nheavy; h += nw) {
+        const int32_t a = heavy[h];
+        for (int64_t s = row[a] + lane; s < ostart[a]; s += 64) {
+            const int32_t b = col[s];
+            wcnt[s] = (int32_t)(lower_bound_g(col, row[b], row[b + 1], a) - row[b]);
+        }
+    }
+}
+
+// tile-table flags: slot p starts a run of equal endpoint tiles in its list
+__global__ void k_tt_flags(const int64_t* __restrict__ row, const int32_t* __restrict__ col, int64_t n, int log2r,
+                           int32_t* __restrict__ flag) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t b = wave; b < n; b += nw) {
+        const int64_t r0 = row[b], r1 = row[b + 1];
+        for (int64_t p = r0 + lane; p < r1; p += 64)
+            flag[p] = (p == r0) || ((col[p] >> log2r) != (col[p - 1] >> log2r));
+    }
+}
+
+__global__ void k_tt_fill(const int64_t* __restrict__ row, const int32_t* __restrict__ col, int64_t n, int64_t nnz,
+                          int log2r, const int32_t* __restrict__ flag, const int64_t* __restrict__ pos,
+                          int64_t total, uint64_t* __restrict__ tt, int64_t* __restrict__ ttrow) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t b = wave; b < n; b += nw) {
+        const int64_t r0 = row[b], r1 = row[b + 1];
+        if (lane == 0) ttrow[b] = r0 < nnz ? pos[r0] : total;
+        for (int64_t p = r0 + lane; p < r1; p += 64)
+            if (flag[p]) tt[pos[p]] = ((uint64_t)(uint32_t)(col[p] >> log2r) << 32) | (uint32_t)(p - r0);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ttrow[n] = total;
+}
+
+// number of endpoint tiles below each heavy hub a: tiles of ranks 0..a-1
+__global__ void k_hub_tiles(const int32_t* __restrict__ heavy, int64_t nheavy, int log2r, int64_t* __restrict__ nt) {
+    GRID_STRIDE(h, nheavy) nt[h] = ((int64_t)(heavy[h] - 1) >> log2r) + 1;
+}
+
+// first tile-table entry of b with tile >= t (entries ascending by tile)
+__device__ __forceinline__ int64_t tt_seek(const uint64_t* __restrict__ tt, int64_t lo, int64_t hi, uint32_t t) {
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((uint32_t)(tt[mid] >> 32) < t) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// segments and wedges per (hub, tile) cell; one block per hub
+template <int NT>
+__global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                                                   const int64_t* __restrict__ ostart,
+                                                   const int32_t* __restrict__ heavy, int64_t nheavy,
+                                                   const int32_t* __restrict__ wcnt, const uint64_t* __restrict__ tt,
+                                                   const int64_t* __restrict__ ttrow, const int64_t* __restrict__ htoff,
+                                                   uint32_t* __restrict__ htseg, uint64_t* __restrict__ htw) {
+    __shared__ uint32_t lseg[kPlanLds];
+    __shared__ unsigned long long lw[kPlanLds];
+    for (int64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
+        const int32_t a = heavy[h];
+        const int64_t c0 = htoff[h], T = htoff[h + 1] - c0;
+        const bool in_lds = T <= kPlanLds;
+        if (in_lds)
+            for (int i = threadIdx.x; i < T; i += NT) { lseg[i] = 0; lw[i] = 0; }
+        __syncthreads();
+        for (int64_t s = row[a] + threadIdx.x; s < ostart[a]; s += NT) {
+            const int32_t b = col[s];
+            const uint32_t w = (uint32_t)wcnt[s];
+            const int64_t j1 = ttrow[b + 1];
+            const uint32_t deg = (uint32_t)(row[b + 1] - row[b]);
+            for (int64_t j = ttrow[b]; j < j1; j++) {
+                const uint64_t e = tt[j];
+                const uint32_t off = (uint32_t)e;
+                if (off >= w) break;
+                const uint32_t nxt = j + 1 < j1 ? (uint32_t)tt[j + 1] : deg;
+                const uint32_t len = min(nxt, w) - off;
+                const uint32_t t = (uint32_t)(e >> 32);
+                if (in_lds) {
+                    atomicAdd(&lseg[t], 1u);
+                    atomicAdd(&lw[t], (unsigned long long)len);
+                } else {
+                    atomicAdd(&htseg[c0 + t], 1u);
+                    atomicAdd((unsigned long long*)&htw[c0 + t], (unsigned long long)len);
+                }
+            }
+        }
+        __syncthreads();
+        if (in_lds)
+            for (int i = threadIdx.x; i < T; i += NT) { htseg[c0 + i] = lseg[i]; htw[c0 + i] = lw[i]; }
+        __syncthreads();
+    }
+}
+
+// cut each hub's tiles into items: at most segcap segments, kTmax tiles and
+// (unless a single tile exceeds it) wcap wedges; empty items are dropped.
+// With out == nullptr only the number of items per hub is written.
+__global__ void k_plan_items(const int64_t* __restrict__ htoff, const uint32_t* __restrict__ htseg,
+                             const uint64_t* __restrict__ htw, int64_t nheavy, int64_t segcap, uint64_t wcap,
+                             int64_t* __restrict__ nitems, const int64_t* __restrict__ ioff,
+                             WedgeItem* __restrict__ out, uint64_t* __restrict__ key) {
+    GRID_STRIDE(h, nheavy) {
+        const int64_t c0 = htoff[h], T = htoff[h + 1] - c0;
+        int64_t cnt = 0, t0 = 0, segs = 0;
+        uint64_t wed = 0;
+        auto close = [&](int64_t t1) {
+            if (wed) {
+                if (out) {
+                    const int64_t i = ioff[h] + cnt;
+                    out[i] = WedgeItem{(int32_t)h, (int32_t)t0, (int32_t)t1, 0};
+                    key[i] = ~wed;  // ascending sort of ~w: largest first
+                }
+                cnt++;
+            }
+        };
+        for (int64_t t = 0; t < T; t++) {
+            const uint32_t s = htseg[c0 + t];
+            const uint64_t w = htw[c0 + t];
+            if (t > t0 && (segs + s > segcap || (wed && wed + w > wcap) || t - t0 >= kTmax)) {
+                close(t);
+                t0 = t;
+                segs = 0;
+                wed = 0;
+            }
+            segs += s;
+            wed += w;
+        }
+        close(T);
+        if (!out) nitems[h] = cnt;
+    }
+}
+
+__global__ void k_gather_items(const uint64_t* __restrict__ order, const WedgeItem* __restrict__ in, int64_t n,
+                               WedgeItem* __restrict__ out) {
+    GRID_STRIDE(i, n) out[i] = in[(uint32_t)order[i]];
+}
+
+__global__ void k_item_keys(const uint64_t* __restrict__ key, int64_t n, uint64_t* __restrict__ k2,
+                            uint64_t* __restrict__ v) {
+    GRID_STRIDE(i, n) {
+        k2[i] = key[i];
+        v[i] = (uint64_t)i;
+    }
+}
+
+// the tiled count/accumulate kernel (persistent; items from the queue)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_wedge_tiled(
+    const int64_t* __restrict__ row, const int32_t* __restrict__ col, const int32_t* __restrict__ eid,
+    const int64_t* __restrict__ ostart, const int32_t* __restrict__ heavy, const int32_t* __restrict__ wcnt,
+    const uint64_t* __restrict__ tt, const int64_t* __restrict__ ttrow, const WedgeItem* __restrict__ items,
+    int64_t nitems, int log2r, WedgeSeg* __restrict__ segs, int64_t segcap, uint32_t* __restrict__ qctr,
+    uint64_t* __restrict__ P, uint64_t* __restrict__ PS, unsigned long long* __restrict__ work) {
+    extern __shared__ uint32_t cnt[];  // 2^log2r endpoint counters
+    __shared__ uint32_t base[kTmax + 1];
+    __shared__ uint32_t cur[kTmax];
+    __shared__ int64_t sstart[NT];
+    __shared__ int32_t pref[NT];
+    __shared__ int32_t seid[NT];
+    __shared__ unsigned long long psum[NT];
+    __shared__ int32_t part[NT];
+    __shared__ int64_t item_sh;
+    const int tid = threadIdx.x;
+    const int R = 1 << log2r;
+    WedgeSeg* seg = segs + (size_t)blockIdx.x * (size_t)segcap;
+    for (;;) {
+        if (tid == 0) item_sh = (int64_t)atomicAdd(qctr, 1u);
+        __syncthreads();
+        const int64_t it = item_sh;
+        __syncthreads();
+        if (it >= nitems) break;
+        const WedgeItem W = items[it];
+        const int32_t a = heavy[W.hub];
+        const int64_t ib = row[a], ie = ostart[a];
+        const int T = W.t1 - W.t0;
+        for (int i = tid; i < T; i += NT) base[i] = 0;
+        __syncthreads();
+        // bucket the hub's segments in [t0, t1) by tile: count, scan, scatter
+        for (int walk = 0; walk < 2; walk++) {
+            for (int64_t s = ib + tid; s < ie; s += NT) {
+                const uint32_t w = (uint32_t)wcnt[s];
+                if (!w) continue;
+                const int32_t b = col[s];
+                const int64_t j1 = ttrow[b + 1];
+                int64_t j = W.t0 ? tt_seek(tt, ttrow[b], j1, (uint32_t)W.t0) : ttrow[b];
+                const uint32_t deg = (uint32_t)(row[b + 1] - row[b]);
+                const int32_t e_ab = walk ? eid[s] : 0;
+                for (; j < j1; j++) {
+                    const uint64_t e = tt[j];
+                    const uint32_t off = (uint32_t)e;
+                    const int t = (int)(e >> 32) - W.t0;
+                    if (off >= w || t >= T) break;
+                    if (walk == 0) {
+                        atomicAdd(&base[t], 1u);
+                    } else {
+                        const uint32_t nxt = j + 1 < j1 ? (uint32_t)tt[j + 1] : deg;
+                        const uint32_t p = atomicAdd(&cur[t], 1u);
+                        seg[p] = WedgeSeg{row[b] + off, (int32_t)(min(nxt, w) - off), e_ab};
+                    }
+                }
+            }
+            __syncthreads();
+            if (walk == 0 && tid == 0) {
+                uint32_t acc = 0;
+                for (int i = 0; i < T; i++) {
+                    const uint32_t x = base[i];
+                    base[i] = acc;
+                    cur[i] = acc;
+                    acc += x;
+                }
+                base[T] = acc;
+            }
+            __syncthreads();
+        }
+        // tile by tile: count the endpoints, then accumulate cnt-1 per wedge
+        for (int t = 0; t < T; t++) {
+            const uint32_t sb = base[t], se = base[t + 1];
+            if (sb == se) continue;
+            const int32_t c0 = (W.t0 + t) << log2r;
+            for (int i = tid; i < R; i += NT) cnt[i] = 0;
+            __syncthreads();
+            for (int pass = 0; pass < 2; pass++) {
+                for (uint32_t cb = sb; cb < se; cb += NT) {
+                    const int kc = (int)min<uint32_t>(NT, se - cb);
+                    if (tid < kc) {
+                        const WedgeSeg g = seg[cb + tid];
+                        sstart[tid] = g.start;
+                        pref[tid] = g.len;
+                        seid[tid] = g.eid;
+                        psum[tid] = 0;
+                    }
+                    __syncthreads();
+                    block_incl_scan<NT>(pref, kc, part);
+                    const int32_t total = pref[kc - 1];
+                    if (pass == 0 && tid == 0) atomicAdd(work, (unsigned long long)total);
+                    for (int32_t q = tid; q < total; q += NT) {
+                        const int o = owner_of(pref, kc, q);
+                        const int32_t ex = o ? pref[o - 1] : 0;
+                        const int64_t slot = sstart[o] + (q - ex);
+                        const int32_t c = col[slot] - c0;
+                        if (pass == 0) {
+                            atomicAdd(&cnt[c], 1u);
+                        } else {
+                            const uint32_t v = cnt[c] - 1;
+                            if (v) {
+                                atomicAdd((unsigned long long*)&PS[slot], (unsigned long long)v);
+                                atomicAdd(&psum[o], (unsigned long long)v);
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    if (pass == 1 && tid < kc && psum[tid])
+                        atomicAdd((unsigned long long*)&P[seid[tid]], psum[tid]);
+                    __syncthreads();
+                }
+            }
+        }
+    }
+}
+
+// fold the per-slot accumulator into P by edge id
+__global__ void k_ps_fold(const uint64_t* __restrict__ PS, const int32_t* __restrict__ eid, int64_t nnz,
+                          uint64_t* __restrict__ P) {
+    GRID_STRIDE(s, nnz) {
+        const uint64_t v = PS[s];
+        if (v) atomicAdd((unsigned long long*)&P[eid[s]], (unsigned long long)v);
+    }
+}
+
+template <int NT>
+void launch_tiled(pgd_ctx* ctx, OrientState& st, unsigned long long* work) {
+    WedgePlan& wp = st.wp;
+    const size_t dyn = (size_t)4 << wp.log2r;
+    hipLaunchKernelGGL(k_wedge_tiled<NT>, dim3(wp.grid), dim3(NT), dyn, ctx->stream, st.rrow.get(), st.rcol.get(),
+                       st.reid.get(), st.ostart.get(), st.b_heavy.get(), wp.wcnt.get(), wp.tt.get(), wp.ttrow.get(),
+                       wp.items.get(), wp.nitems, wp.log2r, wp.segs.get(), wp.segcap, wp.qctr.get(), st.P.get(),
+                       wp.ps.get(), work);
+}
+
+template <int NT>
+int tiled_blocks_per_cu(pgd_ctx* ctx, int log2r) {
+    const size_t dyn = (size_t)4 << log2r;
+    PGD_HIP(hipFuncSetAttribute((const void*)k_wedge_tiled<NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)dyn));
+    int nb = 0;
+    PGD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_wedge_tiled<NT>, NT, dyn));
+    return nb;
+}
+
+}  // namespace
+
+// plan of the heavy 4-cycle pass: pair wedge counts, tile table, per-cell
+// sizes, items sorted largest first, per-block buffers (host synchronisations)
+void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
+    hipStream_t s = ctx->stream;
+    WedgePlan& wp = st.wp;
+    const OrientParams& prm = st.prm;
+    const int64_t n = st.n, nnz = 2 * st.m, nh = st.n_heavy;
+    PGD_CHECK(prm.wedge_log2r >= 8 && prm.wedge_log2r <= 15, PGD_ERR_INVALID, "wedge tile must be 2^8..2^15");
+    PGD_CHECK(prm.wedge_nt == 256 || prm.wedge_nt == 512 || prm.wedge_nt == 1024, PGD_ERR_INVALID,
+              "wedge block size must be 256, 512 or 1024");
+    wp.log2r = prm.wedge_log2r;
+    // pair wedge counts
+    ensure(wp.wcnt, ctx, nnz);
+    hipLaunchKernelGGL(k_pair_wedges, dim3(grid_cap(ctx, (nh * 64 + 255) / 256, 16)), dim3(256), 0, s, st.rrow.get(),
+                       st.rcol.get(), st.ostart.get(), st.b_heavy.get(), nh, wp.wcnt.get());
+    PGD_HIP(hipGetLastError());
+    // tile table (st.v0 and st.k0 are free scratch of 2m entries after the relabel)
+    {
+        int32_t* flag = st.v0.get();
+        int64_t* pos = (int64_t*)st.k0.get();
+        const int gw = grid_cap(ctx, (n * 64 + 255) / 256, 16);
+        hipLaunchKernelGGL(k_tt_flags, dim3(gw), dim3(256), 0, s, st.rrow.get(), st.rcol.get(), n, wp.log2r, flag);
+        PGD_HIP(hipGetLastError());
+        excl_sum(ctx, flag, pos, (size_t)nnz);
+        int64_t last_pos = 0;
+        int32_t last_flag = 0;
+        PGD_HIP(hipMemcpyAsync(&last_pos, pos + (nnz - 1), sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        PGD_HIP(hipMemcpyAsync(&last_flag, flag + (nnz - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        PGD_HIP(hipStreamSynchronize(s));
+        const int64_t total = last_pos + last_flag;
+        ensure(wp.tt, ctx, total);
+        ensure(wp.ttrow, ctx, n + 1);
+        hipLaunchKernelGGL(k_tt_fill, dim3(gw), dim3(256), 0, s, st.rrow.get(), st.rcol.get(), n, nnz, wp.log2r,
+                           flag, pos, total, wp.tt.get(), wp.ttrow.get());
+        PGD_HIP(hipGetLastError());
+    }
+    // (hub, tile) cells
+    ensure(wp.htoff, ctx, nh + 1);
+    int64_t ncell = 0;
+    {
+        DBuf<int64_t> nt;
+        nt.alloc(ctx, nh + 1);
+        hipLaunchKernelGGL(k_hub_tiles, dim3(grid_cap(ctx, (nh + 255) / 256, 16)), dim3(256), 0, s,
+                           st.b_heavy.get(), nh, wp.log2r, nt.get());
+        hipLaunchKernelGGL(w_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)nt.get() + nh, (int64_t)1);
+        PGD_HIP(hipGetLastError());
+        excl_sum(ctx, nt.get(), wp.htoff.get(), (size_t)(nh + 1));
+        PGD_HIP(hipMemcpyAsync(&ncell, wp.htoff.get() + nh, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        PGD_HIP(hipStreamSynchronize(s));
+    }
+    ensure(wp.htseg, ctx, ncell);
+    ensure(wp.htw, ctx, ncell);
+    {
+        const int g = grid_cap(ctx, (ncell + 255) / 256, 16);
+        hipLaunchKernelGGL(w_fill_u32, dim3(g), dim3(256), 0, s, wp.htseg.get(), ncell);
+        hipLaunchKernelGGL(w_fill_u64, dim3(g), dim3(256), 0, s, wp.htw.get(), ncell);
+        hipLaunchKernelGGL(k_plan_cells<256>, dim3(grid_cap(ctx, nh, 8)), dim3(256), 0, s, st.rrow.get(),
+                           st.rcol.get(), st.ostart.get(), st.b_heavy.get(), nh, wp.wcnt.get(), wp.tt.get(),
+                           wp.ttrow.get(), wp.htoff.get(), wp.htseg.get(), wp.htw.get());
+        PGD_HIP(hipGetLastError());
+    }
+    const uint32_t maxcell = device_max(ctx, wp.htseg.get(), (size_t)ncell);
+    const uint64_t segtot = device_sum<uint64_t>(ctx, wp.htseg.get(), (size_t)ncell);
+    wp.wedges = (int64_t)device_sum<uint64_t>(ctx, wp.htw.get(), (size_t)ncell);
+    // grid from the kernel's occupancy at this tile size
+    int per_cu = 0;
+    if (prm.wedge_nt == 256) per_cu = tiled_blocks_per_cu<256>(ctx, wp.log2r);
+    else if (prm.wedge_nt == 512) per_cu = tiled_blocks_per_cu<512>(ctx, wp.log2r);
+    else per_cu = tiled_blocks_per_cu<1024>(ctx, wp.log2r);
+    PGD_CHECK(per_cu > 0, PGD_ERR_INVALID, "wedge tile and block size exceed the LDS of a CU");
+    wp.grid = ctx->num_cus * per_cu;
+    wp.segcap = (int64_t)std::max<uint64_t>(maxcell, std::min<uint64_t>(segtot, (uint64_t)1 << 18));
+    wp.segcap = std::max<int64_t>(wp.segcap, 1);
+    const uint64_t wcap =
+        std::max<uint64_t>((uint64_t)wp.wedges / ((uint64_t)wp.grid * std::max(1, prm.wedge_split)), 1 << 16);
+    // items: count, offsets, write, sort by size
+    {
+        DBuf<int64_t> per, ioff;
+        per.alloc(ctx, nh + 1);
+        ioff.alloc(ctx, nh + 1);
+        const int g = grid_cap(ctx, (nh + 255) / 256, 16);
+        hipLaunchKernelGGL(w_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)per.get() + nh, (int64_t)1);
+        hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, s, wp.htoff.get(), wp.htseg.get(), wp.htw.get(), nh,
+                           wp.segcap, wcap, per.get(), (const int64_t*)nullptr, (WedgeItem*)nullptr,
+                           (uint64_t*)nullptr);
+        PGD_HIP(hipGetLastError());
+        excl_sum(ctx, per.get(), ioff.get(), (size_t)(nh + 1));
+        PGD_HIP(hipMemcpyAsync(&wp.nitems, ioff.get() + nh, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        PGD_HIP(hipStreamSynchronize(s));
+        const int64_t ni = wp.nitems;
+        DBuf<WedgeItem> raw;
+        DBuf<uint64_t> key, k2, v0, v1;
+        raw.alloc(ctx, ni);
+        key.alloc(ctx, ni);
+        k2.alloc(ctx, ni);
+        v0.alloc(ctx, ni);
+        v1.alloc(ctx, ni);
+        hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, s, wp.htoff.get(), wp.htseg.get(), wp.htw.get(), nh,
+                           wp.segcap, wcap, per.get(), ioff.get(), raw.get(), key.get());
+        PGD_HIP(hipGetLastError());
+        ensure(wp.items, ctx, ni);
+        if (ni) {
+            const int gi = grid_cap(ctx, (ni + 255) / 256, 16);
+            hipLaunchKernelGGL(k_item_keys, dim3(gi), dim3(256), 0, s, key.get(), ni, k2.get(), v0.get());
+            sort_kv(ctx, k2.get(), key.get(), v0.get(), v1.get(), (size_t)ni, 64u);
+            hipLaunchKernelGGL(k_gather_items, dim3(gi), dim3(256), 0, s, v1.get(), raw.get(), ni, wp.items.get());
+            PGD_HIP(hipGetLastError());
+        }
+        PGD_HIP(hipStreamSynchronize(s));
+    }
+    ensure(wp.segs, ctx, (size_t)wp.grid * (size_t)wp.segcap);
+    ensure(wp.ps, ctx, nnz);
+    ensure(wp.qctr, ctx, 1);
+}
+
+// the heavy 4-cycle pass (capturable: fills, the persistent kernel, the fold)
+void wedge_plan_count(pgd_ctx* ctx, OrientState& st, unsigned long long* work) {
+    hipStream_t s = ctx->stream;
+    WedgePlan& wp = st.wp;
+    const int64_t nnz = 2 * st.m;
+    const int g = grid_cap(ctx, (nnz + 255) / 256, 16);
+    hipLaunchKernelGGL(w_fill_u64, dim3(g), dim3(256), 0, s, wp.ps.get(), nnz);
+    hipLaunchKernelGGL(w_fill_u32, dim3(1), dim3(64), 0, s, wp.qctr.get(), (int64_t)1);
+    PGD_HIP(hipGetLastError());
+    if (wp.nitems) {
+        if (st.prm.wedge_nt == 256) launch_tiled<256>(ctx, st, work);
+        else if (st.prm.wedge_nt == 512) launch_tiled<512>(ctx, st, work);
+        else launch_tiled<1024>(ctx, st, work);
+        PGD_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_ps_fold, dim3(g), dim3(256), 0, s, wp.ps.get(), st.reid.get(), nnz, st.P.get());
+    PGD_HIP(hipGetLastError());
+}
+
+}  // namespace pgd

@@ -119,14 +119,33 @@ struct OrientParams {
     int mid_k = 256;        // up to which a 256-thread block does (1024 threads above)
     int grid_small = 32;    // resident blocks per CU for the wavefront bins
     int grid_mid = 8;       // resident blocks per CU for the 256-thread bin
-    int heavy_per_cu = 2;   // heavy-wedge blocks (each with a dense counter slice) per CU
+    int wedge_log2r = 15;   // endpoint tile of the heavy 4-cycle pass: 2^log2r LDS counters
+    int wedge_nt = 512;     // threads per block of the heavy 4-cycle pass (256, 512 or 1024)
+    int wedge_split = 4;    // work items per block the heavy wedges are cut into (load balance)
 };
 OrientState* orient_state_new();
 OrientParams& orient_params(OrientState* s);
 void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st);
 void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer);
 void engine_edge(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t);
-void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t, OrientState** cache);
+void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t, OrientState** cache,
+                   const OrientParams& prm);
+inline OrientParams params_from(const pgd_tuning& t) {
+    OrientParams p;
+    p.small_k = t.small_k;
+    p.mid_k = t.mid_k;
+    p.grid_small = t.grid_small;
+    p.grid_mid = t.grid_mid;
+    p.wedge_log2r = t.wedge_log2r;
+    p.wedge_nt = t.wedge_nt;
+    p.wedge_split = t.wedge_split;
+    return p;
+}
+inline pgd_tuning tuning_from(const OrientParams& p) {
+    return pgd_tuning{p.small_k, p.mid_k, p.grid_small, p.grid_mid, p.wedge_log2r, p.wedge_nt, p.wedge_split};
+}
+// empty string when t is usable, else what is wrong
+const char* tuning_problem(const pgd_tuning& t);
 void orient_state_free(pgd_ctx* ctx, OrientState* s);
 bool engine_dense_ok(const GraphView& g);
 void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t);
@@ -161,6 +180,7 @@ struct pgd_graph {
     int last_engine = 0;
     pgd::Timing timing;
     pgd::OrientState* orient = nullptr;
+    pgd::OrientParams prm;
     pgd::GraphView view() const {
         return pgd::GraphView{n, m, nnz, max_deg, row.get(), col.get(), src.get(), deg.get(),
                               ev.get(), eu.get(), order.get()};

@@ -1,0 +1,239 @@
+// pgd_orient.hpp -- shared pieces of the degree-oriented engine: launch and
+// rocPRIM helpers, block-level device helpers, and the engine's persistent
+// per-graph state (relabelled CSR, vertex bins, the tiled wedge plan).
+#pragma once
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "pgd_internal.hpp"
+
+namespace pgd {
+
+#define GRID_STRIDE(i, n) \
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+inline int grid_cap(pgd_ctx* ctx, int64_t items, int per_cu) {
+    int64_t g = std::min<int64_t>(items, (int64_t)ctx->num_cus * per_cu);
+    return (int)std::max<int64_t>(g, 1);
+}
+
+template <class T, class U>
+void excl_sum(pgd_ctx* ctx, const T* in, U* out, size_t n) {
+    if (!n) return;
+    size_t bytes = 0;
+    PGD_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, U(0), n, rocprim::plus<U>(), ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::exclusive_scan(t, bytes, in, out, U(0), n, rocprim::plus<U>(), ctx->stream));
+}
+
+template <class K, class V>
+void sort_kv(pgd_ctx* ctx, K* kin, K* kout, V* vin, V* vout, size_t n, unsigned b1) {
+    if (!n) return;
+    size_t bytes = 0;
+    PGD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, b1, ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, 0u, b1, ctx->stream));
+}
+
+template <class K>
+void sort_k(pgd_ctx* ctx, K* kin, K* kout, size_t n, unsigned b1) {
+    if (!n) return;
+    size_t bytes = 0;
+    PGD_HIP(rocprim::radix_sort_keys(nullptr, bytes, kin, kout, n, 0u, b1, ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::radix_sort_keys(t, bytes, kin, kout, n, 0u, b1, ctx->stream));
+}
+
+// maximum of in[0..n) read back to the host (synchronises the stream)
+template <class T>
+T device_max(pgd_ctx* ctx, const T* in, size_t n) {
+    if (!n) return T(0);
+    DBuf<T> mx;
+    mx.alloc(ctx, 1);
+    size_t bytes = 0;
+    PGD_HIP(rocprim::reduce(nullptr, bytes, in, mx.get(), T(0), n, rocprim::maximum<T>(), ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::reduce(t, bytes, in, mx.get(), T(0), n, rocprim::maximum<T>(), ctx->stream));
+    T h = 0;
+    PGD_HIP(hipMemcpyAsync(&h, mx.get(), sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    PGD_HIP(hipStreamSynchronize(ctx->stream));
+    return h;
+}
+
+// sum of in[0..n) (as type S) read back to the host (synchronises the stream)
+template <class S, class T>
+S device_sum(pgd_ctx* ctx, const T* in, size_t n) {
+    if (!n) return S(0);
+    DBuf<S> r;
+    r.alloc(ctx, 1);
+    size_t bytes = 0;
+    PGD_HIP(rocprim::reduce(nullptr, bytes, in, r.get(), S(0), n, rocprim::plus<S>(), ctx->stream));
+    void* t = tmp_storage(ctx, bytes);
+    PGD_HIP(rocprim::reduce(t, bytes, in, r.get(), S(0), n, rocprim::plus<S>(), ctx->stream));
+    S h = 0;
+    PGD_HIP(hipMemcpyAsync(&h, r.get(), sizeof(S), hipMemcpyDeviceToHost, ctx->stream));
+    PGD_HIP(hipStreamSynchronize(ctx->stream));
+    return h;
+}
+
+inline int bitsfor(int64_t v) {
+    int b = 1;
+    while (b < 63 && ((int64_t)1 << b) <= v) b++;
+    return b;
+}
+
+// capacity-reusing allocation: keeps the buffer when it is already large enough
+template <class T>
+void ensure(DBuf<T>& b, pgd_ctx* ctx, size_t n) {
+    if (n == 0) n = 1;
+    if (b.get() == nullptr || b.n < n) b.alloc(ctx, n);
+}
+
+// ---------------------------------------------------------------------------
+// block-level device helpers (blockDim.x == NT)
+// ---------------------------------------------------------------------------
+// inclusive scan of s[0..k) in place, NT threads, k arbitrary
+template <int NT, class T>
+__device__ void block_incl_scan(T* s, int k, T* part) {
+    const int tid = threadIdx.x;
+    const int per = (k + NT - 1) / NT;
+    const int b = tid * per, e = min(b + per, k);
+    T acc = 0;
+    for (int i = b; i < e; i++) { acc += s[i]; s[i] = acc; }
+    part[tid] = acc;
+    __syncthreads();
+    for (int off = 1; off < NT; off <<= 1) {
+        T y = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += y;
+        __syncthreads();
+    }
+    T base = tid ? part[tid - 1] : 0;
+    for (int i = b; i < e; i++) s[i] += base;
+    __syncthreads();
+}
+
+// first index i in [0,k) with incl[i] > t (incl nondecreasing, incl[k-1] > t)
+template <class T>
+__device__ __forceinline__ int owner_of(const T* incl, int k, T t) {
+    int lo = 0, hi = k - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (incl[mid] > t) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// index of c in L[lo..hi) (ascending) or -1
+__device__ __forceinline__ int find_in(const int32_t* L, int lo, int hi, int32_t c) {
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        int32_t x = L[mid];
+        if (x < c) lo = mid + 1;
+        else if (x > c) hi = mid;
+        else return mid;
+    }
+    return -1;
+}
+
+// first position in col[lo..hi) (ascending) holding a value >= x
+__device__ __forceinline__ int64_t lower_bound_g(const int32_t* __restrict__ col, int64_t lo, int64_t hi, int32_t x) {
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (col[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// per-pass HIP event timer on the context's stream
+// ---------------------------------------------------------------------------
+struct Timer {
+    pgd_ctx* ctx;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<const char*> names;
+    void begin(const char* name) {
+        hipEvent_t a, b;
+        PGD_HIP(hipEventCreate(&a));
+        PGD_HIP(hipEventCreate(&b));
+        PGD_HIP(hipEventRecord(a, ctx->stream));
+        ev.push_back({a, b});
+        names.push_back(name);
+    }
+    void end() { PGD_HIP(hipEventRecord(ev.back().second, ctx->stream)); }
+    ~Timer() {
+        for (auto& p : ev) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// tiled wedge plan (heavy vertices of the 4-cycle pass)
+// ---------------------------------------------------------------------------
+// One segment: the run of N(b) (ascending CSR slots [start, start+len)) whose
+// endpoints c fall in one endpoint tile, for the in-neighbour b of a hub a;
+// eid is the edge id of (a, b).
+struct WedgeSeg {
+    int64_t start;
+    int32_t len;
+    int32_t eid;
+};
+
+// one unit of work of the tiled kernel: hub (index into the heavy list) and
+// its endpoint tiles [t0, t1)
+struct WedgeItem {
+    int32_t hub, t0, t1, pad;
+};
+
+struct WedgePlan {
+    int log2r = 15;                 // endpoint tile = 2^log2r consecutive ranks (LDS counters)
+    int tmax = 128;                 // tiles per item
+    int64_t nitems = 0, segcap = 0, wedges = 0;
+    int grid = 0;
+    DBuf<int32_t> wcnt;             // per in-slot: |N(b) below a| for the pair (a,b)
+    DBuf<uint64_t> tt;              // tile table entries: (tile << 32) | offset in N(b)
+    DBuf<int64_t> ttrow;            // first tile-table entry of each vertex (n+1)
+    DBuf<int64_t> htoff;            // first (hub, tile) cell of each heavy hub (+1)
+    DBuf<uint32_t> htseg;           // segments per (hub, tile) cell
+    DBuf<uint64_t> htw;             // wedges per (hub, tile) cell
+    DBuf<WedgeItem> items;          // work queue, largest first
+    DBuf<WedgeSeg> segs;            // per-block segment buffers, segcap each
+    DBuf<uint64_t> ps;              // per-slot P accumulator (ascending CSR slots)
+    DBuf<uint32_t> qctr;            // queue head
+};
+
+// Persistent state of the oriented engine for one graph: the relabelled CSR,
+// the vertex bins, scratch and accumulators.  Buffers are kept across calls
+// (their contents are rebuilt every call), so the counting launches can be
+// captured once in a HIP graph and replayed.
+struct OrientState {
+    OrientParams prm;
+    int64_t n = 0, m = 0;
+    DBuf<int32_t> rank, rdeg, rcol, reid, outdeg, v0, flags;
+    DBuf<int64_t> rrow, ostart, off, ub;
+    DBuf<uint64_t> k0, k1;
+    DBuf<int32_t> b_small, b_mid, b_large, b_huge, b_light, b_heavy;
+    int64_t n_small = 0, n_mid = 0, n_large = 0, n_huge = 0, n_light = 0, n_heavy = 0;
+    int kmax_lds = 0, maxk = 0, ghuge = 0;
+    size_t small_lds = 0, mid_lds = 0, large_lds = 0;
+    DBuf<char> hscratch;
+    DBuf<uint64_t> P, Q;
+    DBuf<unsigned long long> work;
+    WedgePlan wp;
+};
+
+// heavy-vertex 4-cycle pass (pgd_engine_wedge.hip)
+void wedge_plan_build(pgd_ctx* ctx, OrientState& st);
+void wedge_plan_count(pgd_ctx* ctx, OrientState& st, unsigned long long* work);
+
+}  // namespace pgd

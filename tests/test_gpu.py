@@ -131,6 +131,51 @@ def test_deterministic_repeat(engine, ctx):
     g.free()
 
 
+TUNINGS = [
+    dict(wedge_log2r=8),                                     # many endpoint tiles, items cut at 128 tiles
+    dict(wedge_log2r=10, wedge_nt=256, wedge_split=64),      # small items
+    dict(wedge_log2r=14, wedge_nt=1024, small_k=32, mid_k=128),
+    dict(wedge_log2r=12, wedge_nt=512, grid_small=4, grid_mid=1, wedge_split=1),
+]
+
+
+@pytest.mark.parametrize("ti", range(len(TUNINGS)))
+def test_tuning_never_changes_counts(ti, ctx):
+    """Launch parameters change the schedule, never a count: per-edge T/K4/C4
+    under each tuning equal the default's (and the oracle's on a sample)."""
+    tun = TUNINGS[ti]
+    say(f"[gpu] tuning {tun}")
+    s, d = pgd_amd.gen_rmat(14, 16, seed=5)
+    n = int(max(s.max(), d.max())) + 1
+    g = pgd_amd.GraphletCore(s, d, n=n, ctx=ctx)
+    g.sort_edges("deg")
+    base_sums = g.decompose(engine="orient")
+    base = g.edge_counts()
+    g.set_tuning(**tun)
+    assert g.tuning().as_dict() == {**pgd_amd.default_tuning().as_dict(), **tun}
+    sums = g.decompose(engine="orient")
+    got = g.edge_counts()
+    assert sums == base_sums
+    for a, b in zip(base, got):
+        assert np.array_equal(a, b)
+    og = OracleGraph.from_edges(s, d, n)
+    og.order_neighbors("deg")
+    ev, eu = og.edge_arrays()
+    sample = np.sort(np.random.default_rng(ti).choice(len(ev), size=1500, replace=False)).astype(np.int32)
+    res = og.decompose(ev, eu, sample, micro=True)
+    assert np.array_equal(got[2][sample], res["c4"])
+    g.free()
+
+
+def test_tuning_rejects_bad_values(ctx):
+    s, d = pgd_amd.gen_gnp(100, 0.1, 1)
+    g = pgd_amd.GraphletCore(s, d, ctx=ctx)
+    for bad in (dict(wedge_log2r=16), dict(wedge_nt=128), dict(small_k=65), dict(mid_k=512), dict(wedge_split=0)):
+        with pytest.raises(pgd_amd.PGDError):
+            g.set_tuning(**bad)
+    g.free()
+
+
 def test_engines_agree_per_edge(ctx):
     if len(ENGINES) < 2:
         pytest.skip("one engine")
