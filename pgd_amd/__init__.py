@@ -336,3 +336,92 @@ class GraphletCore:
         if rows is None:
             return head
         return head + "\n".join(",".join(str(int(x)) for x in r) for r in rows) + "\n"
+
+
+# ---------------------------------------------------------------------------
+# batched form
+# ---------------------------------------------------------------------------
+class Batch:
+    """Many independent graphs counted together (pgd_batch_*).
+
+    graphs: sequence of (n, src, dst) with ids local to each graph.  Each
+    graph gets the plain path's preprocessing (deg/deg defaults) and E_ordered;
+    run() replays one captured launch sequence for all of them."""
+
+    def __init__(self, graphs, ctx: Context | None = None, micro: bool = True, dedup: bool = False):
+        self.ctx = ctx or default_context()
+        ng = len(graphs)
+        n = np.ascontiguousarray([g[0] for g in graphs], dtype=np.int32)
+        lens = np.array([len(g[1]) for g in graphs], dtype=np.int64)
+        line_off = np.zeros(ng + 1, dtype=np.int64)
+        np.cumsum(lens, out=line_off[1:])
+        L = int(line_off[-1])
+        src = np.empty(max(L, 1), dtype=np.int32)
+        dst = np.empty(max(L, 1), dtype=np.int32)
+        for i, g in enumerate(graphs):
+            src[line_off[i]:line_off[i + 1]] = g[1]
+            dst[line_off[i]:line_off[i + 1]] = g[2]
+        h = C.c_void_p()
+        flags = (MICRO if micro else 0) | (DEDUP if dedup else 0)
+        check(lib().pgd_batch_create(self.ctx.h, ng, ptr(n, C.c_int32), ptr(line_off, C.c_int64),
+                                     ptr(src, C.c_int32), ptr(dst, C.c_int32), flags, C.byref(h)),
+              self.ctx.h, "pgd_batch_create")
+        self.h = h
+        self.ctx._graphs.add(self)
+        self.micro_enabled = micro
+        self.ngraphs, self.n_total, self.m_total = self.info()
+        self.n = n.astype(np.int64)
+        self.m = None
+
+    def info(self):
+        g, nv, ne = C.c_int32(), C.c_int64(), C.c_int64()
+        check(lib().pgd_batch_info(self.h, C.byref(g), C.byref(nv), C.byref(ne)), self.ctx.h)
+        return g.value, nv.value, ne.value
+
+    def run(self):
+        """Replay the counting; returns (sums[ng,15], totals[ng,17]) as uint64."""
+        s = np.zeros((self.ngraphs, NSUMS), dtype=np.uint64)
+        t = np.zeros((self.ngraphs, NTOTALS), dtype=np.uint64)
+        check(lib().pgd_batch_run(self.h, ptr(s, C.c_uint64), ptr(t, C.c_uint64)), self.ctx.h, "pgd_batch_run")
+        return s, t
+
+    def last_ms(self) -> float:
+        v = C.c_double()
+        check(lib().pgd_batch_last_ms(self.h, C.byref(v)), self.ctx.h)
+        return v.value
+
+    def micro(self, gi: int, column: int | str, m: int) -> np.ndarray:
+        """Micro column of graph gi (m = its edge count), E_ordered order."""
+        if isinstance(column, str):
+            column = MICRO_COLUMNS.index(column)
+        out = np.zeros(max(m, 1), dtype=np.uint64)
+        check(lib().pgd_batch_micro(self.h, gi, column, ptr(out, C.c_uint64)), self.ctx.h, "pgd_batch_micro")
+        return out[:m]
+
+    def set_tuning(self, tuning: Tuning | None = None, **overrides) -> None:
+        t = tuning if tuning is not None else self.tuning()
+        for k, v in overrides.items():
+            setattr(t, k, int(v))
+        check(lib().pgd_batch_set_tuning(self.h, C.byref(t)), self.ctx.h, "pgd_batch_set_tuning")
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        check(lib().pgd_batch_get_tuning(self.h, C.byref(t)), self.ctx.h)
+        return t
+
+    def autotune(self, trials: int = 3):
+        """Built-in coordinate search; returns (best Tuning, its median ms)."""
+        t, ms = Tuning(), C.c_double()
+        check(lib().pgd_batch_autotune(self.h, trials, C.byref(t), C.byref(ms)), self.ctx.h, "pgd_batch_autotune")
+        return t, ms.value
+
+    def free(self) -> None:
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
+            lib().pgd_batch_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

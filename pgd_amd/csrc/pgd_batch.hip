@@ -1,20 +1,460 @@
-// pgd_batch.hip -- batched and served forms (placeholder until written).
-#include "pgd_internal.hpp"
+// pgd_batch.hip -- batched form: thousands of independent graphs counted
+// together.
+//
+// A batch is the disjoint union of its graphs: graph g's vertices are the
+// union ids [voff_g, voff_g + n_g).  Everything the plain path computes per
+// graph (CSR, neighbour order, e_v/e_u, edge ids, E_ordered, per-edge counts)
+// is a per-component property, so running the plain pipeline on the union
+// gives every graph's own result: its edge ids are the union ids
+// [eoff_g, eoff_g + m_g) in the same order, and its E_ordered is the union
+// E_ordered restricted to its edges (stable partition by graph).  Two things
+// are per graph: the ranks of the oriented engine are grouped by graph (so a
+// graph's vertices form one rank range and its wedge endpoints one or two
+// LDS tiles), and the epilogue, which needs each graph's n and m, is a
+// persistent kernel pulling (graph, edge-range) tiles from a device queue.
+//
+// The counting sequence (the oriented engine's passes, the segmented
+// epilogue) has no host synchronisation, allocation or memset, and is
+// captured once in a HIP graph that pgd_batch_run replays.  Launch
+// parameters come from pgd_batch_set_tuning or the built-in autotuner.
+#include <chrono>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pgd_epilogue.hpp"
+#include "pgd_orient.hpp"
+
+using namespace pgd;
+
+namespace {
+
+constexpr int kEpiTile = 4096;  // E_ordered positions per epilogue tile
+
+struct EpiItem {
+    int32_t graph, pad;
+    int64_t p0, p1;
+};
+
+__global__ void b_fill_u64(uint64_t* __restrict__ p, int64_t n) {
+    GRID_STRIDE(i, n) p[i] = 0;
+}
+__global__ void b_fill_u32(uint32_t* __restrict__ p, int64_t n) {
+    GRID_STRIDE(i, n) p[i] = 0;
+}
+
+__global__ void k_vertex_group(const int64_t* __restrict__ voff, int32_t ng, int32_t* __restrict__ group) {
+    for (int64_t g = blockIdx.x; g < ng; g += gridDim.x)
+        for (int64_t v = voff[g] + threadIdx.x; v < voff[g + 1]; v += blockDim.x) group[v] = (int32_t)g;
+}
+
+// row offsets at the graph boundaries (edge offsets = row / 2)
+__global__ void k_graph_rows(const int64_t* __restrict__ row, const int64_t* __restrict__ voff, int32_t ng,
+                             int64_t* __restrict__ out) {
+    GRID_STRIDE(g, (int64_t)ng + 1) out[g] = row[voff[g]];
+}
+
+// key of E_ordered position p: the graph of its edge
+__global__ void k_order_group(const int32_t* __restrict__ order, const int32_t* __restrict__ ev,
+                              const int32_t* __restrict__ group, int64_t m, uint32_t* __restrict__ key) {
+    GRID_STRIDE(p, m) key[p] = (uint32_t)group[ev[order[p]]];
+}
+
+// segmented epilogue: per-graph sums and micro columns, tiles from a queue
+__global__ __launch_bounds__(256) void k_epilogue_batch(
+    const EpiItem* __restrict__ items, int64_t nitems, uint32_t* __restrict__ qctr, const int32_t* __restrict__ order,
+    const int32_t* __restrict__ ev, const int32_t* __restrict__ eu, const int64_t* __restrict__ row,
+    const uint32_t* __restrict__ Tc, const uint64_t* __restrict__ K4c, const uint64_t* __restrict__ C4c,
+    const int64_t* __restrict__ gn, const int64_t* __restrict__ gm, uint64_t* __restrict__ sums,
+    uint64_t* __restrict__ micro, int64_t mstride, uint32_t* __restrict__ inexact_out) {
+    __shared__ int64_t it_sh;
+    uint32_t inexact = 0;
+    for (;;) {
+        if (threadIdx.x == 0) it_sh = (int64_t)atomicAdd(qctr, 1u);
+        __syncthreads();
+        const int64_t it = it_sh;
+        __syncthreads();
+        if (it >= nitems) break;
+        const EpiItem I = items[it];
+        const int64_t n = gn[I.graph], m = gm[I.graph];
+        uint64_t acc[PGD_NSUMS];
+#pragma unroll
+        for (int k = 0; k < PGD_NSUMS; k++) acc[k] = 0;
+        for (int64_t p = I.p0 + threadIdx.x; p < I.p1; p += blockDim.x) {
+            const int32_t id = order[p];
+            const int32_t v = ev[id], u = eu[id];
+            inexact += edge_terms(row[v + 1] - row[v], row[u + 1] - row[u], Tc[id], K4c[id], C4c[id], n, m, acc,
+                                  micro, mstride, p);
+        }
+        block_add_sums(acc, sums + (size_t)I.graph * PGD_NSUMS);
+    }
+    if (inexact) atomicAdd(inexact_out, inexact);
+}
+
+}  // namespace
+
+struct pgd_batch {
+    pgd_ctx* ctx = nullptr;
+    int32_t ng = 0;
+    uint32_t flags = 0;
+    int64_t N = 0, M = 0;
+    std::vector<int64_t> n, voff, m, eoff;
+    pgd_graph* U = nullptr;
+    DBuf<int32_t> group;
+    DBuf<int64_t> gn, gm;
+    DBuf<EpiItem> items;
+    int64_t nitems = 0;
+    DBuf<uint64_t> sums;
+    DBuf<uint32_t> ctl;  // [0] inexact micro values, [1] epilogue queue head
+    OrientState* st = nullptr;
+    OrientParams prm;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double last_ms = 0;
+    std::vector<uint64_t> host_sums;
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~pgd_batch() {
+        drop_graph();
+        if (st) orient_state_free(ctx, st);
+        if (U) pgd_graph_free(U);
+    }
+};
+
+namespace {
+
+template <class F>
+pgd_status batch_guard(pgd_ctx* ctx, F&& f) {
+    try {
+        f();
+        return PGD_OK;
+    } catch (const Error& e) {
+        ctx->last_err = e.what;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        ctx->last_err = "host allocation failed";
+        return PGD_ERR_OOM;
+    }
+}
+
+void check_status(pgd_ctx* ctx, pgd_status s, const char* what) {
+    if (s != PGD_OK) throw Error{s, std::string(what) + ": " + ctx->last_err};
+}
+
+GraphView union_view(pgd_batch* b) {
+    GraphView v = b->U->view();
+    v.group = b->group.get();
+    v.ngroups = b->ng;
+    return v;
+}
+
+// the capturable counting sequence
+void batch_count(pgd_batch* b) {
+    pgd_ctx* ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    pgd_graph* U = b->U;
+    if (b->M) orient_count(ctx, *b->st, EdgeCounts{U->T.get(), U->K4.get(), U->C4.get()}, nullptr);
+    const int64_t ns = (int64_t)b->ng * PGD_NSUMS;
+    hipLaunchKernelGGL(b_fill_u64, dim3(grid_cap(ctx, (ns + 255) / 256, 4)), dim3(256), 0, s, b->sums.get(), ns);
+    hipLaunchKernelGGL(b_fill_u32, dim3(1), dim3(64), 0, s, b->ctl.get(), (int64_t)2);
+    PGD_HIP(hipGetLastError());
+    if (b->nitems) {
+        hipLaunchKernelGGL(k_epilogue_batch, dim3(grid_cap(ctx, b->nitems, 8)), dim3(256), 0, s, b->items.get(),
+                           b->nitems, b->ctl.get() + 1, U->order.get(), U->ev.get(), U->eu.get(), U->row.get(),
+                           U->T.get(), U->K4.get(), U->C4.get(), b->gn.get(), b->gm.get(), b->sums.get(),
+                           (b->flags & PGD_MICRO) ? U->micro.get() : nullptr, b->M, b->ctl.get());
+        PGD_HIP(hipGetLastError());
+    }
+}
+
+// plan the engine for the current parameters and capture the sequence
+void batch_plan(pgd_batch* b) {
+    pgd_ctx* ctx = b->ctx;
+    b->drop_graph();
+    if (b->M) {
+        if (!b->st) b->st = orient_state_new();
+        orient_params(b->st) = b->prm;
+        orient_build(ctx, union_view(b), *b->st);
+    }
+    PGD_HIP(hipStreamSynchronize(ctx->stream));
+    PGD_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        batch_count(b);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    PGD_HIP(hipStreamEndCapture(ctx->stream, &b->graph));
+    PGD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+}
+
+// one replay; sums to host; returns device ms
+double batch_replay(pgd_batch* b) {
+    pgd_ctx* ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    if (!b->exec) batch_plan(b);
+    hipEvent_t e0, e1;
+    PGD_HIP(hipEventCreate(&e0));
+    PGD_HIP(hipEventCreate(&e1));
+    PGD_HIP(hipEventRecord(e0, s));
+    PGD_HIP(hipGraphLaunch(b->exec, s));
+    PGD_HIP(hipEventRecord(e1, s));
+    b->host_sums.resize((size_t)b->ng * PGD_NSUMS);
+    uint32_t ctl[2] = {0, 0};
+    PGD_HIP(hipMemcpyAsync(b->host_sums.data(), b->sums.get(), b->host_sums.size() * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, s));
+    PGD_HIP(hipMemcpyAsync(ctl, b->ctl.get(), sizeof(ctl), hipMemcpyDeviceToHost, s));
+    PGD_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    PGD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->last_ms = ms;
+    PGD_CHECK(!(b->flags & PGD_MICRO) || ctl[0] == 0, PGD_ERR_UNSUPPORTED,
+              "a per-edge micro value exceeds 2^53; the exact double emulation for it is not implemented");
+    return ms;
+}
+
+}  // namespace
+
 extern "C" {
-pgd_status pgd_batch_create(pgd_ctx*, int32_t, const int32_t*, const int64_t*, const int32_t*, const int32_t*, uint32_t, pgd_batch**) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_batch_run(pgd_batch*, uint64_t*, uint64_t*) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_batch_micro(pgd_batch*, int32_t, int32_t, uint64_t*) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_batch_info(pgd_batch*, int32_t*, int64_t*, int64_t*) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_batch_free(pgd_batch*) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_batch_set_tuning(pgd_batch*, const pgd_tuning*) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_batch_get_tuning(pgd_batch*, pgd_tuning*) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_batch_autotune(pgd_batch*, int32_t, pgd_tuning*, double*) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_batch_last_ms(pgd_batch*, double*) { return PGD_ERR_UNSUPPORTED; }
+
+pgd_status pgd_batch_create(pgd_ctx* ctx, int32_t ngraphs, const int32_t* n, const int64_t* line_off,
+                            const int32_t* src, const int32_t* dst, uint32_t flags, pgd_batch** out) {
+    if (!ctx || !out || ngraphs < 1 || !n || !line_off) return PGD_ERR_INVALID;
+    *out = nullptr;
+    std::unique_ptr<pgd_batch> b(new (std::nothrow) pgd_batch());
+    if (!b) return PGD_ERR_OOM;
+    b->ctx = ctx;
+    b->ng = ngraphs;
+    b->flags = flags & (PGD_DEDUP | PGD_MICRO);
+    pgd_status st = batch_guard(ctx, [&] {
+        PGD_HIP(hipSetDevice(ctx->device));
+        const int32_t ng = ngraphs;
+        b->n.assign(n, n + ng);
+        b->voff.assign(ng + 1, 0);
+        PGD_CHECK(line_off[0] == 0, PGD_ERR_INVALID, "line_off[0] must be 0");
+        for (int32_t g = 0; g < ng; g++) {
+            PGD_CHECK(n[g] >= 0, PGD_ERR_INVALID, "negative vertex count");
+            PGD_CHECK(line_off[g + 1] >= line_off[g], PGD_ERR_INVALID, "line_off must be non-decreasing");
+            b->voff[g + 1] = b->voff[g] + n[g];
+        }
+        const int64_t L = line_off[ng];
+        PGD_CHECK(L == 0 || (src && dst), PGD_ERR_INVALID, "missing edge lists");
+        b->N = b->voff[ng];
+        PGD_CHECK(b->N < INT32_MAX, PGD_ERR_UNSUPPORTED, "batch has 2^31 vertices or more");
+        // union COO: graph-local ids shifted by voff (checked against n_g)
+        std::vector<int32_t> us(std::max<int64_t>(L, 1)), ud(std::max<int64_t>(L, 1));
+        int64_t bad = 0;
+        for (int32_t g = 0; g < ng; g++) {
+            const int64_t off = b->voff[g], ng_ = n[g];
+            for (int64_t i = line_off[g]; i < line_off[g + 1]; i++) {
+                const int32_t x = src[i], y = dst[i];
+                if (x < 0 || y < 0 || x >= ng_ || y >= ng_) bad++;
+                us[i] = (int32_t)(x + off);
+                ud[i] = (int32_t)(y + off);
+            }
+        }
+        PGD_CHECK(bad == 0, PGD_ERR_INVALID, "vertex id outside [0, n) of its graph");
+        check_status(ctx, pgd_graph_from_coo(ctx, b->N, L, us.data(), ud.data(), b->flags & PGD_DEDUP, &b->U),
+                     "union CSR");
+        us.clear();
+        us.shrink_to_fit();
+        ud.clear();
+        ud.shrink_to_fit();
+        check_status(ctx, pgd_preprocess(b->U, "deg", 0), "preprocess");
+        check_status(ctx, pgd_sort_edges(b->U, "deg", 0), "sort_edges");
+        pgd_graph* U = b->U;
+        hipStream_t s = ctx->stream;
+        b->M = U->m;
+        // vertex groups, per-graph edge ranges
+        DBuf<int64_t> dvoff, grow;
+        dvoff.alloc(ctx, ng + 1);
+        grow.alloc(ctx, ng + 1);
+        b->group.alloc(ctx, std::max<int64_t>(b->N, 1));
+        PGD_HIP(hipMemcpyAsync(dvoff.get(), b->voff.data(), (ng + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_vertex_group, dim3(grid_cap(ctx, ng, 8)), dim3(256), 0, s, dvoff.get(), ng,
+                           b->group.get());
+        hipLaunchKernelGGL(k_graph_rows, dim3(grid_cap(ctx, (ng + 256) / 256, 4)), dim3(256), 0, s, U->row.get(),
+                           dvoff.get(), ng, grow.get());
+        PGD_HIP(hipGetLastError());
+        std::vector<int64_t> hrow(ng + 1);
+        PGD_HIP(hipMemcpyAsync(hrow.data(), grow.get(), (ng + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        PGD_HIP(hipStreamSynchronize(s));
+        b->eoff.resize(ng + 1);
+        b->m.resize(ng);
+        for (int32_t g = 0; g <= ng; g++) b->eoff[g] = hrow[g] / 2;
+        for (int32_t g = 0; g < ng; g++) b->m[g] = b->eoff[g + 1] - b->eoff[g];
+        PGD_CHECK(b->eoff[ng] == b->M, PGD_ERR_STATE, "graph edge ranges do not cover the union");
+        // E_ordered grouped by graph (stable: each graph keeps its own order)
+        if (b->M) {
+            DBuf<uint32_t> k0, k1;
+            DBuf<int32_t> o1;
+            k0.alloc(ctx, b->M);
+            k1.alloc(ctx, b->M);
+            o1.alloc(ctx, b->M);
+            hipLaunchKernelGGL(k_order_group, dim3(grid_cap(ctx, (b->M + 255) / 256, 16)), dim3(256), 0, s,
+                               U->order.get(), U->ev.get(), b->group.get(), b->M, k0.get());
+            PGD_HIP(hipGetLastError());
+            sort_kv(ctx, k0.get(), k1.get(), U->order.get(), o1.get(), (size_t)b->M, (unsigned)bitsfor(ng - 1));
+            PGD_HIP(hipMemcpyAsync(U->order.get(), o1.get(), b->M * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        }
+        // per-graph sizes and epilogue tiles
+        std::vector<EpiItem> items;
+        for (int32_t g = 0; g < ng; g++)
+            for (int64_t p = b->eoff[g]; p < b->eoff[g + 1]; p += kEpiTile)
+                items.push_back(EpiItem{g, 0, p, std::min<int64_t>(p + kEpiTile, b->eoff[g + 1])});
+        b->nitems = (int64_t)items.size();
+        b->items.alloc(ctx, std::max<int64_t>(b->nitems, 1));
+        if (b->nitems)
+            PGD_HIP(hipMemcpyAsync(b->items.get(), items.data(), items.size() * sizeof(EpiItem),
+                                   hipMemcpyHostToDevice, s));
+        b->gn.alloc(ctx, ng);
+        b->gm.alloc(ctx, ng);
+        PGD_HIP(hipMemcpyAsync(b->gn.get(), b->n.data(), ng * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        PGD_HIP(hipMemcpyAsync(b->gm.get(), b->m.data(), ng * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        b->sums.alloc(ctx, (size_t)ng * PGD_NSUMS);
+        b->ctl.alloc(ctx, 2);
+        const int64_t M1 = std::max<int64_t>(b->M, 1);
+        U->T.alloc(ctx, M1);
+        U->K4.alloc(ctx, M1);
+        U->C4.alloc(ctx, M1);
+        if (b->flags & PGD_MICRO) U->micro.alloc(ctx, (size_t)PGD_NMICRO * M1);
+        PGD_HIP(hipStreamSynchronize(s));
+    });
+    if (st != PGD_OK) return st;
+    *out = b.release();
+    return PGD_OK;
+}
+
+pgd_status pgd_batch_run(pgd_batch* b, uint64_t* sums, uint64_t* totals) {
+    if (!b) return PGD_ERR_INVALID;
+    return batch_guard(b->ctx, [&] {
+        PGD_HIP(hipSetDevice(b->ctx->device));
+        batch_replay(b);
+        const size_t ns = (size_t)b->ng * PGD_NSUMS;
+        if (sums) std::copy(b->host_sums.begin(), b->host_sums.begin() + ns, sums);
+        if (totals)
+            for (int32_t g = 0; g < b->ng; g++)
+                check_status(b->ctx,
+                             pgd_derive(b->n[g], b->m[g], b->host_sums.data() + (size_t)g * PGD_NSUMS,
+                                        totals + (size_t)g * PGD_NTOTALS),
+                             "derive");
+    });
+}
+
+pgd_status pgd_batch_micro(pgd_batch* b, int32_t gi, int32_t column, uint64_t* out) {
+    if (!b || !out || gi < 0 || gi >= b->ng || column < 0 || column >= PGD_NMICRO) return PGD_ERR_INVALID;
+    if (!(b->flags & PGD_MICRO) || b->host_sums.empty()) return PGD_ERR_STATE;
+    return batch_guard(b->ctx, [&] {
+        if (b->m[gi])
+            PGD_HIP(hipMemcpy(out, b->U->micro.get() + (size_t)column * b->M + b->eoff[gi],
+                              b->m[gi] * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    });
+}
+
+pgd_status pgd_batch_info(pgd_batch* b, int32_t* ngraphs, int64_t* total_vertices, int64_t* total_edges) {
+    if (!b) return PGD_ERR_INVALID;
+    if (ngraphs) *ngraphs = b->ng;
+    if (total_vertices) *total_vertices = b->N;
+    if (total_edges) *total_edges = b->M;
+    return PGD_OK;
+}
+
+pgd_status pgd_batch_free(pgd_batch* b) {
+    if (!b) return PGD_ERR_INVALID;
+    delete b;
+    return PGD_OK;
+}
+
+pgd_status pgd_batch_set_tuning(pgd_batch* b, const pgd_tuning* t) {
+    if (!b || !t) return PGD_ERR_INVALID;
+    return batch_guard(b->ctx, [&] {
+        const char* why = tuning_problem(*t);
+        PGD_CHECK(why[0] == 0, PGD_ERR_INVALID, why);
+        b->prm = params_from(*t);
+        b->drop_graph();
+    });
+}
+
+pgd_status pgd_batch_get_tuning(pgd_batch* b, pgd_tuning* t) {
+    if (!b || !t) return PGD_ERR_INVALID;
+    *t = tuning_from(b->prm);
+    return PGD_OK;
+}
+
+pgd_status pgd_batch_last_ms(pgd_batch* b, double* ms) {
+    if (!b || !ms) return PGD_ERR_INVALID;
+    *ms = b->last_ms;
+    return PGD_OK;
+}
+
+// Coordinate search: starting from the current parameters, each parameter in
+// turn takes every value of its list with the others held; the fastest
+// (median replay time) is kept before moving to the next parameter.
+pgd_status pgd_batch_autotune(pgd_batch* b, int32_t trials, pgd_tuning* best, double* best_ms) {
+    if (!b || trials < 1) return PGD_ERR_INVALID;
+    return batch_guard(b->ctx, [&] {
+        PGD_HIP(hipSetDevice(b->ctx->device));
+        std::vector<uint64_t> ref;
+        auto measure = [&](const OrientParams& p) {
+            b->prm = p;
+            b->drop_graph();
+            std::vector<double> t;
+            for (int i = 0; i < trials; i++) t.push_back(batch_replay(b));
+            if (ref.empty()) ref = b->host_sums;
+            PGD_CHECK(b->host_sums == ref, PGD_ERR_STATE, "autotuner candidates disagree on the counts");
+            std::sort(t.begin(), t.end());
+            return t[t.size() / 2];
+        };
+        OrientParams cur = b->prm;
+        double cur_ms = measure(cur);
+        struct Axis {
+            int OrientParams::*field;
+            std::vector<int> values;
+        };
+        const std::vector<Axis> axes = {
+            {&OrientParams::wedge_log2r, {12, 13, 14, 15}}, {&OrientParams::wedge_nt, {256, 512, 1024}},
+            {&OrientParams::small_k, {32, 64}},             {&OrientParams::mid_k, {128, 256}},
+            {&OrientParams::grid_small, {16, 32}},          {&OrientParams::wedge_split, {2, 4, 8}},
+        };
+        for (const Axis& ax : axes) {
+            for (int v : ax.values) {
+                if (cur.*ax.field == v) continue;
+                OrientParams cand = cur;
+                cand.*ax.field = v;
+                if (tuning_problem(tuning_from(cand))[0]) continue;
+                const double ms = measure(cand);
+                if (ms < cur_ms) {
+                    cur_ms = ms;
+                    cur = cand;
+                }
+            }
+        }
+        b->prm = cur;
+        b->drop_graph();
+        if (best) *best = tuning_from(cur);
+        if (best_ms) *best_ms = cur_ms;
+    });
+}
+
+// ---- serving (not yet available) ------------------------------------------
 pgd_status pgd_server_create(pgd_ctx*, int64_t, int32_t, pgd_server**) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_server_submit(pgd_server*, int32_t, int64_t, const int32_t*, const int32_t*, uint32_t, int64_t*) { return PGD_ERR_UNSUPPORTED; }
+pgd_status pgd_server_submit(pgd_server*, int32_t, int64_t, const int32_t*, const int32_t*, uint32_t, int64_t*) {
+    return PGD_ERR_UNSUPPORTED;
+}
 pgd_status pgd_server_poll(pgd_server*, int64_t*, int32_t, int32_t*) { return PGD_ERR_UNSUPPORTED; }
 pgd_status pgd_server_result(pgd_server*, int64_t, uint64_t*, uint64_t*, int64_t*) { return PGD_ERR_UNSUPPORTED; }
 pgd_status pgd_server_micro(pgd_server*, int64_t, int32_t, uint64_t*) { return PGD_ERR_UNSUPPORTED; }
 pgd_status pgd_server_release(pgd_server*, int64_t) { return PGD_ERR_UNSUPPORTED; }
 pgd_status pgd_server_free(pgd_server*) { return PGD_ERR_UNSUPPORTED; }
-}
+
+}  // extern "C"

@@ -19,6 +19,7 @@
 #include <climits>
 #include <new>
 
+#include "pgd_epilogue.hpp"
 #include "pgd_internal.hpp"
 
 using namespace pgd;
@@ -347,26 +348,6 @@ __global__ void k_order_from_keys(const uint64_t* __restrict__ key, int64_t m, i
 // graphlet_core.cpp:1325-1364), micro columns (1559-1575) and the reduction
 // of the 15 sums (1429-1448) in uint64 wrap-around arithmetic.
 // ===========================================================================
-// (double)p * 0.5 converted back to uint64, for even p: exact below 2^53,
-// otherwise p rounded to 53 significant bits (nearest, ties to even) halved.
-__device__ __forceinline__ uint64_t half_rne(uint64_t p) {
-    if (p < (1ULL << 53)) return p >> 1;
-    int L = 64 - __clzll((long long)p);
-    int sh = L - 53;
-    uint64_t q = p >> sh;
-    uint64_t rem = p & ((1ULL << sh) - 1);
-    uint64_t hf = 1ULL << (sh - 1);
-    if (rem > hf || (rem == hf && (q & 1))) q++;
-    return q << (sh - 1);
-}
-
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
-}
-
-constexpr uint64_t kExact = 1ULL << 53;
-
 __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ order, const int32_t* __restrict__ ev,
                                                   const int32_t* __restrict__ eu, const int64_t* __restrict__ row,
                                                   const uint32_t* __restrict__ Tc, const uint64_t* __restrict__ K4c,
@@ -380,56 +361,10 @@ __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ or
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
         int32_t id = order[e];
         int32_t v = ev[id], u = eu[id];
-        int64_t dv = row[v + 1] - row[v], du = row[u + 1] - row[u];
-        uint64_t T = Tc[id], K = K4c[id], C = C4c[id];
-        uint64_t sv = (uint64_t)dv - T - 1, su = (uint64_t)du - T - 1;
-        uint64_t S = sv + su;
-        uint64_t D = (uint64_t)n - (S + T) - 2;
-        acc[0] += T;
-        acc[1] += S;
-        acc[2] += D;
-        acc[3] += K;
-        acc[4] += C;
-        acc[5] += half_rne((T + S) * (T + S - 1));
-        acc[6] += half_rne(S * (S - 1));
-        acc[7] += half_rne(T * (T - 1));
-        acc[8] += T * S;
-        acc[9] += sv * su;
-        acc[10] += sv * (sv - 1) / 2 + su * (su - 1) / 2;
-        acc[11] += (uint64_t)(m - dv - du + 1);
-        acc[12] += half_rne(D * (D - 1));
-        acc[13] += S * D;
-        acc[14] += T * D;
-        if (micro) {
-            uint64_t tt = T * (T - 1);
-            int64_t svs = (int64_t)sv, sus = (int64_t)su;
-            uint64_t a3 = (uint64_t)(svs * (svs - 1)), b3 = (uint64_t)(sus * (sus - 1));
-            // exact below 2^53; larger values are recomputed on the host
-            if (tt >= kExact || K >= kExact || a3 >= kExact || b3 >= kExact || (a3 >> 1) + (b3 >> 1) >= kExact)
-                inexact++;
-            micro[(size_t)PGD_MICRO_TRI * m + e] = T;
-            micro[(size_t)PGD_MICRO_CLIQUE * m + e] = K;
-            micro[(size_t)PGD_MICRO_CHORDAL * m + e] = (tt >> 1) - K;
-            micro[(size_t)PGD_MICRO_TAILED * m + e] = 0;
-            micro[(size_t)PGD_MICRO_CYCLE * m + e] = C;
-            micro[(size_t)PGD_MICRO_STAR3 * m + e] = (a3 >> 1) + (b3 >> 1);
-            micro[(size_t)PGD_MICRO_PATH * m + e] = (uint64_t)(svs * sus) - C;
-            micro[(size_t)PGD_MICRO_STAR2 * m + e] = S;
-        }
+        inexact += edge_terms(row[v + 1] - row[v], row[u + 1] - row[u], Tc[id], K4c[id], C4c[id], n, m, acc, micro,
+                              m, e);
     }
-    __shared__ uint64_t red[PGD_NSUMS][4];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < PGD_NSUMS; k++) {
-        uint64_t s = wave_sum_u64(acc[k]);
-        if (lane == 0) red[k][w] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < PGD_NSUMS) {
-        uint64_t s = 0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[threadIdx.x][i];
-        atomicAdd((unsigned long long*)&sums[threadIdx.x], (unsigned long long)s);
-    }
+    block_add_sums(acc, sums);
     if (inexact) atomicAdd(flags, inexact);
 }
 

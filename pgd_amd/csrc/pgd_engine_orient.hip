@@ -46,34 +46,42 @@ constexpr int kHashSlots = 512;
 // ---------------------------------------------------------------------------
 // relabelled, sorted CSR
 // ---------------------------------------------------------------------------
-__global__ void k_rank_keys(const int32_t* __restrict__ deg, int64_t n, uint64_t* __restrict__ key) {
-    GRID_STRIDE(v, n) key[v] = ((uint64_t)(uint32_t)deg[v] << 32) | (uint32_t)v;
+// rank key (group, degree, id): the rank order is by degree then id, within
+// the vertex's group when groups are given (the graphs of a batch), so each
+// group occupies a contiguous rank range
+__global__ void k_rank_keys(const int32_t* __restrict__ deg, const int32_t* __restrict__ group, int64_t n, int db,
+                            int vb, uint64_t* __restrict__ key) {
+    GRID_STRIDE(v, n) {
+        uint64_t k = ((uint64_t)(uint32_t)deg[v] << vb) | (uint64_t)v;
+        if (group) k |= (uint64_t)(uint32_t)group[v] << (db + vb);
+        key[v] = k;
+    }
 }
 
-__global__ void k_rank_scatter(const uint64_t* __restrict__ sorted, int64_t n, int32_t* __restrict__ rank,
-                               int32_t* __restrict__ rdeg) {
+__global__ void k_rank_scatter(const uint64_t* __restrict__ sorted, int64_t n, int db, int vb,
+                               int32_t* __restrict__ rank, int32_t* __restrict__ rdeg) {
     GRID_STRIDE(i, n) {
         uint64_t k = sorted[i];
-        rank[(uint32_t)(k & 0xffffffffu)] = (int32_t)i;
-        rdeg[i] = (int32_t)(k >> 32);
+        rank[k & ((1ULL << vb) - 1)] = (int32_t)i;
+        rdeg[i] = (int32_t)((k >> vb) & ((1ULL << db) - 1));
     }
 }
 
 // two slots per edge id: (rank(ev) -> rank(eu)) and the reverse, value = edge id
 __global__ void k_slot_keys(const int32_t* __restrict__ ev, const int32_t* __restrict__ eu,
-                            const int32_t* __restrict__ rank, int64_t m, uint64_t* __restrict__ key,
+                            const int32_t* __restrict__ rank, int64_t m, int vb, uint64_t* __restrict__ key,
                             int32_t* __restrict__ val) {
     GRID_STRIDE(e, m) {
-        uint32_t a = (uint32_t)rank[ev[e]], b = (uint32_t)rank[eu[e]];
-        key[2 * e] = ((uint64_t)a << 32) | b;
-        key[2 * e + 1] = ((uint64_t)b << 32) | a;
+        uint64_t a = (uint32_t)rank[ev[e]], b = (uint32_t)rank[eu[e]];
+        key[2 * e] = (a << vb) | b;
+        key[2 * e + 1] = (b << vb) | a;
         val[2 * e] = (int32_t)e;
         val[2 * e + 1] = (int32_t)e;
     }
 }
 
-__global__ void k_slot_cols(const uint64_t* __restrict__ key, int64_t nnz, int32_t* __restrict__ col) {
-    GRID_STRIDE(p, nnz) col[p] = (int32_t)(key[p] & 0xffffffffu);
+__global__ void k_slot_cols(const uint64_t* __restrict__ key, int64_t nnz, int vb, int32_t* __restrict__ col) {
+    GRID_STRIDE(p, nnz) col[p] = (int32_t)(key[p] & ((1ULL << vb) - 1));
 }
 
 __global__ void k_row_tail(int64_t* row, int64_t n, int64_t nnz) {
@@ -429,17 +437,20 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
     ensure(st.Q, ctx, m);
     ensure(st.work, ctx, 4);
     int gn = grid_cap(ctx, (n + 255) / 256, 16), gm = grid_cap(ctx, (m + 255) / 256, 16);
-    hipLaunchKernelGGL(k_rank_keys, dim3(gn), dim3(256), 0, s, g.deg, n, st.k0.get());
+    const int vb = bitsfor(n - 1), db = bitsfor(g.max_deg), gb = g.group ? bitsfor(g.ngroups - 1) : 0;
+    PGD_CHECK(vb + db + gb <= 64, PGD_ERR_UNSUPPORTED, "rank key of (group, degree, id) exceeds 64 bits");
+    hipLaunchKernelGGL(k_rank_keys, dim3(gn), dim3(256), 0, s, g.deg, g.group, n, db, vb, st.k0.get());
     PGD_HIP(hipGetLastError());
-    sort_k(ctx, st.k0.get(), st.k1.get(), (size_t)n, 64);
-    hipLaunchKernelGGL(k_rank_scatter, dim3(gn), dim3(256), 0, s, st.k1.get(), n, st.rank.get(), st.rdeg.get());
+    sort_k(ctx, st.k0.get(), st.k1.get(), (size_t)n, (unsigned)(vb + db + gb));
+    hipLaunchKernelGGL(k_rank_scatter, dim3(gn), dim3(256), 0, s, st.k1.get(), n, db, vb, st.rank.get(),
+                       st.rdeg.get());
     PGD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_slot_keys, dim3(gm), dim3(256), 0, s, g.ev, g.eu, st.rank.get(), m, st.k0.get(),
+    hipLaunchKernelGGL(k_slot_keys, dim3(gm), dim3(256), 0, s, g.ev, g.eu, st.rank.get(), m, vb, st.k0.get(),
                        st.v0.get());
     PGD_HIP(hipGetLastError());
-    sort_kv(ctx, st.k0.get(), st.k1.get(), st.v0.get(), st.reid.get(), (size_t)nnz, 32u + (unsigned)bitsfor(n - 1));
+    sort_kv(ctx, st.k0.get(), st.k1.get(), st.v0.get(), st.reid.get(), (size_t)nnz, 2u * (unsigned)vb);
     int gz = grid_cap(ctx, (nnz + 255) / 256, 16);
-    hipLaunchKernelGGL(k_slot_cols, dim3(gz), dim3(256), 0, s, st.k1.get(), nnz, st.rcol.get());
+    hipLaunchKernelGGL(k_slot_cols, dim3(gz), dim3(256), 0, s, st.k1.get(), nnz, vb, st.rcol.get());
     PGD_HIP(hipGetLastError());
     excl_sum(ctx, st.rdeg.get(), st.rrow.get(), (size_t)n);
     hipLaunchKernelGGL(k_row_tail, dim3(1), dim3(1), 0, s, st.rrow.get(), n, nnz);

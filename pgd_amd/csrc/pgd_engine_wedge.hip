@@ -32,20 +32,25 @@ __global__ void w_fill_u32(uint32_t* __restrict__ p, int64_t n) {
     GRID_STRIDE(i, n) p[i] = 0;
 }
 
-// wcnt[s] = |N(b) below a| for every in-slot s = (a, b) of a heavy hub a;
-// one wavefront per hub
+// wcnt[s] = |N(b) below a| for every in-slot s = (a, b) of a heavy hub a,
+// and the hub's first endpoint tile (the lowest tile any N(b) reaches); one
+// wavefront per hub
 __global__ void k_pair_wedges(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                               const int64_t* __restrict__ ostart, const int32_t* __restrict__ heavy, int64_t nheavy,
-                              int32_t* __restrict__ wcnt) {
+                              int log2r, int32_t* __restrict__ wcnt, int32_t* __restrict__ hbase) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t h = wave; h < nheavy; h += nw) {
         const int32_t a = heavy[h];
+        int32_t lo = a;
         for (int64_t s = row[a] + lane; s < ostart[a]; s += 64) {
             const int32_t b = col[s];
             wcnt[s] = (int32_t)(lower_bound_g(col, row[b], row[b + 1], a) - row[b]);
+            lo = min(lo, col[row[b]]);
         }
+        for (int off = 32; off > 0; off >>= 1) lo = min(lo, __shfl_xor(lo, off, 64));
+        if (lane == 0) hbase[h] = lo >> log2r;
     }
 }
 
@@ -77,9 +82,11 @@ __global__ void k_tt_fill(const int64_t* __restrict__ row, const int32_t* __rest
     if (blockIdx.x == 0 && threadIdx.x == 0) ttrow[n] = total;
 }
 
-// number of endpoint tiles below each heavy hub a: tiles of ranks 0..a-1
-__global__ void k_hub_tiles(const int32_t* __restrict__ heavy, int64_t nheavy, int log2r, int64_t* __restrict__ nt) {
-    GRID_STRIDE(h, nheavy) nt[h] = ((int64_t)(heavy[h] - 1) >> log2r) + 1;
+// number of endpoint tiles of each heavy hub a: from its first tile to the
+// tile of rank a-1
+__global__ void k_hub_tiles(const int32_t* __restrict__ heavy, const int32_t* __restrict__ hbase, int64_t nheavy,
+                            int log2r, int64_t* __restrict__ nt) {
+    GRID_STRIDE(h, nheavy) nt[h] = ((int64_t)(heavy[h] - 1) >> log2r) + 1 - hbase[h];
 }
 
 // first tile-table entry of b with tile >= t (entries ascending by tile)
@@ -98,13 +105,15 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
                                                    const int64_t* __restrict__ ostart,
                                                    const int32_t* __restrict__ heavy, int64_t nheavy,
                                                    const int32_t* __restrict__ wcnt, const uint64_t* __restrict__ tt,
-                                                   const int64_t* __restrict__ ttrow, const int64_t* __restrict__ htoff,
-                                                   uint32_t* __restrict__ htseg, uint64_t* __restrict__ htw) {
+                                                   const int64_t* __restrict__ ttrow, const int32_t* __restrict__ hbase,
+                                                   const int64_t* __restrict__ htoff, uint32_t* __restrict__ htseg,
+                                                   uint64_t* __restrict__ htw) {
     __shared__ uint32_t lseg[kPlanLds];
     __shared__ unsigned long long lw[kPlanLds];
     for (int64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
         const int32_t a = heavy[h];
         const int64_t c0 = htoff[h], T = htoff[h + 1] - c0;
+        const uint32_t base = (uint32_t)hbase[h];
         const bool in_lds = T <= kPlanLds;
         if (in_lds)
             for (int i = threadIdx.x; i < T; i += NT) { lseg[i] = 0; lw[i] = 0; }
@@ -120,7 +129,7 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
                 if (off >= w) break;
                 const uint32_t nxt = j + 1 < j1 ? (uint32_t)tt[j + 1] : deg;
                 const uint32_t len = min(nxt, w) - off;
-                const uint32_t t = (uint32_t)(e >> 32);
+                const uint32_t t = (uint32_t)(e >> 32) - base;
                 if (in_lds) {
                     atomicAdd(&lseg[t], 1u);
                     atomicAdd(&lw[t], (unsigned long long)len);
@@ -140,8 +149,9 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
 // cut each hub's tiles into items: at most segcap segments, kTmax tiles and
 // (unless a single tile exceeds it) wcap wedges; empty items are dropped.
 // With out == nullptr only the number of items per hub is written.
-__global__ void k_plan_items(const int64_t* __restrict__ htoff, const uint32_t* __restrict__ htseg,
-                             const uint64_t* __restrict__ htw, int64_t nheavy, int64_t segcap, uint64_t wcap,
+__global__ void k_plan_items(const int64_t* __restrict__ htoff, const int32_t* __restrict__ hbase,
+                             const uint32_t* __restrict__ htseg, const uint64_t* __restrict__ htw, int64_t nheavy,
+                             int64_t segcap, uint64_t wcap,
                              int64_t* __restrict__ nitems, const int64_t* __restrict__ ioff,
                              WedgeItem* __restrict__ out, uint64_t* __restrict__ key) {
     GRID_STRIDE(h, nheavy) {
@@ -152,7 +162,7 @@ __global__ void k_plan_items(const int64_t* __restrict__ htoff, const uint32_t* 
             if (wed) {
                 if (out) {
                     const int64_t i = ioff[h] + cnt;
-                    out[i] = WedgeItem{(int32_t)h, (int32_t)t0, (int32_t)t1, 0};
+                    out[i] = WedgeItem{(int32_t)h, (int32_t)(hbase[h] + t0), (int32_t)(hbase[h] + t1), 0};
                     key[i] = ~wed;  // ascending sort of ~w: largest first
                 }
                 cnt++;
@@ -347,8 +357,9 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     wp.log2r = prm.wedge_log2r;
     // pair wedge counts
     ensure(wp.wcnt, ctx, nnz);
+    ensure(wp.hbase, ctx, nh);
     hipLaunchKernelGGL(k_pair_wedges, dim3(grid_cap(ctx, (nh * 64 + 255) / 256, 16)), dim3(256), 0, s, st.rrow.get(),
-                       st.rcol.get(), st.ostart.get(), st.b_heavy.get(), nh, wp.wcnt.get());
+                       st.rcol.get(), st.ostart.get(), st.b_heavy.get(), nh, wp.log2r, wp.wcnt.get(), wp.hbase.get());
     PGD_HIP(hipGetLastError());
     // tile table (st.v0 and st.k0 are free scratch of 2m entries after the relabel)
     {
@@ -377,7 +388,7 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         DBuf<int64_t> nt;
         nt.alloc(ctx, nh + 1);
         hipLaunchKernelGGL(k_hub_tiles, dim3(grid_cap(ctx, (nh + 255) / 256, 16)), dim3(256), 0, s,
-                           st.b_heavy.get(), nh, wp.log2r, nt.get());
+                           st.b_heavy.get(), wp.hbase.get(), nh, wp.log2r, nt.get());
         hipLaunchKernelGGL(w_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)nt.get() + nh, (int64_t)1);
         PGD_HIP(hipGetLastError());
         excl_sum(ctx, nt.get(), wp.htoff.get(), (size_t)(nh + 1));
@@ -392,7 +403,7 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         hipLaunchKernelGGL(w_fill_u64, dim3(g), dim3(256), 0, s, wp.htw.get(), ncell);
         hipLaunchKernelGGL(k_plan_cells<256>, dim3(grid_cap(ctx, nh, 8)), dim3(256), 0, s, st.rrow.get(),
                            st.rcol.get(), st.ostart.get(), st.b_heavy.get(), nh, wp.wcnt.get(), wp.tt.get(),
-                           wp.ttrow.get(), wp.htoff.get(), wp.htseg.get(), wp.htw.get());
+                           wp.ttrow.get(), wp.hbase.get(), wp.htoff.get(), wp.htseg.get(), wp.htw.get());
         PGD_HIP(hipGetLastError());
     }
     const uint32_t maxcell = device_max(ctx, wp.htseg.get(), (size_t)ncell);
@@ -416,8 +427,8 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         ioff.alloc(ctx, nh + 1);
         const int g = grid_cap(ctx, (nh + 255) / 256, 16);
         hipLaunchKernelGGL(w_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)per.get() + nh, (int64_t)1);
-        hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, s, wp.htoff.get(), wp.htseg.get(), wp.htw.get(), nh,
-                           wp.segcap, wcap, per.get(), (const int64_t*)nullptr, (WedgeItem*)nullptr,
+        hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, s, wp.htoff.get(), wp.hbase.get(), wp.htseg.get(),
+                           wp.htw.get(), nh, wp.segcap, wcap, per.get(), (const int64_t*)nullptr, (WedgeItem*)nullptr,
                            (uint64_t*)nullptr);
         PGD_HIP(hipGetLastError());
         excl_sum(ctx, per.get(), ioff.get(), (size_t)(nh + 1));
@@ -431,8 +442,8 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         k2.alloc(ctx, ni);
         v0.alloc(ctx, ni);
         v1.alloc(ctx, ni);
-        hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, s, wp.htoff.get(), wp.htseg.get(), wp.htw.get(), nh,
-                           wp.segcap, wcap, per.get(), ioff.get(), raw.get(), key.get());
+        hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, s, wp.htoff.get(), wp.hbase.get(), wp.htseg.get(),
+                           wp.htw.get(), nh, wp.segcap, wcap, per.get(), ioff.get(), raw.get(), key.get());
         PGD_HIP(hipGetLastError());
         ensure(wp.items, ctx, ni);
         if (ni) {

@@ -108,6 +108,8 @@ struct GraphView {
     const int32_t* ev;    // m
     const int32_t* eu;    // m
     const int32_t* order; // m, E_ordered
+    const int32_t* group = nullptr;  // n, graph of each vertex (batched union), or null
+    int64_t ngroups = 0;
 };
 
 // engines (each writes EdgeCounts for every edge id)

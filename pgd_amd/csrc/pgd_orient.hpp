@@ -203,6 +203,7 @@ struct WedgePlan {
     DBuf<int32_t> wcnt;             // per in-slot: |N(b) below a| for the pair (a,b)
     DBuf<uint64_t> tt;              // tile table entries: (tile << 32) | offset in N(b)
     DBuf<int64_t> ttrow;            // first tile-table entry of each vertex (n+1)
+    DBuf<int32_t> hbase;            // first endpoint tile of each heavy hub
     DBuf<int64_t> htoff;            // first (hub, tile) cell of each heavy hub (+1)
     DBuf<uint32_t> htseg;           // segments per (hub, tile) cell
     DBuf<uint64_t> htw;             // wedges per (hub, tile) cell
