@@ -206,23 +206,44 @@ pgd_status pgd_batch_autotune(pgd_batch *b, int32_t trials, pgd_tuning *best, do
 pgd_status pgd_batch_last_ms(pgd_batch *b, double *ms);
 
 /* ---- serving ----------------------------------------------------------
- * A request loop with continuous batching: requests (one graph each) are
- * admitted when their estimated HBM footprint fits the server's budget and a
- * slot is free; admitted requests are counted by the persistent kernel while
- * later ones queue.  pgd_server_poll returns finished request ids. */
+ * A request loop with continuous batching (no reference counterpart).  A
+ * worker thread owns two private streams on the context's device.  Requests
+ * (one graph each, ids local to the graph) queue in arrival order; the
+ * worker admits queued requests while their estimated HBM footprint
+ * (pgd_server_footprint) fits what the in-flight batches leave of the budget
+ * and fewer than max_inflight requests are in flight, builds them into one
+ * batch on one stream while the previous batch counts on the other, and
+ * stores each finished request's counts on the host until released.
+ * Requests with different flags go to different batches.  Latency is
+ * submission to completion, on the host's steady clock. */
 typedef struct pgd_server pgd_server;
 pgd_status pgd_server_create(pgd_ctx *ctx, int64_t hbm_budget_bytes, int32_t max_inflight, pgd_server **out);
-/* Submit a graph; returns its request id.  PGD_ERR_BUSY if it can never
- * fit the budget. */
+/* Estimated HBM bytes of a request: CSR 8m + 8(n+1), per-edge scratch and
+ * outputs 64m (m = nlines), and its share of the tile queues. */
+int64_t    pgd_server_footprint(int32_t n, int64_t nlines);
+/* Queue a graph (flags: PGD_DEDUP, PGD_MICRO); returns its request id.
+ * PGD_ERR_BUSY if its footprint exceeds the whole budget. */
 pgd_status pgd_server_submit(pgd_server *s, int32_t n, int64_t nlines, const int32_t *src,
                              const int32_t *dst, uint32_t flags, int64_t *req_id);
-/* Drive the loop: admit queued requests, launch, collect finished ones.
- * Writes up to cap finished request ids into done (count in *ndone). */
+/* Ids of requests finished since the last poll (up to cap; *ndone of them).
+ * Never blocks. */
 pgd_status pgd_server_poll(pgd_server *s, int64_t *done, int32_t cap, int32_t *ndone);
-/* Result of a finished request: sums (PGD_NSUMS), totals (PGD_NTOTALS). */
-pgd_status pgd_server_result(pgd_server *s, int64_t req_id, uint64_t *sums, uint64_t *totals, int64_t *m);
+/* Block until every queued request has finished (timeout_ms < 0: no limit;
+ * PGD_ERR_BUSY on timeout). */
+pgd_status pgd_server_drain(pgd_server *s, int64_t timeout_ms);
+/* Result of a finished request: sums (PGD_NSUMS), totals (PGD_NTOTALS), its
+ * edge count and latency in ms (any pointer may be NULL).  PGD_ERR_STATE if
+ * it has not finished; its own failure status if it failed. */
+pgd_status pgd_server_result(pgd_server *s, int64_t req_id, uint64_t *sums, uint64_t *totals, int64_t *m,
+                             double *latency_ms);
+/* Micro column of a finished PGD_MICRO request (m values, E_ordered order). */
 pgd_status pgd_server_micro(pgd_server *s, int64_t req_id, int32_t column, uint64_t *out);
 pgd_status pgd_server_release(pgd_server *s, int64_t req_id);
+/* Totals since creation: finished requests, their edges, seconds from the
+ * first submission to the last completion, latency percentiles (ms), and
+ * the number of batches launched. */
+pgd_status pgd_server_stats(pgd_server *s, int64_t *finished, int64_t *edges, double *elapsed_s, double *p50_ms,
+                            double *p99_ms, int64_t *batches);
 pgd_status pgd_server_free(pgd_server *s);
 
 /* ---- synthetic inputs (host) ------------------------------------------ */

@@ -425,3 +425,84 @@ class Batch:
             self.free()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# served form
+# ---------------------------------------------------------------------------
+def server_footprint(n: int, nlines: int) -> int:
+    """Estimated HBM bytes the server charges a request (pgd_server_footprint)."""
+    return int(lib().pgd_server_footprint(n, nlines))
+
+
+class Server:
+    """Request loop with continuous batching (pgd_server_*).
+
+    submit() queues a graph and returns at once; a worker thread admits
+    queued requests by estimated HBM footprint, counts them in batches on two
+    overlapping streams, and keeps each result until release()."""
+
+    def __init__(self, ctx: Context | None = None, hbm_budget: int = 32 << 30, max_inflight: int = 16384):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        check(lib().pgd_server_create(self.ctx.h, hbm_budget, max_inflight, C.byref(h)), self.ctx.h,
+              "pgd_server_create")
+        self.h = h
+        self.ctx._graphs.add(self)
+
+    def submit(self, n: int, src, dst, micro: bool = False, dedup: bool = False) -> int:
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        dst = np.ascontiguousarray(dst, dtype=np.int32)
+        rid = C.c_int64()
+        flags = (MICRO if micro else 0) | (DEDUP if dedup else 0)
+        check(lib().pgd_server_submit(self.h, n, len(src), ptr(src, C.c_int32), ptr(dst, C.c_int32), flags,
+                                      C.byref(rid)), self.ctx.h, "pgd_server_submit")
+        return rid.value
+
+    def poll(self, cap: int = 65536) -> list[int]:
+        buf = np.zeros(cap, dtype=np.int64)
+        k = C.c_int32()
+        check(lib().pgd_server_poll(self.h, ptr(buf, C.c_int64), cap, C.byref(k)), self.ctx.h)
+        return [int(x) for x in buf[: k.value]]
+
+    def drain(self, timeout_ms: int = -1) -> None:
+        check(lib().pgd_server_drain(self.h, timeout_ms), self.ctx.h, "pgd_server_drain")
+
+    def result(self, rid: int) -> dict:
+        s = np.zeros(NSUMS, dtype=np.uint64)
+        t = np.zeros(NTOTALS, dtype=np.uint64)
+        m, lat = C.c_int64(), C.c_double()
+        check(lib().pgd_server_result(self.h, rid, ptr(s, C.c_uint64), ptr(t, C.c_uint64), C.byref(m),
+                                      C.byref(lat)), self.ctx.h, "pgd_server_result")
+        return {"sums": [int(x) for x in s], "totals": [int(x) for x in t], "m": m.value, "latency_ms": lat.value}
+
+    def micro(self, rid: int, column: int | str, m: int) -> np.ndarray:
+        if isinstance(column, str):
+            column = MICRO_COLUMNS.index(column)
+        out = np.zeros(max(m, 1), dtype=np.uint64)
+        check(lib().pgd_server_micro(self.h, rid, column, ptr(out, C.c_uint64)), self.ctx.h, "pgd_server_micro")
+        return out[:m]
+
+    def release(self, rid: int) -> None:
+        check(lib().pgd_server_release(self.h, rid), self.ctx.h, "pgd_server_release")
+
+    def stats(self) -> dict:
+        f, e, b = C.c_int64(), C.c_int64(), C.c_int64()
+        el, p50, p99 = C.c_double(), C.c_double(), C.c_double()
+        check(lib().pgd_server_stats(self.h, C.byref(f), C.byref(e), C.byref(el), C.byref(p50), C.byref(p99),
+                                     C.byref(b)), self.ctx.h)
+        return {"finished": f.value, "edges": e.value, "elapsed_s": el.value, "p50_ms": p50.value,
+                "p99_ms": p99.value, "batches": b.value}
+
+    def free(self) -> None:
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
+            lib().pgd_server_free(self.h)
+        self.h = None
+
+    close = free
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

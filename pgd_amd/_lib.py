@@ -38,7 +38,7 @@ EXPORTS = [
     "pgd_batch_micro", "pgd_batch_info", "pgd_batch_free", "pgd_batch_set_tuning", "pgd_batch_get_tuning",
     "pgd_batch_autotune", "pgd_batch_last_ms", "pgd_tuning_default", "pgd_graph_set_tuning", "pgd_graph_get_tuning",
     "pgd_server_create", "pgd_server_submit", "pgd_server_poll", "pgd_server_result", "pgd_server_micro",
-    "pgd_server_release", "pgd_server_free", "pgd_gen_gnp", "pgd_gen_ba", "pgd_gen_rmat", "pgd_write_mtx",
+    "pgd_server_release", "pgd_server_free", "pgd_server_footprint", "pgd_server_drain", "pgd_server_stats", "pgd_gen_gnp", "pgd_gen_ba", "pgd_gen_rmat", "pgd_write_mtx",
     "pgd_read_graph", "pgd_free",
 ]
 
@@ -111,7 +111,10 @@ def lib() -> C.CDLL:
         "pgd_server_create": (I32, [P, I64, I32, C.POINTER(P)]),
         "pgd_server_submit": (I32, [P, I32, I64, PI32, PI32, U32, PI64]),
         "pgd_server_poll": (I32, [P, PI64, I32, PI32]),
-        "pgd_server_result": (I32, [P, I64, PU64, PU64, PI64]),
+        "pgd_server_result": (I32, [P, I64, PU64, PU64, PI64, C.POINTER(D)]),
+        "pgd_server_footprint": (I64, [I32, I64]),
+        "pgd_server_drain": (I32, [P, I64]),
+        "pgd_server_stats": (I32, [P, PI64, PI64, C.POINTER(D), C.POINTER(D), C.POINTER(D), PI64]),
         "pgd_server_micro": (I32, [P, I64, I32, PU64]),
         "pgd_server_release": (I32, [P, I64]),
         "pgd_server_free": (I32, [P]),

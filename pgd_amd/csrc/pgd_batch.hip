@@ -17,10 +17,16 @@
 // epilogue) has no host synchronisation, allocation or memset, and is
 // captured once in a HIP graph that pgd_batch_run replays.  Launch
 // parameters come from pgd_batch_set_tuning or the built-in autotuner.
+#include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "pgd_epilogue.hpp"
@@ -225,6 +231,108 @@ double batch_replay(pgd_batch* b) {
 
 }  // namespace
 
+namespace {
+
+// the union, its preprocessing, grouped E_ordered and per-graph tables
+void batch_build(pgd_batch* b, int32_t ngraphs, const int32_t* n, const int64_t* line_off, const int32_t* src,
+                 const int32_t* dst) {
+    pgd_ctx* ctx = b->ctx;
+    PGD_HIP(hipSetDevice(ctx->device));
+    const int32_t ng = ngraphs;
+    b->n.assign(n, n + ng);
+    b->voff.assign(ng + 1, 0);
+    PGD_CHECK(line_off[0] == 0, PGD_ERR_INVALID, "line_off[0] must be 0");
+    for (int32_t g = 0; g < ng; g++) {
+        PGD_CHECK(n[g] >= 0, PGD_ERR_INVALID, "negative vertex count");
+        PGD_CHECK(line_off[g + 1] >= line_off[g], PGD_ERR_INVALID, "line_off must be non-decreasing");
+        b->voff[g + 1] = b->voff[g] + n[g];
+    }
+    const int64_t L = line_off[ng];
+    PGD_CHECK(L == 0 || (src && dst), PGD_ERR_INVALID, "missing edge lists");
+    b->N = b->voff[ng];
+    PGD_CHECK(b->N < INT32_MAX, PGD_ERR_UNSUPPORTED, "batch has 2^31 vertices or more");
+    // union COO: graph-local ids shifted by voff (checked against n_g)
+    std::vector<int32_t> us(std::max<int64_t>(L, 1)), ud(std::max<int64_t>(L, 1));
+    int64_t bad = 0;
+    for (int32_t g = 0; g < ng; g++) {
+        const int64_t off = b->voff[g], ng_ = n[g];
+        for (int64_t i = line_off[g]; i < line_off[g + 1]; i++) {
+            const int32_t x = src[i], y = dst[i];
+            if (x < 0 || y < 0 || x >= ng_ || y >= ng_) bad++;
+            us[i] = (int32_t)(x + off);
+            ud[i] = (int32_t)(y + off);
+        }
+    }
+    PGD_CHECK(bad == 0, PGD_ERR_INVALID, "vertex id outside [0, n) of its graph");
+    check_status(ctx, pgd_graph_from_coo(ctx, b->N, L, us.data(), ud.data(), b->flags & PGD_DEDUP, &b->U),
+                 "union CSR");
+    us.clear();
+    us.shrink_to_fit();
+    ud.clear();
+    ud.shrink_to_fit();
+    check_status(ctx, pgd_preprocess(b->U, "deg", 0), "preprocess");
+    check_status(ctx, pgd_sort_edges(b->U, "deg", 0), "sort_edges");
+    pgd_graph* U = b->U;
+    hipStream_t s = ctx->stream;
+    b->M = U->m;
+    // vertex groups, per-graph edge ranges
+    DBuf<int64_t> dvoff, grow;
+    dvoff.alloc(ctx, ng + 1);
+    grow.alloc(ctx, ng + 1);
+    b->group.alloc(ctx, std::max<int64_t>(b->N, 1));
+    PGD_HIP(hipMemcpyAsync(dvoff.get(), b->voff.data(), (ng + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_vertex_group, dim3(grid_cap(ctx, ng, 8)), dim3(256), 0, s, dvoff.get(), ng,
+                       b->group.get());
+    hipLaunchKernelGGL(k_graph_rows, dim3(grid_cap(ctx, (ng + 256) / 256, 4)), dim3(256), 0, s, U->row.get(),
+                       dvoff.get(), ng, grow.get());
+    PGD_HIP(hipGetLastError());
+    std::vector<int64_t> hrow(ng + 1);
+    PGD_HIP(hipMemcpyAsync(hrow.data(), grow.get(), (ng + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PGD_HIP(hipStreamSynchronize(s));
+    b->eoff.resize(ng + 1);
+    b->m.resize(ng);
+    for (int32_t g = 0; g <= ng; g++) b->eoff[g] = hrow[g] / 2;
+    for (int32_t g = 0; g < ng; g++) b->m[g] = b->eoff[g + 1] - b->eoff[g];
+    PGD_CHECK(b->eoff[ng] == b->M, PGD_ERR_STATE, "graph edge ranges do not cover the union");
+    // E_ordered grouped by graph (stable: each graph keeps its own order)
+    if (b->M) {
+        DBuf<uint32_t> k0, k1;
+        DBuf<int32_t> o1;
+        k0.alloc(ctx, b->M);
+        k1.alloc(ctx, b->M);
+        o1.alloc(ctx, b->M);
+        hipLaunchKernelGGL(k_order_group, dim3(grid_cap(ctx, (b->M + 255) / 256, 16)), dim3(256), 0, s,
+                           U->order.get(), U->ev.get(), b->group.get(), b->M, k0.get());
+        PGD_HIP(hipGetLastError());
+        sort_kv(ctx, k0.get(), k1.get(), U->order.get(), o1.get(), (size_t)b->M, (unsigned)bitsfor(ng - 1));
+        PGD_HIP(hipMemcpyAsync(U->order.get(), o1.get(), b->M * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    }
+    // per-graph sizes and epilogue tiles
+    std::vector<EpiItem> items;
+    for (int32_t g = 0; g < ng; g++)
+        for (int64_t p = b->eoff[g]; p < b->eoff[g + 1]; p += kEpiTile)
+            items.push_back(EpiItem{g, 0, p, std::min<int64_t>(p + kEpiTile, b->eoff[g + 1])});
+    b->nitems = (int64_t)items.size();
+    b->items.alloc(ctx, std::max<int64_t>(b->nitems, 1));
+    if (b->nitems)
+        PGD_HIP(hipMemcpyAsync(b->items.get(), items.data(), items.size() * sizeof(EpiItem),
+                               hipMemcpyHostToDevice, s));
+    b->gn.alloc(ctx, ng);
+    b->gm.alloc(ctx, ng);
+    PGD_HIP(hipMemcpyAsync(b->gn.get(), b->n.data(), ng * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    PGD_HIP(hipMemcpyAsync(b->gm.get(), b->m.data(), ng * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    b->sums.alloc(ctx, (size_t)ng * PGD_NSUMS);
+    b->ctl.alloc(ctx, 2);
+    const int64_t M1 = std::max<int64_t>(b->M, 1);
+    U->T.alloc(ctx, M1);
+    U->K4.alloc(ctx, M1);
+    U->C4.alloc(ctx, M1);
+    if (b->flags & PGD_MICRO) U->micro.alloc(ctx, (size_t)PGD_NMICRO * M1);
+    PGD_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
 extern "C" {
 
 pgd_status pgd_batch_create(pgd_ctx* ctx, int32_t ngraphs, const int32_t* n, const int64_t* line_off,
@@ -236,100 +344,7 @@ pgd_status pgd_batch_create(pgd_ctx* ctx, int32_t ngraphs, const int32_t* n, con
     b->ctx = ctx;
     b->ng = ngraphs;
     b->flags = flags & (PGD_DEDUP | PGD_MICRO);
-    pgd_status st = batch_guard(ctx, [&] {
-        PGD_HIP(hipSetDevice(ctx->device));
-        const int32_t ng = ngraphs;
-        b->n.assign(n, n + ng);
-        b->voff.assign(ng + 1, 0);
-        PGD_CHECK(line_off[0] == 0, PGD_ERR_INVALID, "line_off[0] must be 0");
-        for (int32_t g = 0; g < ng; g++) {
-            PGD_CHECK(n[g] >= 0, PGD_ERR_INVALID, "negative vertex count");
-            PGD_CHECK(line_off[g + 1] >= line_off[g], PGD_ERR_INVALID, "line_off must be non-decreasing");
-            b->voff[g + 1] = b->voff[g] + n[g];
-        }
-        const int64_t L = line_off[ng];
-        PGD_CHECK(L == 0 || (src && dst), PGD_ERR_INVALID, "missing edge lists");
-        b->N = b->voff[ng];
-        PGD_CHECK(b->N < INT32_MAX, PGD_ERR_UNSUPPORTED, "batch has 2^31 vertices or more");
-        // union COO: graph-local ids shifted by voff (checked against n_g)
-        std::vector<int32_t> us(std::max<int64_t>(L, 1)), ud(std::max<int64_t>(L, 1));
-        int64_t bad = 0;
-        for (int32_t g = 0; g < ng; g++) {
-            const int64_t off = b->voff[g], ng_ = n[g];
-            for (int64_t i = line_off[g]; i < line_off[g + 1]; i++) {
-                const int32_t x = src[i], y = dst[i];
-                if (x < 0 || y < 0 || x >= ng_ || y >= ng_) bad++;
-                us[i] = (int32_t)(x + off);
-                ud[i] = (int32_t)(y + off);
-            }
-        }
-        PGD_CHECK(bad == 0, PGD_ERR_INVALID, "vertex id outside [0, n) of its graph");
-        check_status(ctx, pgd_graph_from_coo(ctx, b->N, L, us.data(), ud.data(), b->flags & PGD_DEDUP, &b->U),
-                     "union CSR");
-        us.clear();
-        us.shrink_to_fit();
-        ud.clear();
-        ud.shrink_to_fit();
-        check_status(ctx, pgd_preprocess(b->U, "deg", 0), "preprocess");
-        check_status(ctx, pgd_sort_edges(b->U, "deg", 0), "sort_edges");
-        pgd_graph* U = b->U;
-        hipStream_t s = ctx->stream;
-        b->M = U->m;
-        // vertex groups, per-graph edge ranges
-        DBuf<int64_t> dvoff, grow;
-        dvoff.alloc(ctx, ng + 1);
-        grow.alloc(ctx, ng + 1);
-        b->group.alloc(ctx, std::max<int64_t>(b->N, 1));
-        PGD_HIP(hipMemcpyAsync(dvoff.get(), b->voff.data(), (ng + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_vertex_group, dim3(grid_cap(ctx, ng, 8)), dim3(256), 0, s, dvoff.get(), ng,
-                           b->group.get());
-        hipLaunchKernelGGL(k_graph_rows, dim3(grid_cap(ctx, (ng + 256) / 256, 4)), dim3(256), 0, s, U->row.get(),
-                           dvoff.get(), ng, grow.get());
-        PGD_HIP(hipGetLastError());
-        std::vector<int64_t> hrow(ng + 1);
-        PGD_HIP(hipMemcpyAsync(hrow.data(), grow.get(), (ng + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        PGD_HIP(hipStreamSynchronize(s));
-        b->eoff.resize(ng + 1);
-        b->m.resize(ng);
-        for (int32_t g = 0; g <= ng; g++) b->eoff[g] = hrow[g] / 2;
-        for (int32_t g = 0; g < ng; g++) b->m[g] = b->eoff[g + 1] - b->eoff[g];
-        PGD_CHECK(b->eoff[ng] == b->M, PGD_ERR_STATE, "graph edge ranges do not cover the union");
-        // E_ordered grouped by graph (stable: each graph keeps its own order)
-        if (b->M) {
-            DBuf<uint32_t> k0, k1;
-            DBuf<int32_t> o1;
-            k0.alloc(ctx, b->M);
-            k1.alloc(ctx, b->M);
-            o1.alloc(ctx, b->M);
-            hipLaunchKernelGGL(k_order_group, dim3(grid_cap(ctx, (b->M + 255) / 256, 16)), dim3(256), 0, s,
-                               U->order.get(), U->ev.get(), b->group.get(), b->M, k0.get());
-            PGD_HIP(hipGetLastError());
-            sort_kv(ctx, k0.get(), k1.get(), U->order.get(), o1.get(), (size_t)b->M, (unsigned)bitsfor(ng - 1));
-            PGD_HIP(hipMemcpyAsync(U->order.get(), o1.get(), b->M * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
-        }
-        // per-graph sizes and epilogue tiles
-        std::vector<EpiItem> items;
-        for (int32_t g = 0; g < ng; g++)
-            for (int64_t p = b->eoff[g]; p < b->eoff[g + 1]; p += kEpiTile)
-                items.push_back(EpiItem{g, 0, p, std::min<int64_t>(p + kEpiTile, b->eoff[g + 1])});
-        b->nitems = (int64_t)items.size();
-        b->items.alloc(ctx, std::max<int64_t>(b->nitems, 1));
-        if (b->nitems)
-            PGD_HIP(hipMemcpyAsync(b->items.get(), items.data(), items.size() * sizeof(EpiItem),
-                                   hipMemcpyHostToDevice, s));
-        b->gn.alloc(ctx, ng);
-        b->gm.alloc(ctx, ng);
-        PGD_HIP(hipMemcpyAsync(b->gn.get(), b->n.data(), ng * sizeof(int64_t), hipMemcpyHostToDevice, s));
-        PGD_HIP(hipMemcpyAsync(b->gm.get(), b->m.data(), ng * sizeof(int64_t), hipMemcpyHostToDevice, s));
-        b->sums.alloc(ctx, (size_t)ng * PGD_NSUMS);
-        b->ctl.alloc(ctx, 2);
-        const int64_t M1 = std::max<int64_t>(b->M, 1);
-        U->T.alloc(ctx, M1);
-        U->K4.alloc(ctx, M1);
-        U->C4.alloc(ctx, M1);
-        if (b->flags & PGD_MICRO) U->micro.alloc(ctx, (size_t)PGD_NMICRO * M1);
-        PGD_HIP(hipStreamSynchronize(s));
-    });
+    pgd_status st = batch_guard(ctx, [&] { batch_build(b.get(), ngraphs, n, line_off, src, dst); });
     if (st != PGD_OK) return st;
     *out = b.release();
     return PGD_OK;
@@ -446,15 +461,386 @@ pgd_status pgd_batch_autotune(pgd_batch* b, int32_t trials, pgd_tuning* best, do
     });
 }
 
-// ---- serving (not yet available) ------------------------------------------
-pgd_status pgd_server_create(pgd_ctx*, int64_t, int32_t, pgd_server**) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_server_submit(pgd_server*, int32_t, int64_t, const int32_t*, const int32_t*, uint32_t, int64_t*) {
-    return PGD_ERR_UNSUPPORTED;
+}  // extern "C"
+
+// ===========================================================================
+// serving: continuous batching over two private streams
+// ===========================================================================
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    int64_t id = 0;
+    int32_t n = 0;
+    uint32_t flags = 0;
+    std::vector<int32_t> src, dst;
+    int64_t footprint = 0;
+    Clock::time_point t_submit, t_done;
+    bool done = false;
+    pgd_status status = PGD_OK;
+    std::string err;
+    int64_t m = 0;
+    uint64_t sums[PGD_NSUMS] = {};
+    uint64_t totals[PGD_NTOTALS] = {};
+    std::vector<uint64_t> micro;  // PGD_NMICRO x m when requested
+};
+
+// launch the counting of a built batch without capture (one use)
+void batch_launch_eager(pgd_batch* b) {
+    if (b->M) {
+        if (!b->st) b->st = orient_state_new();
+        orient_params(b->st) = b->prm;
+        orient_build(b->ctx, union_view(b), *b->st);
+    }
+    batch_count(b);
 }
-pgd_status pgd_server_poll(pgd_server*, int64_t*, int32_t, int32_t*) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_server_result(pgd_server*, int64_t, uint64_t*, uint64_t*, int64_t*) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_server_micro(pgd_server*, int64_t, int32_t, uint64_t*) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_server_release(pgd_server*, int64_t) { return PGD_ERR_UNSUPPORTED; }
-pgd_status pgd_server_free(pgd_server*) { return PGD_ERR_UNSUPPORTED; }
+
+// wait for the batch's stream; per-request results to the host
+void batch_collect(pgd_batch* b, std::vector<Request*>& reqs) {
+    pgd_ctx* ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    b->host_sums.resize((size_t)b->ng * PGD_NSUMS);
+    uint32_t ctl[2] = {0, 0};
+    PGD_HIP(hipMemcpyAsync(b->host_sums.data(), b->sums.get(), b->host_sums.size() * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, s));
+    PGD_HIP(hipMemcpyAsync(ctl, b->ctl.get(), sizeof(ctl), hipMemcpyDeviceToHost, s));
+    std::vector<uint64_t> micro;
+    if ((b->flags & PGD_MICRO) && b->M) {
+        micro.resize((size_t)PGD_NMICRO * b->M);
+        PGD_HIP(hipMemcpyAsync(micro.data(), b->U->micro.get(), micro.size() * sizeof(uint64_t),
+                               hipMemcpyDeviceToHost, s));
+    }
+    PGD_HIP(hipStreamSynchronize(s));
+    PGD_CHECK(!(b->flags & PGD_MICRO) || ctl[0] == 0, PGD_ERR_UNSUPPORTED,
+              "a per-edge micro value exceeds 2^53; the exact double emulation for it is not implemented");
+    for (int32_t g = 0; g < b->ng; g++) {
+        Request* r = reqs[g];
+        r->m = b->m[g];
+        std::copy(b->host_sums.begin() + (size_t)g * PGD_NSUMS, b->host_sums.begin() + (size_t)(g + 1) * PGD_NSUMS,
+                  r->sums);
+        check_status(ctx, pgd_derive(b->n[g], b->m[g], r->sums, r->totals), "derive");
+        if (b->flags & PGD_MICRO) {
+            r->micro.resize((size_t)PGD_NMICRO * r->m);
+            for (int c = 0; c < PGD_NMICRO; c++)
+                std::copy(micro.begin() + (size_t)c * b->M + b->eoff[g],
+                          micro.begin() + (size_t)c * b->M + b->eoff[g] + r->m, r->micro.begin() + (size_t)c * r->m);
+        }
+    }
+}
+
+}  // namespace
+
+struct pgd_server {
+    pgd_ctx* owner = nullptr;
+    pgd_ctx* lane[2] = {nullptr, nullptr};  // private contexts: one stream each
+    int64_t budget = 0;
+    int32_t max_inflight = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> pending;
+    std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
+    std::vector<int64_t> unreported;
+    int64_t next_id = 0, inflight_bytes = 0, inflight_reqs = 0, outstanding = 0;
+    bool stop = false;
+    std::thread worker;
+    // statistics
+    std::vector<double> latencies;
+    int64_t finished = 0, edges = 0, batches = 0;
+    bool any_submit = false;
+    Clock::time_point first_submit, last_done;
+
+    void run();
+    // take queued requests for one batch (caller holds mu)
+    std::vector<Request*> admit();
+};
+
+std::vector<Request*> pgd_server::admit() {
+    std::vector<Request*> out;
+    int64_t bytes = 0;
+    while (!pending.empty()) {
+        Request* r = pending.front();
+        if (!out.empty() && r->flags != out[0]->flags) break;
+        if ((int64_t)out.size() + inflight_reqs >= max_inflight && !(out.empty() && inflight_reqs == 0)) break;
+        if (inflight_bytes + bytes + r->footprint > budget && !(out.empty() && inflight_reqs == 0)) break;
+        out.push_back(r);
+        bytes += r->footprint;
+        pending.pop_front();
+    }
+    inflight_bytes += bytes;
+    inflight_reqs += (int64_t)out.size();
+    return out;
+}
+
+void pgd_server::run() {
+    struct InFlight {
+        std::unique_ptr<pgd_batch> b;
+        std::vector<Request*> reqs;
+        int64_t bytes = 0;
+        pgd_status status = PGD_OK;
+        std::string err;
+    };
+    std::unique_ptr<InFlight> prev;
+    int turn = 0;
+    // build a batch of the given requests on ctx and launch its counting
+    auto start = [&](pgd_batch* b, pgd_ctx* ctx, const std::vector<Request*>& rs) {
+        b->ctx = ctx;
+        b->ng = (int32_t)rs.size();
+        b->flags = rs[0]->flags & (PGD_DEDUP | PGD_MICRO);
+        return batch_guard(ctx, [&] {
+            PGD_HIP(hipSetDevice(ctx->device));
+            std::vector<int32_t> n(rs.size());
+            std::vector<int64_t> off(rs.size() + 1, 0);
+            for (size_t i = 0; i < rs.size(); i++) {
+                n[i] = rs[i]->n;
+                off[i + 1] = off[i] + (int64_t)rs[i]->src.size();
+            }
+            std::vector<int32_t> src(std::max<int64_t>(off.back(), 1)), dst(std::max<int64_t>(off.back(), 1));
+            for (size_t i = 0; i < rs.size(); i++) {
+                std::copy(rs[i]->src.begin(), rs[i]->src.end(), src.begin() + off[i]);
+                std::copy(rs[i]->dst.begin(), rs[i]->dst.end(), dst.begin() + off[i]);
+            }
+            batch_build(b, b->ng, n.data(), off.data(), src.data(), dst.data());
+            batch_launch_eager(b);
+        });
+    };
+    auto finish = [&](InFlight& f) {
+        if (f.status == PGD_OK && f.b) {
+            f.status = batch_guard(f.b->ctx, [&] { batch_collect(f.b.get(), f.reqs); });
+            if (f.status != PGD_OK) f.err = f.b->ctx->last_err;
+        }
+        pgd_ctx* lctx = f.b ? f.b->ctx : lane[0];
+        if (f.b) (void)hipStreamSynchronize(lctx->stream);  // nothing of it still runs
+        f.b.reset();
+        // a failed batch of several requests: count each alone, so that one
+        // request's failure (an invalid graph) does not fail the others
+        std::vector<pgd_status> own(f.reqs.size(), f.status);
+        std::vector<std::string> own_err(f.reqs.size(), f.err);
+        if (f.status != PGD_OK && f.reqs.size() > 1) {
+            for (size_t i = 0; i < f.reqs.size(); i++) {
+                std::vector<Request*> one{f.reqs[i]};
+                std::unique_ptr<pgd_batch> b1(new pgd_batch());
+                own[i] = start(b1.get(), lctx, one);
+                if (own[i] == PGD_OK) own[i] = batch_guard(lctx, [&] { batch_collect(b1.get(), one); });
+                own_err[i] = own[i] == PGD_OK ? std::string() : lctx->last_err;
+                (void)hipStreamSynchronize(lctx->stream);
+            }
+        }
+        const Clock::time_point now = Clock::now();
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t i = 0; i < f.reqs.size(); i++) {
+            Request* r = f.reqs[i];
+            r->status = own[i];
+            r->err = own_err[i];
+            r->t_done = now;
+            r->done = true;
+            r->src.clear();
+            r->src.shrink_to_fit();
+            r->dst.clear();
+            r->dst.shrink_to_fit();
+            unreported.push_back(r->id);
+            if (r->status == PGD_OK) {
+                latencies.push_back(std::chrono::duration<double, std::milli>(now - r->t_submit).count());
+                finished++;
+                edges += r->m;
+            }
+            last_done = now;
+            outstanding--;
+        }
+        inflight_bytes -= f.bytes;
+        inflight_reqs -= (int64_t)f.reqs.size();
+        cv_done.notify_all();
+    };
+    for (;;) {
+        std::vector<Request*> batch;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            if (!prev) cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+            if (stop && pending.empty() && !prev) return;
+            batch = admit();
+        }
+        std::unique_ptr<InFlight> cur;
+        if (!batch.empty()) {
+            cur.reset(new InFlight());
+            cur->reqs = batch;
+            for (Request* r : batch) cur->bytes += r->footprint;
+            pgd_ctx* ctx = lane[turn];
+            turn ^= 1;
+            cur->b.reset(new pgd_batch());
+            cur->status = start(cur->b.get(), ctx, batch);  // counting runs while the previous batch is collected
+            if (cur->status != PGD_OK) cur->err = ctx->last_err;
+            std::lock_guard<std::mutex> lk(mu);
+            batches++;
+        }
+        if (prev) finish(*prev);
+        prev = std::move(cur);
+    }
+}
+
+extern "C" {
+
+int64_t pgd_server_footprint(int32_t n, int64_t nlines) {
+    if (n < 0 || nlines < 0) return -1;
+    return 8 * nlines + 8 * ((int64_t)n + 1) + 64 * nlines + 4096;
+}
+
+pgd_status pgd_server_create(pgd_ctx* ctx, int64_t hbm_budget_bytes, int32_t max_inflight, pgd_server** out) {
+    if (!ctx || !out || hbm_budget_bytes <= 0 || max_inflight < 1) return PGD_ERR_INVALID;
+    *out = nullptr;
+    std::unique_ptr<pgd_server> s(new (std::nothrow) pgd_server());
+    if (!s) return PGD_ERR_OOM;
+    s->owner = ctx;
+    s->budget = hbm_budget_bytes;
+    s->max_inflight = max_inflight;
+    for (int i = 0; i < 2; i++) {
+        pgd_status st = pgd_open(ctx->device, &s->lane[i]);
+        if (st != PGD_OK) {
+            ctx->last_err = "cannot open a private stream for the server";
+            for (int j = 0; j < i; j++) pgd_close(s->lane[j]);
+            return st;
+        }
+        pgd_set_mem_cap(s->lane[i], ctx->mem_cap);
+    }
+    pgd_server* p = s.get();
+    try {
+        s->worker = std::thread([p] { p->run(); });
+    } catch (...) {
+        for (int i = 0; i < 2; i++) pgd_close(s->lane[i]);
+        return PGD_ERR_OOM;
+    }
+    *out = s.release();
+    return PGD_OK;
+}
+
+pgd_status pgd_server_submit(pgd_server* s, int32_t n, int64_t nlines, const int32_t* src, const int32_t* dst,
+                             uint32_t flags, int64_t* req_id) {
+    if (!s || !req_id || n < 0 || nlines < 0 || (nlines > 0 && (!src || !dst))) return PGD_ERR_INVALID;
+    const int64_t fp = pgd_server_footprint(n, nlines);
+    if (fp > s->budget) return PGD_ERR_BUSY;
+    std::unique_ptr<Request> r(new (std::nothrow) Request());
+    if (!r) return PGD_ERR_OOM;
+    try {
+        r->src.assign(src, src + nlines);
+        r->dst.assign(dst, dst + nlines);
+    } catch (const std::bad_alloc&) {
+        return PGD_ERR_OOM;
+    }
+    r->n = n;
+    r->flags = flags & (PGD_DEDUP | PGD_MICRO);
+    r->footprint = fp;
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->id = s->next_id++;
+        *req_id = r->id;
+        if (!s->any_submit) {
+            s->any_submit = true;
+            s->first_submit = r->t_submit;
+        }
+        s->pending.push_back(r.get());
+        s->outstanding++;
+        s->reqs.emplace(r->id, std::move(r));
+    }
+    s->cv_work.notify_one();
+    return PGD_OK;
+}
+
+pgd_status pgd_server_poll(pgd_server* s, int64_t* done, int32_t cap, int32_t* ndone) {
+    if (!s || !ndone || cap < 0 || (cap > 0 && !done)) return PGD_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int32_t k = (int32_t)std::min<size_t>((size_t)cap, s->unreported.size());
+    std::copy(s->unreported.begin(), s->unreported.begin() + k, done);
+    s->unreported.erase(s->unreported.begin(), s->unreported.begin() + k);
+    *ndone = k;
+    return PGD_OK;
+}
+
+pgd_status pgd_server_drain(pgd_server* s, int64_t timeout_ms) {
+    if (!s) return PGD_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto idle = [&] { return s->outstanding == 0; };
+    if (timeout_ms < 0) {
+        s->cv_done.wait(lk, idle);
+        return PGD_OK;
+    }
+    return s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), idle) ? PGD_OK : PGD_ERR_BUSY;
+}
+
+pgd_status pgd_server_result(pgd_server* s, int64_t req_id, uint64_t* sums, uint64_t* totals, int64_t* m,
+                             double* latency_ms) {
+    if (!s) return PGD_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(req_id);
+    if (it == s->reqs.end()) return PGD_ERR_INVALID;
+    Request* r = it->second.get();
+    if (!r->done) return PGD_ERR_STATE;
+    if (r->status != PGD_OK) {
+        s->owner->last_err = r->err;
+        return r->status;
+    }
+    if (sums) std::copy(r->sums, r->sums + PGD_NSUMS, sums);
+    if (totals) std::copy(r->totals, r->totals + PGD_NTOTALS, totals);
+    if (m) *m = r->m;
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    return PGD_OK;
+}
+
+pgd_status pgd_server_micro(pgd_server* s, int64_t req_id, int32_t column, uint64_t* out) {
+    if (!s || !out || column < 0 || column >= PGD_NMICRO) return PGD_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(req_id);
+    if (it == s->reqs.end()) return PGD_ERR_INVALID;
+    Request* r = it->second.get();
+    if (!r->done || r->status != PGD_OK || !(r->flags & PGD_MICRO)) return PGD_ERR_STATE;
+    std::copy(r->micro.begin() + (size_t)column * r->m, r->micro.begin() + (size_t)(column + 1) * r->m, out);
+    return PGD_OK;
+}
+
+pgd_status pgd_server_release(pgd_server* s, int64_t req_id) {
+    if (!s) return PGD_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(req_id);
+    if (it == s->reqs.end()) return PGD_ERR_INVALID;
+    if (!it->second->done) return PGD_ERR_STATE;
+    s->reqs.erase(it);
+    return PGD_OK;
+}
+
+pgd_status pgd_server_stats(pgd_server* s, int64_t* finished, int64_t* edges, double* elapsed_s, double* p50_ms,
+                            double* p99_ms, int64_t* batches) {
+    if (!s) return PGD_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::vector<double> l = s->latencies;
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double q) {
+        if (l.empty()) return 0.0;
+        size_t i = (size_t)std::min<double>((double)l.size() - 1, std::ceil(q * (double)l.size()) - 1);
+        return l[i];
+    };
+    if (finished) *finished = s->finished;
+    if (edges) *edges = s->edges;
+    if (elapsed_s)
+        *elapsed_s = s->finished ? std::chrono::duration<double>(s->last_done - s->first_submit).count() : 0.0;
+    if (p50_ms) *p50_ms = pct(0.50);
+    if (p99_ms) *p99_ms = pct(0.99);
+    if (batches) *batches = s->batches;
+    return PGD_OK;
+}
+
+pgd_status pgd_server_free(pgd_server* s) {
+    if (!s) return PGD_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (auto& kv : s->reqs) (void)kv;
+    s->reqs.clear();
+    for (int i = 0; i < 2; i++)
+        if (s->lane[i]) pgd_close(s->lane[i]);
+    delete s;
+    return PGD_OK;
+}
 
 }  // extern "C"

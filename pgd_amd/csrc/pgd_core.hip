@@ -33,11 +33,13 @@ void* dmalloc(pgd_ctx* ctx, size_t bytes) {
     if (ctx->mem_cap > 0 && ctx->mem_used + (int64_t)bytes > ctx->mem_cap)
         throw Error{PGD_ERR_OOM, "device memory cap exceeded (" + std::to_string(ctx->mem_used + bytes) +
                                      " > " + std::to_string(ctx->mem_cap) + " bytes)"};
+    // stream-ordered: allocation and release follow the context's stream, so
+    // freeing a buffer never waits for work on other streams
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = hipMallocAsync(&p, bytes, ctx->stream);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        throw Error{PGD_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e)};
+        throw Error{PGD_ERR_OOM, "hipMallocAsync(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e)};
     }
     ctx->mem_used += (int64_t)bytes;
     return p;
@@ -45,7 +47,7 @@ void* dmalloc(pgd_ctx* ctx, size_t bytes) {
 
 void dfree(pgd_ctx* ctx, void* p, size_t bytes) {
     if (!p) return;
-    (void)hipFree(p);
+    (void)hipFreeAsync(p, ctx->stream);
     ctx->mem_used -= (int64_t)bytes;
 }
 
@@ -529,6 +531,12 @@ pgd_status pgd_open(int32_t device, pgd_ctx** out) {
         PGD_HIP(hipGetDeviceProperties(&prop, device));
         ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         PGD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        // keep released blocks in the device's pool instead of returning them
+        // to the driver at every synchronisation
+        hipMemPool_t pool = nullptr;
+        PGD_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+        uint64_t keep = UINT64_MAX;
+        PGD_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
     });
     if (st != PGD_OK) {
         delete ctx;
@@ -542,7 +550,10 @@ pgd_status pgd_close(pgd_ctx* ctx) {
     if (!ctx) return PGD_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
     if (ctx->tmp) dfree(ctx, ctx->tmp, ctx->tmp_bytes);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
     delete ctx;
     return PGD_OK;
 }
