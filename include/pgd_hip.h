@@ -155,8 +155,8 @@ typedef struct pgd_tuning {
     int32_t small_k;     /* out-degree up to which one wavefront owns a local graph (64) */
     int32_t mid_k;       /* out-degree up to which a 256-thread block does, 1024 threads above (256) */
     int32_t grid_small;  /* resident wavefront blocks per CU for the local passes (32) */
-    int32_t grid_mid;    /* resident 256-thread blocks per CU (8) */
-    int32_t wedge_log2r; /* endpoint tile of the heavy 4-cycle pass, 2^r LDS counters, r in 8..15 (15) */
+    int32_t grid_mid;    /* resident 256-thread blocks per CU (16) */
+    int32_t wedge_log2r; /* endpoint tile of the heavy 4-cycle pass, 2^r LDS counters, r in 8..15 (13) */
     int32_t wedge_nt;    /* threads per block of that pass: 256, 512 or 1024 (512) */
     int32_t wedge_split; /* work items per block the heavy wedges are cut into (4) */
 } pgd_tuning;

@@ -190,6 +190,24 @@ __device__ int64_t load_local(const int64_t* __restrict__ row, const int32_t* __
     return A.pref[k - 1];
 }
 
+constexpr int kU = 4;  // flattened positions per thread per round (independent searches and loads)
+
+// owner i and CSR slot of the flattened positions t0, t0+NT, ... (i = -1 past the end)
+template <int NT>
+__device__ __forceinline__ void locate(const LocalArrays& A, int k, int64_t total, int64_t t0, int* i,
+                                       int64_t* slot) {
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+        const int64_t t = t0 + (int64_t)NT * u;
+        i[u] = -1;
+        if (t < total) {
+            i[u] = owner_of(A.pref, k, t);
+            const int64_t ex = i[u] ? A.pref[i[u] - 1] : 0;
+            slot[u] = A.obeg[i[u]] + (t - ex);
+        }
+    }
+}
+
 // pass 1: triangles.  Local edge (i,j), i<j <=> triangle {a, L[i], L[j]}.
 template <int NT, bool kGlobal>
 __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
@@ -207,16 +225,23 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
         const int k = outdeg[a];
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
         if (threadIdx.x == 0) atomicAdd(work, (unsigned long long)total);
-        for (int64_t t = threadIdx.x; t < total; t += NT) {
-            const int i = owner_of(A.pref, k, t);
-            const int64_t ex = i ? A.pref[i - 1] : 0;
-            const int64_t slot = A.obeg[i] + (t - ex);
-            const int j = find_in(A.L, i + 1, k, col[slot]);
-            if (j >= 0) {
-                atomicAdd(&T[eid[slot]], 1u);
-                atomicAdd(&A.cnt[i], 1u);
-                atomicAdd(&A.cnt[j], 1u);
-            }
+        for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
+            int i[kU], j[kU];
+            int64_t slot[kU];
+            int32_t x[kU];
+            locate<NT>(A, k, total, t0, i, slot);
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                if (i[u] >= 0) x[u] = col[slot[u]];
+#pragma unroll
+            for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_in(A.L, i[u] + 1, k, x[u]) : -1;
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                if (j[u] >= 0) {
+                    atomicAdd(&T[eid[slot[u]]], 1u);
+                    atomicAdd(&A.cnt[i[u]], 1u);
+                    atomicAdd(&A.cnt[j[u]], 1u);
+                }
         }
         __syncthreads();
         for (int i = threadIdx.x; i < k; i += NT)
@@ -246,26 +271,41 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
         if (threadIdx.x == 0) atomicAdd(work, (unsigned long long)total);
         // build the symmetric adjacency of the local graph
-        for (int64_t t = threadIdx.x; t < total; t += NT) {
-            const int i = owner_of(A.pref, k, t);
-            const int64_t ex = i ? A.pref[i - 1] : 0;
-            const int64_t slot = A.obeg[i] + (t - ex);
-            const int j = find_in(A.L, i + 1, k, col[slot]);
-            if (j >= 0) {
-                atomicOr(&A.M[i * W + (j >> 5)], 1u << (j & 31));
-                atomicOr(&A.M[j * W + (i >> 5)], 1u << (i & 31));
-            }
+        for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
+            int i[kU], j[kU];
+            int64_t slot[kU];
+            int32_t x[kU];
+            locate<NT>(A, k, total, t0, i, slot);
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                if (i[u] >= 0) x[u] = col[slot[u]];
+#pragma unroll
+            for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_in(A.L, i[u] + 1, k, x[u]) : -1;
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                if (j[u] >= 0) {
+                    atomicOr(&A.M[i[u] * W + (j[u] >> 5)], 1u << (j[u] & 31));
+                    atomicOr(&A.M[j[u] * W + (i[u] >> 5)], 1u << (i[u] & 31));
+                }
         }
         __syncthreads();
-        for (int64_t t = threadIdx.x; t < total; t += NT) {
-            const int i = owner_of(A.pref, k, t);
-            const int64_t ex = i ? A.pref[i - 1] : 0;
-            const int64_t slot = A.obeg[i] + (t - ex);
-            const int j = find_in(A.L, i + 1, k, col[slot]);
-            if (j >= 0) {
+        for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
+            int iu[kU], ju[kU];
+            int64_t su[kU];
+            int32_t x[kU];
+            locate<NT>(A, k, total, t0, iu, su);
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                if (iu[u] >= 0) x[u] = col[su[u]];
+#pragma unroll
+            for (int u = 0; u < kU; u++) ju[u] = iu[u] >= 0 ? find_in(A.L, iu[u] + 1, k, x[u]) : -1;
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int i = iu[u], j = ju[u];
+                if (j < 0) continue;
                 uint32_t kij = 0;
                 for (int w = 0; w < W; w++) kij += __popc(A.M[i * W + w] & A.M[j * W + w]);
-                const int32_t eij = eid[slot];
+                const int32_t eij = eid[su[u]];
                 const uint64_t tai = T[A.Le[i]], taj = T[A.Le[j]], tij = T[eij];
                 if (kij) {
                     atomicAdd((unsigned long long*)&K4[eij], (unsigned long long)kij);

@@ -198,25 +198,31 @@ __global__ void k_item_keys(const uint64_t* __restrict__ key, int64_t n, uint64_
     }
 }
 
-// the tiled count/accumulate kernel (persistent; items from the queue)
+// the tiled count/accumulate kernel (persistent; items from the queue).
+// Within a tile each wavefront takes 64 segments at a time, scans their
+// lengths in registers and walks the flattened wedges 64 lanes at a time
+// (consecutive lanes read consecutive slots of a list); the block only
+// synchronises between the count and the accumulate pass of a tile.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_wedge_tiled(
-    const int64_t* __restrict__ row, const int32_t* __restrict__ col, const int32_t* __restrict__ eid,
-    const int64_t* __restrict__ ostart, const int32_t* __restrict__ heavy, const int32_t* __restrict__ wcnt,
-    const uint64_t* __restrict__ tt, const int64_t* __restrict__ ttrow, const WedgeItem* __restrict__ items,
-    int64_t nitems, int log2r, WedgeSeg* __restrict__ segs, int64_t segcap, uint32_t* __restrict__ qctr,
-    uint64_t* __restrict__ P, uint64_t* __restrict__ PS, unsigned long long* __restrict__ work) {
+    const int64_t* __restrict__ row, const int32_t* __restrict__ col, const int64_t* __restrict__ ostart,
+    const int32_t* __restrict__ heavy, const int32_t* __restrict__ wcnt, const uint64_t* __restrict__ tt,
+    const int64_t* __restrict__ ttrow, const int32_t* __restrict__ hbase, const int64_t* __restrict__ htoff,
+    const uint32_t* __restrict__ htseg, const WedgeItem* __restrict__ items, int64_t nitems, int log2r,
+    WedgeSeg* __restrict__ segs, int64_t segcap, uint32_t* __restrict__ qctr, uint64_t* __restrict__ PS,
+    unsigned long long* __restrict__ work) {
+    constexpr int NW = NT / 64;
     extern __shared__ uint32_t cnt[];  // 2^log2r endpoint counters
     __shared__ uint32_t base[kTmax + 1];
     __shared__ uint32_t cur[kTmax];
-    __shared__ int64_t sstart[NT];
-    __shared__ int32_t pref[NT];
-    __shared__ int32_t seid[NT];
-    __shared__ unsigned long long psum[NT];
-    __shared__ int32_t part[NT];
+    __shared__ int64_t wstart[NW][64];
+    __shared__ int32_t wpref[NW][64];
+    __shared__ int32_t wslot[NW][64];
+    __shared__ unsigned long long wpsum[NW][64];
     __shared__ int64_t item_sh;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int R = 1 << log2r;
+    unsigned long long done = 0;
     WedgeSeg* seg = segs + (size_t)blockIdx.x * (size_t)segcap;
     for (;;) {
         if (tid == 0) item_sh = (int64_t)atomicAdd(qctr, 1u);
@@ -228,45 +234,37 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
         const int32_t a = heavy[W.hub];
         const int64_t ib = row[a], ie = ostart[a];
         const int T = W.t1 - W.t0;
-        for (int i = tid; i < T; i += NT) base[i] = 0;
-        __syncthreads();
-        // bucket the hub's segments in [t0, t1) by tile: count, scan, scatter
-        for (int walk = 0; walk < 2; walk++) {
-            for (int64_t s = ib + tid; s < ie; s += NT) {
-                const uint32_t w = (uint32_t)wcnt[s];
-                if (!w) continue;
-                const int32_t b = col[s];
-                const int64_t j1 = ttrow[b + 1];
-                int64_t j = W.t0 ? tt_seek(tt, ttrow[b], j1, (uint32_t)W.t0) : ttrow[b];
-                const uint32_t deg = (uint32_t)(row[b + 1] - row[b]);
-                const int32_t e_ab = walk ? eid[s] : 0;
-                for (; j < j1; j++) {
-                    const uint64_t e = tt[j];
-                    const uint32_t off = (uint32_t)e;
-                    const int t = (int)(e >> 32) - W.t0;
-                    if (off >= w || t >= T) break;
-                    if (walk == 0) {
-                        atomicAdd(&base[t], 1u);
-                    } else {
-                        const uint32_t nxt = j + 1 < j1 ? (uint32_t)tt[j + 1] : deg;
-                        const uint32_t p = atomicAdd(&cur[t], 1u);
-                        seg[p] = WedgeSeg{row[b] + off, (int32_t)(min(nxt, w) - off), e_ab};
-                    }
-                }
+        // bucket the hub's segments in [t0, t1) by tile: offsets from the plan's
+        // per-cell counts, then one walk scatters
+        if (tid == 0) {
+            const uint32_t* cells = htseg + htoff[W.hub] + (W.t0 - hbase[W.hub]);
+            uint32_t acc = 0;
+            for (int i = 0; i < T; i++) {
+                base[i] = acc;
+                cur[i] = acc;
+                acc += cells[i];
             }
-            __syncthreads();
-            if (walk == 0 && tid == 0) {
-                uint32_t acc = 0;
-                for (int i = 0; i < T; i++) {
-                    const uint32_t x = base[i];
-                    base[i] = acc;
-                    cur[i] = acc;
-                    acc += x;
-                }
-                base[T] = acc;
-            }
-            __syncthreads();
+            base[T] = acc;
         }
+        __syncthreads();
+        for (int64_t s = ib + tid; s < ie; s += NT) {
+            const uint32_t wc = (uint32_t)wcnt[s];
+            if (!wc) continue;
+            const int32_t b = col[s];
+            const int64_t j1 = ttrow[b + 1];
+            int64_t j = W.t0 ? tt_seek(tt, ttrow[b], j1, (uint32_t)W.t0) : ttrow[b];
+            const uint32_t deg = (uint32_t)(row[b + 1] - row[b]);
+            for (; j < j1; j++) {
+                const uint64_t e = tt[j];
+                const uint32_t off = (uint32_t)e;
+                const int t = (int)(e >> 32) - W.t0;
+                if (off >= wc || t >= T) break;
+                const uint32_t nxt = j + 1 < j1 ? (uint32_t)tt[j + 1] : deg;
+                const uint32_t p = atomicAdd(&cur[t], 1u);
+                seg[p] = WedgeSeg{row[b] + off, (int32_t)(min(nxt, wc) - off), (int32_t)s};
+            }
+        }
+        __syncthreads();
         // tile by tile: count the endpoints, then accumulate cnt-1 per wedge
         for (int t = 0; t < T; t++) {
             const uint32_t sb = base[t], se = base[t + 1];
@@ -275,42 +273,73 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
             for (int i = tid; i < R; i += NT) cnt[i] = 0;
             __syncthreads();
             for (int pass = 0; pass < 2; pass++) {
-                for (uint32_t cb = sb; cb < se; cb += NT) {
-                    const int kc = (int)min<uint32_t>(NT, se - cb);
-                    if (tid < kc) {
-                        const WedgeSeg g = seg[cb + tid];
-                        sstart[tid] = g.start;
-                        pref[tid] = g.len;
-                        seid[tid] = g.eid;
-                        psum[tid] = 0;
+                for (uint32_t gb = sb + (uint32_t)w * 64; gb < se; gb += NT) {
+                    const int kc = (int)min<uint32_t>(64, se - gb);
+                    int32_t len = 0, sl = 0;
+                    int64_t st = 0;
+                    if (lane < kc) {
+                        const WedgeSeg g = seg[gb + lane];
+                        len = g.len;
+                        st = g.start;
+                        sl = g.sab;
                     }
-                    __syncthreads();
-                    block_incl_scan<NT>(pref, kc, part);
-                    const int32_t total = pref[kc - 1];
-                    if (pass == 0 && tid == 0) atomicAdd(work, (unsigned long long)total);
-                    for (int32_t q = tid; q < total; q += NT) {
-                        const int o = owner_of(pref, kc, q);
-                        const int32_t ex = o ? pref[o - 1] : 0;
-                        const int64_t slot = sstart[o] + (q - ex);
-                        const int32_t c = col[slot] - c0;
-                        if (pass == 0) {
-                            atomicAdd(&cnt[c], 1u);
-                        } else {
-                            const uint32_t v = cnt[c] - 1;
-                            if (v) {
-                                atomicAdd((unsigned long long*)&PS[slot], (unsigned long long)v);
-                                atomicAdd(&psum[o], (unsigned long long)v);
+                    int32_t x = len;
+#pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const int32_t y = __shfl_up(x, off, 64);
+                        if (lane >= off) x += y;
+                    }
+                    wpref[w][lane] = x;
+                    wstart[w][lane] = st;
+                    wslot[w][lane] = sl;
+                    wpsum[w][lane] = 0;
+                    __builtin_amdgcn_wave_barrier();
+                    const int32_t total = __shfl(x, 63, 64);
+                    if (pass == 0 && lane == 0) done += (unsigned long long)total;
+                    // kU positions per lane per round: their owner searches and
+                    // endpoint loads are independent and overlap
+                    constexpr int kU = 4;
+                    for (int32_t q0 = lane; q0 < total; q0 += 64 * kU) {
+                        int o[kU];
+                        int64_t slot[kU];
+                        int32_t c[kU];
+#pragma unroll
+                        for (int u = 0; u < kU; u++) {
+                            const int32_t q = q0 + 64 * u;
+                            o[u] = -1;
+                            if (q < total) {
+                                o[u] = owner_of(wpref[w], kc, q);
+                                const int32_t ex = o[u] ? wpref[w][o[u] - 1] : 0;
+                                slot[u] = wstart[w][o[u]] + (q - ex);
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < kU; u++)
+                            if (o[u] >= 0) c[u] = col[slot[u]] - c0;
+#pragma unroll
+                        for (int u = 0; u < kU; u++) {
+                            if (o[u] < 0) continue;
+                            if (pass == 0) {
+                                atomicAdd(&cnt[c[u]], 1u);
+                            } else {
+                                const uint32_t v = cnt[c[u]] - 1;
+                                if (v) {
+                                    atomicAdd((unsigned long long*)&PS[slot[u]], (unsigned long long)v);
+                                    atomicAdd(&wpsum[w][o[u]], (unsigned long long)v);
+                                }
                             }
                         }
                     }
-                    __syncthreads();
-                    if (pass == 1 && tid < kc && psum[tid])
-                        atomicAdd((unsigned long long*)&P[seid[tid]], psum[tid]);
-                    __syncthreads();
+                    __builtin_amdgcn_wave_barrier();
+                    if (pass == 1 && lane < kc && wpsum[w][lane])
+                        atomicAdd((unsigned long long*)&PS[wslot[w][lane]], wpsum[w][lane]);
+                    __builtin_amdgcn_wave_barrier();
                 }
+                __syncthreads();
             }
         }
     }
+    if (done) atomicAdd(work, done);
 }
 
 // fold the per-slot accumulator into P by edge id
@@ -327,9 +356,9 @@ void launch_tiled(pgd_ctx* ctx, OrientState& st, unsigned long long* work) {
     WedgePlan& wp = st.wp;
     const size_t dyn = (size_t)4 << wp.log2r;
     hipLaunchKernelGGL(k_wedge_tiled<NT>, dim3(wp.grid), dim3(NT), dyn, ctx->stream, st.rrow.get(), st.rcol.get(),
-                       st.reid.get(), st.ostart.get(), st.b_heavy.get(), wp.wcnt.get(), wp.tt.get(), wp.ttrow.get(),
-                       wp.items.get(), wp.nitems, wp.log2r, wp.segs.get(), wp.segcap, wp.qctr.get(), st.P.get(),
-                       wp.ps.get(), work);
+                       st.ostart.get(), st.b_heavy.get(), wp.wcnt.get(), wp.tt.get(), wp.ttrow.get(), wp.hbase.get(),
+                       wp.htoff.get(), wp.htseg.get(), wp.items.get(), wp.nitems, wp.log2r, wp.segs.get(), wp.segcap,
+                       wp.qctr.get(), wp.ps.get(), work);
 }
 
 template <int NT>
@@ -354,6 +383,7 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     PGD_CHECK(prm.wedge_log2r >= 8 && prm.wedge_log2r <= 15, PGD_ERR_INVALID, "wedge tile must be 2^8..2^15");
     PGD_CHECK(prm.wedge_nt == 256 || prm.wedge_nt == 512 || prm.wedge_nt == 1024, PGD_ERR_INVALID,
               "wedge block size must be 256, 512 or 1024");
+    PGD_CHECK(nnz < INT32_MAX, PGD_ERR_UNSUPPORTED, "2m >= 2^31 slots: the wedge plan indexes slots as int32");
     wp.log2r = prm.wedge_log2r;
     // pair wedge counts
     ensure(wp.wcnt, ctx, nnz);

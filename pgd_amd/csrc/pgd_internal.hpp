@@ -120,8 +120,8 @@ struct OrientParams {
     int small_k = 64;       // out-degree up to which one wavefront owns a local graph
     int mid_k = 256;        // up to which a 256-thread block does (1024 threads above)
     int grid_small = 32;    // resident blocks per CU for the wavefront bins
-    int grid_mid = 8;       // resident blocks per CU for the 256-thread bin
-    int wedge_log2r = 15;   // endpoint tile of the heavy 4-cycle pass: 2^log2r LDS counters
+    int grid_mid = 16;      // resident blocks per CU for the 256-thread bin
+    int wedge_log2r = 13;   // endpoint tile of the heavy 4-cycle pass: 2^log2r LDS counters
     int wedge_nt = 512;     // threads per block of the heavy 4-cycle pass (256, 512 or 1024)
     int wedge_split = 4;    // work items per block the heavy wedges are cut into (load balance)
 };

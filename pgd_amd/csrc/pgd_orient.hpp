@@ -182,11 +182,11 @@ struct Timer {
 // ---------------------------------------------------------------------------
 // One segment: the run of N(b) (ascending CSR slots [start, start+len)) whose
 // endpoints c fall in one endpoint tile, for the in-neighbour b of a hub a;
-// eid is the edge id of (a, b).
+// sab is the slot of b in N(a) (the edge (a, b)).
 struct WedgeSeg {
     int64_t start;
     int32_t len;
-    int32_t eid;
+    int32_t sab;
 };
 
 // one unit of work of the tiled kernel: hub (index into the heavy list) and
