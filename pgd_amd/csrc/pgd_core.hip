@@ -917,7 +917,7 @@ pgd_status pgd_decompose(pgd_graph* g, uint32_t flags, uint64_t* sums) {
             if (engine == 1) engine_edge(ctx, v, c, &t);
             else if (engine == 2) engine_orient(ctx, v, c, &t, &g->orient, g->prm);
             else {
-                PGD_CHECK(engine_dense_ok(v), PGD_ERR_UNSUPPORTED, "graph too large for the dense engine");
+                PGD_CHECK(engine_dense_fits(v), PGD_ERR_UNSUPPORTED, "graph too large for the dense engine");
                 engine_dense(ctx, v, c, &t);
             }
         }

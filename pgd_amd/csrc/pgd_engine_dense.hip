@@ -1,8 +1,215 @@
-// pgd_engine_dense.hip -- dense-tier engine (placeholder until written).
-#include "pgd_internal.hpp"
+// pgd_engine_dense.hip -- dense-graph tier on the int8 matrix cores.
+//
+// For a dense graph the per-edge counts are contractions of the adjacency
+// matrix A (0/1, held as int8, zero-padded to N = 64*ceil(n/64)):
+//
+//   S    = A.A                  codegrees; T(e) = S_vu for the edge e = (v,u)
+//   A3   = A.S                  P(e) = A3_vu - d_v - d_u + 1 (4-cycles through e)
+//   AT   = A.T, T = S o A       Q(e) = AT_vu + AT_uv  (sum over triangles {v,u,w}
+//                               of T(v,w) + T(u,w))
+//   K4(e) = edges among the common neighbours of v and u (bit rows, popcounts)
+//   C4(e) = P - Q + 2T + 2K4    (the oriented engine's identity)
+//
+// Every product is an int8 x int8 -> int32 MFMA (v_mfma_i32_32x32x32_i8).
+// S and T are at most n-2 < 2^14, so they enter the second products as two
+// base-128 digits (each an int8 in [0, 127]); every int32 accumulator stays
+// below n * 127 * 1 < 2^31 and the digits recombine exactly in int64.  All
+// operands are symmetric, so both MFMA operands are read along rows.
+#include "pgd_orient.hpp"
+
 namespace pgd {
-bool engine_dense_ok(const GraphView&) { return false; }
-void engine_dense(pgd_ctx*, const GraphView&, EdgeCounts, Timing*) {
-    throw Error{PGD_ERR_UNSUPPORTED, "dense engine not built"};
+namespace {
+
+constexpr int kDenseMaxN = 8192;  // int32 products of N^2 entries, bit rows of N/32 words
+constexpr int kTile = 64;         // output tile of one block (2 x 2 waves of 32 x 32)
+
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__global__ void d_fill_u32(uint32_t* __restrict__ p, int64_t n) {
+    GRID_STRIDE(i, n) p[i] = 0;
 }
+
+// A (int8, N x N) and the bit rows (N x Wd words) from the CSR
+__global__ void k_dense_adj(const int64_t* __restrict__ row, const int32_t* __restrict__ col, int64_t n, int64_t N,
+                            int Wd, int8_t* __restrict__ A, uint32_t* __restrict__ bits) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t v = wave; v < n; v += nw)
+        for (int64_t p = row[v] + lane; p < row[v + 1]; p += 64) {
+            const int32_t u = col[p];
+            A[v * N + u] = 1;
+            atomicOr(&bits[v * Wd + (u >> 5)], 1u << (u & 31));
+        }
+}
+
+// C = A . X^T for N x N int8 row-major A and X (N a multiple of 64): one
+// block per 64 x 64 output tile, one wavefront per 32 x 32 quarter.  Lane l
+// feeds row (l mod 32) of its A rows and of its X rows, 16 bytes of k each
+// step (the same k positions for both operands).
+__global__ __launch_bounds__(256) void k_gemm_i8(const int8_t* __restrict__ A, const int8_t* __restrict__ X,
+                                                 int32_t* __restrict__ C, int64_t N) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t ti = (int64_t)blockIdx.y * kTile + (w >> 1) * 32;
+    const int64_t tj = (int64_t)blockIdx.x * kTile + (w & 1) * 32;
+    const int8_t* pa = A + (ti + (lane & 31)) * N + 16 * (lane >> 5);
+    const int8_t* px = X + (tj + (lane & 31)) * N + 16 * (lane >> 5);
+    v16i acc0 = {}, acc1 = {};
+    int64_t k = 0;
+    for (; k + 64 <= N; k += 64) {  // two independent accumulators in flight
+        const v4i a0 = *(const v4i*)(pa + k), x0 = *(const v4i*)(px + k);
+        const v4i a1 = *(const v4i*)(pa + k + 32), x1 = *(const v4i*)(px + k + 32);
+        acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, x0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, x1, acc1, 0, 0, 0);
+    }
+    for (; k < N; k += 32) {
+        const v4i a0 = *(const v4i*)(pa + k), x0 = *(const v4i*)(px + k);
+        acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, x0, acc0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int64_t i = ti + (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5), j = tj + (lane & 31);
+        C[i * N + j] = acc0[r] + acc1[r];
+    }
+}
+
+// base-128 digits of S and of T = S o A
+__global__ void k_dense_digits(const int32_t* __restrict__ S, const int8_t* __restrict__ A, int64_t NN,
+                               int8_t* __restrict__ s0, int8_t* __restrict__ s1, int8_t* __restrict__ t0,
+                               int8_t* __restrict__ t1) {
+    GRID_STRIDE(i, NN) {
+        const int32_t s = S[i];
+        const int32_t t = A[i] ? s : 0;
+        s0[i] = (int8_t)(s & 127);
+        s1[i] = (int8_t)(s >> 7);
+        t0[i] = (int8_t)(t & 127);
+        t1[i] = (int8_t)(t >> 7);
+    }
+}
+
+// per-edge counts; one wavefront per edge for the 4-clique popcounts
+__global__ void k_dense_edges(const int32_t* __restrict__ ev, const int32_t* __restrict__ eu,
+                              const int64_t* __restrict__ row, int64_t m, int64_t N, int Wd,
+                              const int32_t* __restrict__ S, const int32_t* __restrict__ P0,
+                              const int32_t* __restrict__ P1, const int32_t* __restrict__ Q0,
+                              const int32_t* __restrict__ Q1, const uint32_t* __restrict__ bits,
+                              uint32_t* __restrict__ T, uint64_t* __restrict__ K4, uint64_t* __restrict__ C4) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t e = wave; e < m; e += nw) {
+        const int64_t v = ev[e], u = eu[e];
+        const uint32_t* bv = bits + v * Wd;
+        const uint32_t* bu = bits + u * Wd;
+        // sum over common neighbours w of |N(w) & N(v) & N(u)|: each edge of
+        // the common neighbourhood is seen from both of its ends
+        uint64_t twice = 0;
+        for (int wd = lane; wd < Wd; wd += 64) {
+            uint32_t cw = bv[wd] & bu[wd];
+            while (cw) {
+                const int b = __ffs(cw) - 1;
+                cw &= cw - 1;
+                const uint32_t* bw = bits + ((int64_t)wd * 32 + b) * Wd;
+                uint32_t s = 0;
+                for (int x = 0; x < Wd; x++) s += __popc(bw[x] & bv[x] & bu[x]);
+                twice += s;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) twice += __shfl_xor(twice, off, 64);
+        if (lane == 0) {
+            const int64_t vu = v * N + u, uv = u * N + v;
+            const uint64_t t = (uint64_t)S[vu];
+            const uint64_t a3 = (uint64_t)P0[vu] + 128 * (uint64_t)P1[vu];
+            const uint64_t dv = (uint64_t)(row[v + 1] - row[v]), du = (uint64_t)(row[u + 1] - row[u]);
+            const uint64_t p = a3 - dv - du + 1;
+            const uint64_t q = (uint64_t)Q0[vu] + 128 * (uint64_t)Q1[vu] + (uint64_t)Q0[uv] + 128 * (uint64_t)Q1[uv];
+            const uint64_t k4 = twice >> 1;
+            T[e] = (uint32_t)t;
+            K4[e] = k4;
+            C4[e] = p - q + 2 * t + 2 * k4;
+        }
+    }
+}
+
+}  // namespace
+
+bool engine_dense_fits(const GraphView& g) { return g.n <= kDenseMaxN; }
+
+bool engine_dense_ok(const GraphView& g) {
+    // dense: at least a quarter of all pairs are edges, and the N x N
+    // products stay modest
+    return g.n >= 64 && g.n <= kDenseMaxN && 8 * g.m >= g.n * g.n;
+}
+
+void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm) {
+    PGD_CHECK(g.n <= kDenseMaxN, PGD_ERR_UNSUPPORTED, "graph too large for the dense engine");
+    hipStream_t s = ctx->stream;
+    const int64_t n = g.n, m = g.m;
+    const int64_t N = (n + kTile - 1) / kTile * kTile, NN = N * N;
+    const int Wd = (int)(N / 32);
+    DBuf<int8_t> A, s0, s1, t0, t1;
+    DBuf<int32_t> S, P0, P1, Q0, Q1;
+    DBuf<uint32_t> bits;
+    A.alloc(ctx, NN);
+    bits.alloc(ctx, N * Wd);
+    S.alloc(ctx, NN);
+    Timer timer{ctx};
+    timer.begin("k_dense_adj");
+    const int gf = grid_cap(ctx, (NN / 4 + 255) / 256, 16);
+    hipLaunchKernelGGL(d_fill_u32, dim3(gf), dim3(256), 0, s, (uint32_t*)A.get(), NN / 4);
+    hipLaunchKernelGGL(d_fill_u32, dim3(grid_cap(ctx, (N * Wd + 255) / 256, 16)), dim3(256), 0, s, bits.get(),
+                       N * Wd);
+    hipLaunchKernelGGL(k_dense_adj, dim3(grid_cap(ctx, (n * 64 + 255) / 256, 16)), dim3(256), 0, s, g.row, g.col, n,
+                       N, Wd, A.get(), bits.get());
+    PGD_HIP(hipGetLastError());
+    timer.end();
+    const dim3 gg((unsigned)(N / kTile), (unsigned)(N / kTile));
+    timer.begin("k_gemm_i8");
+    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, A.get(), A.get(), S.get(), N);
+    PGD_HIP(hipGetLastError());
+    s0.alloc(ctx, NN);
+    s1.alloc(ctx, NN);
+    t0.alloc(ctx, NN);
+    t1.alloc(ctx, NN);
+    hipLaunchKernelGGL(k_dense_digits, dim3(grid_cap(ctx, (NN + 255) / 256, 16)), dim3(256), 0, s, S.get(), A.get(),
+                       NN, s0.get(), s1.get(), t0.get(), t1.get());
+    P0.alloc(ctx, NN);
+    P1.alloc(ctx, NN);
+    Q0.alloc(ctx, NN);
+    Q1.alloc(ctx, NN);
+    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, A.get(), s0.get(), P0.get(), N);
+    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, A.get(), s1.get(), P1.get(), N);
+    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, A.get(), t0.get(), Q0.get(), N);
+    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, A.get(), t1.get(), Q1.get(), N);
+    PGD_HIP(hipGetLastError());
+    timer.end();
+    timer.begin("k_dense_edges");
+    if (m)
+        hipLaunchKernelGGL(k_dense_edges, dim3(grid_cap(ctx, (m * 64 + 255) / 256, 16)), dim3(256), 0, s, g.ev, g.eu,
+                           g.row, m, N, Wd, S.get(), P0.get(), P1.get(), Q0.get(), Q1.get(), bits.get(), out.T,
+                           out.K4, out.C4);
+    PGD_HIP(hipGetLastError());
+    timer.end();
+    PGD_HIP(hipStreamSynchronize(s));
+    // work models: the MFMA products are 5 x 2 N^3 int8 operations; the adjacency
+    // pass writes N^2 + N^2/8 bytes; the edge pass reads its five int32 entries
+    // and, per common neighbour, three bit rows
+    const double flops = 5.0 * 2.0 * (double)N * N * N;
+    tm->phases.clear();
+    double best = 0;
+    for (size_t i = 0; i < timer.ev.size(); i++) {
+        float ms = 0;
+        PGD_HIP(hipEventElapsedTime(&ms, timer.ev[i].first, timer.ev[i].second));
+        double bytes = 0, units = 0;
+        if (i == 0) { bytes = (double)NN + NN / 8.0; units = (double)NN; }
+        if (i == 1) { bytes = flops; units = 5; }  // for the MFMA pass: operations, not bytes
+        if (i == 2) { bytes = 28.0 * m; units = (double)m; }
+        tm->phases.push_back(Phase{timer.names[i], ms, bytes, units});
+        if (ms > best) best = ms;
+    }
+    tm->dominant_ms = best;
+    tm->dominant_launches = 1;
+}
+
 }  // namespace pgd

@@ -1,0 +1,101 @@
+"""Dense tier: per-edge counts from int8 MFMA products equal the sparse
+oriented engine's and the oracle's, bit for bit, on the dense configurations
+G(500, 0.9) and G(2000, 0.5) and on dense graphs of awkward sizes."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import pgd_amd
+from tests.oracle_lib import OracleGraph
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def both_engines(ctx, s, d, n, micro=True):
+    g = pgd_amd.GraphletCore(s, d, n=n, ctx=ctx)
+    g.sort_edges("deg")
+    out = {}
+    for eng in ("orient", "dense"):
+        sums = g.decompose(micro=micro, engine=eng)
+        assert g.last_engine() == eng
+        out[eng] = (sums, g.edge_counts(), [g.micro(c) for c in range(8)] if micro else None)
+    return g, out
+
+
+def assert_same(out):
+    a, b = out["orient"], out["dense"]
+    assert a[0] == b[0]
+    for x, y in zip(a[1], b[1]):
+        assert np.array_equal(x, y)
+    if a[2] is not None:
+        for x, y in zip(a[2], b[2]):
+            assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("n,p,seed", [(500, 0.9, 1), (2000, 0.5, 1)])
+def test_dense_configurations(n, p, seed, ctx):
+    say(f"[gpu] dense G({n},{p}) seed {seed}")
+    s, d = pgd_amd.gen_gnp(n, p, seed)
+    g, out = both_engines(ctx, s, d, n)
+    assert_same(out)
+    # left to itself the library picks the dense tier for these graphs
+    g.decompose(micro=False)
+    assert g.last_engine() == "dense"
+    # the oracle on a sample of edges
+    T, K4, C4 = out["dense"][1]
+    og = OracleGraph.from_edges(s, d, n)
+    og.order_neighbors("deg")
+    ev, eu = og.edge_arrays()
+    sample = np.sort(np.random.default_rng(seed).choice(len(ev), size=min(len(ev), 1500), replace=False))
+    res = og.decompose(ev, eu, sample.astype(np.int32), micro=True)
+    assert np.array_equal(T[sample], res["tri"])
+    assert np.array_equal(K4[sample], res["k4"])
+    assert np.array_equal(C4[sample], res["c4"])
+    g.free()
+
+
+def complete(n):
+    i, j = np.triu_indices(n, 1)
+    return i.astype(np.int32), j.astype(np.int32)
+
+
+@pytest.mark.parametrize("case", ["k64", "k100", "gnp65", "gnp130_0.7", "gnp777_0.3", "gnp4000_0.3", "sparse_ba"])
+def test_dense_matches_sparse(case, ctx):
+    say(f"[gpu] dense vs sparse {case}")
+    if case == "k64":
+        n, (s, d) = 64, complete(64)
+    elif case == "k100":
+        n, (s, d) = 100, complete(100)
+    elif case == "gnp65":
+        n, (s, d) = 65, pgd_amd.gen_gnp(65, 0.5, 3)
+    elif case == "sparse_ba":
+        n, (s, d) = 3000, pgd_amd.gen_ba(3000, 3, 8)    # requested explicitly on a sparse graph
+    else:
+        _, nn, pp = case.split("_")[0][:3], int(case.split("_")[0][3:]), float(case.split("_")[1])
+        n, (s, d) = nn, pgd_amd.gen_gnp(nn, pp, 5)
+    g, out = both_engines(ctx, s, d, n, micro=(case != "gnp4000_0.3"))
+    assert_same(out)
+    if case.startswith("k"):
+        # closed forms of the complete graph K_n, per edge
+        T, K4, C4 = out["dense"][1]
+        assert np.all(T == n - 2)
+        assert np.all(K4 == (n - 2) * (n - 3) // 2)
+        assert np.all(C4 == 0)
+    g.free()
+
+
+def test_dense_rejects_large(ctx):
+    s, d = pgd_amd.gen_ba(9000, 2, 1)
+    g = pgd_amd.GraphletCore(s, d, ctx=ctx)
+    g.sort_edges("deg")
+    with pytest.raises(pgd_amd.PGDError) as e:
+        g.decompose(engine="dense")
+    assert e.value.code == -5
+    g.decompose()
+    assert g.last_engine() == "orient"
+    g.free()
