@@ -530,6 +530,7 @@ pgd_status pgd_open(int32_t device, pgd_ctx** out) {
         hipDeviceProp_t prop;
         PGD_HIP(hipGetDeviceProperties(&prop, device));
         ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        ctx->lds_per_block = prop.sharedMemPerBlock;
         PGD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         // keep released blocks in the device's pool instead of returning them
         // to the driver at every synchronisation

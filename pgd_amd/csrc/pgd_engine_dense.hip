@@ -88,35 +88,52 @@ __global__ void k_dense_digits(const int32_t* __restrict__ S, const int8_t* __re
     }
 }
 
-// per-edge counts; one wavefront per edge for the 4-clique popcounts
-__global__ void k_dense_edges(const int32_t* __restrict__ ev, const int32_t* __restrict__ eu,
-                              const int64_t* __restrict__ row, int64_t m, int64_t N, int Wd,
-                              const int32_t* __restrict__ S, const int32_t* __restrict__ P0,
-                              const int32_t* __restrict__ P1, const int32_t* __restrict__ Q0,
-                              const int32_t* __restrict__ Q1, const uint32_t* __restrict__ bits,
-                              uint32_t* __restrict__ T, uint64_t* __restrict__ K4, uint64_t* __restrict__ C4) {
+// per-edge counts; one wavefront per edge.  The 4-cliques on e = (v,u) are
+// the edges among C = N(v) & N(u): lane l keeps words l, l+64, ... of C in
+// registers, the wavefront walks the members w of C in order (a uniform
+// loop), and each lane adds popcount(N(w) & C) over its words: every edge of
+// C is seen from both ends.
+constexpr int kMaxWordsPerLane = kDenseMaxN / 32 / 64;
+
+__global__ __launch_bounds__(256) void k_dense_edges(
+    const int32_t* __restrict__ ev, const int32_t* __restrict__ eu, const int64_t* __restrict__ row, int64_t m,
+    int64_t N, int Wd, const int32_t* __restrict__ S, const int32_t* __restrict__ P0, const int32_t* __restrict__ P1,
+    const int32_t* __restrict__ Q0, const int32_t* __restrict__ Q1, const uint32_t* __restrict__ bits,
+    uint32_t* __restrict__ T, uint64_t* __restrict__ K4, uint64_t* __restrict__ C4) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int per = (Wd + 63) / 64;  // words of C per lane (<= kMaxWordsPerLane)
     for (int64_t e = wave; e < m; e += nw) {
         const int64_t v = ev[e], u = eu[e];
         const uint32_t* bv = bits + v * Wd;
         const uint32_t* bu = bits + u * Wd;
-        // sum over common neighbours w of |N(w) & N(v) & N(u)|: each edge of
-        // the common neighbourhood is seen from both of its ends
-        uint64_t twice = 0;
-        for (int wd = lane; wd < Wd; wd += 64) {
-            uint32_t cw = bv[wd] & bu[wd];
-            while (cw) {
-                const int b = __ffs(cw) - 1;
-                cw &= cw - 1;
-                const uint32_t* bw = bits + ((int64_t)wd * 32 + b) * Wd;
-                uint32_t s = 0;
-                for (int x = 0; x < Wd; x++) s += __popc(bw[x] & bv[x] & bu[x]);
-                twice += s;
+        uint32_t c[kMaxWordsPerLane];
+#pragma unroll
+        for (int q = 0; q < kMaxWordsPerLane; q++) {
+            const int x = lane + 64 * q;
+            c[q] = (q < per && x < Wd) ? (bv[x] & bu[x]) : 0u;
+        }
+        uint32_t twice = 0;
+        for (int j = 0; j < Wd; j++) {
+            // word j of C, broadcast from its lane (uniform across the wavefront)
+            uint32_t cj = 0;
+#pragma unroll
+            for (int q = 0; q < kMaxWordsPerLane; q++)
+                if (q == j / 64) cj = __shfl(c[q], j % 64, 64);
+            while (cj) {
+                const int b = __ffs(cj) - 1;
+                cj &= cj - 1;
+                const uint32_t* bw = bits + ((int64_t)j * 32 + b) * Wd;
+#pragma unroll
+                for (int q = 0; q < kMaxWordsPerLane; q++) {
+                    const int x = lane + 64 * q;
+                    if (q < per && x < Wd) twice += __popc(bw[x] & c[q]);
+                }
             }
         }
-        for (int off = 32; off > 0; off >>= 1) twice += __shfl_xor(twice, off, 64);
+        uint64_t tw = twice;
+        for (int off = 32; off > 0; off >>= 1) tw += __shfl_xor(tw, off, 64);
         if (lane == 0) {
             const int64_t vu = v * N + u, uv = u * N + v;
             const uint64_t t = (uint64_t)S[vu];
@@ -124,7 +141,7 @@ __global__ void k_dense_edges(const int32_t* __restrict__ ev, const int32_t* __r
             const uint64_t dv = (uint64_t)(row[v + 1] - row[v]), du = (uint64_t)(row[u + 1] - row[u]);
             const uint64_t p = a3 - dv - du + 1;
             const uint64_t q = (uint64_t)Q0[vu] + 128 * (uint64_t)Q1[vu] + (uint64_t)Q0[uv] + 128 * (uint64_t)Q1[uv];
-            const uint64_t k4 = twice >> 1;
+            const uint64_t k4 = tw >> 1;
             T[e] = (uint32_t)t;
             K4[e] = k4;
             C4[e] = p - q + 2 * t + 2 * k4;

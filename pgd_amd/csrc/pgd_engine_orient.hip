@@ -499,9 +499,7 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
     PGD_HIP(hipGetLastError());
     st.maxk = device_max(ctx, st.outdeg.get(), (size_t)n);
     // the largest local graph whose arrays and bit matrix fit one block's LDS
-    hipDeviceProp_t prop;
-    PGD_HIP(hipGetDeviceProperties(&prop, ctx->device));
-    const size_t lds_cap = std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024) - 8192;
+    const size_t lds_cap = std::min<size_t>(ctx->lds_per_block, 160 * 1024) - 8192;
     int kl = 32;
     while (carve_bytes(kl + 32, (kl + 32 + 31) / 32) <= lds_cap) kl += 32;
     st.kmax_lds = std::max(kl, prm.mid_k);

@@ -363,11 +363,14 @@ void launch_tiled(pgd_ctx* ctx, OrientState& st, unsigned long long* work) {
 
 template <int NT>
 int tiled_blocks_per_cu(pgd_ctx* ctx, int log2r) {
+    int& slot = ctx->tiled_per_cu[NT == 256 ? 0 : NT == 512 ? 1 : 2][log2r - 8];
+    if (slot) return slot;  // the attribute and the occupancy are per process and device
     const size_t dyn = (size_t)4 << log2r;
     PGD_HIP(hipFuncSetAttribute((const void*)k_wedge_tiled<NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)dyn));
     int nb = 0;
     PGD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_wedge_tiled<NT>, NT, dyn));
+    slot = nb;
     return nb;
 }
 
@@ -446,7 +449,10 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     else per_cu = tiled_blocks_per_cu<1024>(ctx, wp.log2r);
     PGD_CHECK(per_cu > 0, PGD_ERR_INVALID, "wedge tile and block size exceed the LDS of a CU");
     wp.grid = ctx->num_cus * per_cu;
-    wp.segcap = (int64_t)std::max<uint64_t>(maxcell, std::min<uint64_t>(segtot, (uint64_t)1 << 18));
+    // per-block segment buffer: the largest cell, or an even share of all
+    // segments (items are cut to fit it), at most 2^18
+    wp.segcap = (int64_t)std::max<uint64_t>(
+        maxcell, std::min<uint64_t>((uint64_t)1 << 18, 2 * (segtot + (uint64_t)wp.grid - 1) / (uint64_t)wp.grid));
     wp.segcap = std::max<int64_t>(wp.segcap, 1);
     const uint64_t wcap =
         std::max<uint64_t>((uint64_t)wp.wedges / ((uint64_t)wp.grid * std::max(1, prm.wedge_split)), 1 << 16);

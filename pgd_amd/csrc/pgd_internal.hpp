@@ -39,6 +39,8 @@ struct pgd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cus = 256;
+    size_t lds_per_block = 65536;  // hipDeviceProp_t::sharedMemPerBlock, read once at open
+    int tiled_per_cu[3][8] = {};   // occupancy of the tiled wedge kernel by (block size, tile), 0 = not asked yet
     int64_t mem_cap = 0;   // 0: no cap
     int64_t mem_used = 0;
     std::string last_err;
