@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0   # dense int8 MFMA, MI355X
 
 
 def log(*a):
@@ -133,6 +134,123 @@ def cpu_baseline(g, target_s: float, threads: int) -> dict:
     }
 
 
+def bench_dense(pgd_amd, ctx) -> dict:
+    """Dense tier on G(500, 0.9) and G(2000, 0.5): device time of the int8
+    MFMA engine and of the sparse oriented engine on the same graph, and the
+    MFMA products' achieved int8 rate (5 products of 2 N^3 operations each)."""
+    out = {}
+    for n, p in ((500, 0.9), (2000, 0.5)):
+        s, d = pgd_amd.gen_gnp(n, p, 1)
+        g = pgd_amd.GraphletCore(s, d, n=n, ctx=ctx)
+        g.sort_edges("deg")
+        res = {}
+        for eng in ("dense", "orient"):
+            g.decompose(micro=True, engine=eng)  # warm-up (allocations, plan)
+            ms, gemm = [], []
+            for _ in range(3):
+                g.decompose(micro=True, engine=eng)
+                ms.append(g.last_timing()[0])
+                ph = {q["name"]: q for q in g.last_phases()}
+                if "k_gemm_i8" in ph:
+                    gemm.append((ph["k_gemm_i8"]["ms"], ph["k_gemm_i8"]["bytes"]))
+            res[eng] = float(np.median(ms))
+            if gemm:
+                gms, ops = sorted(gemm)[len(gemm) // 2]
+                res["mfma"] = {"bound": "mfma", "achieved": ops / (gms / 1e3) / 1e12, "peak": INT8_PEAK_TOPS,
+                               "unit": "TOP/s", "frac": ops / (gms / 1e3) / 1e12 / INT8_PEAK_TOPS,
+                               "ops": ops, "ms": gms}
+        out[f"gnp_{n}_{p}_seed1"] = {"m": g.m, "dense_ms": res["dense"], "orient_ms": res["orient"],
+                                     "edges_per_s": g.m / (res["dense"] / 1e3), "mfma": res.get("mfma")}
+        g.free()
+        log(f"[bench] dense G({n},{p}): dense {res['dense']:.2f} ms, sparse {res['orient']:.2f} ms")
+    return out
+
+
+def bench_batched(pgd_amd, ctx, count: int = 8192) -> dict:
+    """Batched form on the batched workload (8192 graphs): one captured
+    launch sequence replayed, against plain one-graph-per-launch on a sample."""
+    from pgd_amd.workloads import batched_graphs
+    graphs = batched_graphs(count, seed=1)
+    t0 = time.perf_counter()
+    b = pgd_amd.Batch(graphs, ctx=ctx, micro=True)
+    t_create = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s0, _ = b.run()
+    t_first = time.perf_counter() - t0
+    best, tune_ms = b.autotune(trials=2)
+    reps = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        s, _ = b.run()
+        reps.append((time.perf_counter() - t0, b.last_ms()))
+        assert np.array_equal(s, s0)
+    wall, dev = sorted(reps)[1]
+    idx = list(range(0, count, max(1, count // 256)))
+    t0 = time.perf_counter()
+    for i in idx:
+        n, src, dst = graphs[i]
+        g = pgd_amd.GraphletCore(src, dst, n=n, ctx=ctx)
+        g.sort_edges("deg")
+        sm = g.decompose(micro=True)
+        assert [int(x) for x in s0[i]] == sm, i
+        g.free()
+    plain_gps = len(idx) / (time.perf_counter() - t0)
+    M = b.m_total
+    out = {"graphs": count, "edges": M, "vertices": b.n_total,
+           "workload": "batched: BA(m in [2,8]) / G(n,p) avg degree [4,32], n log-uniform [20,5000], seed 1",
+           "create_s": t_create, "first_run_s": t_first, "replay_ms": dev, "replay_wall_s": wall,
+           "graphs_per_s": count / wall, "edges_per_s": M / wall,
+           "one_graph_per_launch_graphs_per_s": plain_gps, "one_graph_per_launch_sample": len(idx),
+           "speedup_vs_one_graph_per_launch": (count / wall) / plain_gps,
+           "end_to_end_graphs_per_s": count / (t_create + t_first),
+           "tuning": best.as_dict(), "autotuned_ms": tune_ms}
+    b.free()
+    log(f"[bench] batched: {out['graphs_per_s']:.0f} graphs/s vs {plain_gps:.0f} one-graph-per-launch")
+    return out
+
+
+def bench_served(pgd_amd, ctx, rate: float = 2000.0, seconds: float = 4.0, large_frac: float = 0.01) -> dict:
+    """Served form: Poisson arrivals of batched-workload graphs mixed with
+    R-MAT scale-18 graphs, per-edge results kept, counted by the server."""
+    from pgd_amd.workloads import batched_graph
+    pool = [batched_graph(i, seed=9)[:3] for i in range(1500)]
+    large = []
+    for sd in (1, 2):
+        s, d = pgd_amd.gen_rmat(18, 16, 0.57, 0.19, 0.19, sd)
+        large.append((int(max(s.max(), d.max())) + 1, s, d))
+    srv = pgd_amd.Server(ctx=ctx, hbm_budget=96 << 30)
+    rng = np.random.default_rng(1)
+    t0 = time.perf_counter()
+    t, k, nl = 0.0, 0, 0
+    while t < seconds:
+        t += rng.exponential(1.0 / rate)
+        now = time.perf_counter() - t0
+        if t > now:
+            time.sleep(t - now)
+        if rng.random() < large_frac:
+            g = large[nl % 2]
+            nl += 1
+        else:
+            g = pool[k % len(pool)]
+        srv.submit(*g, micro=True)
+        k += 1
+        if k % 256 == 0:
+            for rid in srv.poll():
+                srv.release(rid)
+    t_sub = time.perf_counter() - t0
+    srv.drain(600000)
+    for rid in srv.poll():
+        srv.release(rid)
+    st = srv.stats()
+    srv.free()
+    out = {"offered_req_per_s": rate, "large_fraction": large_frac, "submitted": k, "large": nl,
+           "submit_s": t_sub, "graphs_per_s": st["finished"] / st["elapsed_s"],
+           "edges_per_s": st["edges"] / st["elapsed_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+           "batches": st["batches"], "elapsed_s": st["elapsed_s"]}
+    log(f"[bench] served: {out['graphs_per_s']:.0f} graphs/s p50 {out['p50_ms']:.1f} ms p99 {out['p99_ms']:.1f} ms")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +261,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sections", default="dense,batched,served",
+                    help="extra measurements after the headline: any of dense,batched,served (or none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (scale for the PMC byte counters)")
     args = ap.parse_args()
@@ -212,6 +332,12 @@ def main() -> int:
         "survey_model": {"bytes": B, "achieved_GBs": B / dt / 1e9, "frac_of_hbm_peak": B / dt / 1e9 / HBM_PEAK_GBS},
         "totals": dict(zip(pgd_amd.TOTAL_NAMES, totals)),
     }
+    for sec, fn in (("dense", bench_dense), ("batched", bench_batched), ("served", bench_served)):
+        if sec in args.sections.split(","):
+            try:
+                out[sec] = fn(pgd_amd, ctx)
+            except Exception as e:  # a section must not hide the headline
+                out[sec] = {"error": repr(e)}
     if not args.no_cpu_baseline:
         try:
             threads = min(os.cpu_count() or 1, 16)

@@ -483,7 +483,10 @@ struct Request {
     int64_t m = 0;
     uint64_t sums[PGD_NSUMS] = {};
     uint64_t totals[PGD_NTOTALS] = {};
-    std::vector<uint64_t> micro;  // PGD_NMICRO x m when requested
+    // micro columns when requested: kept on the device in the batch's block
+    // (column c at block + c * stride + offset), shared by the batch's requests
+    std::shared_ptr<DBuf<uint64_t>> micro;
+    int64_t micro_stride = 0, micro_offset = 0;
 };
 
 // launch the counting of a built batch without capture (one use)
@@ -496,7 +499,8 @@ void batch_launch_eager(pgd_batch* b) {
     batch_count(b);
 }
 
-// wait for the batch's stream; per-request results to the host
+// wait for the batch's stream; per-request sums and totals to the host, and
+// the micro columns handed to the requests on the device
 void batch_collect(pgd_batch* b, std::vector<Request*>& reqs) {
     pgd_ctx* ctx = b->ctx;
     hipStream_t s = ctx->stream;
@@ -505,74 +509,95 @@ void batch_collect(pgd_batch* b, std::vector<Request*>& reqs) {
     PGD_HIP(hipMemcpyAsync(b->host_sums.data(), b->sums.get(), b->host_sums.size() * sizeof(uint64_t),
                            hipMemcpyDeviceToHost, s));
     PGD_HIP(hipMemcpyAsync(ctl, b->ctl.get(), sizeof(ctl), hipMemcpyDeviceToHost, s));
-    std::vector<uint64_t> micro;
-    if ((b->flags & PGD_MICRO) && b->M) {
-        micro.resize((size_t)PGD_NMICRO * b->M);
-        PGD_HIP(hipMemcpyAsync(micro.data(), b->U->micro.get(), micro.size() * sizeof(uint64_t),
-                               hipMemcpyDeviceToHost, s));
-    }
     PGD_HIP(hipStreamSynchronize(s));
     PGD_CHECK(!(b->flags & PGD_MICRO) || ctl[0] == 0, PGD_ERR_UNSUPPORTED,
               "a per-edge micro value exceeds 2^53; the exact double emulation for it is not implemented");
+    std::shared_ptr<DBuf<uint64_t>> block;
+    if ((b->flags & PGD_MICRO) && b->M) {
+        block = std::make_shared<DBuf<uint64_t>>();
+        std::swap(block->ctx, b->U->micro.ctx);
+        std::swap(block->p, b->U->micro.p);
+        std::swap(block->n, b->U->micro.n);
+    }
     for (int32_t g = 0; g < b->ng; g++) {
         Request* r = reqs[g];
         r->m = b->m[g];
         std::copy(b->host_sums.begin() + (size_t)g * PGD_NSUMS, b->host_sums.begin() + (size_t)(g + 1) * PGD_NSUMS,
                   r->sums);
         check_status(ctx, pgd_derive(b->n[g], b->m[g], r->sums, r->totals), "derive");
-        if (b->flags & PGD_MICRO) {
-            r->micro.resize((size_t)PGD_NMICRO * r->m);
-            for (int c = 0; c < PGD_NMICRO; c++)
-                std::copy(micro.begin() + (size_t)c * b->M + b->eoff[g],
-                          micro.begin() + (size_t)c * b->M + b->eoff[g] + r->m, r->micro.begin() + (size_t)c * r->m);
+        if (block) {
+            r->micro = block;
+            r->micro_stride = b->M;
+            r->micro_offset = b->eoff[g];
         }
     }
 }
 
 }  // namespace
 
+// Requests are served by two workers by size class (small graphs; graphs of
+// kLargeLines lines or more), so a large graph never holds small requests
+// back; each worker alternates two private streams, building one batch while
+// the previous one counts.  Both share the HBM budget and the statistics.
+constexpr int64_t kLargeLines = (int64_t)1 << 18;
+
+struct ServerWorker {
+    pgd_ctx* lane[2] = {nullptr, nullptr};  // private contexts: one stream each
+    std::deque<Request*> pending;
+    std::thread th;
+};
+
 struct pgd_server {
     pgd_ctx* owner = nullptr;
-    pgd_ctx* lane[2] = {nullptr, nullptr};  // private contexts: one stream each
+    ServerWorker cls[2];
     int64_t budget = 0;
     int32_t max_inflight = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<Request*> pending;
     std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
     std::vector<int64_t> unreported;
     int64_t next_id = 0, inflight_bytes = 0, inflight_reqs = 0, outstanding = 0;
     bool stop = false;
-    std::thread worker;
     // statistics
     std::vector<double> latencies;
     int64_t finished = 0, edges = 0, batches = 0;
     bool any_submit = false;
     Clock::time_point first_submit, last_done;
 
-    void run();
-    // take queued requests for one batch (caller holds mu)
-    std::vector<Request*> admit();
+    void run(int c);
+    // the head of class c's queue can be admitted now (caller holds mu)
+    bool head_admissible(int c) const;
+    // take queued requests of class c for one batch (caller holds mu)
+    std::vector<Request*> admit(int c);
 };
 
-std::vector<Request*> pgd_server::admit() {
+bool pgd_server::head_admissible(int c) const {
+    const std::deque<Request*>& q = cls[c].pending;
+    if (q.empty()) return false;
+    if (inflight_reqs == 0) return true;
+    return inflight_reqs < max_inflight && inflight_bytes + q.front()->footprint <= budget;
+}
+
+std::vector<Request*> pgd_server::admit(int c) {
+    std::deque<Request*>& q = cls[c].pending;
     std::vector<Request*> out;
     int64_t bytes = 0;
-    while (!pending.empty()) {
-        Request* r = pending.front();
+    while (!q.empty()) {
+        Request* r = q.front();
+        const bool idle = out.empty() && inflight_reqs == 0;  // the head alone always fits an empty device
         if (!out.empty() && r->flags != out[0]->flags) break;
-        if ((int64_t)out.size() + inflight_reqs >= max_inflight && !(out.empty() && inflight_reqs == 0)) break;
-        if (inflight_bytes + bytes + r->footprint > budget && !(out.empty() && inflight_reqs == 0)) break;
+        if ((int64_t)out.size() + inflight_reqs >= max_inflight && !idle) break;
+        if (inflight_bytes + bytes + r->footprint > budget && !idle) break;
         out.push_back(r);
         bytes += r->footprint;
-        pending.pop_front();
+        q.pop_front();
     }
     inflight_bytes += bytes;
     inflight_reqs += (int64_t)out.size();
     return out;
 }
 
-void pgd_server::run() {
+void pgd_server::run(int c) {
     struct InFlight {
         std::unique_ptr<pgd_batch> b;
         std::vector<Request*> reqs;
@@ -609,7 +634,7 @@ void pgd_server::run() {
             f.status = batch_guard(f.b->ctx, [&] { batch_collect(f.b.get(), f.reqs); });
             if (f.status != PGD_OK) f.err = f.b->ctx->last_err;
         }
-        pgd_ctx* lctx = f.b ? f.b->ctx : lane[0];
+        pgd_ctx* lctx = f.b ? f.b->ctx : cls[c].lane[0];
         if (f.b) (void)hipStreamSynchronize(lctx->stream);  // nothing of it still runs
         f.b.reset();
         // a failed batch of several requests: count each alone, so that one
@@ -650,21 +675,22 @@ void pgd_server::run() {
         inflight_bytes -= f.bytes;
         inflight_reqs -= (int64_t)f.reqs.size();
         cv_done.notify_all();
+        cv_work.notify_all();  // the budget freed may admit the other class's head
     };
     for (;;) {
         std::vector<Request*> batch;
         {
             std::unique_lock<std::mutex> lk(mu);
-            if (!prev) cv_work.wait(lk, [&] { return stop || !pending.empty(); });
-            if (stop && pending.empty() && !prev) return;
-            batch = admit();
+            if (!prev) cv_work.wait(lk, [&] { return (stop && cls[c].pending.empty()) || head_admissible(c); });
+            if (stop && cls[c].pending.empty() && !prev) return;
+            batch = admit(c);
         }
         std::unique_ptr<InFlight> cur;
         if (!batch.empty()) {
             cur.reset(new InFlight());
             cur->reqs = batch;
             for (Request* r : batch) cur->bytes += r->footprint;
-            pgd_ctx* ctx = lane[turn];
+            pgd_ctx* ctx = cls[c].lane[turn];
             turn ^= 1;
             cur->b.reset(new pgd_batch());
             cur->status = start(cur->b.get(), ctx, batch);  // counting runs while the previous batch is collected
@@ -692,20 +718,31 @@ pgd_status pgd_server_create(pgd_ctx* ctx, int64_t hbm_budget_bytes, int32_t max
     s->owner = ctx;
     s->budget = hbm_budget_bytes;
     s->max_inflight = max_inflight;
-    for (int i = 0; i < 2; i++) {
-        pgd_status st = pgd_open(ctx->device, &s->lane[i]);
-        if (st != PGD_OK) {
-            ctx->last_err = "cannot open a private stream for the server";
-            for (int j = 0; j < i; j++) pgd_close(s->lane[j]);
-            return st;
+    for (int c = 0; c < 2; c++)
+        for (int i = 0; i < 2; i++) {
+            pgd_status st = pgd_open(ctx->device, &s->cls[c].lane[i]);
+            if (st != PGD_OK) {
+                ctx->last_err = "cannot open a private stream for the server";
+                for (int c2 = 0; c2 < 2; c2++)
+                    for (int j = 0; j < 2; j++)
+                        if (s->cls[c2].lane[j]) pgd_close(s->cls[c2].lane[j]);
+                return st;
+            }
+            pgd_set_mem_cap(s->cls[c].lane[i], ctx->mem_cap);
         }
-        pgd_set_mem_cap(s->lane[i], ctx->mem_cap);
-    }
     pgd_server* p = s.get();
     try {
-        s->worker = std::thread([p] { p->run(); });
+        for (int c = 0; c < 2; c++) s->cls[c].th = std::thread([p, c] { p->run(c); });
     } catch (...) {
-        for (int i = 0; i < 2; i++) pgd_close(s->lane[i]);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        for (int c = 0; c < 2; c++)
+            if (s->cls[c].th.joinable()) s->cls[c].th.join();
+        for (int c = 0; c < 2; c++)
+            for (int i = 0; i < 2; i++) pgd_close(s->cls[c].lane[i]);
         return PGD_ERR_OOM;
     }
     *out = s.release();
@@ -737,11 +774,11 @@ pgd_status pgd_server_submit(pgd_server* s, int32_t n, int64_t nlines, const int
             s->any_submit = true;
             s->first_submit = r->t_submit;
         }
-        s->pending.push_back(r.get());
+        s->cls[nlines >= kLargeLines ? 1 : 0].pending.push_back(r.get());
         s->outstanding++;
         s->reqs.emplace(r->id, std::move(r));
     }
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();
     return PGD_OK;
 }
 
@@ -787,12 +824,24 @@ pgd_status pgd_server_result(pgd_server* s, int64_t req_id, uint64_t* sums, uint
 
 pgd_status pgd_server_micro(pgd_server* s, int64_t req_id, int32_t column, uint64_t* out) {
     if (!s || !out || column < 0 || column >= PGD_NMICRO) return PGD_ERR_INVALID;
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(req_id);
-    if (it == s->reqs.end()) return PGD_ERR_INVALID;
-    Request* r = it->second.get();
-    if (!r->done || r->status != PGD_OK || !(r->flags & PGD_MICRO)) return PGD_ERR_STATE;
-    std::copy(r->micro.begin() + (size_t)column * r->m, r->micro.begin() + (size_t)(column + 1) * r->m, out);
+    std::shared_ptr<DBuf<uint64_t>> block;
+    int64_t off = 0, m = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(req_id);
+        if (it == s->reqs.end()) return PGD_ERR_INVALID;
+        Request* r = it->second.get();
+        if (!r->done || r->status != PGD_OK || !(r->flags & PGD_MICRO)) return PGD_ERR_STATE;
+        block = r->micro;
+        off = column * r->micro_stride + r->micro_offset;
+        m = r->m;
+    }
+    if (!m) return PGD_OK;
+    hipError_t e = hipMemcpy(out, block->get() + off, m * sizeof(uint64_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        s->owner->last_err = std::string("hipMemcpy: ") + hipGetErrorString(e);
+        return PGD_ERR_HIP;
+    }
     return PGD_OK;
 }
 
@@ -834,11 +883,12 @@ pgd_status pgd_server_free(pgd_server* s) {
         s->stop = true;
     }
     s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
-    for (auto& kv : s->reqs) (void)kv;
+    for (int c = 0; c < 2; c++)
+        if (s->cls[c].th.joinable()) s->cls[c].th.join();
     s->reqs.clear();
-    for (int i = 0; i < 2; i++)
-        if (s->lane[i]) pgd_close(s->lane[i]);
+    for (int c = 0; c < 2; c++)
+        for (int i = 0; i < 2; i++)
+            if (s->cls[c].lane[i]) pgd_close(s->cls[c].lane[i]);
     delete s;
     return PGD_OK;
 }

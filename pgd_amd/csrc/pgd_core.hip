@@ -30,8 +30,8 @@ using namespace pgd;
 namespace pgd {
 
 void* dmalloc(pgd_ctx* ctx, size_t bytes) {
-    if (ctx->mem_cap > 0 && ctx->mem_used + (int64_t)bytes > ctx->mem_cap)
-        throw Error{PGD_ERR_OOM, "device memory cap exceeded (" + std::to_string(ctx->mem_used + bytes) +
+    if (ctx->mem_cap > 0 && ctx->mem_used.load() + (int64_t)bytes > ctx->mem_cap)
+        throw Error{PGD_ERR_OOM, "device memory cap exceeded (" + std::to_string(ctx->mem_used.load() + bytes) +
                                      " > " + std::to_string(ctx->mem_cap) + " bytes)"};
     // stream-ordered: allocation and release follow the context's stream, so
     // freeing a buffer never waits for work on other streams
@@ -567,7 +567,7 @@ pgd_status pgd_set_mem_cap(pgd_ctx* ctx, int64_t bytes) {
 
 pgd_status pgd_mem_in_use(pgd_ctx* ctx, int64_t* bytes) {
     if (!ctx || !bytes) return PGD_ERR_INVALID;
-    *bytes = ctx->mem_used;
+    *bytes = ctx->mem_used.load();
     return PGD_OK;
 }
 

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -42,7 +43,7 @@ struct pgd_ctx {
     size_t lds_per_block = 65536;  // hipDeviceProp_t::sharedMemPerBlock, read once at open
     int tiled_per_cu[3][8] = {};   // occupancy of the tiled wedge kernel by (block size, tile), 0 = not asked yet
     int64_t mem_cap = 0;   // 0: no cap
-    int64_t mem_used = 0;
+    std::atomic<int64_t> mem_used{0};  // also released from other threads (served results)
     std::string last_err;
     // rocPRIM temporary storage, grown on demand
     void* tmp = nullptr;
