@@ -290,6 +290,14 @@ def main() -> int:
     ctx = pgd_amd.Context(0)
     g = pgd_amd.GraphletCore(s, d, n=n, ctx=ctx, preprocess=False)
     log(f"[bench] device CSR built; max degree {g.max_degree}")
+    # host edge list -> device CSR (upload over PCIe, dedup, CSR assembly), timed
+    # on a second construction so the first-use costs are out of it
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g2 = pgd_amd.GraphletCore(s, d, n=n, ctx=ctx, preprocess=False)
+    torch.cuda.synchronize()
+    t_upload = time.perf_counter() - t0
+    g2.free()
 
     def step():
         g.preprocess("deg", False)
@@ -326,6 +334,8 @@ def main() -> int:
         "config": {"workload": f"rmat_scale{args.scale}_ef{args.ef}_a0.57_b0.19_c0.19_seed{args.seed}",
                    "n": g.n, "m": m, "max_degree": g.max_degree, "micro": True, "engine": g.last_engine()},
         "counting_ms": float(np.mean(tot)),
+        "upload_and_csr_ms": t_upload * 1e3,
+        "pcie_inclusive_edges_per_s": m / (dt + t_upload),
         "roofline": roofline(phases, f"rmat_scale{args.scale}_ef{args.ef}_a0.57_b0.19_c0.19_seed{args.seed}"),
         "phases": [{"name": p["name"], "ms": round(float(np.mean([ph[j]["ms"] for ph in phases])), 3),
                     "GB": round(p["bytes"] / 1e9, 3), "units": p["units"]} for j, p in enumerate(phases[0])],
