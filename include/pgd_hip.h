@@ -174,15 +174,18 @@ pgd_status pgd_graph_get_tuning(pgd_graph *g, pgd_tuning *t);
 pgd_status pgd_derive(int64_t n, int64_t m, const uint64_t *sums, uint64_t *totals);
 
 /* ---- batched form -----------------------------------------------------
- * Thousands of small graphs counted in one persistent launch.  The caller
- * packs the graphs' COO lists back to back (ids local to each graph);
- * graph g owns lines [line_off[g], line_off[g+1]) and has n[g] vertices.
- * The batch builds each graph's CSR, preprocessing and E_ordered as the
- * plain path does (deg/deg defaults), then counts every graph with one
- * persistent kernel that pulls (graph, edge-range) tiles from a device
- * queue; the launch is captured in a HIP graph and replayed by
- * pgd_batch_run.  sums receives PGD_NSUMS values per graph and totals
- * PGD_NTOTALS per graph (either may be NULL). */
+ * Thousands of small graphs counted together (no reference counterpart).
+ * The caller packs the graphs' COO lists back to back (ids local to each
+ * graph); graph g owns lines [line_off[g], line_off[g+1]) and has n[g]
+ * vertices; flags: PGD_DEDUP, PGD_MICRO.  The batch builds each graph's CSR,
+ * preprocessing and E_ordered as the plain path does (deg/deg defaults) on
+ * the disjoint union of the graphs; each counting pass is one launch over all
+ * graphs (the 4-cycle pass and the epilogue pull their tiles from device
+ * queues, the epilogue's tiles being (graph, edge-range) pairs), and the
+ * whole counting sequence is captured in a HIP graph that pgd_batch_run
+ * replays.  sums receives PGD_NSUMS values per graph and totals PGD_NTOTALS
+ * per graph (either may be NULL); each equals the plain path's for that
+ * graph alone. */
 typedef struct pgd_batch pgd_batch;
 pgd_status pgd_batch_create(pgd_ctx *ctx, int32_t ngraphs, const int32_t *n,
                             const int64_t *line_off, const int32_t *src, const int32_t *dst,
