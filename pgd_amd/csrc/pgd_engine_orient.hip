@@ -147,34 +147,64 @@ struct LocalArrays {
     int64_t* obeg;     // first out-slot of L[i]     [kmax]
     int64_t* pref;     // inclusive scan of |N+(L[i])| [kmax]
     uint32_t* cnt;     // per-local-vertex counter   [kmax]
+    uint32_t* ta;      // T of the edge a -> L[i]    [kmax]
     uint64_t* acc;     // per-local-vertex u64 accumulator [kmax]
+    int32_t* hk;       // hash of N+(a): vertex id (-1 empty) [H]
+    int32_t* hv;       //                local index          [H]
     uint32_t* M;       // bit matrix, kmax x W        [kmax*W]
+    int hmask;
 };
+
+// hash slots for kmax local vertices: a power of two, at most half full
+__host__ __device__ inline int hash_slots(int kmax) {
+    int h = 64;
+    while (h < 2 * kmax) h <<= 1;
+    return h;
+}
 
 __device__ __forceinline__ LocalArrays carve(char* base, int kmax, int W) {
     LocalArrays r;
     char* p = base;
+    const int H = hash_slots(kmax);
     r.pref = (int64_t*)p; p += (size_t)kmax * 8;
     r.obeg = (int64_t*)p; p += (size_t)kmax * 8;
     r.acc = (uint64_t*)p; p += (size_t)kmax * 8;
     r.L = (int32_t*)p; p += (size_t)kmax * 4;
     r.Le = (int32_t*)p; p += (size_t)kmax * 4;
     r.cnt = (uint32_t*)p; p += (size_t)kmax * 4;
+    r.ta = (uint32_t*)p; p += (size_t)kmax * 4;
+    r.hk = (int32_t*)p; p += (size_t)H * 4;
+    r.hv = (int32_t*)p; p += (size_t)H * 4;
     r.M = (uint32_t*)p;
+    r.hmask = H - 1;
     (void)W;
     return r;
 }
 // rounded to 256 B so every per-block HBM slice keeps 8-byte alignment for the u64 atomics
 __host__ __device__ inline size_t carve_bytes(int kmax, int W) {
-    return ((size_t)kmax * 36 + (size_t)kmax * W * 4 + 255) & ~(size_t)255;
+    return ((size_t)kmax * 40 + (size_t)hash_slots(kmax) * 8 + (size_t)kmax * W * 4 + 255) & ~(size_t)255;
 }
 
-// load N+(a) and the flattened out-list prefix; returns total local work
+__device__ __forceinline__ uint32_t vhash(int32_t c) { return (uint32_t)c * 2654435761u; }
+
+// local index of vertex c in N+(a), or -1
+__device__ __forceinline__ int find_local(const LocalArrays& A, int32_t c) {
+    uint32_t h = (vhash(c) >> 7) & (uint32_t)A.hmask;
+    for (;;) {
+        const int32_t key = A.hk[h];
+        if (key == c) return A.hv[h];
+        if (key < 0) return -1;
+        h = (h + 1) & (uint32_t)A.hmask;
+    }
+}
+
+// load N+(a), its hash and the flattened out-list prefix; returns total local work
 template <int NT>
 __device__ int64_t load_local(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                               const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart, int32_t a,
                               int k, LocalArrays& A, int64_t* part) {
     const int64_t s0 = ostart[a];
+    for (int h = threadIdx.x; h <= A.hmask; h += NT) A.hk[h] = -1;
     for (int i = threadIdx.x; i < k; i += NT) {
         int32_t b = col[s0 + i];
         A.L[i] = b;
@@ -184,6 +214,13 @@ __device__ int64_t load_local(const int64_t* __restrict__ row, const int32_t* __
         A.pref[i] = row[b + 1] - ob;
         A.cnt[i] = 0;
         A.acc[i] = 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += NT) {
+        const int32_t b = A.L[i];
+        uint32_t h = (vhash(b) >> 7) & (uint32_t)A.hmask;
+        while (atomicCAS(&A.hk[h], -1, b) != -1) h = (h + 1) & (uint32_t)A.hmask;
+        A.hv[h] = i;
     }
     __syncthreads();
     block_incl_scan<NT>(A.pref, k, part);
@@ -234,7 +271,7 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
             for (int u = 0; u < kU; u++)
                 if (i[u] >= 0) x[u] = col[slot[u]];
 #pragma unroll
-            for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_in(A.L, i[u] + 1, k, x[u]) : -1;
+            for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_local(A, x[u]) : -1;
 #pragma unroll
             for (int u = 0; u < kU; u++)
                 if (j[u] >= 0) {
@@ -250,27 +287,54 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
     }
 }
 
-// pass 2: 4-cliques and Q, with the local bit matrix
+// pass 2: 4-cliques and Q, with the local bit matrix.  The pass that builds
+// the bit matrix also records every local edge (i, j, edge id of (L[i],
+// L[j])) in the block's slice of `elist` (capacity k(k-1)/2 for the bin's
+// largest k, so it cannot overflow); the scan then reads that list instead
+// of searching again.  Without a list (the HBM-scratch bin) the scan
+// searches again.
+__device__ __forceinline__ void edge_terms_k4q(const LocalArrays& A, int W, int i, int j, int32_t eij,
+                                               const uint32_t* __restrict__ T, uint64_t* __restrict__ K4,
+                                               uint64_t* __restrict__ Q) {
+    uint32_t kij = 0;
+    for (int w = 0; w < W; w++) kij += __popc(A.M[i * W + w] & A.M[j * W + w]);
+    const uint64_t tai = A.ta[i], taj = A.ta[j], tij = T[eij];
+    if (kij) {
+        atomicAdd((unsigned long long*)&K4[eij], (unsigned long long)kij);
+        atomicAdd(&A.cnt[i], kij);
+        atomicAdd(&A.cnt[j], kij);
+    }
+    atomicAdd((unsigned long long*)&Q[eij], (unsigned long long)(tai + taj));
+    atomicAdd((unsigned long long*)&A.acc[i], (unsigned long long)(taj + tij));
+    atomicAdd((unsigned long long*)&A.acc[j], (unsigned long long)(tai + tij));
+}
+
 template <int NT, bool kGlobal>
 __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                                                   const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart,
                                                   const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
                                                   int64_t cnt, int kmax, char* __restrict__ gscratch,
+                                                  uint64_t* __restrict__ elist, int64_t ecap,
                                                   const uint32_t* __restrict__ T, uint64_t* __restrict__ K4,
                                                   uint64_t* __restrict__ Q, unsigned long long* __restrict__ work) {
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
+    __shared__ int ecount;
     const int Wmax = (kmax + 31) / 32;
+    const int lane = threadIdx.x & 63;
     char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, Wmax) : lds;
     LocalArrays A = carve(base, kmax, Wmax);
+    uint64_t* el = elist ? elist + (size_t)blockIdx.x * (size_t)ecap : nullptr;
     for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
         const int32_t a = list[li];
         const int k = outdeg[a];
         const int W = (k + 31) / 32;
         for (int w = threadIdx.x; w < k * W; w += NT) A.M[w] = 0;
+        if (threadIdx.x == 0) ecount = 0;
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
         if (threadIdx.x == 0) atomicAdd(work, (unsigned long long)total);
-        // build the symmetric adjacency of the local graph
+        for (int i = threadIdx.x; i < k; i += NT) A.ta[i] = T[A.Le[i]];
+        // build the symmetric adjacency of the local graph (and record its edges)
         for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
             int i[kU], j[kU];
             int64_t slot[kU];
@@ -280,41 +344,50 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
             for (int u = 0; u < kU; u++)
                 if (i[u] >= 0) x[u] = col[slot[u]];
 #pragma unroll
-            for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_in(A.L, i[u] + 1, k, x[u]) : -1;
+            for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_local(A, x[u]) : -1;
 #pragma unroll
-            for (int u = 0; u < kU; u++)
-                if (j[u] >= 0) {
+            for (int u = 0; u < kU; u++) {
+                const bool hit = j[u] >= 0;
+                if (hit) {
                     atomicOr(&A.M[i[u] * W + (j[u] >> 5)], 1u << (j[u] & 31));
                     atomicOr(&A.M[j[u] * W + (i[u] >> 5)], 1u << (i[u] & 31));
                 }
+                if (el) {
+                    // one LDS reservation per wavefront for its hits
+                    const uint64_t active = __ballot(1);
+                    const uint64_t hits = __ballot(hit);
+                    const int leader = __ffsll((unsigned long long)active) - 1;
+                    int basepos = 0;
+                    if (lane == leader && hits) basepos = atomicAdd(&ecount, __popcll(hits));
+                    basepos = __shfl(basepos, leader, 64);
+                    if (hit) {
+                        const int pos = basepos + __popcll(hits & ((1ULL << lane) - 1));
+                        el[pos] = ((uint64_t)(uint32_t)eid[slot[u]] << 32) | ((uint32_t)j[u] << 16) | (uint32_t)i[u];
+                    }
+                }
+            }
         }
         __syncthreads();
-        for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
-            int iu[kU], ju[kU];
-            int64_t su[kU];
-            int32_t x[kU];
-            locate<NT>(A, k, total, t0, iu, su);
+        if (el) {
+            const int ne = ecount;
+            for (int p = threadIdx.x; p < ne; p += NT) {
+                const uint64_t e = el[p];
+                edge_terms_k4q(A, W, (int)(e & 0xffff), (int)((e >> 16) & 0xffff), (int32_t)(e >> 32), T, K4, Q);
+            }
+        } else {
+            for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
+                int iu[kU], ju[kU];
+                int64_t su[kU];
+                int32_t x[kU];
+                locate<NT>(A, k, total, t0, iu, su);
 #pragma unroll
-            for (int u = 0; u < kU; u++)
-                if (iu[u] >= 0) x[u] = col[su[u]];
+                for (int u = 0; u < kU; u++)
+                    if (iu[u] >= 0) x[u] = col[su[u]];
 #pragma unroll
-            for (int u = 0; u < kU; u++) ju[u] = iu[u] >= 0 ? find_in(A.L, iu[u] + 1, k, x[u]) : -1;
+                for (int u = 0; u < kU; u++) ju[u] = iu[u] >= 0 ? find_local(A, x[u]) : -1;
 #pragma unroll
-            for (int u = 0; u < kU; u++) {
-                const int i = iu[u], j = ju[u];
-                if (j < 0) continue;
-                uint32_t kij = 0;
-                for (int w = 0; w < W; w++) kij += __popc(A.M[i * W + w] & A.M[j * W + w]);
-                const int32_t eij = eid[su[u]];
-                const uint64_t tai = T[A.Le[i]], taj = T[A.Le[j]], tij = T[eij];
-                if (kij) {
-                    atomicAdd((unsigned long long*)&K4[eij], (unsigned long long)kij);
-                    atomicAdd(&A.cnt[i], kij);
-                    atomicAdd(&A.cnt[j], kij);
-                }
-                atomicAdd((unsigned long long*)&Q[eij], (unsigned long long)(tai + taj));
-                atomicAdd((unsigned long long*)&A.acc[i], (unsigned long long)(taj + tij));
-                atomicAdd((unsigned long long*)&A.acc[j], (unsigned long long)(tai + tij));
+                for (int u = 0; u < kU; u++)
+                    if (ju[u] >= 0) edge_terms_k4q(A, W, iu[u], ju[u], eid[su[u]], T, K4, Q);
             }
         }
         __syncthreads();
@@ -528,6 +601,20 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
         st.ghuge = (int)std::min<int64_t>(st.n_huge, ctx->num_cus);
         ensure(st.hscratch, ctx, (size_t)st.ghuge * carve_bytes(st.maxk, (st.maxk + 31) / 32));
     }
+    {
+        // local-edge lists of the clique pass: per block, the bin's k(k-1)/2 at most
+        auto cap = [](int64_t k) { return std::max<int64_t>(1, k * (k - 1) / 2); };
+        st.ecap_small = cap(prm.small_k);
+        st.ecap_mid = cap(prm.mid_k);
+        st.ecap_large = cap(st.kmax_lds);
+        const int64_t bs = st.n_small ? grid_cap(ctx, st.n_small, prm.grid_small) : 0;
+        const int64_t bm = st.n_mid ? grid_cap(ctx, st.n_mid, prm.grid_mid) : 0;
+        const int64_t bl = st.n_large ? grid_cap(ctx, st.n_large, 1) : 0;
+        ensure(st.elist, ctx, (size_t)(bs * st.ecap_small + bm * st.ecap_mid + bl * st.ecap_large));
+        st.el_small = st.elist.get();
+        st.el_mid = st.el_small + bs * st.ecap_small;
+        st.el_large = st.el_mid + bm * st.ecap_mid;
+    }
     if (st.n_heavy) wedge_plan_build(ctx, st);
 }
 
@@ -570,22 +657,22 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     if (st.n_small)
         hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (char*)nullptr, out.T, out.K4,
-                           st.Q.get(), w + 1);
+                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (char*)nullptr, st.el_small,
+                           st.ecap_small, out.T, out.K4, st.Q.get(), w + 1);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (char*)nullptr, out.T, out.K4,
-                           st.Q.get(), w + 1);
+                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (char*)nullptr, st.el_mid,
+                           st.ecap_mid, out.T, out.K4, st.Q.get(), w + 1);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kmax_lds, (char*)nullptr, out.T, out.K4,
-                           st.Q.get(), w + 1);
+                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kmax_lds, (char*)nullptr, st.el_large,
+                           st.ecap_large, out.T, out.K4, st.Q.get(), w + 1);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_k4q<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.hscratch.get(), out.T, out.K4, st.Q.get(), w + 1);
+                           st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, out.T, out.K4, st.Q.get(), w + 1);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (timer) timer->begin("k_wedge_light");

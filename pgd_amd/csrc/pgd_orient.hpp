@@ -228,6 +228,9 @@ struct OrientState {
     int kmax_lds = 0, maxk = 0, ghuge = 0;
     size_t small_lds = 0, mid_lds = 0, large_lds = 0;
     DBuf<char> hscratch;
+    DBuf<uint64_t> elist;  // local-edge lists of the clique pass (per block of each LDS bin)
+    uint64_t *el_small = nullptr, *el_mid = nullptr, *el_large = nullptr;
+    int64_t ecap_small = 0, ecap_mid = 0, ecap_large = 0;
     DBuf<uint64_t> P, Q;
     DBuf<unsigned long long> work;
     WedgePlan wp;
