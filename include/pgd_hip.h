@@ -159,6 +159,8 @@ typedef struct pgd_tuning {
     int32_t wedge_log2r; /* endpoint tile of the heavy 4-cycle pass, 2^r LDS counters, r in 8..15 (13) */
     int32_t wedge_nt;    /* threads per block of that pass: 256, 512 or 1024 (512) */
     int32_t wedge_split; /* work items per block the heavy wedges are cut into (4) */
+    int32_t tri_lists;   /* local-edge lists kept from the triangle pass for the clique pass, in
+                            entries per edge (0: off; the clique pass searches again) (64) */
 } pgd_tuning;
 pgd_status pgd_tuning_default(pgd_tuning *t);
 /* Parameters of later pgd_decompose calls on g. */

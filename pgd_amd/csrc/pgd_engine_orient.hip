@@ -245,21 +245,52 @@ __device__ __forceinline__ void locate(const LocalArrays& A, int k, int64_t tota
     }
 }
 
+// Record the hits of one round of flattened positions in the block's local
+// edge list: one LDS reservation per wavefront.  Entry = (edge id of
+// (L[i], L[j]) << 32) | (j << 16) | i.
+__device__ __forceinline__ void record_hits(uint64_t* __restrict__ el, int* ecount, bool hit, int i, int j,
+                                            int32_t e) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t active = __ballot(1);
+    const uint64_t hits = __ballot(hit);
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    int basepos = 0;
+    if (lane == leader && hits) basepos = atomicAdd(ecount, __popcll(hits));
+    basepos = __shfl(basepos, leader, 64);
+    if (hit) el[basepos + __popcll(hits & ((1ULL << lane) - 1))] = ((uint64_t)(uint32_t)e << 32) | ((uint32_t)j << 16) | (uint32_t)i;
+}
+
+// Lists of local edges kept for the clique pass: the triangle pass appends
+// vertex a's list (from its block scratch) to `G` when it fits and records
+// its offset and length (length -1: not kept, the clique pass searches again).
+struct TriLists {
+    uint64_t* G;              // all kept lists
+    int64_t gcap;             // entries of G
+    unsigned long long* cur;  // next free entry
+    int64_t* off;             // per vertex rank: first entry of its list
+    int32_t* len;             // per vertex rank: entries, or -1
+};
+
 // pass 1: triangles.  Local edge (i,j), i<j <=> triangle {a, L[i], L[j]}.
 template <int NT, bool kGlobal>
 __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                                                   const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart,
                                                   const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
                                                   int64_t cnt, int kmax, char* __restrict__ gscratch,
+                                                  uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
                                                   uint32_t* __restrict__ T, unsigned long long* __restrict__ work) {
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
+    __shared__ int ecount;
+    __shared__ int64_t gpos;
     const int W = (kmax + 31) / 32;
     char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, W) : lds;
     LocalArrays A = carve(base, kmax, W);
+    uint64_t* el = (elist && tl.G) ? elist + (size_t)blockIdx.x * (size_t)ecap : nullptr;
     for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
         const int32_t a = list[li];
         const int k = outdeg[a];
+        if (threadIdx.x == 0) ecount = 0;
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
         if (threadIdx.x == 0) atomicAdd(work, (unsigned long long)total);
         for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
@@ -273,16 +304,35 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
 #pragma unroll
             for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_local(A, x[u]) : -1;
 #pragma unroll
-            for (int u = 0; u < kU; u++)
-                if (j[u] >= 0) {
-                    atomicAdd(&T[eid[slot[u]]], 1u);
+            for (int u = 0; u < kU; u++) {
+                const bool hit = j[u] >= 0;
+                int32_t e = 0;
+                if (hit) {
+                    e = eid[slot[u]];
+                    atomicAdd(&T[e], 1u);
                     atomicAdd(&A.cnt[i[u]], 1u);
                     atomicAdd(&A.cnt[j[u]], 1u);
                 }
+                if (el) record_hits(el, &ecount, hit, i[u], j[u], e);
+            }
         }
         __syncthreads();
         for (int i = threadIdx.x; i < k; i += NT)
             if (A.cnt[i]) atomicAdd(&T[A.Le[i]], A.cnt[i]);
+        if (el) {
+            const int ne = ecount;
+            if (threadIdx.x == 0) {
+                int64_t p = (int64_t)atomicAdd(tl.cur, (unsigned long long)ne);
+                if (p + ne > tl.gcap) p = -1;
+                gpos = p;
+                tl.off[a] = p;
+                tl.len[a] = p < 0 ? -1 : ne;
+            }
+            __syncthreads();
+            const int64_t p = gpos;
+            if (p >= 0)
+                for (int q = threadIdx.x; q < ne; q += NT) tl.G[p + q] = el[q];
+        }
         __syncthreads();
     }
 }
@@ -314,14 +364,13 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                                                   const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart,
                                                   const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
                                                   int64_t cnt, int kmax, char* __restrict__ gscratch,
-                                                  uint64_t* __restrict__ elist, int64_t ecap,
+                                                  uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
                                                   const uint32_t* __restrict__ T, uint64_t* __restrict__ K4,
                                                   uint64_t* __restrict__ Q, unsigned long long* __restrict__ work) {
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
     __shared__ int ecount;
     const int Wmax = (kmax + 31) / 32;
-    const int lane = threadIdx.x & 63;
     char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, Wmax) : lds;
     LocalArrays A = carve(base, kmax, Wmax);
     uint64_t* el = elist ? elist + (size_t)blockIdx.x * (size_t)ecap : nullptr;
@@ -334,8 +383,19 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
         if (threadIdx.x == 0) atomicAdd(work, (unsigned long long)total);
         for (int i = threadIdx.x; i < k; i += NT) A.ta[i] = T[A.Le[i]];
+        // the triangle pass's list of a's local edges, when it kept one
+        const int kept = tl.G ? tl.len[a] : -1;
+        const uint64_t* gl = kept >= 0 ? tl.G + tl.off[a] : nullptr;
+        if (gl) {
+            for (int p = threadIdx.x; p < kept; p += NT) {
+                const uint64_t e = gl[p];
+                const int i = (int)(e & 0xffff), j = (int)((e >> 16) & 0xffff);
+                atomicOr(&A.M[i * W + (j >> 5)], 1u << (j & 31));
+                atomicOr(&A.M[j * W + (i >> 5)], 1u << (i & 31));
+            }
+        }
         // build the symmetric adjacency of the local graph (and record its edges)
-        for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
+        for (int64_t t0 = threadIdx.x; t0 < (gl ? 0 : total); t0 += NT * kU) {
             int i[kU], j[kU];
             int64_t slot[kU];
             int32_t x[kU];
@@ -352,26 +412,15 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                     atomicOr(&A.M[i[u] * W + (j[u] >> 5)], 1u << (j[u] & 31));
                     atomicOr(&A.M[j[u] * W + (i[u] >> 5)], 1u << (i[u] & 31));
                 }
-                if (el) {
-                    // one LDS reservation per wavefront for its hits
-                    const uint64_t active = __ballot(1);
-                    const uint64_t hits = __ballot(hit);
-                    const int leader = __ffsll((unsigned long long)active) - 1;
-                    int basepos = 0;
-                    if (lane == leader && hits) basepos = atomicAdd(&ecount, __popcll(hits));
-                    basepos = __shfl(basepos, leader, 64);
-                    if (hit) {
-                        const int pos = basepos + __popcll(hits & ((1ULL << lane) - 1));
-                        el[pos] = ((uint64_t)(uint32_t)eid[slot[u]] << 32) | ((uint32_t)j[u] << 16) | (uint32_t)i[u];
-                    }
-                }
+                if (el) record_hits(el, &ecount, hit, i[u], j[u], hit ? eid[slot[u]] : 0);
             }
         }
         __syncthreads();
-        if (el) {
-            const int ne = ecount;
+        if (gl || el) {
+            const uint64_t* src = gl ? gl : el;
+            const int ne = gl ? kept : ecount;
             for (int p = threadIdx.x; p < ne; p += NT) {
-                const uint64_t e = el[p];
+                const uint64_t e = src[p];
                 edge_terms_k4q(A, W, (int)(e & 0xffff), (int)((e >> 16) & 0xffff), (int32_t)(e >> 32), T, K4, Q);
             }
         } else {
@@ -516,6 +565,9 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, int64_t n) {
 __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n) {
     GRID_STRIDE(i, n) p[i] = 0;
 }
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+    GRID_STRIDE(i, n) p[i] = v;
+}
 
 }  // namespace
 
@@ -615,6 +667,25 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
         st.el_mid = st.el_small + bs * st.ecap_small;
         st.el_large = st.el_mid + bm * st.ecap_mid;
     }
+    {
+        // global local-edge lists (triangle pass -> clique pass), as many
+        // entries as a third of the free memory holds, at most tri_lists per
+        // edge; a vertex whose list does not fit is searched again
+        st.tcap = 0;
+        if (prm.tri_lists > 0 && m >= ((int64_t)1 << 12)) {
+            size_t fr = 0, tot = 0;
+            PGD_HIP(hipMemGetInfo(&fr, &tot));
+            int64_t avail = (int64_t)(fr / 3);
+            if (ctx->mem_cap > 0) avail = std::min<int64_t>(avail, (ctx->mem_cap - ctx->mem_used.load()) / 3);
+            st.tcap = std::min<int64_t>(avail / 8, (int64_t)prm.tri_lists * m);
+        }
+        if (st.tcap > 0) {
+            ensure(st.tlist, ctx, (size_t)st.tcap);
+            ensure(st.toff, ctx, n);
+            ensure(st.tlen, ctx, n);
+            ensure(st.tcur, ctx, 1);
+        }
+    }
     if (st.n_heavy) wedge_plan_build(ctx, st);
 }
 
@@ -634,23 +705,35 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     unsigned long long* w = st.work.get();
     const int32_t *rr = nullptr;
     (void)rr;
+    TriLists tl{st.tlist.get(), st.tcap, (unsigned long long*)st.tcur.get(), st.toff.get(), st.tlen.get()};
+    if (st.tcap > 0) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, st.tcur.get(), (int64_t)1);
+        hipLaunchKernelGGL(k_fill_i32, dim3(grid_cap(ctx, (n + 255) / 256, 16)), dim3(256), 0, s, st.tlen.get(), n,
+                           -1);  // not kept, until the triangle pass writes a list
+        PGD_HIP(hipGetLastError());
+    } else {
+        tl.G = nullptr;
+    }
     if (timer) timer->begin("k_local_tri");
     if (st.n_small)
         hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (char*)nullptr, out.T, w);
+                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (char*)nullptr, st.el_small,
+                           st.ecap_small, tl, out.T, w);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (char*)nullptr, out.T, w);
+                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (char*)nullptr, st.el_mid,
+                           st.ecap_mid, tl, out.T, w);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_tri<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kmax_lds, (char*)nullptr, out.T, w);
+                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kmax_lds, (char*)nullptr, st.el_large,
+                           st.ecap_large, tl, out.T, w);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_tri<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.hscratch.get(), out.T, w);
+                           st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, out.T, w);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (timer) timer->begin("k_local_k4q");
@@ -658,21 +741,21 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
         hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (char*)nullptr, st.el_small,
-                           st.ecap_small, out.T, out.K4, st.Q.get(), w + 1);
+                           st.ecap_small, tl, out.T, out.K4, st.Q.get(), w + 1);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (char*)nullptr, st.el_mid,
-                           st.ecap_mid, out.T, out.K4, st.Q.get(), w + 1);
+                           st.ecap_mid, tl, out.T, out.K4, st.Q.get(), w + 1);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_large.get(), st.n_large, st.kmax_lds, (char*)nullptr, st.el_large,
-                           st.ecap_large, out.T, out.K4, st.Q.get(), w + 1);
+                           st.ecap_large, tl, out.T, out.K4, st.Q.get(), w + 1);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_k4q<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, out.T, out.K4, st.Q.get(), w + 1);
+                           st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, out.T, out.K4, st.Q.get(), w + 1);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (timer) timer->begin("k_wedge_light");
@@ -742,6 +825,7 @@ const char* tuning_problem(const pgd_tuning& t) {
     if (t.wedge_log2r < 8 || t.wedge_log2r > 15) return "wedge_log2r must be in 8..15";
     if (t.wedge_nt != 256 && t.wedge_nt != 512 && t.wedge_nt != 1024) return "wedge_nt must be 256, 512 or 1024";
     if (t.wedge_split < 1 || t.wedge_split > 64) return "wedge_split must be in 1..64";
+    if (t.tri_lists < 0 || t.tri_lists > 1024) return "tri_lists must be in 0..1024";
     return "";
 }
 

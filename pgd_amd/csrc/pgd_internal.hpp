@@ -127,6 +127,7 @@ struct OrientParams {
     int wedge_log2r = 13;   // endpoint tile of the heavy 4-cycle pass: 2^log2r LDS counters
     int wedge_nt = 512;     // threads per block of the heavy 4-cycle pass (256, 512 or 1024)
     int wedge_split = 4;    // work items per block the heavy wedges are cut into (load balance)
+    int tri_lists = 64;     // kept local-edge lists, entries per edge (0: off)
 };
 OrientState* orient_state_new();
 OrientParams& orient_params(OrientState* s);
@@ -144,10 +145,12 @@ inline OrientParams params_from(const pgd_tuning& t) {
     p.wedge_log2r = t.wedge_log2r;
     p.wedge_nt = t.wedge_nt;
     p.wedge_split = t.wedge_split;
+    p.tri_lists = t.tri_lists;
     return p;
 }
 inline pgd_tuning tuning_from(const OrientParams& p) {
-    return pgd_tuning{p.small_k, p.mid_k, p.grid_small, p.grid_mid, p.wedge_log2r, p.wedge_nt, p.wedge_split};
+    return pgd_tuning{p.small_k,     p.mid_k,    p.grid_small,  p.grid_mid,
+                      p.wedge_log2r, p.wedge_nt, p.wedge_split, p.tri_lists};
 }
 // empty string when t is usable, else what is wrong
 const char* tuning_problem(const pgd_tuning& t);

@@ -144,6 +144,8 @@ TUNINGS = [
     dict(wedge_log2r=10, wedge_nt=256, wedge_split=64),      # small items
     dict(wedge_log2r=14, wedge_nt=1024, small_k=32, mid_k=128),
     dict(wedge_log2r=12, wedge_nt=512, grid_small=4, grid_mid=1, wedge_split=1),
+    dict(tri_lists=0),                                       # clique pass searches again
+    dict(tri_lists=1),                                       # lists kept until the buffer is full
 ]
 
 
@@ -178,7 +180,8 @@ def test_tuning_never_changes_counts(ti, ctx):
 def test_tuning_rejects_bad_values(ctx):
     s, d = pgd_amd.gen_gnp(100, 0.1, 1)
     g = pgd_amd.GraphletCore(s, d, ctx=ctx)
-    for bad in (dict(wedge_log2r=16), dict(wedge_nt=128), dict(small_k=65), dict(mid_k=512), dict(wedge_split=0)):
+    for bad in (dict(wedge_log2r=16), dict(wedge_nt=128), dict(small_k=65), dict(mid_k=512), dict(wedge_split=0),
+                dict(tri_lists=-1)):
         with pytest.raises(pgd_amd.PGDError):
             g.set_tuning(**bad)
     g.free()
