@@ -198,6 +198,96 @@ __global__ void k_item_keys(const uint64_t* __restrict__ key, int64_t n, uint64_
     }
 }
 
+// One pass of one wavefront over the segments of a tile (groups of 64
+// segments, their lengths scanned in registers, flattened 64 lanes at a
+// time).  pass 0 counts endpoints, pass 1 adds cnt-1 per wedge to the slot's
+// accumulator and, per segment, to the edge (a, b).  mode 0: 16-bit counters
+// over the whole tile (a count reaching 65536 sets *ovf); mode 1/2: 32-bit
+// counters over the lower/upper half of the tile, other endpoints skipped.
+template <int NT>
+__device__ __forceinline__ void tile_pass(int pass, int mode, uint32_t sb, uint32_t se, int32_t c0, int log2r,
+                                          const WedgeSeg* __restrict__ seg, const int32_t* __restrict__ col,
+                                          uint32_t* cnt, uint64_t* __restrict__ PS, int32_t* wpref, int64_t* wstart,
+                                          int32_t* wslot, unsigned long long* wpsum, int* ovf, int lane, int w,
+                                          unsigned long long& done) {
+    const int32_t half = 1 << (log2r - 1);
+    const int32_t lo = mode == 2 ? half : 0;
+    for (uint32_t gb = sb + (uint32_t)w * 64; gb < se; gb += NT) {
+        const int kc = (int)min<uint32_t>(64, se - gb);
+        int32_t len = 0, sl = 0;
+        int64_t st = 0;
+        if (lane < kc) {
+            const WedgeSeg g = seg[gb + lane];
+            len = g.len;
+            st = g.start;
+            sl = g.sab;
+        }
+        int32_t x = len;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int32_t y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        wpref[lane] = x;
+        wstart[lane] = st;
+        wslot[lane] = sl;
+        wpsum[lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+        const int32_t total = __shfl(x, 63, 64);
+        if (pass == 0 && mode < 2 && lane == 0) done += (unsigned long long)total;
+        // kU positions per lane per round: their owner searches and endpoint
+        // loads are independent and overlap
+        constexpr int kU = 4;
+        for (int32_t q0 = lane; q0 < total; q0 += 64 * kU) {
+            int o[kU];
+            int64_t slot[kU];
+            int32_t c[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int32_t q = q0 + 64 * u;
+                o[u] = -1;
+                if (q < total) {
+                    o[u] = owner_of(wpref, kc, q);
+                    const int32_t ex = o[u] ? wpref[o[u] - 1] : 0;
+                    slot[u] = wstart[o[u]] + (q - ex);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                if (o[u] >= 0) c[u] = col[slot[u]] - c0;
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                if (o[u] < 0) continue;
+                uint32_t v;
+                if (mode == 0) {
+                    const int sh = (c[u] & 1) * 16;
+                    if (pass == 0) {
+                        const uint32_t old = atomicAdd(&cnt[c[u] >> 1], 1u << sh);
+                        if (((old >> sh) & 0xffffu) == 0xffffu) *ovf = 1;
+                        continue;
+                    }
+                    v = ((cnt[c[u] >> 1] >> sh) & 0xffffu) - 1;
+                } else {
+                    const int32_t ci = c[u] - lo;
+                    if (ci < 0 || ci >= half) continue;
+                    if (pass == 0) {
+                        atomicAdd(&cnt[ci], 1u);
+                        continue;
+                    }
+                    v = cnt[ci] - 1;
+                }
+                if (v) {
+                    atomicAdd((unsigned long long*)&PS[slot[u]], (unsigned long long)v);
+                    atomicAdd(&wpsum[o[u]], (unsigned long long)v);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (pass == 1 && lane < kc && wpsum[lane]) atomicAdd((unsigned long long*)&PS[wslot[lane]], wpsum[lane]);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // the tiled count/accumulate kernel (persistent; items from the queue).
 // Within a tile each wavefront takes 64 segments at a time, scans their
 // lengths in registers and walks the flattened wedges 64 lanes at a time
@@ -212,7 +302,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     WedgeSeg* __restrict__ segs, int64_t segcap, uint32_t* __restrict__ qctr, uint64_t* __restrict__ PS,
     unsigned long long* __restrict__ work) {
     constexpr int NW = NT / 64;
-    extern __shared__ uint32_t cnt[];  // 2^log2r endpoint counters
+    extern __shared__ uint32_t cnt[];  // 2^log2r 16-bit endpoint counters (2^(log2r-1) words)
     __shared__ uint32_t base[kTmax + 1];
     __shared__ uint32_t cur[kTmax];
     __shared__ int64_t wstart[NW][64];
@@ -220,6 +310,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     __shared__ int32_t wslot[NW][64];
     __shared__ unsigned long long wpsum[NW][64];
     __shared__ int64_t item_sh;
+    __shared__ int ovf;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int R = 1 << log2r;
     unsigned long long done = 0;
@@ -265,77 +356,36 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
             }
         }
         __syncthreads();
-        // tile by tile: count the endpoints, then accumulate cnt-1 per wedge
+        // tile by tile: count the endpoints, then accumulate cnt-1 per wedge.
+        // Counters are 16-bit, two per word (mode 0); a count that would pass
+        // 65535 (a hub with that many in-neighbours on one endpoint) flags the
+        // tile, which is then recounted as two half tiles with 32-bit
+        // counters in the same LDS (modes 1 and 2).
         for (int t = 0; t < T; t++) {
             const uint32_t sb = base[t], se = base[t + 1];
             if (sb == se) continue;
             const int32_t c0 = (W.t0 + t) << log2r;
-            for (int i = tid; i < R; i += NT) cnt[i] = 0;
+            for (int i = tid; i < R / 2; i += NT) cnt[i] = 0;
+            if (tid == 0) ovf = 0;
             __syncthreads();
-            for (int pass = 0; pass < 2; pass++) {
-                for (uint32_t gb = sb + (uint32_t)w * 64; gb < se; gb += NT) {
-                    const int kc = (int)min<uint32_t>(64, se - gb);
-                    int32_t len = 0, sl = 0;
-                    int64_t st = 0;
-                    if (lane < kc) {
-                        const WedgeSeg g = seg[gb + lane];
-                        len = g.len;
-                        st = g.start;
-                        sl = g.sab;
-                    }
-                    int32_t x = len;
-#pragma unroll
-                    for (int off = 1; off < 64; off <<= 1) {
-                        const int32_t y = __shfl_up(x, off, 64);
-                        if (lane >= off) x += y;
-                    }
-                    wpref[w][lane] = x;
-                    wstart[w][lane] = st;
-                    wslot[w][lane] = sl;
-                    wpsum[w][lane] = 0;
-                    __builtin_amdgcn_wave_barrier();
-                    const int32_t total = __shfl(x, 63, 64);
-                    if (pass == 0 && lane == 0) done += (unsigned long long)total;
-                    // kU positions per lane per round: their owner searches and
-                    // endpoint loads are independent and overlap
-                    constexpr int kU = 4;
-                    for (int32_t q0 = lane; q0 < total; q0 += 64 * kU) {
-                        int o[kU];
-                        int64_t slot[kU];
-                        int32_t c[kU];
-#pragma unroll
-                        for (int u = 0; u < kU; u++) {
-                            const int32_t q = q0 + 64 * u;
-                            o[u] = -1;
-                            if (q < total) {
-                                o[u] = owner_of(wpref[w], kc, q);
-                                const int32_t ex = o[u] ? wpref[w][o[u] - 1] : 0;
-                                slot[u] = wstart[w][o[u]] + (q - ex);
-                            }
-                        }
-#pragma unroll
-                        for (int u = 0; u < kU; u++)
-                            if (o[u] >= 0) c[u] = col[slot[u]] - c0;
-#pragma unroll
-                        for (int u = 0; u < kU; u++) {
-                            if (o[u] < 0) continue;
-                            if (pass == 0) {
-                                atomicAdd(&cnt[c[u]], 1u);
-                            } else {
-                                const uint32_t v = cnt[c[u]] - 1;
-                                if (v) {
-                                    atomicAdd((unsigned long long*)&PS[slot[u]], (unsigned long long)v);
-                                    atomicAdd(&wpsum[w][o[u]], (unsigned long long)v);
-                                }
-                            }
-                        }
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    if (pass == 1 && lane < kc && wpsum[w][lane])
-                        atomicAdd((unsigned long long*)&PS[wslot[w][lane]], wpsum[w][lane]);
-                    __builtin_amdgcn_wave_barrier();
-                }
+            tile_pass<NT>(0, 0, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w], wpsum[w],
+                          &ovf, lane, w, done);
+            __syncthreads();
+            if (!ovf) {
+                tile_pass<NT>(1, 0, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w], wpsum[w],
+                              &ovf, lane, w, done);
                 __syncthreads();
+            } else {
+                for (int mode = 1; mode <= 2; mode++) {
+                    for (int i = tid; i < R / 2; i += NT) cnt[i] = 0;
+                    __syncthreads();
+                    tile_pass<NT>(0, mode, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w],
+                                  wpsum[w], &ovf, lane, w, done);
+                    __syncthreads();
+                    tile_pass<NT>(1, mode, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w],
+                                  wpsum[w], &ovf, lane, w, done);
+                    __syncthreads();
+                }
             }
         }
     }
@@ -354,7 +404,7 @@ __global__ void k_ps_fold(const uint64_t* __restrict__ PS, const int32_t* __rest
 template <int NT>
 void launch_tiled(pgd_ctx* ctx, OrientState& st, unsigned long long* work) {
     WedgePlan& wp = st.wp;
-    const size_t dyn = (size_t)4 << wp.log2r;
+    const size_t dyn = (size_t)2 << wp.log2r;
     hipLaunchKernelGGL(k_wedge_tiled<NT>, dim3(wp.grid), dim3(NT), dyn, ctx->stream, st.rrow.get(), st.rcol.get(),
                        st.ostart.get(), st.b_heavy.get(), wp.wcnt.get(), wp.tt.get(), wp.ttrow.get(), wp.hbase.get(),
                        wp.htoff.get(), wp.htseg.get(), wp.items.get(), wp.nitems, wp.log2r, wp.segs.get(), wp.segcap,
@@ -365,7 +415,7 @@ template <int NT>
 int tiled_blocks_per_cu(pgd_ctx* ctx, int log2r) {
     int& slot = ctx->tiled_per_cu[NT == 256 ? 0 : NT == 512 ? 1 : 2][log2r - 8];
     if (slot) return slot;  // the attribute and the occupancy are per process and device
-    const size_t dyn = (size_t)4 << log2r;
+    const size_t dyn = (size_t)2 << log2r;
     PGD_HIP(hipFuncSetAttribute((const void*)k_wedge_tiled<NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)dyn));
     int nb = 0;

@@ -124,7 +124,7 @@ struct OrientParams {
     int mid_k = 256;        // up to which a 256-thread block does (1024 threads above)
     int grid_small = 32;    // resident blocks per CU for the wavefront bins
     int grid_mid = 16;      // resident blocks per CU for the 256-thread bin
-    int wedge_log2r = 13;   // endpoint tile of the heavy 4-cycle pass: 2^log2r LDS counters
+    int wedge_log2r = 14;   // endpoint tile of the heavy 4-cycle pass: 2^log2r 16-bit LDS counters
     int wedge_nt = 512;     // threads per block of the heavy 4-cycle pass (256, 512 or 1024)
     int wedge_split = 4;    // work items per block the heavy wedges are cut into (load balance)
     int tri_lists = 64;     // kept local-edge lists, entries per edge (0: off)

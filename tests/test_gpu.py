@@ -177,6 +177,30 @@ def test_tuning_never_changes_counts(ti, ctx):
     g.free()
 
 
+def test_wedge_counter_overflow(ctx):
+    """A hub with 70,000 in-neighbours that all reach one other vertex: the
+    16-bit endpoint counter of that vertex overflows and its tile is
+    recounted with 32-bit counters.  Closed forms: every wedge a-b-c closes
+    69,999 induced 4-cycles a-b-c-b'."""
+    k = 70000
+    a, c, x = 0, 1, 2            # x: an extra neighbour so that a is the top rank
+    bs = np.arange(3, 3 + k, dtype=np.int32)
+    src = np.concatenate([np.full(k, a, np.int32), np.full(k, c, np.int32), np.array([a], np.int32)])
+    dst = np.concatenate([bs, bs, np.array([x], np.int32)])
+    n = 3 + k
+    g = pgd_amd.GraphletCore(src, dst, n=n, ctx=ctx)
+    g.sort_edges("deg")
+    for log2r in (8, 14):
+        g.set_tuning(wedge_log2r=log2r)
+        g.decompose(engine="orient")
+        T, K4, C4 = g.edge_counts()
+        ev, eu = g.edge_arrays()
+        assert int(T.sum()) == 0 and int(K4.sum()) == 0
+        want = np.where((ev == x) | (eu == x), 0, k - 1).astype(np.uint64)
+        assert np.array_equal(C4, want), log2r
+    g.free()
+
+
 def test_tuning_rejects_bad_values(ctx):
     s, d = pgd_amd.gen_gnp(100, 0.1, 1)
     g = pgd_amd.GraphletCore(s, d, ctx=ctx)
