@@ -651,6 +651,10 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
     }
     if (st.n_huge) {
         st.ghuge = (int)std::min<int64_t>(st.n_huge, ctx->num_cus);
+        // its local-edge lists (kept for the clique pass) take maxk(maxk-1)/2
+        // entries per block: fewer blocks when that exceeds 4 GiB in all
+        const int64_t per = std::max<int64_t>(1, (int64_t)st.maxk * (st.maxk - 1) / 2) * 8;
+        st.ghuge = (int)std::max<int64_t>(1, std::min<int64_t>(st.ghuge, ((int64_t)4 << 30) / per));
         ensure(st.hscratch, ctx, (size_t)st.ghuge * carve_bytes(st.maxk, (st.maxk + 31) / 32));
     }
     {
@@ -662,10 +666,14 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
         const int64_t bs = st.n_small ? grid_cap(ctx, st.n_small, prm.grid_small) : 0;
         const int64_t bm = st.n_mid ? grid_cap(ctx, st.n_mid, prm.grid_mid) : 0;
         const int64_t bl = st.n_large ? grid_cap(ctx, st.n_large, 1) : 0;
-        ensure(st.elist, ctx, (size_t)(bs * st.ecap_small + bm * st.ecap_mid + bl * st.ecap_large));
+        st.ecap_huge = st.n_huge ? cap(st.maxk) : 0;
+        const int64_t bh = st.n_huge ? st.ghuge : 0;
+        ensure(st.elist,
+               ctx, (size_t)(bs * st.ecap_small + bm * st.ecap_mid + bl * st.ecap_large + bh * st.ecap_huge));
         st.el_small = st.elist.get();
         st.el_mid = st.el_small + bs * st.ecap_small;
         st.el_large = st.el_mid + bm * st.ecap_mid;
+        st.el_huge = st.el_large + bl * st.ecap_large;
     }
     {
         // global local-edge lists (triangle pass -> clique pass), as many
@@ -733,7 +741,7 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_tri<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, out.T, w);
+                           st.hscratch.get(), st.el_huge, st.ecap_huge, tl, out.T, w);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (timer) timer->begin("k_local_k4q");

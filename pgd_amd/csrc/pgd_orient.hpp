@@ -229,8 +229,8 @@ struct OrientState {
     size_t small_lds = 0, mid_lds = 0, large_lds = 0;
     DBuf<char> hscratch;
     DBuf<uint64_t> elist;  // local-edge lists of the clique pass (per block of each LDS bin)
-    uint64_t *el_small = nullptr, *el_mid = nullptr, *el_large = nullptr;
-    int64_t ecap_small = 0, ecap_mid = 0, ecap_large = 0;
+    uint64_t *el_small = nullptr, *el_mid = nullptr, *el_large = nullptr, *el_huge = nullptr;
+    int64_t ecap_small = 0, ecap_mid = 0, ecap_large = 0, ecap_huge = 0;
     DBuf<uint64_t> tlist, tcur;  // local-edge lists kept from the triangle pass for the clique pass
     DBuf<int64_t> toff;
     DBuf<int32_t> tlen;

@@ -201,6 +201,24 @@ def test_wedge_counter_overflow(ctx):
     g.free()
 
 
+@pytest.mark.parametrize("lists", [64, 0])
+def test_local_graphs_beyond_lds(lists, ctx):
+    """K_1000: the lowest-ranked vertices have out-degree up to 999, beyond
+    what one block's LDS holds, so their local graphs run from HBM scratch.
+    Closed forms of K_n per edge: T = n-2, K4 = C(n-2, 2), C4 = 0."""
+    n = 1000
+    i, j = np.triu_indices(n, 1)
+    g = pgd_amd.GraphletCore(i.astype(np.int32), j.astype(np.int32), n=n, ctx=ctx)
+    g.sort_edges("deg")
+    g.set_tuning(tri_lists=lists)
+    g.decompose(engine="orient")
+    T, K4, C4 = g.edge_counts()
+    assert np.all(T == n - 2)
+    assert np.all(K4 == (n - 2) * (n - 3) // 2)
+    assert np.all(C4 == 0)
+    g.free()
+
+
 def test_tuning_rejects_bad_values(ctx):
     s, d = pgd_amd.gen_gnp(100, 0.1, 1)
     g = pgd_amd.GraphletCore(s, d, ctx=ctx)
