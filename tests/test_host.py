@@ -131,3 +131,36 @@ def test_generators_deterministic():
     assert len(set(zip(np.minimum(s, d).tolist(), np.maximum(s, d).tolist()))) == len(s)
     m = len(pgd_amd.gen_gnp(2000, 0.01, 1)[0])
     assert 18000 < m < 22000
+
+
+def test_tuning_defaults():
+    """pgd_tuning_default (host only) returns the documented defaults."""
+    t = pgd_amd.default_tuning().as_dict()
+    assert t == {"small_k": 64, "mid_k": 256, "grid_small": 32, "grid_mid": 16, "wedge_log2r": 14,
+                 "wedge_nt": 512, "wedge_split": 4, "tri_lists": 64}
+    with pytest.raises(KeyError):
+        pgd_amd.default_tuning(no_such_field=1)
+
+
+def test_server_footprint_formula():
+    """Admission estimate: CSR 8m + 8(n+1), per-edge scratch and outputs 64m, 4 KiB."""
+    for n, m in ((0, 0), (20, 50), (5000, 80000), (262144, 3805169)):
+        assert pgd_amd.server_footprint(n, m) == 8 * m + 8 * (n + 1) + 64 * m + 4096
+    assert pgd_amd.server_footprint(-1, 5) < 0
+
+
+def test_batched_workload_is_seeded():
+    """The batched workload: same seed, same graphs; the size law and families."""
+    from pgd_amd.workloads import batched_graph, batched_graphs
+    a = batched_graphs(24, seed=3)
+    b = batched_graphs(24, seed=3)
+    for (n1, s1, d1), (n2, s2, d2) in zip(a, b):
+        assert n1 == n2 and np.array_equal(s1, s2) and np.array_equal(d1, d2)
+    fams = [batched_graph(i, seed=3)[3] for i in range(24)]
+    assert fams[::2] == ["ba"] * 12 and fams[1::2] == ["gnp"] * 12
+    ns = np.array([g[0] for g in batched_graphs(400, seed=5)])
+    assert ns.min() >= 20 and ns.max() <= 5000
+    # log-uniform on [20, 5000]: the median is near sqrt(20 * 5000) = 316
+    assert 200 < np.median(ns) < 500
+    for n, s, d in a:
+        assert np.all(s != d) and (len(s) == 0 or (s.max() < n and d.max() < n))
