@@ -272,13 +272,16 @@ struct TriLists {
 };
 
 // pass 1: triangles.  Local edge (i,j), i<j <=> triangle {a, L[i], L[j]}.
+// The edge (L[i], L[j]) is counted in TS at its out-slot (folded into T by
+// edge id afterwards); the edges (a, L[i]) through the block's counters.
 template <int NT, bool kGlobal>
 __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                                                   const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart,
                                                   const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
                                                   int64_t cnt, int kmax, char* __restrict__ gscratch,
                                                   uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
-                                                  uint32_t* __restrict__ T, unsigned long long* __restrict__ work) {
+                                                  uint32_t* __restrict__ TS, uint32_t* __restrict__ T,
+                                                  unsigned long long* __restrict__ work) {
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
     __shared__ int ecount;
@@ -306,14 +309,12 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
 #pragma unroll
             for (int u = 0; u < kU; u++) {
                 const bool hit = j[u] >= 0;
-                int32_t e = 0;
                 if (hit) {
-                    e = eid[slot[u]];
-                    atomicAdd(&T[e], 1u);
+                    atomicAdd(&TS[slot[u]], 1u);  // the edge (L[i], L[j]) by its out-slot: consecutive lanes
                     atomicAdd(&A.cnt[i[u]], 1u);
                     atomicAdd(&A.cnt[j[u]], 1u);
                 }
-                if (el) record_hits(el, &ecount, hit, i[u], j[u], e);
+                if (el) record_hits(el, &ecount, hit, i[u], j[u], hit ? eid[slot[u]] : 0);
             }
         }
         __syncthreads();
@@ -568,6 +569,14 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n) {
 __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
     GRID_STRIDE(i, n) p[i] = v;
 }
+// T[eid[s]] += TS[s] for every slot s with a count
+__global__ void k_fold_u32(const uint32_t* __restrict__ TS, const int32_t* __restrict__ eid, int64_t nnz,
+                           uint32_t* __restrict__ T) {
+    GRID_STRIDE(s, nnz) {
+        const uint32_t v = TS[s];
+        if (v) atomicAdd(&T[eid[s]], v);
+    }
+}
 
 }  // namespace
 
@@ -600,6 +609,7 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
     ensure(st.ub, ctx, n);
     ensure(st.P, ctx, m);
     ensure(st.Q, ctx, m);
+    ensure(st.TS, ctx, nnz);
     ensure(st.work, ctx, 4);
     int gn = grid_cap(ctx, (n + 255) / 256, 16), gm = grid_cap(ctx, (m + 255) / 256, 16);
     const int vb = bitsfor(n - 1), db = bitsfor(g.max_deg), gb = g.group ? bitsfor(g.ngroups - 1) : 0;
@@ -722,26 +732,30 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     } else {
         tl.G = nullptr;
     }
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_cap(ctx, (2 * m + 255) / 256, 16)), dim3(256), 0, s, st.TS.get(),
+                       2 * m);
     if (timer) timer->begin("k_local_tri");
     if (st.n_small)
         hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (char*)nullptr, st.el_small,
-                           st.ecap_small, tl, out.T, w);
+                           st.ecap_small, tl, st.TS.get(), out.T, w);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (char*)nullptr, st.el_mid,
-                           st.ecap_mid, tl, out.T, w);
+                           st.ecap_mid, tl, st.TS.get(), out.T, w);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_tri<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_large.get(), st.n_large, st.kmax_lds, (char*)nullptr, st.el_large,
-                           st.ecap_large, tl, out.T, w);
+                           st.ecap_large, tl, st.TS.get(), out.T, w);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_tri<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.hscratch.get(), st.el_huge, st.ecap_huge, tl, out.T, w);
+                           st.hscratch.get(), st.el_huge, st.ecap_huge, tl, st.TS.get(), out.T, w);
+    hipLaunchKernelGGL(k_fold_u32, dim3(grid_cap(ctx, (2 * m + 255) / 256, 16)), dim3(256), 0, s, st.TS.get(),
+                       st.reid.get(), 2 * m, out.T);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (timer) timer->begin("k_local_k4q");

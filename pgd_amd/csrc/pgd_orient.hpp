@@ -236,6 +236,7 @@ struct OrientState {
     DBuf<int32_t> tlen;
     int64_t tcap = 0;
     DBuf<uint64_t> P, Q;
+    DBuf<uint32_t> TS;  // per-slot triangle counts of the triangle pass
     DBuf<unsigned long long> work;
     WedgePlan wp;
 };
