@@ -32,11 +32,23 @@ __global__ void w_fill_u32(uint32_t* __restrict__ p, int64_t n) {
     GRID_STRIDE(i, n) p[i] = 0;
 }
 
-// wcnt[s] = |N(b) below a| for every in-slot s = (a, b) of a heavy hub a,
-// and the hub's first endpoint tile (the lowest tile any N(b) reaches); one
-// wavefront per hub
+// outslot[e] = the slot of edge e in the list of its lower-ranked end (the
+// out-slots of v are [ostart[v], row[v+1])); one wavefront per vertex
+__global__ void k_out_slots(const int64_t* __restrict__ row, const int64_t* __restrict__ ostart,
+                            const int32_t* __restrict__ eid, int64_t n, int32_t* __restrict__ outslot) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t v = wave; v < n; v += nw)
+        for (int64_t s = ostart[v] + lane; s < row[v + 1]; s += 64) outslot[eid[s]] = (int32_t)s;
+}
+
+// wcnt[s] = |N(b) below a| for every in-slot s = (a, b) of a heavy hub a --
+// the offset, within N(b), of the twin slot b -> a -- and the hub's first
+// endpoint tile (the lowest tile any N(b) reaches); one wavefront per hub
 __global__ void k_pair_wedges(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
-                              const int64_t* __restrict__ ostart, const int32_t* __restrict__ heavy, int64_t nheavy,
+                              const int64_t* __restrict__ ostart, const int32_t* __restrict__ eid,
+                              const int32_t* __restrict__ outslot, const int32_t* __restrict__ heavy, int64_t nheavy,
                               int log2r, int32_t* __restrict__ wcnt, int32_t* __restrict__ hbase) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -46,8 +58,9 @@ __global__ void k_pair_wedges(const int64_t* __restrict__ row, const int32_t* __
         int32_t lo = a;
         for (int64_t s = row[a] + lane; s < ostart[a]; s += 64) {
             const int32_t b = col[s];
-            wcnt[s] = (int32_t)(lower_bound_g(col, row[b], row[b + 1], a) - row[b]);
-            lo = min(lo, col[row[b]]);
+            const int64_t rb = row[b];
+            wcnt[s] = (int32_t)(outslot[eid[s]] - rb);
+            lo = min(lo, col[rb]);
         }
         for (int off = 32; off > 0; off >>= 1) lo = min(lo, __shfl_xor(lo, off, 64));
         if (lane == 0) hbase[h] = lo >> log2r;
@@ -441,8 +454,16 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     // pair wedge counts
     ensure(wp.wcnt, ctx, nnz);
     ensure(wp.hbase, ctx, nh);
-    hipLaunchKernelGGL(k_pair_wedges, dim3(grid_cap(ctx, (nh * 64 + 255) / 256, 16)), dim3(256), 0, s, st.rrow.get(),
-                       st.rcol.get(), st.ostart.get(), st.b_heavy.get(), nh, wp.log2r, wp.wcnt.get(), wp.hbase.get());
+    {
+        // st.v0 (2m int32, free after the relabel) holds the out-slot of every edge
+        int32_t* outslot = st.v0.get();
+        hipLaunchKernelGGL(k_out_slots, dim3(grid_cap(ctx, (n * 64 + 255) / 256, 16)), dim3(256), 0, s,
+                           st.rrow.get(), st.ostart.get(), st.reid.get(), n, outslot);
+        hipLaunchKernelGGL(k_pair_wedges, dim3(grid_cap(ctx, (nh * 64 + 255) / 256, 16)), dim3(256), 0, s,
+                           st.rrow.get(), st.rcol.get(), st.ostart.get(), st.reid.get(), outslot, st.b_heavy.get(),
+                           nh, wp.log2r, wp.wcnt.get(), wp.hbase.get());
+        PGD_HIP(hipGetLastError());
+    }
     PGD_HIP(hipGetLastError());
     // tile table (st.v0 and st.k0 are free scratch of 2m entries after the relabel)
     {
