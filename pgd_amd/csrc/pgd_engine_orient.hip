@@ -228,6 +228,7 @@ __device__ int64_t load_local(const int64_t* __restrict__ row, const int32_t* __
 }
 
 constexpr int kU = 4;  // flattened positions per thread per round (independent searches and loads)
+constexpr int kListChunk = 2048;  // kept-list entries a block reserves at a time
 
 // owner i and CSR slot of the flattened positions t0, t0+NT, ... (i = -1 past the end)
 template <int NT>
@@ -290,12 +291,15 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
     char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, W) : lds;
     LocalArrays A = carve(base, kmax, W);
     uint64_t* el = (elist && tl.G) ? elist + (size_t)blockIdx.x * (size_t)ecap : nullptr;
+    unsigned long long done = 0;       // local work of this block, added once at the end
+    __shared__ int64_t chunk_pos, chunk_end;  // the block's reserved part of the kept lists
+    if (threadIdx.x == 0) chunk_pos = chunk_end = 0;
     for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
         const int32_t a = list[li];
         const int k = outdeg[a];
         if (threadIdx.x == 0) ecount = 0;
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
-        if (threadIdx.x == 0) atomicAdd(work, (unsigned long long)total);
+        if (threadIdx.x == 0) done += (unsigned long long)total;
         for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
             int i[kU], j[kU];
             int64_t slot[kU];
@@ -323,8 +327,19 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
         if (el) {
             const int ne = ecount;
             if (threadIdx.x == 0) {
-                int64_t p = (int64_t)atomicAdd(tl.cur, (unsigned long long)ne);
-                if (p + ne > tl.gcap) p = -1;
+                // lists are placed in chunks the block reserves (one global
+                // atomic per chunk, not per vertex)
+                if (chunk_pos + ne > chunk_end) {
+                    const int64_t want = max((int64_t)ne, (int64_t)kListChunk);
+                    const int64_t c = (int64_t)atomicAdd(tl.cur, (unsigned long long)want);
+                    chunk_pos = c;
+                    chunk_end = c + want <= tl.gcap ? c + want : c;  // an empty chunk once G is full
+                }
+                int64_t p = -1;
+                if (chunk_pos + ne <= chunk_end) {
+                    p = chunk_pos;
+                    chunk_pos += ne;
+                }
                 gpos = p;
                 tl.off[a] = p;
                 tl.len[a] = p < 0 ? -1 : ne;
@@ -336,6 +351,7 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
         }
         __syncthreads();
     }
+    if (done) atomicAdd(work, done);
 }
 
 // pass 2: 4-cliques and Q, with the local bit matrix.  The pass that builds
@@ -375,6 +391,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
     char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, Wmax) : lds;
     LocalArrays A = carve(base, kmax, Wmax);
     uint64_t* el = elist ? elist + (size_t)blockIdx.x * (size_t)ecap : nullptr;
+    unsigned long long done = 0;
     for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
         const int32_t a = list[li];
         const int k = outdeg[a];
@@ -382,7 +399,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
         for (int w = threadIdx.x; w < k * W; w += NT) A.M[w] = 0;
         if (threadIdx.x == 0) ecount = 0;
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
-        if (threadIdx.x == 0) atomicAdd(work, (unsigned long long)total);
+        if (threadIdx.x == 0) done += (unsigned long long)total;
         for (int i = threadIdx.x; i < k; i += NT) A.ta[i] = T[A.Le[i]];
         // the triangle pass's list of a's local edges, when it kept one
         const int kept = tl.G ? tl.len[a] : -1;
@@ -448,6 +465,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
         }
         __syncthreads();
     }
+    if (done) atomicAdd(work, done);
 }
 
 // ---------------------------------------------------------------------------
@@ -472,6 +490,7 @@ __global__ __launch_bounds__(64) void k_wedge_light(const int64_t* __restrict__ 
     __shared__ uint64_t psum[64];
     __shared__ int64_t part[64];
     const int tid = threadIdx.x;
+    unsigned long long done = 0;
     for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
         const int32_t a = list[li];
         const int64_t ib = row[a], ie = ostart[a];
@@ -490,7 +509,7 @@ __global__ __launch_bounds__(64) void k_wedge_light(const int64_t* __restrict__ 
                 __syncthreads();
                 block_incl_scan<64>(pref, kc, part);
                 const int64_t total = pref[kc - 1];
-                if (pass == 0 && tid == 0) atomicAdd(work, (unsigned long long)total);
+                if (pass == 0 && tid == 0) done += (unsigned long long)total;
                 for (int64_t t = tid; t < total; t += 64) {
                     const int i = owner_of(pref, kc, t);
                     const int64_t ex = i ? pref[i - 1] : 0;
@@ -519,6 +538,7 @@ __global__ __launch_bounds__(64) void k_wedge_light(const int64_t* __restrict__ 
             }
         }
     }
+    if (done) atomicAdd(work, done);
 }
 
 // C4 = P - Q + 2T + 2K4 (uint64 wrap-around; the true value is >= 0)
