@@ -221,8 +221,8 @@ template <int NT>
 __device__ __forceinline__ void tile_pass(int pass, int mode, uint32_t sb, uint32_t se, int32_t c0, int log2r,
                                           const WedgeSeg* __restrict__ seg, const int32_t* __restrict__ col,
                                           uint32_t* cnt, uint64_t* __restrict__ PS, int32_t* wpref, int64_t* wstart,
-                                          int32_t* wslot, unsigned long long* wpsum, int* ovf, int lane, int w,
-                                          unsigned long long& done) {
+                                          int32_t* wslot, unsigned long long* wpsum, uint8_t* wflag, int* ovf,
+                                          int lane, int w, unsigned long long& done) {
     const int32_t half = 1 << (log2r - 1);
     const int32_t lo = mode == 2 ? half : 0;
     for (uint32_t gb = sb + (uint32_t)w * 64; gb < se; gb += NT) {
@@ -250,20 +250,34 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, uint32_t sb, uint3
         if (pass == 0 && mode < 2 && lane == 0) done += (unsigned long long)total;
         // kU positions per lane per round: their owner searches and endpoint
         // loads are independent and overlap
+        // (the owner of position q, the segment holding it, is the last segment
+        // starting at or before q: the segment starts inside a step are
+        // flagged in LDS and read back as one 64-bit ballot per round, so the
+        // owner is a popcount, not a search -- segments are non-empty, so their
+        // starts are distinct)
         constexpr int kU = 4;
-        for (int32_t q0 = lane; q0 < total; q0 += 64 * kU) {
+        const int32_t es = lane < kc ? x - len : INT32_MAX;  // exclusive start of segment `lane`
+        for (int32_t rb0 = 0; rb0 < total; rb0 += 64 * kU) {
             int o[kU];
             int64_t slot[kU];
             int32_t c[kU];
 #pragma unroll
+            for (int u = 0; u < kU; u++) wflag[u * 64 + lane] = 0;
+            __builtin_amdgcn_wave_barrier();
+            if (es >= rb0 && es < rb0 + 64 * kU) wflag[es - rb0] = 1;
+            __builtin_amdgcn_wave_barrier();
+            int cum = __popcll(__ballot(es < rb0)) - 1;  // last segment starting before this step
+#pragma unroll
             for (int u = 0; u < kU; u++) {
-                const int32_t q = q0 + 64 * u;
+                const uint64_t starts = __ballot(wflag[u * 64 + lane] != 0);
+                const int32_t q = rb0 + 64 * u + lane;
                 o[u] = -1;
                 if (q < total) {
-                    o[u] = owner_of(wpref, kc, q);
+                    o[u] = cum + __popcll(starts & ((2ULL << lane) - 1));
                     const int32_t ex = o[u] ? wpref[o[u] - 1] : 0;
                     slot[u] = wstart[o[u]] + (q - ex);
                 }
+                cum += __popcll(starts);
             }
 #pragma unroll
             for (int u = 0; u < kU; u++)
@@ -321,6 +335,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     __shared__ int64_t wstart[NW][64];
     __shared__ int32_t wpref[NW][64];
     __shared__ int32_t wslot[NW][64];
+    __shared__ uint8_t wflag[NW][4 * 64];  // segment-start flags of one step of a wavefront
     __shared__ unsigned long long wpsum[NW][64];
     __shared__ int64_t item_sh;
     __shared__ int ovf;
@@ -382,21 +397,21 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
             if (tid == 0) ovf = 0;
             __syncthreads();
             tile_pass<NT>(0, 0, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w], wpsum[w],
-                          &ovf, lane, w, done);
+                          wflag[w], &ovf, lane, w, done);
             __syncthreads();
             if (!ovf) {
                 tile_pass<NT>(1, 0, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w], wpsum[w],
-                              &ovf, lane, w, done);
+                              wflag[w], &ovf, lane, w, done);
                 __syncthreads();
             } else {
                 for (int mode = 1; mode <= 2; mode++) {
                     for (int i = tid; i < R / 2; i += NT) cnt[i] = 0;
                     __syncthreads();
                     tile_pass<NT>(0, mode, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w],
-                                  wpsum[w], &ovf, lane, w, done);
+                                  wpsum[w], wflag[w], &ovf, lane, w, done);
                     __syncthreads();
                     tile_pass<NT>(1, mode, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w],
-                                  wpsum[w], &ovf, lane, w, done);
+                                  wpsum[w], wflag[w], &ovf, lane, w, done);
                     __syncthreads();
                 }
             }
