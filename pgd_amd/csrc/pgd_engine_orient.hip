@@ -246,19 +246,34 @@ __device__ __forceinline__ void locate(const LocalArrays& A, int k, int64_t tota
     }
 }
 
-// Record the hits of one round of flattened positions in the block's local
-// edge list: one LDS reservation per wavefront.  Entry = (edge id of
-// (L[i], L[j]) << 32) | (j << 16) | i.
-__device__ __forceinline__ void record_hits(uint64_t* __restrict__ el, int* ecount, bool hit, int i, int j,
-                                            int32_t e) {
+// Record the hits of one round (kU positions per thread) in the block's
+// local edge list: one LDS reservation per wavefront per round.  Entry =
+// (edge id of (L[i], L[j]) << 32) | (j << 16) | i.
+template <int KU>
+__device__ __forceinline__ void record_hits(uint64_t* __restrict__ el, int* ecount, const int* i, const int* j,
+                                            const int32_t* e) {
     const int lane = threadIdx.x & 63;
+    int mine = 0;
+#pragma unroll
+    for (int u = 0; u < KU; u++) mine += j[u] >= 0;
+    // exclusive scan of the lanes' hit counts
+    int x = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
     const uint64_t active = __ballot(1);
-    const uint64_t hits = __ballot(hit);
+    const int last = 63 - __clzll((long long)active);
+    const int wave_total = __shfl(x, last, 64);
     const int leader = __ffsll((unsigned long long)active) - 1;
     int basepos = 0;
-    if (lane == leader && hits) basepos = atomicAdd(ecount, __popcll(hits));
+    if (lane == leader && wave_total) basepos = atomicAdd(ecount, wave_total);
     basepos = __shfl(basepos, leader, 64);
-    if (hit) el[basepos + __popcll(hits & ((1ULL << lane) - 1))] = ((uint64_t)(uint32_t)e << 32) | ((uint32_t)j << 16) | (uint32_t)i;
+    int pos = basepos + x - mine;
+#pragma unroll
+    for (int u = 0; u < KU; u++)
+        if (j[u] >= 0) el[pos++] = ((uint64_t)(uint32_t)e[u] << 32) | ((uint32_t)j[u] << 16) | (uint32_t)i[u];
 }
 
 // Lists of local edges kept for the clique pass: the triangle pass appends
@@ -310,16 +325,17 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
                 if (i[u] >= 0) x[u] = col[slot[u]];
 #pragma unroll
             for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_local(A, x[u]) : -1;
+            int32_t e[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
-                const bool hit = j[u] >= 0;
-                if (hit) {
+                if (j[u] >= 0) {
                     atomicAdd(&TS[slot[u]], 1u);  // the edge (L[i], L[j]) by its out-slot: consecutive lanes
                     atomicAdd(&A.cnt[i[u]], 1u);
                     atomicAdd(&A.cnt[j[u]], 1u);
+                    if (el) e[u] = eid[slot[u]];
                 }
-                if (el) record_hits(el, &ecount, hit, i[u], j[u], hit ? eid[slot[u]] : 0);
             }
+            if (el) record_hits<kU>(el, &ecount, i, j, e);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < k; i += NT)
@@ -423,15 +439,16 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                 if (i[u] >= 0) x[u] = col[slot[u]];
 #pragma unroll
             for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_local(A, x[u]) : -1;
+            int32_t e[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
-                const bool hit = j[u] >= 0;
-                if (hit) {
+                if (j[u] >= 0) {
                     atomicOr(&A.M[i[u] * W + (j[u] >> 5)], 1u << (j[u] & 31));
                     atomicOr(&A.M[j[u] * W + (i[u] >> 5)], 1u << (i[u] & 31));
+                    if (el) e[u] = eid[slot[u]];
                 }
-                if (el) record_hits(el, &ecount, hit, i[u], j[u], hit ? eid[slot[u]] : 0);
             }
+            if (el) record_hits<kU>(el, &ecount, i, j, e);
         }
         __syncthreads();
         if (gl || el) {
