@@ -112,7 +112,88 @@ __device__ __forceinline__ int64_t tt_seek(const uint64_t* __restrict__ tt, int6
     return lo;
 }
 
-// segments and wedges per (hub, tile) cell; one block per hub
+// first tile-table entry of b, in [lo, hi), whose offset is >= w (entries
+// ascending by offset)
+__device__ __forceinline__ int64_t tt_seek_off(const uint64_t* __restrict__ tt, int64_t lo, int64_t hi, uint32_t w) {
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((uint32_t)tt[mid] < w) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The tile-table entries of 64 pairs (a, b), flattened across a wavefront.
+// Lane i describes pair i: entries [jb, je) of b's table (possibly empty),
+// the end of b's table, |N(b)| and the pair's bound w.  fn(pair, tile,
+// offset, len) is called once per entry, consecutive lanes on consecutive
+// entries of a pair; pair is the rank of the lane among the lanes with
+// entries.  flag: 64*4 bytes of the wavefront's LDS; pj, pe, pw, pd: 64
+// entries each, indexed by that rank.
+template <class F>
+__device__ __forceinline__ void walk_entries(int64_t jb, int64_t je, int64_t tend, uint32_t deg, uint32_t w,
+                                             int64_t* pj, int64_t* pe, uint32_t* pw, uint32_t* pd, uint8_t* flag,
+                                             const uint64_t* __restrict__ tt, int lane, F&& fn) {
+    const int32_t cnt = (int32_t)(je > jb ? je - jb : 0);
+    int32_t x = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    const int32_t total = __shfl(x, 63, 64);
+    const uint64_t nz = __ballot(cnt > 0);
+    if (cnt > 0) {
+        // owners are counted over the lanes with entries only
+        const int r = __popcll(nz & ((1ULL << lane) - 1));
+        pj[r] = jb - (x - cnt);  // entry index = pj[owner] + flattened position
+        pe[r] = tend;
+        pw[r] = w;
+        pd[r] = deg;
+    }
+    const int32_t es = cnt ? x - cnt : INT32_MAX;
+    __builtin_amdgcn_wave_barrier();
+    constexpr int kU = 4;
+    for (int32_t rb0 = 0; rb0 < total; rb0 += 64 * kU) {
+#pragma unroll
+        for (int u = 0; u < kU; u++) flag[u * 64 + lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+        if (es >= rb0 && es < rb0 + 64 * kU) flag[es - rb0] = 1;
+        __builtin_amdgcn_wave_barrier();
+        int cum = __popcll(__ballot(es < rb0)) - 1;
+        int o[kU];
+        int64_t j[kU];
+        uint64_t e[kU], en[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const uint64_t starts = __ballot(flag[u * 64 + lane] != 0);
+            const int32_t q = rb0 + 64 * u + lane;
+            o[u] = -1;
+            if (q < total) {
+                o[u] = cum + __popcll(starts & ((2ULL << lane) - 1));
+                j[u] = pj[o[u]] + q;
+            }
+            cum += __popcll(starts);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+            if (o[u] >= 0) {
+                e[u] = tt[j[u]];
+                en[u] = j[u] + 1 < pe[o[u]] ? tt[j[u] + 1] : (uint64_t)pd[o[u]];
+            }
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+            if (o[u] >= 0) {
+                const uint32_t off = (uint32_t)e[u];
+                const uint32_t len = min((uint32_t)en[u], pw[o[u]]) - off;
+                fn(o[u], (uint32_t)(e[u] >> 32), off, len);
+            }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// segments and wedges per (hub, tile) cell; one block per hub, its pairs
+// 64 per wavefront
 template <int NT>
 __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                                                    const int64_t* __restrict__ ostart,
@@ -121,8 +202,13 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
                                                    const int64_t* __restrict__ ttrow, const int32_t* __restrict__ hbase,
                                                    const int64_t* __restrict__ htoff, uint32_t* __restrict__ htseg,
                                                    uint64_t* __restrict__ htw) {
+    constexpr int NW = NT / 64;
     __shared__ uint32_t lseg[kPlanLds];
     __shared__ unsigned long long lw[kPlanLds];
+    __shared__ int64_t pj[NW][64], pe[NW][64];
+    __shared__ uint32_t pw[NW][64], pd[NW][64];
+    __shared__ uint8_t flag[NW][4 * 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
         const int32_t a = heavy[h];
         const int64_t c0 = htoff[h], T = htoff[h + 1] - c0;
@@ -131,26 +217,30 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
         if (in_lds)
             for (int i = threadIdx.x; i < T; i += NT) { lseg[i] = 0; lw[i] = 0; }
         __syncthreads();
-        for (int64_t s = row[a] + threadIdx.x; s < ostart[a]; s += NT) {
-            const int32_t b = col[s];
-            const uint32_t w = (uint32_t)wcnt[s];
-            const int64_t j1 = ttrow[b + 1];
-            const uint32_t deg = (uint32_t)(row[b + 1] - row[b]);
-            for (int64_t j = ttrow[b]; j < j1; j++) {
-                const uint64_t e = tt[j];
-                const uint32_t off = (uint32_t)e;
-                if (off >= w) break;
-                const uint32_t nxt = j + 1 < j1 ? (uint32_t)tt[j + 1] : deg;
-                const uint32_t len = min(nxt, w) - off;
-                const uint32_t t = (uint32_t)(e >> 32) - base;
-                if (in_lds) {
-                    atomicAdd(&lseg[t], 1u);
-                    atomicAdd(&lw[t], (unsigned long long)len);
-                } else {
-                    atomicAdd(&htseg[c0 + t], 1u);
-                    atomicAdd((unsigned long long*)&htw[c0 + t], (unsigned long long)len);
-                }
+        const int64_t ib = row[a], ie = ostart[a];
+        for (int64_t g0 = ib + (int64_t)wv * 64; g0 < ie; g0 += NT) {
+            const int64_t s = g0 + lane;
+            int64_t jb = 0, je = 0, tend = 0;
+            uint32_t w = 0, deg = 0;
+            if (s < ie) {
+                const int32_t b = col[s];
+                w = (uint32_t)wcnt[s];
+                jb = ttrow[b];
+                tend = ttrow[b + 1];
+                deg = (uint32_t)(row[b + 1] - row[b]);
+                je = tt_seek_off(tt, jb, tend, w);  // entries starting below the bound
             }
+            walk_entries(jb, je, tend, deg, w, pj[wv], pe[wv], pw[wv], pd[wv], flag[wv], tt, lane,
+                         [&](int, uint32_t tile, uint32_t, uint32_t len) {
+                             const uint32_t t = tile - base;
+                             if (in_lds) {
+                                 atomicAdd(&lseg[t], 1u);
+                                 atomicAdd(&lw[t], (unsigned long long)len);
+                             } else {
+                                 atomicAdd(&htseg[c0 + t], 1u);
+                                 atomicAdd((unsigned long long*)&htw[c0 + t], (unsigned long long)len);
+                             }
+                         });
         }
         __syncthreads();
         if (in_lds)
