@@ -301,18 +301,27 @@ __global__ void k_item_keys(const uint64_t* __restrict__ key, int64_t n, uint64_
     }
 }
 
-// One pass of one wavefront over the segments of a tile (groups of 64
-// segments, their lengths scanned in registers, flattened 64 lanes at a
-// time).  pass 0 counts endpoints, pass 1 adds cnt-1 per wedge to the slot's
+// One pass of one wavefront over its share of the segments of a tile.
+// pass 0 counts endpoints, pass 1 adds cnt-1 per wedge to the slot's
 // accumulator and, per segment, to the edge (a, b).  mode 0: 16-bit counters
-// over the whole tile (a count reaching 65536 sets *ovf); mode 1/2: 32-bit
-// counters over the lower/upper half of the tile, other endpoints skipped.
+// over the whole tile; with chk, a count reaching 65536 sets *ovf (without
+// chk the hub has fewer than 65536 in-neighbours, so no count can); mode
+// 1/2: 32-bit counters over the lower/upper half of the tile, other
+// endpoints skipped.
+//
+// Each wavefront takes groups of 64 segments, scans their lengths in
+// registers and walks the flattened wedges; every lane computes kU positions
+// per round.  Positions past the end are clamped to the last one (same
+// owner, same slot), so the owner lookups, the endpoint loads and the
+// counter reads are issued for all lanes without branches and overlap; only
+// the updates are predicated.
 template <int NT>
-__device__ __forceinline__ void tile_pass(int pass, int mode, uint32_t sb, uint32_t se, int32_t c0, int log2r,
-                                          const WedgeSeg* __restrict__ seg, const int32_t* __restrict__ col,
-                                          uint32_t* cnt, uint64_t* __restrict__ PS, int32_t* wpref, int64_t* wstart,
-                                          int32_t* wslot, unsigned long long* wpsum, uint8_t* wflag, int* ovf,
-                                          int lane, int w, unsigned long long& done) {
+__device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t sb, uint32_t se, int32_t c0,
+                                          int log2r, const WedgeSeg* __restrict__ seg,
+                                          const int32_t* __restrict__ col, uint32_t* cnt,
+                                          uint64_t* __restrict__ PS, int64_t* wbase, int32_t* wslot,
+                                          unsigned long long* wpsum, uint8_t* wflag, int* ovf, int lane, int w,
+                                          unsigned long long& done) {
     const int32_t half = 1 << (log2r - 1);
     const int32_t lo = mode == 2 ? half : 0;
     for (uint32_t gb = sb + (uint32_t)w * 64; gb < se; gb += NT) {
@@ -331,24 +340,24 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, uint32_t sb, uint3
             const int32_t y = __shfl_up(x, off, 64);
             if (lane >= off) x += y;
         }
-        wpref[lane] = x;
-        wstart[lane] = st;
+        // slot of flattened position q in segment i: wbase[i] + q
+        wbase[lane] = st - (int64_t)(x - len);
         wslot[lane] = sl;
         wpsum[lane] = 0;
         __builtin_amdgcn_wave_barrier();
         const int32_t total = __shfl(x, 63, 64);
         if (pass == 0 && mode < 2 && lane == 0) done += (unsigned long long)total;
-        // kU positions per lane per round: their owner searches and endpoint
-        // loads are independent and overlap
         // (the owner of position q, the segment holding it, is the last segment
-        // starting at or before q: the segment starts inside a step are
-        // flagged in LDS and read back as one 64-bit ballot per round, so the
+        // starting at or before q: the segment starts inside a round are
+        // flagged in LDS and read back as one 64-bit ballot per step, so the
         // owner is a popcount, not a search -- segments are non-empty, so their
         // starts are distinct)
         constexpr int kU = 4;
         const int32_t es = lane < kc ? x - len : INT32_MAX;  // exclusive start of segment `lane`
+        const uint64_t upto = (2ULL << lane) - 1;             // lanes 0..lane
         for (int32_t rb0 = 0; rb0 < total; rb0 += 64 * kU) {
             int o[kU];
+            bool ok[kU];
             int64_t slot[kU];
             int32_t c[kU];
 #pragma unroll
@@ -356,47 +365,73 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, uint32_t sb, uint3
             __builtin_amdgcn_wave_barrier();
             if (es >= rb0 && es < rb0 + 64 * kU) wflag[es - rb0] = 1;
             __builtin_amdgcn_wave_barrier();
-            int cum = __popcll(__ballot(es < rb0)) - 1;  // last segment starting before this step
+            uint8_t fl[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) fl[u] = wflag[u * 64 + lane];
+            int cum = __popcll(__ballot(es < rb0)) - 1;  // last segment starting before this round
 #pragma unroll
             for (int u = 0; u < kU; u++) {
-                const uint64_t starts = __ballot(wflag[u * 64 + lane] != 0);
+                const uint64_t starts = __ballot(fl[u] != 0);
                 const int32_t q = rb0 + 64 * u + lane;
-                o[u] = -1;
-                if (q < total) {
-                    o[u] = cum + __popcll(starts & ((2ULL << lane) - 1));
-                    const int32_t ex = o[u] ? wpref[o[u] - 1] : 0;
-                    slot[u] = wstart[o[u]] + (q - ex);
-                }
+                ok[u] = q < total;
+                // past the end: the owner is the last segment; clamp to its last position
+                o[u] = cum + __popcll(starts & upto);
                 cum += __popcll(starts);
+                slot[u] = wbase[o[u]] + (ok[u] ? q : total - 1);
             }
 #pragma unroll
-            for (int u = 0; u < kU; u++)
-                if (o[u] >= 0) c[u] = col[slot[u]] - c0;
+            for (int u = 0; u < kU; u++) c[u] = col[slot[u]] - c0;
+            if (mode == 0) {
+                if (pass == 0) {
+                    if (chk) {
+                        uint32_t old[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) {
-                if (o[u] < 0) continue;
-                uint32_t v;
-                if (mode == 0) {
-                    const int sh = (c[u] & 1) * 16;
-                    if (pass == 0) {
-                        const uint32_t old = atomicAdd(&cnt[c[u] >> 1], 1u << sh);
-                        if (((old >> sh) & 0xffffu) == 0xffffu) *ovf = 1;
-                        continue;
+                        for (int u = 0; u < kU; u++)
+                            old[u] = ok[u] ? atomicAdd(&cnt[c[u] >> 1], 1u << ((c[u] & 1) * 16)) : 0u;
+                        bool hit = false;
+#pragma unroll
+                        for (int u = 0; u < kU; u++) hit |= ((old[u] >> ((c[u] & 1) * 16)) & 0xffffu) == 0xffffu;
+                        if (hit) *ovf = 1;
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < kU; u++)
+                            if (ok[u]) atomicAdd(&cnt[c[u] >> 1], 1u << ((c[u] & 1) * 16));
                     }
-                    v = ((cnt[c[u] >> 1] >> sh) & 0xffffu) - 1;
-                } else {
-                    const int32_t ci = c[u] - lo;
-                    if (ci < 0 || ci >= half) continue;
-                    if (pass == 0) {
-                        atomicAdd(&cnt[ci], 1u);
-                        continue;
+                    continue;
+                }
+                uint32_t v[kU];
+#pragma unroll
+                for (int u = 0; u < kU; u++) v[u] = ((cnt[c[u] >> 1] >> ((c[u] & 1) * 16)) & 0xffffu) - 1;
+#pragma unroll
+                for (int u = 0; u < kU; u++)
+                    if (ok[u] && v[u]) {
+                        atomicAdd((unsigned long long*)&PS[slot[u]], (unsigned long long)v[u]);
+                        atomicAdd(&wpsum[o[u]], (unsigned long long)v[u]);
                     }
-                    v = cnt[ci] - 1;
+            } else {
+                bool in[kU];
+                int32_t ci[kU];
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    ci[u] = c[u] - lo;
+                    in[u] = ok[u] && ci[u] >= 0 && ci[u] < half;
+                    ci[u] = min(max(ci[u], 0), half - 1);
                 }
-                if (v) {
-                    atomicAdd((unsigned long long*)&PS[slot[u]], (unsigned long long)v);
-                    atomicAdd(&wpsum[o[u]], (unsigned long long)v);
+                if (pass == 0) {
+#pragma unroll
+                    for (int u = 0; u < kU; u++)
+                        if (in[u]) atomicAdd(&cnt[ci[u]], 1u);
+                    continue;
                 }
+                uint32_t v[kU];
+#pragma unroll
+                for (int u = 0; u < kU; u++) v[u] = cnt[ci[u]] - 1;
+#pragma unroll
+                for (int u = 0; u < kU; u++)
+                    if (in[u] && v[u]) {
+                        atomicAdd((unsigned long long*)&PS[slot[u]], (unsigned long long)v[u]);
+                        atomicAdd(&wpsum[o[u]], (unsigned long long)v[u]);
+                    }
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -422,8 +457,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     extern __shared__ uint32_t cnt[];  // 2^log2r 16-bit endpoint counters (2^(log2r-1) words)
     __shared__ uint32_t base[kTmax + 1];
     __shared__ uint32_t cur[kTmax];
-    __shared__ int64_t wstart[NW][64];
-    __shared__ int32_t wpref[NW][64];
+    __shared__ int64_t wbase[NW][64];
     __shared__ int32_t wslot[NW][64];
     __shared__ uint8_t wflag[NW][4 * 64];  // segment-start flags of one step of a wavefront
     __shared__ unsigned long long wpsum[NW][64];
@@ -479,6 +513,8 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
         // 65535 (a hub with that many in-neighbours on one endpoint) flags the
         // tile, which is then recounted as two half tiles with 32-bit
         // counters in the same LDS (modes 1 and 2).
+        // a count is at most the hub's in-degree: below 65536 no 16-bit counter can overflow
+        const bool chk = ie - ib > 65535;
         for (int t = 0; t < T; t++) {
             const uint32_t sb = base[t], se = base[t + 1];
             if (sb == se) continue;
@@ -486,22 +522,22 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
             for (int i = tid; i < R / 2; i += NT) cnt[i] = 0;
             if (tid == 0) ovf = 0;
             __syncthreads();
-            tile_pass<NT>(0, 0, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w], wpsum[w],
-                          wflag[w], &ovf, lane, w, done);
+            tile_pass<NT>(0, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, wbase[w], wslot[w], wpsum[w], wflag[w],
+                          &ovf, lane, w, done);
             __syncthreads();
             if (!ovf) {
-                tile_pass<NT>(1, 0, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w], wpsum[w],
+                tile_pass<NT>(1, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, wbase[w], wslot[w], wpsum[w],
                               wflag[w], &ovf, lane, w, done);
                 __syncthreads();
             } else {
                 for (int mode = 1; mode <= 2; mode++) {
                     for (int i = tid; i < R / 2; i += NT) cnt[i] = 0;
                     __syncthreads();
-                    tile_pass<NT>(0, mode, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w],
-                                  wpsum[w], wflag[w], &ovf, lane, w, done);
+                    tile_pass<NT>(0, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, wbase[w], wslot[w], wpsum[w],
+                                  wflag[w], &ovf, lane, w, done);
                     __syncthreads();
-                    tile_pass<NT>(1, mode, sb, se, c0, log2r, seg, col, cnt, PS, wpref[w], wstart[w], wslot[w],
-                                  wpsum[w], wflag[w], &ovf, lane, w, done);
+                    tile_pass<NT>(1, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, wbase[w], wslot[w], wpsum[w],
+                                  wflag[w], &ovf, lane, w, done);
                     __syncthreads();
                 }
             }
