@@ -151,7 +151,7 @@ struct LocalArrays {
     uint64_t* acc;     // per-local-vertex u64 accumulator [kmax]
     int32_t* hk;       // hash of N+(a): vertex id (-1 empty) [H]
     int32_t* hv;       //                local index          [H]
-    uint32_t* M;       // bit matrix, kmax x W        [kmax*W]
+    uint32_t* M;       // bit matrix, k rows of up to mw/k words [mw]
     int hmask;
 };
 
@@ -162,7 +162,9 @@ __host__ __device__ inline int hash_slots(int kmax) {
     return h;
 }
 
-__device__ __forceinline__ LocalArrays carve(char* base, int kmax, int W) {
+// per-vertex arrays of a local graph of at most kmax vertices; the bit
+// matrix (clique pass only) follows them
+__device__ __forceinline__ LocalArrays carve(char* base, int kmax) {
     LocalArrays r;
     char* p = base;
     const int H = hash_slots(kmax);
@@ -177,13 +179,14 @@ __device__ __forceinline__ LocalArrays carve(char* base, int kmax, int W) {
     r.hv = (int32_t*)p; p += (size_t)H * 4;
     r.M = (uint32_t*)p;
     r.hmask = H - 1;
-    (void)W;
     return r;
 }
 // rounded to 256 B so every per-block HBM slice keeps 8-byte alignment for the u64 atomics
-__host__ __device__ inline size_t carve_bytes(int kmax, int W) {
-    return ((size_t)kmax * 40 + (size_t)hash_slots(kmax) * 8 + (size_t)kmax * W * 4 + 255) & ~(size_t)255;
+__host__ __device__ inline size_t carve_bytes(int kmax, int64_t mw) {
+    return ((size_t)kmax * 40 + (size_t)hash_slots(kmax) * 8 + (size_t)mw * 4 + 255) & ~(size_t)255;
 }
+// bit-matrix words that hold all of a local graph of kmax vertices
+__host__ __device__ inline int64_t full_matrix_words(int kmax) { return (int64_t)kmax * ((kmax + 31) / 32); }
 
 __device__ __forceinline__ uint32_t vhash(int32_t c) { return (uint32_t)c * 2654435761u; }
 
@@ -294,7 +297,7 @@ template <int NT, bool kGlobal>
 __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                                                   const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart,
                                                   const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
-                                                  int64_t cnt, int kmax, char* __restrict__ gscratch,
+                                                  int64_t cnt, int kmax, int64_t mw, char* __restrict__ gscratch,
                                                   uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
                                                   uint32_t* __restrict__ TS, uint32_t* __restrict__ T,
                                                   unsigned long long* __restrict__ work) {
@@ -302,9 +305,9 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
     __shared__ int64_t part[NT];
     __shared__ int ecount;
     __shared__ int64_t gpos;
-    const int W = (kmax + 31) / 32;
-    char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, W) : lds;
-    LocalArrays A = carve(base, kmax, W);
+    // mw: bit-matrix words of an HBM slice (the slices are shared with the clique pass)
+    char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, mw) : lds;
+    LocalArrays A = carve(base, kmax);
     uint64_t* el = (elist && tl.G) ? elist + (size_t)blockIdx.x * (size_t)ecap : nullptr;
     unsigned long long done = 0;       // local work of this block, added once at the end
     __shared__ int64_t chunk_pos, chunk_end;  // the block's reserved part of the kept lists
@@ -376,43 +379,55 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
 // largest k, so it cannot overflow); the scan then reads that list instead
 // of searching again.  Without a list (the HBM-scratch bin) the scan
 // searches again.
-__device__ __forceinline__ void edge_terms_k4q(const LocalArrays& A, int W, int i, int j, int32_t eij,
-                                               const uint32_t* __restrict__ T, uint64_t* __restrict__ K4,
-                                               uint64_t* __restrict__ Q) {
+//
+// When the full k x k bit matrix exceeds the block's mw words, it is held in
+// column blocks of wc words per row: the K4 terms of a local edge are summed
+// over the blocks (popcounts of partial rows), the Q terms are added once.
+__device__ __forceinline__ void edge_terms_k4q(const LocalArrays& A, int wn, bool first, int i, int j,
+                                               int32_t eij, const uint32_t* __restrict__ T,
+                                               uint64_t* __restrict__ K4, uint64_t* __restrict__ Q) {
     uint32_t kij = 0;
-    for (int w = 0; w < W; w++) kij += __popc(A.M[i * W + w] & A.M[j * W + w]);
-    const uint64_t tai = A.ta[i], taj = A.ta[j], tij = T[eij];
+    for (int w = 0; w < wn; w++) kij += __popc(A.M[i * wn + w] & A.M[j * wn + w]);
     if (kij) {
         atomicAdd((unsigned long long*)&K4[eij], (unsigned long long)kij);
         atomicAdd(&A.cnt[i], kij);
         atomicAdd(&A.cnt[j], kij);
     }
-    atomicAdd((unsigned long long*)&Q[eij], (unsigned long long)(tai + taj));
-    atomicAdd((unsigned long long*)&A.acc[i], (unsigned long long)(taj + tij));
-    atomicAdd((unsigned long long*)&A.acc[j], (unsigned long long)(tai + tij));
+    if (first) {
+        const uint64_t tai = A.ta[i], taj = A.ta[j], tij = T[eij];
+        atomicAdd((unsigned long long*)&Q[eij], (unsigned long long)(tai + taj));
+        atomicAdd((unsigned long long*)&A.acc[i], (unsigned long long)(taj + tij));
+        atomicAdd((unsigned long long*)&A.acc[j], (unsigned long long)(tai + tij));
+    }
+}
+
+// set the bits of the local edge (i, j) that fall in columns [jlo, jlo + 32 wn)
+__device__ __forceinline__ void set_pair(uint32_t* M, int wn, int jlo, int i, int j) {
+    const int ji = j - jlo, ii = i - jlo;
+    if (ji >= 0 && ji < 32 * wn) atomicOr(&M[i * wn + (ji >> 5)], 1u << (ji & 31));
+    if (ii >= 0 && ii < 32 * wn) atomicOr(&M[j * wn + (ii >> 5)], 1u << (ii & 31));
 }
 
 template <int NT, bool kGlobal>
 __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                                                   const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart,
                                                   const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
-                                                  int64_t cnt, int kmax, char* __restrict__ gscratch,
+                                                  int64_t cnt, int kmax, int64_t mw, char* __restrict__ gscratch,
                                                   uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
                                                   const uint32_t* __restrict__ T, uint64_t* __restrict__ K4,
                                                   uint64_t* __restrict__ Q, unsigned long long* __restrict__ work) {
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
     __shared__ int ecount;
-    const int Wmax = (kmax + 31) / 32;
-    char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, Wmax) : lds;
-    LocalArrays A = carve(base, kmax, Wmax);
+    char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, mw) : lds;
+    LocalArrays A = carve(base, kmax);
     uint64_t* el = elist ? elist + (size_t)blockIdx.x * (size_t)ecap : nullptr;
     unsigned long long done = 0;
     for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
         const int32_t a = list[li];
         const int k = outdeg[a];
         const int W = (k + 31) / 32;
-        for (int w = threadIdx.x; w < k * W; w += NT) A.M[w] = 0;
+        const int wc = (int)min<int64_t>(W, mw / k);  // words per row held at once (>= 1)
         if (threadIdx.x == 0) ecount = 0;
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
         if (threadIdx.x == 0) done += (unsigned long long)total;
@@ -420,61 +435,70 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
         // the triangle pass's list of a's local edges, when it kept one
         const int kept = tl.G ? tl.len[a] : -1;
         const uint64_t* gl = kept >= 0 ? tl.G + tl.off[a] : nullptr;
-        if (gl) {
-            for (int p = threadIdx.x; p < kept; p += NT) {
-                const uint64_t e = gl[p];
-                const int i = (int)(e & 0xffff), j = (int)((e >> 16) & 0xffff);
-                atomicOr(&A.M[i * W + (j >> 5)], 1u << (j & 31));
-                atomicOr(&A.M[j * W + (i >> 5)], 1u << (i & 31));
-            }
-        }
-        // build the symmetric adjacency of the local graph (and record its edges)
-        for (int64_t t0 = threadIdx.x; t0 < (gl ? 0 : total); t0 += NT * kU) {
-            int i[kU], j[kU];
-            int64_t slot[kU];
-            int32_t x[kU];
-            locate<NT>(A, k, total, t0, i, slot);
+        for (int cw = 0; cw < W; cw += wc) {
+            const int wn = min(wc, W - cw), jlo = 32 * cw;
+            const bool first = cw == 0;
+            for (int w = threadIdx.x; w < k * wn; w += NT) A.M[w] = 0;
+            __syncthreads();
+            // local edges: the kept list, this block's list (recorded while
+            // the first column block is built), or a new search
+            const uint64_t* src = gl ? gl : (first ? nullptr : el);
+            const int ne = gl ? kept : ecount;
+            if (src) {
+                for (int p = threadIdx.x; p < ne; p += NT) {
+                    const uint64_t e = src[p];
+                    set_pair(A.M, wn, jlo, (int)(e & 0xffff), (int)((e >> 16) & 0xffff));
+                }
+            } else {
+                for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
+                    int i[kU], j[kU];
+                    int64_t slot[kU];
+                    int32_t x[kU];
+                    locate<NT>(A, k, total, t0, i, slot);
 #pragma unroll
-            for (int u = 0; u < kU; u++)
-                if (i[u] >= 0) x[u] = col[slot[u]];
+                    for (int u = 0; u < kU; u++)
+                        if (i[u] >= 0) x[u] = col[slot[u]];
 #pragma unroll
-            for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_local(A, x[u]) : -1;
-            int32_t e[kU];
+                    for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_local(A, x[u]) : -1;
+                    int32_t e[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) {
-                if (j[u] >= 0) {
-                    atomicOr(&A.M[i[u] * W + (j[u] >> 5)], 1u << (j[u] & 31));
-                    atomicOr(&A.M[j[u] * W + (i[u] >> 5)], 1u << (i[u] & 31));
-                    if (el) e[u] = eid[slot[u]];
+                    for (int u = 0; u < kU; u++) {
+                        if (j[u] >= 0) {
+                            set_pair(A.M, wn, jlo, i[u], j[u]);
+                            if (first && el) e[u] = eid[slot[u]];
+                        }
+                    }
+                    if (first && el) record_hits<kU>(el, &ecount, i, j, e);
                 }
             }
-            if (el) record_hits<kU>(el, &ecount, i, j, e);
-        }
-        __syncthreads();
-        if (gl || el) {
-            const uint64_t* src = gl ? gl : el;
-            const int ne = gl ? kept : ecount;
-            for (int p = threadIdx.x; p < ne; p += NT) {
-                const uint64_t e = src[p];
-                edge_terms_k4q(A, W, (int)(e & 0xffff), (int)((e >> 16) & 0xffff), (int32_t)(e >> 32), T, K4, Q);
+            __syncthreads();
+            const uint64_t* lst = gl ? gl : el;
+            if (lst) {
+                const int nl = gl ? kept : ecount;
+                for (int p = threadIdx.x; p < nl; p += NT) {
+                    const uint64_t e = lst[p];
+                    edge_terms_k4q(A, wn, first, (int)(e & 0xffff), (int)((e >> 16) & 0xffff), (int32_t)(e >> 32), T,
+                                   K4, Q);
+                }
+            } else {
+                // no list: wc == W here (the HBM bin holds the full matrix)
+                for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
+                    int iu[kU], ju[kU];
+                    int64_t su[kU];
+                    int32_t x[kU];
+                    locate<NT>(A, k, total, t0, iu, su);
+#pragma unroll
+                    for (int u = 0; u < kU; u++)
+                        if (iu[u] >= 0) x[u] = col[su[u]];
+#pragma unroll
+                    for (int u = 0; u < kU; u++) ju[u] = iu[u] >= 0 ? find_local(A, x[u]) : -1;
+#pragma unroll
+                    for (int u = 0; u < kU; u++)
+                        if (ju[u] >= 0) edge_terms_k4q(A, wn, first, iu[u], ju[u], eid[su[u]], T, K4, Q);
+                }
             }
-        } else {
-            for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
-                int iu[kU], ju[kU];
-                int64_t su[kU];
-                int32_t x[kU];
-                locate<NT>(A, k, total, t0, iu, su);
-#pragma unroll
-                for (int u = 0; u < kU; u++)
-                    if (iu[u] >= 0) x[u] = col[su[u]];
-#pragma unroll
-                for (int u = 0; u < kU; u++) ju[u] = iu[u] >= 0 ? find_local(A, x[u]) : -1;
-#pragma unroll
-                for (int u = 0; u < kU; u++)
-                    if (ju[u] >= 0) edge_terms_k4q(A, W, iu[u], ju[u], eid[su[u]], T, K4, Q);
-            }
+            __syncthreads();
         }
-        __syncthreads();
         for (int i = threadIdx.x; i < k; i += NT) {
             // each local triangle through i is seen from its two edges at i
             if (A.cnt[i]) atomicAdd((unsigned long long*)&K4[A.Le[i]], (unsigned long long)(A.cnt[i] >> 1));
@@ -670,10 +694,13 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
                        st.outdeg.get());
     PGD_HIP(hipGetLastError());
     st.maxk = device_max(ctx, st.outdeg.get(), (size_t)n);
-    // the largest local graph whose arrays and bit matrix fit one block's LDS
+    // the largest local graph whose arrays fit one block's LDS (the triangle
+    // pass); the clique pass gives the rest of the LDS to its bit matrix,
+    // held in column blocks when the whole matrix does not fit.  Local
+    // indices are 16-bit in the local-edge lists.
     const size_t lds_cap = std::min<size_t>(ctx->lds_per_block, 160 * 1024) - 8192;
     int kl = 32;
-    while (carve_bytes(kl + 32, (kl + 32 + 31) / 32) <= lds_cap) kl += 32;
+    while (kl + 32 < 65536 && carve_bytes(kl + 32, (int64_t)kl + 32) <= lds_cap) kl += 32;
     st.kmax_lds = std::max(kl, prm.mid_k);
     st.n_small = make_bin(ctx, st.outdeg.get(), n, 1, prm.small_k, st.b_small, st.flags, st.off);
     st.n_mid = make_bin(ctx, st.outdeg.get(), n, prm.small_k, prm.mid_k, st.b_mid, st.flags, st.off);
@@ -687,12 +714,21 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
     }
     st.n_light = make_bin(ctx, st.ub.get(), n, 0, kLightBound, st.b_light, st.flags, st.off);
     st.n_heavy = make_bin(ctx, st.ub.get(), n, kLightBound, INT64_MAX, st.b_heavy, st.flags, st.off);
-    st.small_lds = carve_bytes(prm.small_k, (prm.small_k + 31) / 32);
-    st.mid_lds = carve_bytes(prm.mid_k, (prm.mid_k + 31) / 32);
-    st.large_lds = carve_bytes(st.kmax_lds, (st.kmax_lds + 31) / 32);
+    st.mw_small = full_matrix_words(prm.small_k);
+    st.mw_mid = full_matrix_words(prm.mid_k);
+    st.small_lds = carve_bytes(prm.small_k, st.mw_small);
+    st.mid_lds = carve_bytes(prm.mid_k, st.mw_mid);
+    // the large bin's arrays are sized for its largest member
+    st.kq = std::max(prm.mid_k, std::min(st.maxk, st.kmax_lds));
+    {
+        const int64_t spare = ((int64_t)lds_cap - (int64_t)carve_bytes(st.kq, 0)) / 4 - 64;
+        st.mw_large = std::min<int64_t>(full_matrix_words(st.kq), std::max<int64_t>(spare, st.kq));
+    }
+    st.large_lds_tri = carve_bytes(st.kq, 0);
+    st.large_lds = carve_bytes(st.kq, st.mw_large);
     if (st.n_large) {
         PGD_HIP(hipFuncSetAttribute((const void*)k_local_tri<1024, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)st.large_lds));
+                                    (int)st.large_lds_tri));
         PGD_HIP(hipFuncSetAttribute((const void*)k_local_k4q<1024, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)st.large_lds));
     }
@@ -702,14 +738,15 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
         // entries per block: fewer blocks when that exceeds 4 GiB in all
         const int64_t per = std::max<int64_t>(1, (int64_t)st.maxk * (st.maxk - 1) / 2) * 8;
         st.ghuge = (int)std::max<int64_t>(1, std::min<int64_t>(st.ghuge, ((int64_t)4 << 30) / per));
-        ensure(st.hscratch, ctx, (size_t)st.ghuge * carve_bytes(st.maxk, (st.maxk + 31) / 32));
+        st.mw_huge = full_matrix_words(st.maxk);
+        ensure(st.hscratch, ctx, (size_t)st.ghuge * carve_bytes(st.maxk, st.mw_huge));
     }
     {
         // local-edge lists of the clique pass: per block, the bin's k(k-1)/2 at most
         auto cap = [](int64_t k) { return std::max<int64_t>(1, k * (k - 1) / 2); };
         st.ecap_small = cap(prm.small_k);
         st.ecap_mid = cap(prm.mid_k);
-        st.ecap_large = cap(st.kmax_lds);
+        st.ecap_large = cap(st.kq);
         const int64_t bs = st.n_small ? grid_cap(ctx, st.n_small, prm.grid_small) : 0;
         const int64_t bm = st.n_mid ? grid_cap(ctx, st.n_mid, prm.grid_mid) : 0;
         const int64_t bl = st.n_large ? grid_cap(ctx, st.n_large, 1) : 0;
@@ -775,22 +812,22 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     if (st.n_small)
         hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (char*)nullptr, st.el_small,
-                           st.ecap_small, tl, st.TS.get(), out.T, w);
+                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, st.mw_small, (char*)nullptr,
+                           st.el_small, st.ecap_small, tl, st.TS.get(), out.T, w);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (char*)nullptr, st.el_mid,
+                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, st.mw_mid, (char*)nullptr, st.el_mid,
                            st.ecap_mid, tl, st.TS.get(), out.T, w);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_tri<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
-                           st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kmax_lds, (char*)nullptr, st.el_large,
-                           st.ecap_large, tl, st.TS.get(), out.T, w);
+                           st.large_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kq, (int64_t)0, (char*)nullptr,
+                           st.el_large, st.ecap_large, tl, st.TS.get(), out.T, w);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_tri<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.hscratch.get(), st.el_huge, st.ecap_huge, tl, st.TS.get(), out.T, w);
+                           st.mw_huge, st.hscratch.get(), st.el_huge, st.ecap_huge, tl, st.TS.get(), out.T, w);
     hipLaunchKernelGGL(k_fold_u32, dim3(grid_cap(ctx, (2 * m + 255) / 256, 16)), dim3(256), 0, s, st.TS.get(),
                        st.reid.get(), 2 * m, out.T);
     PGD_HIP(hipGetLastError());
@@ -799,22 +836,23 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     if (st.n_small)
         hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (char*)nullptr, st.el_small,
-                           st.ecap_small, tl, out.T, out.K4, st.Q.get(), w + 1);
+                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, st.mw_small, (char*)nullptr,
+                           st.el_small, st.ecap_small, tl, out.T, out.K4, st.Q.get(), w + 1);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (char*)nullptr, st.el_mid,
+                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, st.mw_mid, (char*)nullptr, st.el_mid,
                            st.ecap_mid, tl, out.T, out.K4, st.Q.get(), w + 1);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kmax_lds, (char*)nullptr, st.el_large,
-                           st.ecap_large, tl, out.T, out.K4, st.Q.get(), w + 1);
+                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kq, st.mw_large, (char*)nullptr,
+                           st.el_large, st.ecap_large, tl, out.T, out.K4, st.Q.get(), w + 1);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_k4q<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, out.T, out.K4, st.Q.get(), w + 1);
+                           st.mw_huge, st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, out.T, out.K4,
+                           st.Q.get(), w + 1);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (timer) timer->begin("k_wedge_light");

@@ -201,12 +201,15 @@ def test_wedge_counter_overflow(ctx):
     g.free()
 
 
-@pytest.mark.parametrize("lists", [64, 0])
-def test_local_graphs_beyond_lds(lists, ctx):
-    """K_1000: the lowest-ranked vertices have out-degree up to 999, beyond
-    what one block's LDS holds, so their local graphs run from HBM scratch.
-    Closed forms of K_n per edge: T = n-2, K4 = C(n-2, 2), C4 = 0."""
-    n = 1000
+@pytest.mark.parametrize("n,lists", [(1000, 64), (1000, 0), (2100, 64)])
+def test_local_graphs_beyond_lds(n, lists, ctx):
+    """K_n: the lowest-ranked vertices have out-degree up to n-1.  At n = 1000
+    the clique pass holds their local bit matrices in LDS column blocks (the
+    whole matrix exceeds the LDS); at n = 2100 the largest local graphs exceed
+    the LDS bin and run from HBM scratch.  With lists=0 the clique pass
+    searches the local graphs again instead of reading the triangle pass's
+    lists.  Closed forms of K_n per edge: T = n-2, K4 = C(n-2, 2), C4 = 0."""
+    say(f"[gpu] K_{n} lists={lists}")
     i, j = np.triu_indices(n, 1)
     g = pgd_amd.GraphletCore(i.astype(np.int32), j.astype(np.int32), n=n, ctx=ctx)
     g.sort_edges("deg")
