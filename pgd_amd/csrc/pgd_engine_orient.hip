@@ -251,10 +251,10 @@ __device__ __forceinline__ void locate(const LocalArrays& A, int k, int64_t tota
 
 // Record the hits of one round (kU positions per thread) in the block's
 // local edge list: one LDS reservation per wavefront per round.  Entry =
-// (edge id of (L[i], L[j]) << 32) | (j << 16) | i.
+// (out-slot of the edge (L[i], L[j]) << 32) | (j << 16) | i.
 template <int KU>
 __device__ __forceinline__ void record_hits(uint64_t* __restrict__ el, int* ecount, const int* i, const int* j,
-                                            const int32_t* e) {
+                                            const uint32_t* e) {
     const int lane = threadIdx.x & 63;
     int mine = 0;
 #pragma unroll
@@ -276,7 +276,7 @@ __device__ __forceinline__ void record_hits(uint64_t* __restrict__ el, int* ecou
     int pos = basepos + x - mine;
 #pragma unroll
     for (int u = 0; u < KU; u++)
-        if (j[u] >= 0) el[pos++] = ((uint64_t)(uint32_t)e[u] << 32) | ((uint32_t)j[u] << 16) | (uint32_t)i[u];
+        if (j[u] >= 0) el[pos++] = ((uint64_t)e[u] << 32) | ((uint32_t)j[u] << 16) | (uint32_t)i[u];
 }
 
 // Lists of local edges kept for the clique pass: the triangle pass appends
@@ -328,14 +328,14 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
                 if (i[u] >= 0) x[u] = col[slot[u]];
 #pragma unroll
             for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_local(A, x[u]) : -1;
-            int32_t e[kU];
+            uint32_t e[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
                 if (j[u] >= 0) {
                     atomicAdd(&TS[slot[u]], 1u);  // the edge (L[i], L[j]) by its out-slot: consecutive lanes
                     atomicAdd(&A.cnt[i[u]], 1u);
                     atomicAdd(&A.cnt[j[u]], 1u);
-                    if (el) e[u] = eid[slot[u]];
+                    e[u] = (uint32_t)slot[u];
                 }
             }
             if (el) record_hits<kU>(el, &ecount, i, j, e);
@@ -374,28 +374,34 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
 }
 
 // pass 2: 4-cliques and Q, with the local bit matrix.  The pass that builds
-// the bit matrix also records every local edge (i, j, edge id of (L[i],
+// the bit matrix also records every local edge (i, j, out-slot of (L[i],
 // L[j])) in the block's slice of `elist` (capacity k(k-1)/2 for the bin's
 // largest k, so it cannot overflow); the scan then reads that list instead
 // of searching again.  Without a list (the HBM-scratch bin) the scan
 // searches again.
 //
+// The far edge (L[i], L[j]) of a local edge is addressed by its out-slot:
+// its T is read from TSL (T by slot) and its K4 and Q terms go to the
+// per-slot accumulators K4S and QS, folded into K4 and Q by edge id after
+// the pass.  Local edges come in list order, which follows the out-lists, so
+// consecutive lanes touch nearby slots instead of scattered edge ids.
+//
 // When the full k x k bit matrix exceeds the block's mw words, it is held in
 // column blocks of wc words per row: the K4 terms of a local edge are summed
 // over the blocks (popcounts of partial rows), the Q terms are added once.
 __device__ __forceinline__ void edge_terms_k4q(const LocalArrays& A, int wn, bool first, int i, int j,
-                                               int32_t eij, const uint32_t* __restrict__ T,
-                                               uint64_t* __restrict__ K4, uint64_t* __restrict__ Q) {
+                                               uint32_t sij, const uint32_t* __restrict__ TSL,
+                                               uint64_t* __restrict__ K4S, uint64_t* __restrict__ QS) {
     uint32_t kij = 0;
     for (int w = 0; w < wn; w++) kij += __popc(A.M[i * wn + w] & A.M[j * wn + w]);
     if (kij) {
-        atomicAdd((unsigned long long*)&K4[eij], (unsigned long long)kij);
+        atomicAdd((unsigned long long*)&K4S[sij], (unsigned long long)kij);
         atomicAdd(&A.cnt[i], kij);
         atomicAdd(&A.cnt[j], kij);
     }
     if (first) {
-        const uint64_t tai = A.ta[i], taj = A.ta[j], tij = T[eij];
-        atomicAdd((unsigned long long*)&Q[eij], (unsigned long long)(tai + taj));
+        const uint64_t tai = A.ta[i], taj = A.ta[j], tij = TSL[sij];
+        atomicAdd((unsigned long long*)&QS[sij], (unsigned long long)(tai + taj));
         atomicAdd((unsigned long long*)&A.acc[i], (unsigned long long)(taj + tij));
         atomicAdd((unsigned long long*)&A.acc[j], (unsigned long long)(tai + tij));
     }
@@ -414,8 +420,8 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                                                   const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
                                                   int64_t cnt, int kmax, int64_t mw, char* __restrict__ gscratch,
                                                   uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
-                                                  const uint32_t* __restrict__ T, uint64_t* __restrict__ K4,
-                                                  uint64_t* __restrict__ Q, unsigned long long* __restrict__ work) {
+                                                  const uint32_t* __restrict__ TSL, uint64_t* __restrict__ K4S,
+                                                  uint64_t* __restrict__ QS, unsigned long long* __restrict__ work) {
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
     __shared__ int ecount;
@@ -431,7 +437,9 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
         if (threadIdx.x == 0) ecount = 0;
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
         if (threadIdx.x == 0) done += (unsigned long long)total;
-        for (int i = threadIdx.x; i < k; i += NT) A.ta[i] = T[A.Le[i]];
+        // the edges a -> L[i] are a's out-slots ostart[a] + i
+        const int64_t s0 = ostart[a];
+        for (int i = threadIdx.x; i < k; i += NT) A.ta[i] = TSL[s0 + i];
         // the triangle pass's list of a's local edges, when it kept one
         const int kept = tl.G ? tl.len[a] : -1;
         const uint64_t* gl = kept >= 0 ? tl.G + tl.off[a] : nullptr;
@@ -460,12 +468,12 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                         if (i[u] >= 0) x[u] = col[slot[u]];
 #pragma unroll
                     for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_local(A, x[u]) : -1;
-                    int32_t e[kU];
+                    uint32_t e[kU];
 #pragma unroll
                     for (int u = 0; u < kU; u++) {
                         if (j[u] >= 0) {
                             set_pair(A.M, wn, jlo, i[u], j[u]);
-                            if (first && el) e[u] = eid[slot[u]];
+                            e[u] = (uint32_t)slot[u];
                         }
                     }
                     if (first && el) record_hits<kU>(el, &ecount, i, j, e);
@@ -477,8 +485,8 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                 const int nl = gl ? kept : ecount;
                 for (int p = threadIdx.x; p < nl; p += NT) {
                     const uint64_t e = lst[p];
-                    edge_terms_k4q(A, wn, first, (int)(e & 0xffff), (int)((e >> 16) & 0xffff), (int32_t)(e >> 32), T,
-                                   K4, Q);
+                    edge_terms_k4q(A, wn, first, (int)(e & 0xffff), (int)((e >> 16) & 0xffff), (uint32_t)(e >> 32),
+                                   TSL, K4S, QS);
                 }
             } else {
                 // no list: wc == W here (the HBM bin holds the full matrix)
@@ -494,15 +502,15 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                     for (int u = 0; u < kU; u++) ju[u] = iu[u] >= 0 ? find_local(A, x[u]) : -1;
 #pragma unroll
                     for (int u = 0; u < kU; u++)
-                        if (ju[u] >= 0) edge_terms_k4q(A, wn, first, iu[u], ju[u], eid[su[u]], T, K4, Q);
+                        if (ju[u] >= 0) edge_terms_k4q(A, wn, first, iu[u], ju[u], (uint32_t)su[u], TSL, K4S, QS);
                 }
             }
             __syncthreads();
         }
         for (int i = threadIdx.x; i < k; i += NT) {
             // each local triangle through i is seen from its two edges at i
-            if (A.cnt[i]) atomicAdd((unsigned long long*)&K4[A.Le[i]], (unsigned long long)(A.cnt[i] >> 1));
-            if (A.acc[i]) atomicAdd((unsigned long long*)&Q[A.Le[i]], (unsigned long long)A.acc[i]);
+            if (A.cnt[i]) atomicAdd((unsigned long long*)&K4S[s0 + i], (unsigned long long)(A.cnt[i] >> 1));
+            if (A.acc[i]) atomicAdd((unsigned long long*)&QS[s0 + i], (unsigned long long)A.acc[i]);
         }
         __syncthreads();
     }
@@ -630,6 +638,25 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n) {
 __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
     GRID_STRIDE(i, n) p[i] = v;
 }
+// TSL[s] = T of the edge at slot s
+__global__ void k_slot_gather_u32(const uint32_t* __restrict__ T, const int32_t* __restrict__ eid, int64_t nnz,
+                                  uint32_t* __restrict__ TSL) {
+    GRID_STRIDE(s, nnz) TSL[s] = T[eid[s]];
+}
+// K4[eid[s]] += K4S[s], Q[eid[s]] += QS[s]: the clique pass adds only at
+// out-slots, one per edge, so the adds need no atomics
+__global__ void k_fold_k4q(const uint64_t* __restrict__ K4S, const uint64_t* __restrict__ QS,
+                           const int32_t* __restrict__ eid, int64_t nnz, uint64_t* __restrict__ K4,
+                           uint64_t* __restrict__ Q) {
+    GRID_STRIDE(s, nnz) {
+        const uint64_t k = K4S[s], q = QS[s];
+        if (k | q) {
+            const int32_t e = eid[s];
+            K4[e] += k;
+            Q[e] += q;
+        }
+    }
+}
 // T[eid[s]] += TS[s] for every slot s with a count
 __global__ void k_fold_u32(const uint32_t* __restrict__ TS, const int32_t* __restrict__ eid, int64_t nnz,
                            uint32_t* __restrict__ T) {
@@ -675,6 +702,7 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
     int gn = grid_cap(ctx, (n + 255) / 256, 16), gm = grid_cap(ctx, (m + 255) / 256, 16);
     const int vb = bitsfor(n - 1), db = bitsfor(g.max_deg), gb = g.group ? bitsfor(g.ngroups - 1) : 0;
     PGD_CHECK(vb + db + gb <= 64, PGD_ERR_UNSUPPORTED, "rank key of (group, degree, id) exceeds 64 bits");
+    PGD_CHECK(nnz < ((int64_t)1 << 32), PGD_ERR_UNSUPPORTED, "2m >= 2^32: local-edge lists hold 32-bit slots");
     hipLaunchKernelGGL(k_rank_keys, dim3(gn), dim3(256), 0, s, g.deg, g.group, n, db, vb, st.k0.get());
     PGD_HIP(hipGetLastError());
     sort_k(ctx, st.k0.get(), st.k1.get(), (size_t)n, (unsigned)(vb + db + gb));
@@ -828,31 +856,40 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
         hipLaunchKernelGGL((k_local_tri<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
                            st.mw_huge, st.hscratch.get(), st.el_huge, st.ecap_huge, tl, st.TS.get(), out.T, w);
-    hipLaunchKernelGGL(k_fold_u32, dim3(grid_cap(ctx, (2 * m + 255) / 256, 16)), dim3(256), 0, s, st.TS.get(),
-                       st.reid.get(), 2 * m, out.T);
+    const int gz = grid_cap(ctx, (2 * m + 255) / 256, 16);
+    hipLaunchKernelGGL(k_fold_u32, dim3(gz), dim3(256), 0, s, st.TS.get(), st.reid.get(), 2 * m, out.T);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (timer) timer->begin("k_local_k4q");
+    // the clique pass addresses edges by out-slot: T by slot (in TS, free
+    // after the fold) and per-slot K4/Q accumulators (k0/k1, free after the
+    // build)
+    uint32_t* TSL = st.TS.get();
+    uint64_t *K4S = st.k0.get(), *QS = st.k1.get();
+    hipLaunchKernelGGL(k_slot_gather_u32, dim3(gz), dim3(256), 0, s, out.T, st.reid.get(), 2 * m, TSL);
+    hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, K4S, 2 * m);
+    hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, QS, 2 * m);
+    PGD_HIP(hipGetLastError());
     if (st.n_small)
         hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, st.mw_small, (char*)nullptr,
-                           st.el_small, st.ecap_small, tl, out.T, out.K4, st.Q.get(), w + 1);
+                           st.el_small, st.ecap_small, tl, TSL, K4S, QS, w + 1);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, st.mw_mid, (char*)nullptr, st.el_mid,
-                           st.ecap_mid, tl, out.T, out.K4, st.Q.get(), w + 1);
+                           st.ecap_mid, tl, TSL, K4S, QS, w + 1);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_large.get(), st.n_large, st.kq, st.mw_large, (char*)nullptr,
-                           st.el_large, st.ecap_large, tl, out.T, out.K4, st.Q.get(), w + 1);
+                           st.el_large, st.ecap_large, tl, TSL, K4S, QS, w + 1);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_k4q<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.mw_huge, st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, out.T, out.K4,
-                           st.Q.get(), w + 1);
+                           st.mw_huge, st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, TSL, K4S, QS, w + 1);
+    hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, s, K4S, QS, st.reid.get(), 2 * m, out.K4, st.Q.get());
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (timer) timer->begin("k_wedge_light");
