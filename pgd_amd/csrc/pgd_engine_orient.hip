@@ -682,6 +682,7 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
     const OrientParams& prm = st.prm;
     st.n = n;
     st.m = m;
+    st.max_deg = g.max_deg;
     ensure(st.rank, ctx, n);
     ensure(st.rdeg, ctx, n);
     ensure(st.rrow, ctx, n + 1);

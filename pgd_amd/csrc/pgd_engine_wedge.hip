@@ -301,9 +301,20 @@ __global__ void k_item_keys(const uint64_t* __restrict__ key, int64_t n, uint64_
     }
 }
 
+// add v to the per-slot accumulator of the slot: 32-bit unless its list
+// (segment o of the group) is wide.  A slot (b, c) gains at most d(c) - 1
+// from each of at most d(b) hubs, below 2^32 unless d(b) * max degree
+// reaches it.
+__device__ __forceinline__ void slot_add(uint64_t* __restrict__ PS, uint32_t* __restrict__ PS32, uint64_t wmask,
+                                         int o, int64_t slot, uint32_t v) {
+    if ((wmask >> o) & 1) atomicAdd((unsigned long long*)&PS[slot], (unsigned long long)v);
+    else atomicAdd(&PS32[slot], v);
+}
+
 // One pass of one wavefront over its share of the segments of a tile.
 // pass 0 counts endpoints, pass 1 adds cnt-1 per wedge to the slot's
-// accumulator and, per segment, to the edge (a, b).  mode 0: 16-bit counters
+// accumulator (PS32, or PS for a wide list) and, per segment, to the edge
+// (a, b) (PS).  mode 0: 16-bit counters
 // over the whole tile; with chk, a count reaching 65536 sets *ovf (without
 // chk the hub has fewer than 65536 in-neighbours, so no count can); mode
 // 1/2: 32-bit counters over the lower/upper half of the tile, other
@@ -319,7 +330,8 @@ template <int NT>
 __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t sb, uint32_t se, int32_t c0,
                                           int log2r, const WedgeSeg* __restrict__ seg,
                                           const int32_t* __restrict__ col, uint32_t* cnt,
-                                          uint64_t* __restrict__ PS, int64_t* wbase, int32_t* wslot,
+                                          uint64_t* __restrict__ PS, uint32_t* __restrict__ PS32,
+                                          int64_t* wbase, int32_t* wslot,
                                           unsigned long long* wpsum, uint8_t* wflag, int* ovf, int lane, int w,
                                           unsigned long long& done) {
     const int32_t half = 1 << (log2r - 1);
@@ -328,12 +340,15 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
         const int kc = (int)min<uint32_t>(64, se - gb);
         int32_t len = 0, sl = 0;
         int64_t st = 0;
+        bool wide = false;
         if (lane < kc) {
             const WedgeSeg g = seg[gb + lane];
-            len = g.len;
+            len = g.len & 0x7fffffff;
+            wide = g.len < 0;
             st = g.start;
             sl = g.sab;
         }
+        const uint64_t wmask = __ballot(wide);  // segments of wide lists
         int32_t x = len;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -405,7 +420,7 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
 #pragma unroll
                 for (int u = 0; u < kU; u++)
                     if (ok[u] && v[u]) {
-                        atomicAdd((unsigned long long*)&PS[slot[u]], (unsigned long long)v[u]);
+                        slot_add(PS, PS32, wmask, o[u], slot[u], v[u]);
                         atomicAdd(&wpsum[o[u]], (unsigned long long)v[u]);
                     }
             } else {
@@ -429,7 +444,7 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
 #pragma unroll
                 for (int u = 0; u < kU; u++)
                     if (in[u] && v[u]) {
-                        atomicAdd((unsigned long long*)&PS[slot[u]], (unsigned long long)v[u]);
+                        slot_add(PS, PS32, wmask, o[u], slot[u], v[u]);
                         atomicAdd(&wpsum[o[u]], (unsigned long long)v[u]);
                     }
             }
@@ -452,7 +467,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     const int64_t* __restrict__ ttrow, const int32_t* __restrict__ hbase, const int64_t* __restrict__ htoff,
     const uint32_t* __restrict__ htseg, const WedgeItem* __restrict__ items, int64_t nitems, int log2r,
     WedgeSeg* __restrict__ segs, int64_t segcap, uint32_t* __restrict__ qctr, uint64_t* __restrict__ PS,
-    unsigned long long* __restrict__ work) {
+    uint32_t* __restrict__ PS32, int32_t wide_deg, unsigned long long* __restrict__ work) {
     constexpr int NW = NT / 64;
     extern __shared__ uint32_t cnt[];  // 2^log2r 16-bit endpoint counters (2^(log2r-1) words)
     __shared__ uint32_t base[kTmax + 1];
@@ -504,7 +519,8 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
                 if (off >= wc || t >= T) break;
                 const uint32_t nxt = j + 1 < j1 ? (uint32_t)tt[j + 1] : deg;
                 const uint32_t p = atomicAdd(&cur[t], 1u);
-                seg[p] = WedgeSeg{row[b] + off, (int32_t)(min(nxt, wc) - off), (int32_t)s};
+                const uint32_t lw = (min(nxt, wc) - off) | (deg >= (uint32_t)wide_deg ? 0x80000000u : 0u);
+                seg[p] = WedgeSeg{row[b] + off, (int32_t)lw, (int32_t)s};
             }
         }
         __syncthreads();
@@ -522,21 +538,21 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
             for (int i = tid; i < R / 2; i += NT) cnt[i] = 0;
             if (tid == 0) ovf = 0;
             __syncthreads();
-            tile_pass<NT>(0, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, wbase[w], wslot[w], wpsum[w], wflag[w],
+            tile_pass<NT>(0, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w], wpsum[w], wflag[w],
                           &ovf, lane, w, done);
             __syncthreads();
             if (!ovf) {
-                tile_pass<NT>(1, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, wbase[w], wslot[w], wpsum[w],
+                tile_pass<NT>(1, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w], wpsum[w],
                               wflag[w], &ovf, lane, w, done);
                 __syncthreads();
             } else {
                 for (int mode = 1; mode <= 2; mode++) {
                     for (int i = tid; i < R / 2; i += NT) cnt[i] = 0;
                     __syncthreads();
-                    tile_pass<NT>(0, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, wbase[w], wslot[w], wpsum[w],
+                    tile_pass<NT>(0, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w], wpsum[w],
                                   wflag[w], &ovf, lane, w, done);
                     __syncthreads();
-                    tile_pass<NT>(1, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, wbase[w], wslot[w], wpsum[w],
+                    tile_pass<NT>(1, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w], wpsum[w],
                                   wflag[w], &ovf, lane, w, done);
                     __syncthreads();
                 }
@@ -546,11 +562,11 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     if (done) atomicAdd(work, done);
 }
 
-// fold the per-slot accumulator into P by edge id
-__global__ void k_ps_fold(const uint64_t* __restrict__ PS, const int32_t* __restrict__ eid, int64_t nnz,
-                          uint64_t* __restrict__ P) {
+// fold the per-slot accumulators into P by edge id
+__global__ void k_ps_fold(const uint64_t* __restrict__ PS, const uint32_t* __restrict__ PS32,
+                          const int32_t* __restrict__ eid, int64_t nnz, uint64_t* __restrict__ P) {
     GRID_STRIDE(s, nnz) {
-        const uint64_t v = PS[s];
+        const uint64_t v = PS[s] + PS32[s];
         if (v) atomicAdd((unsigned long long*)&P[eid[s]], (unsigned long long)v);
     }
 }
@@ -562,7 +578,7 @@ void launch_tiled(pgd_ctx* ctx, OrientState& st, unsigned long long* work) {
     hipLaunchKernelGGL(k_wedge_tiled<NT>, dim3(wp.grid), dim3(NT), dyn, ctx->stream, st.rrow.get(), st.rcol.get(),
                        st.ostart.get(), st.b_heavy.get(), wp.wcnt.get(), wp.tt.get(), wp.ttrow.get(), wp.hbase.get(),
                        wp.htoff.get(), wp.htseg.get(), wp.items.get(), wp.nitems, wp.log2r, wp.segs.get(), wp.segcap,
-                       wp.qctr.get(), wp.ps.get(), work);
+                       wp.qctr.get(), wp.ps.get(), wp.ps32.get(), wp.wide_deg, work);
 }
 
 template <int NT>
@@ -705,7 +721,12 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     }
     ensure(wp.segs, ctx, (size_t)wp.grid * (size_t)wp.segcap);
     ensure(wp.ps, ctx, nnz);
+    ensure(wp.ps32, ctx, nnz);
     ensure(wp.qctr, ctx, 1);
+    // smallest degree d with d * (max degree) >= 2^32
+    wp.wide_deg = st.max_deg > 0
+                      ? (int32_t)std::min<int64_t>(INT32_MAX, (((int64_t)1 << 32) + st.max_deg - 1) / st.max_deg)
+                      : INT32_MAX;
 }
 
 // the heavy 4-cycle pass (capturable: fills, the persistent kernel, the fold)
@@ -715,6 +736,7 @@ void wedge_plan_count(pgd_ctx* ctx, OrientState& st, unsigned long long* work) {
     const int64_t nnz = 2 * st.m;
     const int g = grid_cap(ctx, (nnz + 255) / 256, 16);
     hipLaunchKernelGGL(w_fill_u64, dim3(g), dim3(256), 0, s, wp.ps.get(), nnz);
+    hipLaunchKernelGGL(w_fill_u32, dim3(g), dim3(256), 0, s, wp.ps32.get(), nnz);
     hipLaunchKernelGGL(w_fill_u32, dim3(1), dim3(64), 0, s, wp.qctr.get(), (int64_t)1);
     PGD_HIP(hipGetLastError());
     if (wp.nitems) {
@@ -723,7 +745,8 @@ void wedge_plan_count(pgd_ctx* ctx, OrientState& st, unsigned long long* work) {
         else launch_tiled<1024>(ctx, st, work);
         PGD_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_ps_fold, dim3(g), dim3(256), 0, s, wp.ps.get(), st.reid.get(), nnz, st.P.get());
+    hipLaunchKernelGGL(k_ps_fold, dim3(g), dim3(256), 0, s, wp.ps.get(), wp.ps32.get(), st.reid.get(), nnz,
+                       st.P.get());
     PGD_HIP(hipGetLastError());
 }
 

@@ -182,7 +182,8 @@ struct Timer {
 // ---------------------------------------------------------------------------
 // One segment: the run of N(b) (ascending CSR slots [start, start+len)) whose
 // endpoints c fall in one endpoint tile, for the in-neighbour b of a hub a;
-// sab is the slot of b in N(a) (the edge (a, b)).
+// sab is the slot of b in N(a) (the edge (a, b)).  The top bit of len marks
+// a wide list (d(b) * max degree >= 2^32: its slots take 64-bit sums).
 struct WedgeSeg {
     int64_t start;
     int32_t len;
@@ -209,7 +210,9 @@ struct WedgePlan {
     DBuf<uint64_t> htw;             // wedges per (hub, tile) cell
     DBuf<WedgeItem> items;          // work queue, largest first
     DBuf<WedgeSeg> segs;            // per-block segment buffers, segcap each
-    DBuf<uint64_t> ps;              // per-slot P accumulator (ascending CSR slots)
+    DBuf<uint64_t> ps;              // per-slot P accumulator (ascending CSR slots), 64-bit sums
+    DBuf<uint32_t> ps32;            // per-slot P accumulator of the lists that are not wide
+    int32_t wide_deg = 0;           // lists of at least this degree are wide
     DBuf<uint32_t> qctr;            // queue head
 };
 
@@ -225,7 +228,7 @@ struct OrientState {
     DBuf<uint64_t> k0, k1;
     DBuf<int32_t> b_small, b_mid, b_large, b_huge, b_light, b_heavy;
     int64_t n_small = 0, n_mid = 0, n_large = 0, n_huge = 0, n_light = 0, n_heavy = 0;
-    int kmax_lds = 0, maxk = 0, ghuge = 0, kq = 0;  // kq: local-graph size the large bin is carved for
+    int kmax_lds = 0, maxk = 0, ghuge = 0, kq = 0, max_deg = 0;  // kq: local-graph size the large bin is carved for
     size_t small_lds = 0, mid_lds = 0, large_lds = 0, large_lds_tri = 0;
     int64_t mw_small = 0, mw_mid = 0, mw_large = 0, mw_huge = 0;  // bit-matrix words per block
     DBuf<char> hscratch;

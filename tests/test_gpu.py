@@ -201,6 +201,33 @@ def test_wedge_counter_overflow(ctx):
     g.free()
 
 
+def test_wide_list_slot_sums(ctx):
+    """Per-slot 4-cycle sums of a wide list (d(b) * max degree >= 2^32 takes
+    64-bit sums).  Hubs h1 and g (degree K+1) share K common neighbours C;
+    h2 (degree K+3: h1, g and K+1 leaves X) is the top rank, so its wedges
+    h2-h1-c and h2-g-c land on the slots of the wide lists N(h1), N(g); g's
+    K = 65536 in-neighbours also overflow its 16-bit counters.  Closed forms:
+    T = K4 = 0 on every edge; C4 = K on h2-h1, h2-g, h1-c, g-c and 0 on h2-x."""
+    K = 65536
+    h2, h1, g = 0, 1, 2
+    C = np.arange(3, 3 + K, dtype=np.int32)
+    X = np.arange(3 + K, 4 + 2 * K, dtype=np.int32)
+    src = np.concatenate([[h2, h2], np.full(K + 1, h2), np.full(K, h1), np.full(K, g)]).astype(np.int32)
+    dst = np.concatenate([[h1, g], X, C, C]).astype(np.int32)
+    n = 4 + 2 * K
+    say(f"[gpu] wide lists K={K}")
+    gr = pgd_amd.GraphletCore(src, dst, n=n, ctx=ctx)
+    gr.sort_edges("deg")
+    gr.decompose(engine="orient")
+    T, K4, C4 = gr.edge_counts()
+    ev, eu = gr.edge_arrays()
+    assert int(T.sum()) == 0 and int(K4.sum()) == 0
+    leaf = (ev >= 3 + K) | (eu >= 3 + K)
+    want = np.where(leaf, 0, K).astype(np.uint64)
+    assert np.array_equal(C4, want)
+    gr.free()
+
+
 @pytest.mark.parametrize("n,lists", [(1000, 64), (1000, 0), (2100, 64)])
 def test_local_graphs_beyond_lds(n, lists, ctx):
     """K_n: the lowest-ranked vertices have out-degree up to n-1.  At n = 1000
