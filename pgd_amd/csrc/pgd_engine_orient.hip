@@ -290,62 +290,244 @@ struct TriLists {
     int32_t* len;             // per vertex rank: entries, or -1
 };
 
+// ---------------------------------------------------------------------------
 // pass 1: triangles.  Local edge (i,j), i<j <=> triangle {a, L[i], L[j]}.
 // The edge (L[i], L[j]) is counted in TS at its out-slot (folded into T by
 // edge id afterwards); the edges (a, L[i]) through the block's counters.
+//
+// The out-lists N+(L[i]) are walked in groups of 64 lists: a wavefront takes
+// chunks of 256 consecutive positions of a group's flattened lists (chunks
+// of all groups are dealt round-robin to the block's wavefronts).  The list
+// holding a position is found from the list starts flagged in LDS (a ballot
+// and a popcount), and membership of the endpoint x in N+(a) is first tested
+// in a bitmap of hashed members (one LDS word; about 3% false positives) and
+// only then looked up in the hash of (member, local index) pairs.
+// ---------------------------------------------------------------------------
+constexpr int kChunk = 64 * kU;  // positions per wavefront chunk
+
+struct TriArrays {
+    int64_t* obeg;   // first out-slot of L[i]                      [kmax]
+    int64_t* pref;   // inclusive scan of |N+(L[i])|                [kmax]
+    uint64_t* hkv;   // hash of N+(a): (vertex << 32) | local index, ~0 empty [2^hbits]
+    int32_t* L;      // N+(a), ascending                            [kmax]
+    int32_t* Le;     // edge id of a -> L[i]                        [kmax]
+    uint32_t* cnt;   // triangles on the edge a -> L[i]             [kmax]
+    uint32_t* bm;    // bitmap of hashed members                    [2^bbits bits]
+    int32_t* gch;    // first chunk of each group of 64 lists       [groups + 1]
+    int hbits, bbits;
+};
+
+__host__ __device__ inline int tri_hash_bits(int kmax) {  // at most half full
+    int b = 6;
+    while ((1 << b) < 2 * kmax) b++;
+    return b;
+}
+__host__ __device__ inline int tri_bm_bits(int kmax) {  // at least 32 bits per member
+    int b = 11;
+    while ((1 << b) < 32 * kmax) b++;
+    return b;
+}
+__host__ __device__ inline size_t tri_carve_bytes(int kmax) {
+    const size_t groups = (size_t)(kmax + 63) / 64 + 1;
+    return ((size_t)kmax * 28 + ((size_t)8 << tri_hash_bits(kmax)) + ((size_t)1 << tri_bm_bits(kmax)) / 8 +
+            groups * 4 + 255) & ~(size_t)255;
+}
+__device__ __forceinline__ TriArrays tri_carve(char* base, int kmax) {
+    TriArrays r;
+    r.hbits = tri_hash_bits(kmax);
+    r.bbits = tri_bm_bits(kmax);
+    char* p = base;
+    r.obeg = (int64_t*)p; p += (size_t)kmax * 8;
+    r.pref = (int64_t*)p; p += (size_t)kmax * 8;
+    r.hkv = (uint64_t*)p; p += (size_t)8 << r.hbits;
+    r.L = (int32_t*)p; p += (size_t)kmax * 4;
+    r.Le = (int32_t*)p; p += (size_t)kmax * 4;
+    r.cnt = (uint32_t*)p; p += (size_t)kmax * 4;
+    r.bm = (uint32_t*)p; p += ((size_t)1 << r.bbits) / 8;
+    r.gch = (int32_t*)p;
+    return r;
+}
+
+__device__ __forceinline__ uint32_t mhash(int32_t c) { return (uint32_t)c * 0x9E3779B1u; }
+
+// local index of x in N+(a) or -1 (x hashed to h)
+__device__ __forceinline__ int tri_find(const TriArrays& A, int32_t x, uint32_t h) {
+    const uint32_t mask = (1u << A.hbits) - 1;
+    uint32_t s = h >> (32 - A.hbits);
+    for (;;) {
+        const uint64_t kv = A.hkv[s];
+        if (kv == ~0ull) return -1;
+        if ((int32_t)(kv >> 32) == x) return (int)(uint32_t)kv;
+        s = (s + 1) & mask;
+    }
+}
+
 template <int NT, bool kGlobal>
 __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                                                   const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart,
                                                   const int32_t* __restrict__ outdeg, const int32_t* __restrict__ list,
-                                                  int64_t cnt, int kmax, int64_t mw, char* __restrict__ gscratch,
+                                                  int64_t cnt, int kmax, size_t stride, char* __restrict__ gscratch,
                                                   uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
                                                   uint32_t* __restrict__ TS, uint32_t* __restrict__ T,
                                                   unsigned long long* __restrict__ work) {
+    constexpr int NW = NT / 64;
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
     __shared__ int ecount;
     __shared__ int64_t gpos;
-    // mw: bit-matrix words of an HBM slice (the slices are shared with the clique pass)
-    char* base = kGlobal ? gscratch + (size_t)blockIdx.x * carve_bytes(kmax, mw) : lds;
-    LocalArrays A = carve(base, kmax);
+    __shared__ int64_t wbase[NW][64];   // per wavefront: slot of position q of list r = wbase[r] + q
+    __shared__ int32_t wown[NW][64];    //                local index of the r-th non-empty list
+    __shared__ uint32_t wflag[NW][64];  //                list starts of the chunk (bytes)
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // stride: bytes of a block's HBM slice (the HBM bin)
+    char* base = kGlobal ? gscratch + (size_t)blockIdx.x * stride : lds;
+    TriArrays A = tri_carve(base, kmax);
     uint64_t* el = (elist && tl.G) ? elist + (size_t)blockIdx.x * (size_t)ecap : nullptr;
     unsigned long long done = 0;       // local work of this block, added once at the end
     __shared__ int64_t chunk_pos, chunk_end;  // the block's reserved part of the kept lists
-    if (threadIdx.x == 0) chunk_pos = chunk_end = 0;
+    if (tid == 0) chunk_pos = chunk_end = 0;
+    const uint64_t upto = (2ULL << lane) - 1;  // lanes 0..lane
+    uint8_t* fl = (uint8_t*)wflag[w];
     for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
         const int32_t a = list[li];
         const int k = outdeg[a];
-        if (threadIdx.x == 0) ecount = 0;
-        const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
-        if (threadIdx.x == 0) done += (unsigned long long)total;
-        for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
-            int i[kU], j[kU];
+        const int ng = (k + 63) / 64;
+        if (tid == 0) ecount = 0;
+        // N+(a), its hash and bitmap, the flattened out-list prefix
+        {
+            const int64_t s0 = ostart[a];
+            for (int h = tid; h < (1 << A.hbits); h += NT) A.hkv[h] = ~0ull;
+            for (int h = tid; h < (1 << A.bbits) / 32; h += NT) A.bm[h] = 0;
+            for (int i = tid; i < k; i += NT) {
+                const int32_t b = col[s0 + i];
+                A.L[i] = b;
+                A.Le[i] = eid[s0 + i];
+                const int64_t ob = ostart[b];
+                A.obeg[i] = ob;
+                A.pref[i] = row[b + 1] - ob;
+                A.cnt[i] = 0;
+            }
+            __syncthreads();
+            const uint32_t mask = (1u << A.hbits) - 1;
+            for (int i = tid; i < k; i += NT) {
+                const int32_t b = A.L[i];
+                const uint32_t h = mhash(b);
+                atomicOr(&A.bm[h >> (32 - A.bbits + 5) ], 1u << ((h >> (32 - A.bbits)) & 31));
+                const unsigned long long kv = ((unsigned long long)(uint32_t)b << 32) | (uint32_t)i;
+                uint32_t sl = h >> (32 - A.hbits);
+                while (atomicCAS((unsigned long long*)&A.hkv[sl], ~0ull, kv) != ~0ull) sl = (sl + 1) & mask;
+            }
+            __syncthreads();
+            block_incl_scan<NT>(A.pref, k, part);
+            // chunks per group of 64 lists, then their exclusive prefix (one wavefront)
+            if (w == 0) {
+                int carry = 0;
+                for (int g0 = 0; g0 < ng; g0 += 64) {
+                    const int g = g0 + lane;
+                    int nc = 0;
+                    if (g < ng) {
+                        const int64_t lo = g ? A.pref[64 * g - 1] : 0;
+                        const int64_t hi = A.pref[min(64 * g + 64, k) - 1];
+                        nc = (int)((hi - lo + kChunk - 1) / kChunk);
+                    }
+                    int x = nc;
+#pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const int y = __shfl_up(x, off, 64);
+                        if (lane >= off) x += y;
+                    }
+                    if (g < ng) A.gch[g] = carry + x - nc;
+                    carry += __shfl(x, 63, 64);
+                }
+                if (lane == 0) A.gch[ng] = carry;
+            }
+            __syncthreads();
+        }
+        const int64_t total = A.pref[k - 1];
+        if (tid == 0) done += (unsigned long long)total;
+        const int nch = A.gch[ng];
+        for (int cid = w; cid < nch; cid += NW) {
+            // the chunk's group: the last g with gch[g] <= cid
+            int glo = 0, ghi = ng - 1;
+            while (glo < ghi) {
+                const int mid = (glo + ghi + 1) >> 1;
+                if (A.gch[mid] <= cid) glo = mid;
+                else ghi = mid - 1;
+            }
+            const int g = glo;
+            const int32_t R = (cid - A.gch[g]) * kChunk;  // chunk start within the group
+            const int64_t G0 = g ? A.pref[64 * g - 1] : 0;
+            const int32_t glen = (int32_t)(A.pref[min(64 * g + 64, k) - 1] - G0);
+            const int i = 64 * g + lane;
+            int32_t es = INT32_MAX, len = 0;
+            if (i < k) {
+                const int64_t ex = i ? A.pref[i - 1] : 0;
+                len = (int32_t)(A.pref[i] - ex);
+                es = (int32_t)(ex - G0);
+            }
+            // the group's non-empty lists, ranked
+            const uint64_t nz = __ballot(len > 0);
+            if (len > 0) {
+                const int r = __popcll(nz & (upto >> 1));
+                wbase[w][r] = A.obeg[i] - es;
+                wown[w][r] = i;
+            } else {
+                es = INT32_MAX;
+            }
+            wflag[w][lane] = 0;
+            __builtin_amdgcn_wave_barrier();
+            if (es >= R && es < R + kChunk) fl[es - R] = 1;
+            __builtin_amdgcn_wave_barrier();
+            uint8_t f[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) f[u] = fl[u * 64 + lane];
+            int cum = __popcll(__ballot(es < R)) - 1;  // last list starting before the chunk
+            int ii[kU], j[kU];
+            bool ok[kU];
             int64_t slot[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const uint64_t starts = __ballot(f[u] != 0);
+                const int32_t q = R + 64 * u + lane;
+                ok[u] = q < glen;
+                // past the group's end: clamp to its last position (a valid slot)
+                const int o = cum + __popcll(starts & upto);
+                cum += __popcll(starts);
+                slot[u] = wbase[w][o] + (ok[u] ? q : glen - 1);
+                ii[u] = wown[w][o];
+            }
             int32_t x[kU];
-            locate<NT>(A, k, total, t0, i, slot);
 #pragma unroll
-            for (int u = 0; u < kU; u++)
-                if (i[u] >= 0) x[u] = col[slot[u]];
+            for (int u = 0; u < kU; u++) x[u] = col[slot[u]];
+            uint32_t h[kU];
+            bool maybe[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) j[u] = i[u] >= 0 ? find_local(A, x[u]) : -1;
+            for (int u = 0; u < kU; u++) {
+                h[u] = mhash(x[u]);
+                const uint32_t bw = A.bm[h[u] >> (32 - A.bbits + 5)];
+                maybe[u] = ok[u] && ((bw >> ((h[u] >> (32 - A.bbits)) & 31)) & 1);
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++) j[u] = maybe[u] ? tri_find(A, x[u], h[u]) : -1;
             uint32_t e[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
                 if (j[u] >= 0) {
                     atomicAdd(&TS[slot[u]], 1u);  // the edge (L[i], L[j]) by its out-slot: consecutive lanes
-                    atomicAdd(&A.cnt[i[u]], 1u);
+                    atomicAdd(&A.cnt[ii[u]], 1u);
                     atomicAdd(&A.cnt[j[u]], 1u);
                     e[u] = (uint32_t)slot[u];
                 }
             }
-            if (el) record_hits<kU>(el, &ecount, i, j, e);
+            if (el) record_hits<kU>(el, &ecount, ii, j, e);
+            __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < k; i += NT)
+        for (int i = tid; i < k; i += NT)
             if (A.cnt[i]) atomicAdd(&T[A.Le[i]], A.cnt[i]);
         if (el) {
             const int ne = ecount;
-            if (threadIdx.x == 0) {
+            if (tid == 0) {
                 // lists are placed in chunks the block reserves (one global
                 // atomic per chunk, not per vertex)
                 if (chunk_pos + ne > chunk_end) {
@@ -366,7 +548,7 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
             __syncthreads();
             const int64_t p = gpos;
             if (p >= 0)
-                for (int q = threadIdx.x; q < ne; q += NT) tl.G[p + q] = el[q];
+                for (int q = tid; q < ne; q += NT) tl.G[p + q] = el[q];
         }
         __syncthreads();
     }
@@ -727,9 +909,14 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
     // pass); the clique pass gives the rest of the LDS to its bit matrix,
     // held in column blocks when the whole matrix does not fit.  Local
     // indices are 16-bit in the local-edge lists.
+    // (static LDS: 8 KiB of scan partials in the clique pass; 24 KiB of scan
+    // partials and per-wavefront tables in the triangle pass)
     const size_t lds_cap = std::min<size_t>(ctx->lds_per_block, 160 * 1024) - 8192;
+    const size_t lds_cap_tri = std::min<size_t>(ctx->lds_per_block, 160 * 1024) - 26 * 1024;
     int kl = 32;
-    while (kl + 32 < 65536 && carve_bytes(kl + 32, (int64_t)kl + 32) <= lds_cap) kl += 32;
+    while (kl + 32 < 65536 && carve_bytes(kl + 32, (int64_t)kl + 32) <= lds_cap &&
+           tri_carve_bytes(kl + 32) <= lds_cap_tri)
+        kl += 32;
     st.kmax_lds = std::max(kl, prm.mid_k);
     st.n_small = make_bin(ctx, st.outdeg.get(), n, 1, prm.small_k, st.b_small, st.flags, st.off);
     st.n_mid = make_bin(ctx, st.outdeg.get(), n, prm.small_k, prm.mid_k, st.b_mid, st.flags, st.off);
@@ -753,7 +940,9 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
         const int64_t spare = ((int64_t)lds_cap - (int64_t)carve_bytes(st.kq, 0)) / 4 - 64;
         st.mw_large = std::min<int64_t>(full_matrix_words(st.kq), std::max<int64_t>(spare, st.kq));
     }
-    st.large_lds_tri = carve_bytes(st.kq, 0);
+    st.small_lds_tri = tri_carve_bytes(prm.small_k);
+    st.mid_lds_tri = tri_carve_bytes(prm.mid_k);
+    st.large_lds_tri = tri_carve_bytes(st.kq);
     st.large_lds = carve_bytes(st.kq, st.mw_large);
     if (st.n_large) {
         PGD_HIP(hipFuncSetAttribute((const void*)k_local_tri<1024, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -768,7 +957,9 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
         const int64_t per = std::max<int64_t>(1, (int64_t)st.maxk * (st.maxk - 1) / 2) * 8;
         st.ghuge = (int)std::max<int64_t>(1, std::min<int64_t>(st.ghuge, ((int64_t)4 << 30) / per));
         st.mw_huge = full_matrix_words(st.maxk);
-        ensure(st.hscratch, ctx, (size_t)st.ghuge * carve_bytes(st.maxk, st.mw_huge));
+        // one HBM slice per block, shared by the triangle and the clique pass
+        st.huge_stride = std::max(carve_bytes(st.maxk, st.mw_huge), tri_carve_bytes(st.maxk));
+        ensure(st.hscratch, ctx, (size_t)st.ghuge * st.huge_stride);
     }
     {
         // local-edge lists of the clique pass: per block, the bin's k(k-1)/2 at most
@@ -840,23 +1031,23 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     if (timer) timer->begin("k_local_tri");
     if (st.n_small)
         hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
-                           st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, st.mw_small, (char*)nullptr,
+                           st.small_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (size_t)0, (char*)nullptr,
                            st.el_small, st.ecap_small, tl, st.TS.get(), out.T, w);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
-                           st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, st.mw_mid, (char*)nullptr, st.el_mid,
+                           st.mid_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (size_t)0, (char*)nullptr, st.el_mid,
                            st.ecap_mid, tl, st.TS.get(), out.T, w);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_tri<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kq, (int64_t)0, (char*)nullptr,
+                           st.outdeg.get(), st.b_large.get(), st.n_large, st.kq, (size_t)0, (char*)nullptr,
                            st.el_large, st.ecap_large, tl, st.TS.get(), out.T, w);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_tri<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.mw_huge, st.hscratch.get(), st.el_huge, st.ecap_huge, tl, st.TS.get(), out.T, w);
+                           st.huge_stride, st.hscratch.get(), st.el_huge, st.ecap_huge, tl, st.TS.get(), out.T, w);
     const int gz = grid_cap(ctx, (2 * m + 255) / 256, 16);
     hipLaunchKernelGGL(k_fold_u32, dim3(gz), dim3(256), 0, s, st.TS.get(), st.reid.get(), 2 * m, out.T);
     PGD_HIP(hipGetLastError());

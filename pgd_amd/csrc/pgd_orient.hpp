@@ -229,7 +229,9 @@ struct OrientState {
     DBuf<int32_t> b_small, b_mid, b_large, b_huge, b_light, b_heavy;
     int64_t n_small = 0, n_mid = 0, n_large = 0, n_huge = 0, n_light = 0, n_heavy = 0;
     int kmax_lds = 0, maxk = 0, ghuge = 0, kq = 0, max_deg = 0;  // kq: local-graph size the large bin is carved for
-    size_t small_lds = 0, mid_lds = 0, large_lds = 0, large_lds_tri = 0;
+    size_t small_lds = 0, mid_lds = 0, large_lds = 0;              // clique pass, per block
+    size_t small_lds_tri = 0, mid_lds_tri = 0, large_lds_tri = 0;  // triangle pass, per block
+    size_t huge_stride = 0;                                         // HBM slice of a huge-bin block
     int64_t mw_small = 0, mw_mid = 0, mw_large = 0, mw_huge = 0;  // bit-matrix words per block
     DBuf<char> hscratch;
     DBuf<uint64_t> elist;  // local-edge lists of the clique pass (per block of each LDS bin)
