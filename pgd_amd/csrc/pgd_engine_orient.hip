@@ -618,7 +618,6 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
         const int wc = (int)min<int64_t>(W, mw / k);  // words per row held at once (>= 1)
         if (threadIdx.x == 0) ecount = 0;
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
-        if (threadIdx.x == 0) done += (unsigned long long)total;
         // the edges a -> L[i] are a's out-slots ostart[a] + i
         const int64_t s0 = ostart[a];
         for (int i = threadIdx.x; i < k; i += NT) A.ta[i] = TSL[s0 + i];
@@ -665,6 +664,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
             const uint64_t* lst = gl ? gl : el;
             if (lst) {
                 const int nl = gl ? kept : ecount;
+                if (threadIdx.x == 0) done += (unsigned long long)nl;
                 for (int p = threadIdx.x; p < nl; p += NT) {
                     const uint64_t e = lst[p];
                     edge_terms_k4q(A, wn, first, (int)(e & 0xffff), (int)((e >> 16) & 0xffff), (uint32_t)(e >> 32),
@@ -672,6 +672,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                 }
             } else {
                 // no list: wc == W here (the HBM bin holds the full matrix)
+                if (threadIdx.x == 0) done += (unsigned long long)total;
                 for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
                     int iu[kU], ju[kU];
                     int64_t su[kU];
@@ -1105,13 +1106,15 @@ void orient_profile(pgd_ctx* ctx, OrientState& st, Timer& timer, Timing* tm) {
     PGD_HIP(hipMemcpyAsync(wk, st.work.get(), sizeof(wk), hipMemcpyDeviceToHost, ctx->stream));
     PGD_HIP(hipStreamSynchronize(ctx->stream));
     // algorithmic HBM bytes per pass (work models):
-    //   local passes: 4 B per out-list entry read per enumeration (1 for the
-    //     triangle pass, 2 for the clique pass: bit-matrix build + scan);
+    //   triangle pass: 4 B per out-list entry read;
+    //   clique pass: per local edge and column block, its 8 B list entry and
+    //     its two 8 B per-slot sums (an out-list entry searched counts as one
+    //     local edge);
     //   light wedges: the 4 B endpoint id of every wedge, read by the count
     //     and the accumulate pass;
-    //   tiled wedges: the same two 4 B reads plus the 8 B per-slot update;
+    //   tiled wedges: the same two 4 B reads plus the 4 B per-slot sum;
     //   combine: 3 x 8 B + 4 B read and 8 B written per edge.
-    const double bytes_of[5] = {4.0 * wk[0], 8.0 * wk[1], 8.0 * wk[2], 16.0 * wk[3], 36.0 * st.m};
+    const double bytes_of[5] = {4.0 * wk[0], 24.0 * wk[1], 8.0 * wk[2], 12.0 * wk[3], 36.0 * st.m};
     const double units_of[5] = {(double)wk[0], (double)wk[1], (double)wk[2], (double)wk[3], (double)st.m};
     double best = 0;
     tm->phases.clear();
@@ -1148,7 +1151,7 @@ const char* tuning_problem(const pgd_tuning& t) {
     if (t.mid_k < t.small_k || t.mid_k > 256) return "mid_k must be in small_k..256";
     if (t.grid_small < 1 || t.grid_small > 64 || t.grid_mid < 1 || t.grid_mid > 64)
         return "grid factors must be in 1..64 blocks per CU";
-    if (t.wedge_log2r < 8 || t.wedge_log2r > 15) return "wedge_log2r must be in 8..15";
+    if (t.wedge_log2r < 8 || t.wedge_log2r > 16) return "wedge_log2r must be in 8..16";
     if (t.wedge_nt != 256 && t.wedge_nt != 512 && t.wedge_nt != 1024) return "wedge_nt must be 256, 512 or 1024";
     if (t.wedge_split < 1 || t.wedge_split > 64) return "wedge_split must be in 1..64";
     if (t.tri_lists < 0 || t.tri_lists > 1024) return "tri_lists must be in 0..1024";

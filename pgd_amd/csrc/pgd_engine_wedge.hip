@@ -603,7 +603,7 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     WedgePlan& wp = st.wp;
     const OrientParams& prm = st.prm;
     const int64_t n = st.n, nnz = 2 * st.m, nh = st.n_heavy;
-    PGD_CHECK(prm.wedge_log2r >= 8 && prm.wedge_log2r <= 15, PGD_ERR_INVALID, "wedge tile must be 2^8..2^15");
+    PGD_CHECK(prm.wedge_log2r >= 8 && prm.wedge_log2r <= 16, PGD_ERR_INVALID, "wedge tile must be 2^8..2^16");
     PGD_CHECK(prm.wedge_nt == 256 || prm.wedge_nt == 512 || prm.wedge_nt == 1024, PGD_ERR_INVALID,
               "wedge block size must be 256, 512 or 1024");
     PGD_CHECK(nnz < INT32_MAX, PGD_ERR_UNSUPPORTED, "2m >= 2^31 slots: the wedge plan indexes slots as int32");

@@ -41,7 +41,7 @@ struct pgd_ctx {
     hipStream_t stream = nullptr;
     int num_cus = 256;
     size_t lds_per_block = 65536;  // hipDeviceProp_t::sharedMemPerBlock, read once at open
-    int tiled_per_cu[3][8] = {};   // occupancy of the tiled wedge kernel by (block size, tile), 0 = not asked yet
+    int tiled_per_cu[3][9] = {};   // occupancy of the tiled wedge kernel by (block size, tile), 0 = not asked yet
     int64_t mem_cap = 0;   // 0: no cap
     std::atomic<int64_t> mem_used{0};  // also released from other threads (served results)
     std::string last_err;
@@ -121,10 +121,10 @@ struct Timer;
 // tunables of the oriented engine (bin thresholds and per-CU grid factors)
 struct OrientParams {
     int small_k = 64;       // out-degree up to which one wavefront owns a local graph
-    int mid_k = 256;        // up to which a 256-thread block does (1024 threads above)
-    int grid_small = 32;    // resident blocks per CU for the wavefront bins
-    int grid_mid = 16;      // resident blocks per CU for the 256-thread bin
-    int wedge_log2r = 14;   // endpoint tile of the heavy 4-cycle pass: 2^log2r 16-bit LDS counters
+    int mid_k = 192;        // up to which a 256-thread block does (1024 threads above)
+    int grid_small = 64;    // resident blocks per CU for the wavefront bins
+    int grid_mid = 32;      // resident blocks per CU for the 256-thread bin
+    int wedge_log2r = 15;   // endpoint tile of the heavy 4-cycle pass: 2^log2r 16-bit LDS counters
     int wedge_nt = 512;     // threads per block of the heavy 4-cycle pass (256, 512 or 1024)
     int wedge_split = 4;    // work items per block the heavy wedges are cut into (load balance)
     int tri_lists = 64;     // kept local-edge lists, entries per edge (0: off)

@@ -143,6 +143,7 @@ TUNINGS = [
     dict(wedge_log2r=8),                                     # many endpoint tiles, items cut at 128 tiles
     dict(wedge_log2r=10, wedge_nt=256, wedge_split=64),      # small items
     dict(wedge_log2r=14, wedge_nt=1024, small_k=32, mid_k=128),
+    dict(wedge_log2r=16, wedge_nt=1024),                     # 64K-rank tiles: one block per CU
     dict(wedge_log2r=12, wedge_nt=512, grid_small=4, grid_mid=1, wedge_split=1),
     dict(tri_lists=0),                                       # clique pass searches again
     dict(tri_lists=1),                                       # lists kept until the buffer is full
@@ -190,7 +191,7 @@ def test_wedge_counter_overflow(ctx):
     n = 3 + k
     g = pgd_amd.GraphletCore(src, dst, n=n, ctx=ctx)
     g.sort_edges("deg")
-    for log2r in (8, 14):
+    for log2r in (8, 14, 16):
         g.set_tuning(wedge_log2r=log2r)
         g.decompose(engine="orient")
         T, K4, C4 = g.edge_counts()
@@ -252,7 +253,7 @@ def test_local_graphs_beyond_lds(n, lists, ctx):
 def test_tuning_rejects_bad_values(ctx):
     s, d = pgd_amd.gen_gnp(100, 0.1, 1)
     g = pgd_amd.GraphletCore(s, d, ctx=ctx)
-    for bad in (dict(wedge_log2r=16), dict(wedge_nt=128), dict(small_k=65), dict(mid_k=512), dict(wedge_split=0),
+    for bad in (dict(wedge_log2r=17), dict(wedge_nt=128), dict(small_k=65), dict(mid_k=512), dict(wedge_split=0),
                 dict(tri_lists=-1)):
         with pytest.raises(pgd_amd.PGDError):
             g.set_tuning(**bad)
