@@ -301,6 +301,18 @@ __global__ void k_item_keys(const uint64_t* __restrict__ key, int64_t n, uint64_
     }
 }
 
+// inclusive sum over the lanes 0..lane of a wavefront: within rows of 16
+// (DPP row shifts), then the row totals carried by DPP row broadcasts
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);  // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false); // row_bcast:15 -> rows 1, 3
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false); // row_bcast:31 -> rows 2, 3
+    return x;
+}
+
 // add v to the per-slot accumulator of the slot: 32-bit unless its list
 // (segment o of the group) is wide.  A slot (b, c) gains at most d(c) - 1
 // from each of at most d(b) hubs, below 2^32 unless d(b) * max degree
@@ -336,6 +348,10 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
                                           unsigned long long& done) {
     const int32_t half = 1 << (log2r - 1);
     const int32_t lo = mode == 2 ? half : 0;
+    // mode 0: the counter of endpoint c is the low (c < half) or high half of
+    // word c mod half, so consecutive endpoints use consecutive words (banks)
+    const int hb = log2r - 1;
+    const int32_t hm = half - 1;
     for (uint32_t gb = sb + (uint32_t)w * 64; gb < se; gb += NT) {
         const int kc = (int)min<uint32_t>(64, se - gb);
         int32_t len = 0, sl = 0;
@@ -384,14 +400,15 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
 #pragma unroll
             for (int u = 0; u < kU; u++) fl[u] = wflag[u * 64 + lane];
             int cum = __popcll(__ballot(es < rb0)) - 1;  // last segment starting before this round
+            uint64_t starts[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
-                const uint64_t starts = __ballot(fl[u] != 0);
+                starts[u] = __ballot(fl[u] != 0);
                 const int32_t q = rb0 + 64 * u + lane;
                 ok[u] = q < total;
                 // past the end: the owner is the last segment; clamp to its last position
-                o[u] = cum + __popcll(starts & upto);
-                cum += __popcll(starts);
+                o[u] = cum + __popcll(starts[u] & upto);
+                cum += __popcll(starts[u]);
                 slot[u] = wbase[o[u]] + (ok[u] ? q : total - 1);
             }
 #pragma unroll
@@ -402,27 +419,39 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
                         uint32_t old[kU];
 #pragma unroll
                         for (int u = 0; u < kU; u++)
-                            old[u] = ok[u] ? atomicAdd(&cnt[c[u] >> 1], 1u << ((c[u] & 1) * 16)) : 0u;
+                            old[u] = ok[u] ? atomicAdd(&cnt[c[u] & hm], 1u << ((c[u] >> hb) * 16)) : 0u;
                         bool hit = false;
 #pragma unroll
-                        for (int u = 0; u < kU; u++) hit |= ((old[u] >> ((c[u] & 1) * 16)) & 0xffffu) == 0xffffu;
+                        for (int u = 0; u < kU; u++) hit |= ((old[u] >> ((c[u] >> hb) * 16)) & 0xffffu) == 0xffffu;
                         if (hit) *ovf = 1;
                     } else {
 #pragma unroll
                         for (int u = 0; u < kU; u++)
-                            if (ok[u]) atomicAdd(&cnt[c[u] >> 1], 1u << ((c[u] & 1) * 16));
+                            if (ok[u]) atomicAdd(&cnt[c[u] & hm], 1u << ((c[u] >> hb) * 16));
                     }
                     continue;
                 }
                 uint32_t v[kU];
 #pragma unroll
-                for (int u = 0; u < kU; u++) v[u] = ((cnt[c[u] >> 1] >> ((c[u] & 1) * 16)) & 0xffffu) - 1;
+                for (int u = 0; u < kU; u++) v[u] = ((cnt[c[u] & hm] >> ((c[u] >> hb) * 16)) & 0xffffu) - 1;
 #pragma unroll
                 for (int u = 0; u < kU; u++)
-                    if (ok[u] && v[u]) {
-                        slot_add(PS, PS32, wmask, o[u], slot[u], v[u]);
-                        atomicAdd(&wpsum[o[u]], (unsigned long long)v[u]);
-                    }
+                    if (ok[u] && v[u]) slot_add(PS, PS32, wmask, o[u], slot[u], v[u]);
+                // the edge (a, b) of each segment: the lanes of one segment are
+                // a run (delimited by segment starts), summed by a wavefront
+                // scan and added once by the run's last lane (v < 2^16, so a
+                // step's sums fit 32 bits)
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    const uint32_t val = ok[u] ? v[u] : 0u;
+                    const uint32_t P = wave_incl_sum(val);
+                    const uint64_t sm = starts[u] & upto;
+                    const int rs = sm ? 63 - __clzll((long long)sm) : 0;  // run start (lane)
+                    const uint32_t before = __shfl(P, rs > 0 ? rs - 1 : 0, 64);
+                    const uint32_t run = P - (rs > 0 ? before : 0u);
+                    const bool last = lane == 63 || ((starts[u] >> (lane + 1)) & 1);
+                    if (last && run) atomicAdd(&wpsum[o[u]], (unsigned long long)run);
+                }
             } else {
                 bool in[kU];
                 int32_t ci[kU];
@@ -469,7 +498,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     WedgeSeg* __restrict__ segs, int64_t segcap, uint32_t* __restrict__ qctr, uint64_t* __restrict__ PS,
     uint32_t* __restrict__ PS32, int32_t wide_deg, unsigned long long* __restrict__ work) {
     constexpr int NW = NT / 64;
-    extern __shared__ uint32_t cnt[];  // 2^log2r 16-bit endpoint counters (2^(log2r-1) words)
+    extern __shared__ uint32_t cnt[];  // 2^log2r 16-bit endpoint counters (2^(log2r-1) words: c and c + 2^(log2r-1) share one)
     __shared__ uint32_t base[kTmax + 1];
     __shared__ uint32_t cur[kTmax];
     __shared__ int64_t wbase[NW][64];
