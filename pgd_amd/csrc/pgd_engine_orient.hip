@@ -563,29 +563,43 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
 // searches again.
 //
 // The far edge (L[i], L[j]) of a local edge is addressed by its out-slot:
-// its T is read from TSL (T by slot) and its K4 and Q terms go to the
-// per-slot accumulators K4S and QS, folded into K4 and Q by edge id after
-// the pass.  Local edges come in list order, which follows the out-lists, so
+// its T is read from TSL (T by slot) and its K4 and Q terms go to per-slot
+// accumulators, folded into K4 and Q by edge id after the pass.  When an
+// edge's lower endpoint has degree at most kNarrowDeg, its K4 (at most
+// C(kNarrowDeg - 1, 2) < 2^24) and its Q (at most 2 kNarrowDeg max-degree,
+// below 2^40 when packing is enabled) share one 64-bit sum PK = K4 << 40 |
+// Q, one atomic per local edge; other edges use K4S and QS.  Local edges come in list order, which follows the out-lists, so
 // consecutive lanes touch nearby slots instead of scattered edge ids.
 //
 // When the full k x k bit matrix exceeds the block's mw words, it is held in
 // column blocks of wc words per row: the K4 terms of a local edge are summed
 // over the blocks (popcounts of partial rows), the Q terms are added once.
+constexpr int32_t kNarrowDeg = 5792;
+
+// (narrow: the lower endpoint L[i] of the far edge has degree <= kNarrowDeg,
+// kept by the caller in A.Le[i])
 __device__ __forceinline__ void edge_terms_k4q(const LocalArrays& A, int wn, bool first, int i, int j,
                                                uint32_t sij, const uint32_t* __restrict__ TSL,
-                                               uint64_t* __restrict__ K4S, uint64_t* __restrict__ QS) {
+                                               uint64_t* __restrict__ K4S, uint64_t* __restrict__ QS,
+                                               uint64_t* __restrict__ PK, bool pack) {
     uint32_t kij = 0;
     for (int w = 0; w < wn; w++) kij += __popc(A.M[i * wn + w] & A.M[j * wn + w]);
     if (kij) {
-        atomicAdd((unsigned long long*)&K4S[sij], (unsigned long long)kij);
         atomicAdd(&A.cnt[i], kij);
         atomicAdd(&A.cnt[j], kij);
     }
+    uint64_t q = 0;
     if (first) {
         const uint64_t tai = A.ta[i], taj = A.ta[j], tij = TSL[sij];
-        atomicAdd((unsigned long long*)&QS[sij], (unsigned long long)(tai + taj));
+        q = tai + taj;
         atomicAdd((unsigned long long*)&A.acc[i], (unsigned long long)(taj + tij));
         atomicAdd((unsigned long long*)&A.acc[j], (unsigned long long)(tai + tij));
+    }
+    if (pack && A.Le[i] <= kNarrowDeg) {
+        if (kij | q) atomicAdd((unsigned long long*)&PK[sij], (unsigned long long)(((uint64_t)kij << 40) | q));
+    } else {
+        if (kij) atomicAdd((unsigned long long*)&K4S[sij], (unsigned long long)kij);
+        if (q) atomicAdd((unsigned long long*)&QS[sij], (unsigned long long)q);
     }
 }
 
@@ -603,7 +617,8 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                                                   int64_t cnt, int kmax, int64_t mw, char* __restrict__ gscratch,
                                                   uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
                                                   const uint32_t* __restrict__ TSL, uint64_t* __restrict__ K4S,
-                                                  uint64_t* __restrict__ QS, unsigned long long* __restrict__ work) {
+                                                  uint64_t* __restrict__ QS, uint64_t* __restrict__ PK, bool pack,
+                                                  unsigned long long* __restrict__ work) {
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
     __shared__ int ecount;
@@ -620,7 +635,11 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
         const int64_t total = load_local<NT>(row, col, eid, ostart, a, k, A, part);
         // the edges a -> L[i] are a's out-slots ostart[a] + i
         const int64_t s0 = ostart[a];
-        for (int i = threadIdx.x; i < k; i += NT) A.ta[i] = TSL[s0 + i];
+        // (Le, the edge ids, is not needed here: it holds the degree of L[i])
+        for (int i = threadIdx.x; i < k; i += NT) {
+            A.ta[i] = TSL[s0 + i];
+            A.Le[i] = (int32_t)(row[A.L[i] + 1] - row[A.L[i]]);
+        }
         // the triangle pass's list of a's local edges, when it kept one
         const int kept = tl.G ? tl.len[a] : -1;
         const uint64_t* gl = kept >= 0 ? tl.G + tl.off[a] : nullptr;
@@ -668,7 +687,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                 for (int p = threadIdx.x; p < nl; p += NT) {
                     const uint64_t e = lst[p];
                     edge_terms_k4q(A, wn, first, (int)(e & 0xffff), (int)((e >> 16) & 0xffff), (uint32_t)(e >> 32),
-                                   TSL, K4S, QS);
+                                   TSL, K4S, QS, PK, pack);
                 }
             } else {
                 // no list: wc == W here (the HBM bin holds the full matrix)
@@ -685,15 +704,22 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                     for (int u = 0; u < kU; u++) ju[u] = iu[u] >= 0 ? find_local(A, x[u]) : -1;
 #pragma unroll
                     for (int u = 0; u < kU; u++)
-                        if (ju[u] >= 0) edge_terms_k4q(A, wn, first, iu[u], ju[u], (uint32_t)su[u], TSL, K4S, QS);
+                        if (ju[u] >= 0)
+                            edge_terms_k4q(A, wn, first, iu[u], ju[u], (uint32_t)su[u], TSL, K4S, QS, PK, pack);
                 }
             }
             __syncthreads();
         }
         for (int i = threadIdx.x; i < k; i += NT) {
             // each local triangle through i is seen from its two edges at i
-            if (A.cnt[i]) atomicAdd((unsigned long long*)&K4S[s0 + i], (unsigned long long)(A.cnt[i] >> 1));
-            if (A.acc[i]) atomicAdd((unsigned long long*)&QS[s0 + i], (unsigned long long)A.acc[i]);
+            // the edge (a, L[i]): its lower endpoint is a
+            const uint64_t k4 = A.cnt[i] >> 1, q = A.acc[i];
+            if (pack && row[a + 1] - row[a] <= kNarrowDeg) {
+                if (k4 | q) atomicAdd((unsigned long long*)&PK[s0 + i], (unsigned long long)((k4 << 40) | q));
+            } else {
+                if (k4) atomicAdd((unsigned long long*)&K4S[s0 + i], (unsigned long long)k4);
+                if (q) atomicAdd((unsigned long long*)&QS[s0 + i], (unsigned long long)q);
+            }
         }
         __syncthreads();
     }
@@ -826,13 +852,15 @@ __global__ void k_slot_gather_u32(const uint32_t* __restrict__ T, const int32_t*
                                   uint32_t* __restrict__ TSL) {
     GRID_STRIDE(s, nnz) TSL[s] = T[eid[s]];
 }
-// K4[eid[s]] += K4S[s], Q[eid[s]] += QS[s]: the clique pass adds only at
-// out-slots, one per edge, so the adds need no atomics
+// K4[eid[s]] += K4S[s] + PK[s] >> 40, Q[eid[s]] += QS[s] + PK[s] mod 2^40:
+// the clique pass adds only at out-slots, one per edge, so the adds need no
+// atomics
 __global__ void k_fold_k4q(const uint64_t* __restrict__ K4S, const uint64_t* __restrict__ QS,
-                           const int32_t* __restrict__ eid, int64_t nnz, uint64_t* __restrict__ K4,
-                           uint64_t* __restrict__ Q) {
+                           const uint64_t* __restrict__ PK, const int32_t* __restrict__ eid, int64_t nnz,
+                           uint64_t* __restrict__ K4, uint64_t* __restrict__ Q) {
     GRID_STRIDE(s, nnz) {
-        const uint64_t k = K4S[s], q = QS[s];
+        const uint64_t pk = PK[s];
+        const uint64_t k = K4S[s] + (pk >> 40), q = QS[s] + (pk & ((1ULL << 40) - 1));
         if (k | q) {
             const int32_t e = eid[s];
             K4[e] += k;
@@ -873,7 +901,7 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
     ensure(st.outdeg, ctx, n);
     ensure(st.rcol, ctx, nnz);
     ensure(st.reid, ctx, nnz);
-    ensure(st.k0, ctx, std::max(n, nnz));
+    ensure(st.k0, ctx, 2 * std::max(n, nnz));  // (also the clique pass's per-slot K4 and Q sums)
     ensure(st.k1, ctx, std::max(n, nnz));
     ensure(st.v0, ctx, nnz);
     ensure(st.flags, ctx, n);
@@ -1058,31 +1086,33 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     // after the fold) and per-slot K4/Q accumulators (k0/k1, free after the
     // build)
     uint32_t* TSL = st.TS.get();
-    uint64_t *K4S = st.k0.get(), *QS = st.k1.get();
+    uint64_t *K4S = st.k0.get(), *QS = st.k0.get() + 2 * m, *PK = st.k1.get();
+    // packing needs 2 kNarrowDeg (max degree) < 2^40
+    const bool pack = 2.0 * kNarrowDeg * (double)st.max_deg < (double)(1ULL << 40);
     hipLaunchKernelGGL(k_slot_gather_u32, dim3(gz), dim3(256), 0, s, out.T, st.reid.get(), 2 * m, TSL);
-    hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, K4S, 2 * m);
-    hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, QS, 2 * m);
+    hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, K4S, 4 * m);
+    hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, PK, 2 * m);
     PGD_HIP(hipGetLastError());
     if (st.n_small)
         hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, st.mw_small, (char*)nullptr,
-                           st.el_small, st.ecap_small, tl, TSL, K4S, QS, w + 1);
+                           st.el_small, st.ecap_small, tl, TSL, K4S, QS, PK, pack, w + 1);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, st.mw_mid, (char*)nullptr, st.el_mid,
-                           st.ecap_mid, tl, TSL, K4S, QS, w + 1);
+                           st.ecap_mid, tl, TSL, K4S, QS, PK, pack, w + 1);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_large.get(), st.n_large, st.kq, st.mw_large, (char*)nullptr,
-                           st.el_large, st.ecap_large, tl, TSL, K4S, QS, w + 1);
+                           st.el_large, st.ecap_large, tl, TSL, K4S, QS, PK, pack, w + 1);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_k4q<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.mw_huge, st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, TSL, K4S, QS, w + 1);
-    hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, s, K4S, QS, st.reid.get(), 2 * m, out.K4, st.Q.get());
+                           st.mw_huge, st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, TSL, K4S, QS, PK, pack, w + 1);
+    hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, s, K4S, QS, PK, st.reid.get(), 2 * m, out.K4, st.Q.get());
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (timer) timer->begin("k_wedge_light");

@@ -229,6 +229,31 @@ def test_wide_list_slot_sums(ctx):
     gr.free()
 
 
+def test_high_degree_local_edge_sums(ctx):
+    """Clique-pass sums of an edge whose ends both have degree above 5792
+    (64-bit K4 and Q sums instead of the packed ones).  Book graph: hubs x, y
+    adjacent, N pages each adjacent to both, pages matched in pairs.  Closed
+    forms: (x, y) has T = N, K4 = N/2; a hub-page or page-page edge has T = 2,
+    K4 = 1; C4 = 0 on every edge (every 4-cycle has a chord)."""
+    N = 6000
+    x, y = 0, 1
+    w = np.arange(2, 2 + N, dtype=np.int32)
+    src = np.concatenate([[x], np.full(N, x), np.full(N, y), w[0::2]]).astype(np.int32)
+    dst = np.concatenate([[y], w, w, w[1::2]]).astype(np.int32)
+    say(f"[gpu] book graph N={N}")
+    g = pgd_amd.GraphletCore(src, dst, n=N + 2, ctx=ctx)
+    g.sort_edges("deg")
+    g.decompose(engine="orient")
+    T, K4, C4 = g.edge_counts()
+    ev, eu = g.edge_arrays()
+    hub = ((ev == x) & (eu == y)) | ((ev == y) & (eu == x))
+    assert hub.sum() == 1
+    assert np.array_equal(T, np.where(hub, N, 2).astype(np.uint32))
+    assert np.array_equal(K4, np.where(hub, N // 2, 1).astype(np.uint64))
+    assert not C4.any()
+    g.free()
+
+
 @pytest.mark.parametrize("n,lists", [(1000, 64), (1000, 0), (2100, 64)])
 def test_local_graphs_beyond_lds(n, lists, ctx):
     """K_n: the lowest-ranked vertices have out-degree up to n-1.  At n = 1000
