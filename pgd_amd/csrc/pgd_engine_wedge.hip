@@ -484,6 +484,13 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
     }
 }
 
+// zero n words of LDS (n a multiple of 4, p 16-byte aligned) with 16-byte stores
+template <int NT>
+__device__ __forceinline__ void clear_lds(uint32_t* p, int n) {
+    uint4* q = (uint4*)p;
+    for (int i = threadIdx.x; i < n / 4; i += NT) q[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // the tiled count/accumulate kernel (persistent; items from the queue).
 // Within a tile each wavefront takes 64 segments at a time, scans their
 // lengths in registers and walks the flattened wedges 64 lanes at a time
@@ -498,7 +505,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     WedgeSeg* __restrict__ segs, int64_t segcap, uint32_t* __restrict__ qctr, uint64_t* __restrict__ PS,
     uint32_t* __restrict__ PS32, int32_t wide_deg, unsigned long long* __restrict__ work) {
     constexpr int NW = NT / 64;
-    extern __shared__ uint32_t cnt[];  // 2^log2r 16-bit endpoint counters (2^(log2r-1) words: c and c + 2^(log2r-1) share one)
+    extern __shared__ __align__(16) uint32_t cnt[];  // 2^log2r 16-bit endpoint counters (2^(log2r-1) words: c and c + 2^(log2r-1) share one)
     __shared__ uint32_t base[kTmax + 1];
     __shared__ uint32_t cur[kTmax];
     __shared__ int64_t wbase[NW][64];
@@ -564,7 +571,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
             const uint32_t sb = base[t], se = base[t + 1];
             if (sb == se) continue;
             const int32_t c0 = (W.t0 + t) << log2r;
-            for (int i = tid; i < R / 2; i += NT) cnt[i] = 0;
+            clear_lds<NT>(cnt, R / 2);
             if (tid == 0) ovf = 0;
             __syncthreads();
             tile_pass<NT>(0, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w], wpsum[w], wflag[w],
@@ -576,7 +583,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
                 __syncthreads();
             } else {
                 for (int mode = 1; mode <= 2; mode++) {
-                    for (int i = tid; i < R / 2; i += NT) cnt[i] = 0;
+                    clear_lds<NT>(cnt, R / 2);
                     __syncthreads();
                     tile_pass<NT>(0, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w], wpsum[w],
                                   wflag[w], &ovf, lane, w, done);
