@@ -886,8 +886,9 @@ void orient_state_free(pgd_ctx*, OrientState* s) { delete s; }
 
 OrientParams& orient_params(OrientState* s) { return s->prm; }
 
-// relabel, ascending CSR, bins and scratch (host synchronisations; not capturable)
-void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
+// relabel, ascending CSR, bins and scratch, and (with plan) the 4-cycle
+// plan (host synchronisations; not capturable)
+void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st, bool plan) {
     hipStream_t s = ctx->stream;
     const int64_t n = g.n, m = g.m, nnz = 2 * m;
     const OrientParams& prm = st.prm;
@@ -1027,12 +1028,22 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st) {
             ensure(st.tcur, ctx, 1);
         }
     }
-    if (st.n_heavy) wedge_plan_build(ctx, st);
+    if (plan && st.n_heavy) wedge_plan_build(ctx, st);
 }
 
 // the counting launches (no host synchronisation, no allocation, no memset:
 // capturable in a HIP graph).  With a timer, each pass is bracketed by events.
+// Local passes (triangles, 4-cliques, Q), then the 4-cycle passes, which
+// need the plan.
+static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer);
+static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer);
+
 void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
+    orient_count_local(ctx, st, out, timer);
+    orient_count_wedges(ctx, st, out, timer);
+}
+
+static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     hipStream_t s = ctx->stream;
     const int64_t m = st.m, n = st.n;
     const OrientParams& prm = st.prm;
@@ -1115,6 +1126,14 @@ void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, s, K4S, QS, PK, st.reid.get(), 2 * m, out.K4, st.Q.get());
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
+}
+
+static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
+    hipStream_t s = ctx->stream;
+    const int64_t m = st.m;
+    const OrientParams& prm = st.prm;
+    const int gm = grid_cap(ctx, (m + 255) / 256, 16);
+    unsigned long long* w = st.work.get();
     if (timer) timer->begin("k_wedge_light");
     if (st.n_light)
         hipLaunchKernelGGL(k_wedge_light, dim3(grid_cap(ctx, st.n_light, prm.grid_small)), dim3(64), 0, s,
@@ -1168,9 +1187,38 @@ void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm,
         else local.reset(st);
     }
     st->prm = prm;
-    orient_build(ctx, g, *st);
+    // the 4-cycle plan is built on a second stream while the local passes
+    // run: its host synchronisations wait for that stream only
+    orient_build(ctx, g, *st, false);
     Timer timer{ctx};
-    orient_count(ctx, *st, out, &timer);
+    hipEvent_t built = nullptr, planned = nullptr;
+    if (st->n_heavy) {
+        if (!ctx->aux) PGD_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+        PGD_HIP(hipEventCreateWithFlags(&built, hipEventDisableTiming));
+        PGD_HIP(hipEventCreateWithFlags(&planned, hipEventDisableTiming));
+        PGD_HIP(hipEventRecord(built, ctx->stream));
+    }
+    orient_count_local(ctx, *st, out, &timer);
+    if (st->n_heavy) {
+        hipStream_t main = ctx->stream;
+        PGD_HIP(hipStreamWaitEvent(ctx->aux, built, 0));
+        ctx->stream = ctx->aux;
+        try {
+            wedge_plan_build(ctx, *st);
+        } catch (...) {
+            ctx->stream = main;
+            (void)hipStreamSynchronize(ctx->aux);
+            (void)hipEventDestroy(built);
+            (void)hipEventDestroy(planned);
+            throw;
+        }
+        ctx->stream = main;
+        PGD_HIP(hipEventRecord(planned, ctx->aux));
+        PGD_HIP(hipStreamWaitEvent(ctx->stream, planned, 0));
+        (void)hipEventDestroy(built);
+        (void)hipEventDestroy(planned);
+    }
+    orient_count_wedges(ctx, *st, out, &timer);
     orient_profile(ctx, *st, timer, tm);
 }
 

@@ -658,10 +658,14 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         PGD_HIP(hipGetLastError());
     }
     PGD_HIP(hipGetLastError());
-    // tile table (st.v0 and st.k0 are free scratch of 2m entries after the relabel)
+    // tile table (st.v0 is free scratch of 2m entries after the relabel; the
+    // prefix gets its own buffer, since the plan may run beside the clique
+    // pass, which uses st.k0)
     {
         int32_t* flag = st.v0.get();
-        int64_t* pos = (int64_t*)st.k0.get();
+        DBuf<int64_t> posb;
+        posb.alloc(ctx, nnz);
+        int64_t* pos = posb.get();
         const int gw = grid_cap(ctx, (n * 64 + 255) / 256, 16);
         hipLaunchKernelGGL(k_tt_flags, dim3(gw), dim3(256), 0, s, st.rrow.get(), st.rcol.get(), n, wp.log2r, flag);
         PGD_HIP(hipGetLastError());

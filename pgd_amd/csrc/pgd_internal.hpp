@@ -39,6 +39,7 @@ struct Error {
 struct pgd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;  // second stream of the plain oriented engine (its 4-cycle plan), made on first use
     int num_cus = 256;
     size_t lds_per_block = 65536;  // hipDeviceProp_t::sharedMemPerBlock, read once at open
     int tiled_per_cu[3][9] = {};   // occupancy of the tiled wedge kernel by (block size, tile), 0 = not asked yet
@@ -131,7 +132,7 @@ struct OrientParams {
 };
 OrientState* orient_state_new();
 OrientParams& orient_params(OrientState* s);
-void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st);
+void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st, bool plan = true);
 void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer);
 void engine_edge(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t);
 void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t, OrientState** cache,
