@@ -279,6 +279,23 @@ __device__ __forceinline__ void record_hits(uint64_t* __restrict__ el, int* ecou
         if (j[u] >= 0) el[pos++] = ((uint64_t)e[u] << 32) | ((uint32_t)j[u] << 16) | (uint32_t)i[u];
 }
 
+// The next local graph of a block: from the bin's work queue, highest rank
+// (heaviest) first, when the launch has one; otherwise round-robin by block.
+// Returns -1 when the bin is done.  Uniform over the block.
+__device__ __forceinline__ int64_t next_vertex(uint32_t* queue, int64_t cnt, int64_t prev, int64_t* sh) {
+    if (!queue) {
+        const int64_t li = prev < 0 ? (int64_t)blockIdx.x : prev + gridDim.x;
+        return li < cnt ? li : -1;
+    }
+    __syncthreads();  // every thread has read the previous index
+    if (threadIdx.x == 0) {
+        const uint32_t q = atomicAdd(queue, 1u);
+        *sh = (int64_t)q < cnt ? cnt - 1 - (int64_t)q : -1;
+    }
+    __syncthreads();
+    return *sh;
+}
+
 // Lists of local edges kept for the clique pass: the triangle pass appends
 // vertex a's list (from its block scratch) to `G` when it fits and records
 // its offset and length (length -1: not kept, the clique pass searches again).
@@ -369,8 +386,9 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
                                                   int64_t cnt, int kmax, size_t stride, char* __restrict__ gscratch,
                                                   uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
                                                   uint32_t* __restrict__ TS, uint32_t* __restrict__ T,
-                                                  unsigned long long* __restrict__ work) {
+                                                  uint32_t* __restrict__ queue, unsigned long long* __restrict__ work) {
     constexpr int NW = NT / 64;
+    __shared__ int64_t li_sh;
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
     __shared__ int ecount;
@@ -388,7 +406,7 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
     if (tid == 0) chunk_pos = chunk_end = 0;
     const uint64_t upto = (2ULL << lane) - 1;  // lanes 0..lane
     uint8_t* fl = (uint8_t*)wflag[w];
-    for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
+    for (int64_t li = next_vertex(queue, cnt, -1, &li_sh); li >= 0; li = next_vertex(queue, cnt, li, &li_sh)) {
         const int32_t a = list[li];
         const int k = outdeg[a];
         const int ng = (k + 63) / 64;
@@ -568,8 +586,9 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
 // edge's lower endpoint has degree at most kNarrowDeg, its K4 (at most
 // C(kNarrowDeg - 1, 2) < 2^24) and its Q (at most 2 kNarrowDeg max-degree,
 // below 2^40 when packing is enabled) share one 64-bit sum PK = K4 << 40 |
-// Q, one atomic per local edge; other edges use K4S and QS.  Local edges come in list order, which follows the out-lists, so
-// consecutive lanes touch nearby slots instead of scattered edge ids.
+// Q, one atomic per local edge; other edges use K4S and QS.  Local edges
+// come in list order, which follows the out-lists, so consecutive lanes
+// touch nearby slots instead of scattered edge ids.
 //
 // When the full k x k bit matrix exceeds the block's mw words, it is held in
 // column blocks of wc words per row: the K4 terms of a local edge are summed
@@ -618,7 +637,8 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                                                   uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
                                                   const uint32_t* __restrict__ TSL, uint64_t* __restrict__ K4S,
                                                   uint64_t* __restrict__ QS, uint64_t* __restrict__ PK, bool pack,
-                                                  unsigned long long* __restrict__ work) {
+                                                  uint32_t* __restrict__ queue, unsigned long long* __restrict__ work) {
+    __shared__ int64_t li_sh;
     extern __shared__ char lds[];
     __shared__ int64_t part[NT];
     __shared__ int ecount;
@@ -626,7 +646,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
     LocalArrays A = carve(base, kmax);
     uint64_t* el = elist ? elist + (size_t)blockIdx.x * (size_t)ecap : nullptr;
     unsigned long long done = 0;
-    for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
+    for (int64_t li = next_vertex(queue, cnt, -1, &li_sh); li >= 0; li = next_vertex(queue, cnt, li, &li_sh)) {
         const int32_t a = list[li];
         const int k = outdeg[a];
         const int W = (k + 31) / 32;
@@ -912,6 +932,7 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st, bool plan) 
     ensure(st.Q, ctx, m);
     ensure(st.TS, ctx, nnz);
     ensure(st.work, ctx, 4);
+    ensure(st.lq, ctx, 6);
     int gn = grid_cap(ctx, (n + 255) / 256, 16), gm = grid_cap(ctx, (m + 255) / 256, 16);
     const int vb = bitsfor(n - 1), db = bitsfor(g.max_deg), gb = g.group ? bitsfor(g.ngroups - 1) : 0;
     PGD_CHECK(vb + db + gb <= 64, PGD_ERR_UNSUPPORTED, "rank key of (group, degree, id) exceeds 64 bits");
@@ -1053,6 +1074,7 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
     hipLaunchKernelGGL(k_fill_u64, dim3(gm), dim3(256), 0, s, st.P.get(), m);
     hipLaunchKernelGGL(k_fill_u64, dim3(gm), dim3(256), 0, s, st.Q.get(), m);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)st.work.get(), (int64_t)4);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, st.lq.get(), (int64_t)6);  // bin work queues
     PGD_HIP(hipGetLastError());
     unsigned long long* w = st.work.get();
     const int32_t *rr = nullptr;
@@ -1073,21 +1095,22 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
         hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (size_t)0, (char*)nullptr,
-                           st.el_small, st.ecap_small, tl, st.TS.get(), out.T, w);
+                           st.el_small, st.ecap_small, tl, st.TS.get(), out.T, (uint32_t*)nullptr, w);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (size_t)0, (char*)nullptr, st.el_mid,
-                           st.ecap_mid, tl, st.TS.get(), out.T, w);
+                           st.ecap_mid, tl, st.TS.get(), out.T, st.lq.get() + 4, w);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_tri<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_large.get(), st.n_large, st.kq, (size_t)0, (char*)nullptr,
-                           st.el_large, st.ecap_large, tl, st.TS.get(), out.T, w);
+                           st.el_large, st.ecap_large, tl, st.TS.get(), out.T, st.lq.get(), w);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_tri<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.huge_stride, st.hscratch.get(), st.el_huge, st.ecap_huge, tl, st.TS.get(), out.T, w);
+                           st.huge_stride, st.hscratch.get(), st.el_huge, st.ecap_huge, tl, st.TS.get(), out.T,
+                           st.lq.get() + 2, w);
     const int gz = grid_cap(ctx, (2 * m + 255) / 256, 16);
     hipLaunchKernelGGL(k_fold_u32, dim3(gz), dim3(256), 0, s, st.TS.get(), st.reid.get(), 2 * m, out.T);
     PGD_HIP(hipGetLastError());
@@ -1108,21 +1131,22 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
         hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, st.mw_small, (char*)nullptr,
-                           st.el_small, st.ecap_small, tl, TSL, K4S, QS, PK, pack, w + 1);
+                           st.el_small, st.ecap_small, tl, TSL, K4S, QS, PK, pack, (uint32_t*)nullptr, w + 1);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, st.mw_mid, (char*)nullptr, st.el_mid,
-                           st.ecap_mid, tl, TSL, K4S, QS, PK, pack, w + 1);
+                           st.ecap_mid, tl, TSL, K4S, QS, PK, pack, st.lq.get() + 5, w + 1);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_large.get(), st.n_large, st.kq, st.mw_large, (char*)nullptr,
-                           st.el_large, st.ecap_large, tl, TSL, K4S, QS, PK, pack, w + 1);
+                           st.el_large, st.ecap_large, tl, TSL, K4S, QS, PK, pack, st.lq.get() + 1, w + 1);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_k4q<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.mw_huge, st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, TSL, K4S, QS, PK, pack, w + 1);
+                           st.mw_huge, st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, TSL, K4S, QS, PK, pack,
+                           st.lq.get() + 3, w + 1);
     hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, s, K4S, QS, PK, st.reid.get(), 2 * m, out.K4, st.Q.get());
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();

@@ -505,7 +505,8 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     WedgeSeg* __restrict__ segs, int64_t segcap, uint32_t* __restrict__ qctr, uint64_t* __restrict__ PS,
     uint32_t* __restrict__ PS32, int32_t wide_deg, unsigned long long* __restrict__ work) {
     constexpr int NW = NT / 64;
-    extern __shared__ __align__(16) uint32_t cnt[];  // 2^log2r 16-bit endpoint counters (2^(log2r-1) words: c and c + 2^(log2r-1) share one)
+    // 2^log2r 16-bit endpoint counters in 2^(log2r-1) words (c and c + 2^(log2r-1) share one)
+    extern __shared__ __align__(16) uint32_t cnt[];
     __shared__ uint32_t base[kTmax + 1];
     __shared__ uint32_t cur[kTmax];
     __shared__ int64_t wbase[NW][64];
@@ -585,11 +586,11 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
                 for (int mode = 1; mode <= 2; mode++) {
                     clear_lds<NT>(cnt, R / 2);
                     __syncthreads();
-                    tile_pass<NT>(0, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w], wpsum[w],
-                                  wflag[w], &ovf, lane, w, done);
+                    tile_pass<NT>(0, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w],
+                                  wpsum[w], wflag[w], &ovf, lane, w, done);
                     __syncthreads();
-                    tile_pass<NT>(1, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w], wpsum[w],
-                                  wflag[w], &ovf, lane, w, done);
+                    tile_pass<NT>(1, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w],
+                                  wpsum[w], wflag[w], &ovf, lane, w, done);
                     __syncthreads();
                 }
             }

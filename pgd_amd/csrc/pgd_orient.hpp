@@ -244,6 +244,7 @@ struct OrientState {
     DBuf<uint64_t> P, Q;
     DBuf<uint32_t> TS;  // per-slot triangle counts of the triangle pass
     DBuf<unsigned long long> work;
+    DBuf<uint32_t> lq;  // work queues of the mid, large and HBM bins of the two local passes
     WedgePlan wp;
 };
 
