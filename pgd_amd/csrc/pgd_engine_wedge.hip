@@ -527,6 +527,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
         if (it >= nitems) break;
         const WedgeItem W = items[it];
         const int32_t a = heavy[W.hub];
+        const int32_t hb0 = hbase[W.hub];
         const int64_t ib = row[a], ie = ostart[a];
         const int T = W.t1 - W.t0;
         // bucket the hub's segments in [t0, t1) by tile: offsets from the plan's
@@ -547,7 +548,9 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
             if (!wc) continue;
             const int32_t b = col[s];
             const int64_t j1 = ttrow[b + 1];
-            int64_t j = W.t0 ? tt_seek(tt, ttrow[b], j1, (uint32_t)W.t0) : ttrow[b];
+            // every list's first entry is at or above the hub's first tile, so
+            // an item that starts there needs no search
+            int64_t j = W.t0 > hb0 ? tt_seek(tt, ttrow[b], j1, (uint32_t)W.t0) : ttrow[b];
             const uint32_t deg = (uint32_t)(row[b + 1] - row[b]);
             for (; j < j1; j++) {
                 const uint64_t e = tt[j];
