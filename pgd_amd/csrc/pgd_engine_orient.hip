@@ -395,7 +395,7 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
     __shared__ int64_t gpos;
     __shared__ int64_t wbase[NW][64];   // per wavefront: slot of position q of list r = wbase[r] + q
     __shared__ int32_t wown[NW][64];    //                local index of the r-th non-empty list
-    __shared__ uint32_t wflag[NW][64];  //                list starts of the chunk (bytes)
+    __shared__ uint32_t wflag[NW][64];  //                list starts of the chunk (byte p>>6 of word p&63)
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     // stride: bytes of a block's HBM slice (the HBM bin)
     char* base = kGlobal ? gscratch + (size_t)blockIdx.x * stride : lds;
@@ -494,11 +494,12 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
             }
             wflag[w][lane] = 0;
             __builtin_amdgcn_wave_barrier();
-            if (es >= R && es < R + kChunk) fl[es - R] = 1;
+            if (es >= R && es < R + kChunk) fl[((es - R) & 63) * 4 + ((es - R) >> 6)] = 1;
             __builtin_amdgcn_wave_barrier();
-            uint8_t f[kU];
+            const uint32_t f4 = wflag[w][lane];
+            uint32_t f[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) f[u] = fl[u * 64 + lane];
+            for (int u = 0; u < kU; u++) f[u] = (f4 >> (8 * u)) & 0xffu;
             int cum = __popcll(__ballot(es < R)) - 1;  // last list starting before the chunk
             int ii[kU], j[kU];
             bool ok[kU];

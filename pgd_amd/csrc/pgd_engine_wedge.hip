@@ -128,11 +128,12 @@ __device__ __forceinline__ int64_t tt_seek_off(const uint64_t* __restrict__ tt, 
 // the end of b's table, |N(b)| and the pair's bound w.  fn(pair, tile,
 // offset, len) is called once per entry, consecutive lanes on consecutive
 // entries of a pair; pair is the rank of the lane among the lanes with
-// entries.  flag: 64*4 bytes of the wavefront's LDS; pj, pe, pw, pd: 64
-// entries each, indexed by that rank.
+// entries.  flag: 64 words of the wavefront's LDS (the start flag of position
+// p of a round is byte p >> 6 of word p & 63, so each lane clears and reads
+// its own word); pj, pe, pw, pd: 64 entries each, indexed by that rank.
 template <class F>
 __device__ __forceinline__ void walk_entries(int64_t jb, int64_t je, int64_t tend, uint32_t deg, uint32_t w,
-                                             int64_t* pj, int64_t* pe, uint32_t* pw, uint32_t* pd, uint8_t* flag,
+                                             int64_t* pj, int64_t* pe, uint32_t* pw, uint32_t* pd, uint32_t* flag,
                                              const uint64_t* __restrict__ tt, int lane, F&& fn) {
     const int32_t cnt = (int32_t)(je > jb ? je - jb : 0);
     int32_t x = cnt;
@@ -155,18 +156,18 @@ __device__ __forceinline__ void walk_entries(int64_t jb, int64_t je, int64_t ten
     __builtin_amdgcn_wave_barrier();
     constexpr int kU = 4;
     for (int32_t rb0 = 0; rb0 < total; rb0 += 64 * kU) {
-#pragma unroll
-        for (int u = 0; u < kU; u++) flag[u * 64 + lane] = 0;
+        flag[lane] = 0;
         __builtin_amdgcn_wave_barrier();
-        if (es >= rb0 && es < rb0 + 64 * kU) flag[es - rb0] = 1;
+        if (es >= rb0 && es < rb0 + 64 * kU) ((uint8_t*)flag)[((es - rb0) & 63) * 4 + ((es - rb0) >> 6)] = 1;
         __builtin_amdgcn_wave_barrier();
+        const uint32_t f4 = flag[lane];
         int cum = __popcll(__ballot(es < rb0)) - 1;
         int o[kU];
         int64_t j[kU];
         uint64_t e[kU], en[kU];
 #pragma unroll
         for (int u = 0; u < kU; u++) {
-            const uint64_t starts = __ballot(flag[u * 64 + lane] != 0);
+            const uint64_t starts = __ballot(((f4 >> (8 * u)) & 0xffu) != 0);
             const int32_t q = rb0 + 64 * u + lane;
             o[u] = -1;
             if (q < total) {
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
     __shared__ unsigned long long lw[kPlanLds];
     __shared__ int64_t pj[NW][64], pe[NW][64];
     __shared__ uint32_t pw[NW][64], pd[NW][64];
-    __shared__ uint8_t flag[NW][4 * 64];
+    __shared__ uint32_t flag[NW][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
         const int32_t a = heavy[h];
@@ -344,7 +345,7 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
                                           const int32_t* __restrict__ col, uint32_t* cnt,
                                           uint64_t* __restrict__ PS, uint32_t* __restrict__ PS32,
                                           int64_t* wbase, int32_t* wslot,
-                                          unsigned long long* wpsum, uint8_t* wflag, int* ovf, int lane, int w,
+                                          unsigned long long* wpsum, uint32_t* wflag, int* ovf, int lane, int w,
                                           unsigned long long& done) {
     const int32_t half = 1 << (log2r - 1);
     const int32_t lo = mode == 2 ? half : 0;
@@ -391,14 +392,16 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
             bool ok[kU];
             int64_t slot[kU];
             int32_t c[kU];
-#pragma unroll
-            for (int u = 0; u < kU; u++) wflag[u * 64 + lane] = 0;
+            // the start flag of position p is byte p >> 6 of word p & 63: each
+            // lane clears and reads one word
+            wflag[lane] = 0;
             __builtin_amdgcn_wave_barrier();
-            if (es >= rb0 && es < rb0 + 64 * kU) wflag[es - rb0] = 1;
+            if (es >= rb0 && es < rb0 + 64 * kU) ((uint8_t*)wflag)[((es - rb0) & 63) * 4 + ((es - rb0) >> 6)] = 1;
             __builtin_amdgcn_wave_barrier();
-            uint8_t fl[kU];
+            const uint32_t f4 = wflag[lane];
+            uint32_t fl[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) fl[u] = wflag[u * 64 + lane];
+            for (int u = 0; u < kU; u++) fl[u] = (f4 >> (8 * u)) & 0xffu;
             int cum = __popcll(__ballot(es < rb0)) - 1;  // last segment starting before this round
             uint64_t starts[kU];
 #pragma unroll
@@ -511,7 +514,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     __shared__ uint32_t cur[kTmax];
     __shared__ int64_t wbase[NW][64];
     __shared__ int32_t wslot[NW][64];
-    __shared__ uint8_t wflag[NW][4 * 64];  // segment-start flags of one step of a wavefront
+    __shared__ uint32_t wflag[NW][64];  // segment-start flags of one round of a wavefront (4 bytes per lane)
     __shared__ unsigned long long wpsum[NW][64];
     __shared__ int64_t item_sh;
     __shared__ int ovf;
