@@ -563,7 +563,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
                 const uint32_t nxt = j + 1 < j1 ? (uint32_t)tt[j + 1] : deg;
                 const uint32_t p = atomicAdd(&cur[t], 1u);
                 const uint32_t lw = (min(nxt, wc) - off) | (deg >= (uint32_t)wide_deg ? 0x80000000u : 0u);
-                seg[p] = WedgeSeg{row[b] + off, (int32_t)lw, (int32_t)s};
+                seg[p] = WedgeSeg{(uint32_t)(row[b] + off), (int32_t)lw, (int32_t)s};
             }
         }
         __syncthreads();

@@ -184,8 +184,9 @@ struct Timer {
 // endpoints c fall in one endpoint tile, for the in-neighbour b of a hub a;
 // sab is the slot of b in N(a) (the edge (a, b)).  The top bit of len marks
 // a wide list (d(b) * max degree >= 2^32: its slots take 64-bit sums).
+// Slots are below 2^31 (the plan requires 2m < 2^31): 12 bytes a segment.
 struct WedgeSeg {
-    int64_t start;
+    uint32_t start;
     int32_t len;
     int32_t sab;
 };
