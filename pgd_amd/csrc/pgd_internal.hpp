@@ -156,7 +156,7 @@ inline pgd_tuning tuning_from(const OrientParams& p) {
 // empty string when t is usable, else what is wrong
 const char* tuning_problem(const pgd_tuning& t);
 void orient_state_free(pgd_ctx* ctx, OrientState* s);
-bool engine_dense_ok(const GraphView& g);    // chosen automatically (dense and small enough)
+bool engine_dense_ok(const GraphView& g);    // chosen automatically (above 80% density, small enough)
 bool engine_dense_fits(const GraphView& g);  // may be requested (small enough)
 void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t);
 

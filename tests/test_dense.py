@@ -43,9 +43,10 @@ def test_dense_configurations(n, p, seed, ctx):
     s, d = pgd_amd.gen_gnp(n, p, seed)
     g, out = both_engines(ctx, s, d, n)
     assert_same(out)
-    # left to itself the library picks the dense tier for these graphs
+    # left to itself the library picks the dense tier above the reference's
+    # 80% density (create_adj_mat), the oriented engine below
     g.decompose(micro=False)
-    assert g.last_engine() == "dense"
+    assert g.last_engine() == ("dense" if p > 0.8 else "orient")
     # the oracle on a sample of edges
     T, K4, C4 = out["dense"][1]
     og = OracleGraph.from_edges(s, d, n)
