@@ -30,6 +30,7 @@
 // is O(m * arboricity), orders of magnitude below the reference's per-edge
 // 2-hop work.  All counts are exact uint64 (uint32 for T), accumulated with
 // atomics, so the result is independent of scheduling.
+#include <cmath>
 #include <memory>
 
 #include "pgd_orient.hpp"
@@ -584,24 +585,28 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
 // The far edge (L[i], L[j]) of a local edge is addressed by its out-slot:
 // its T is read from TSL (T by slot) and its K4 and Q terms go to per-slot
 // accumulators, folded into K4 and Q by edge id after the pass.  When an
-// edge's lower endpoint has degree at most kNarrowDeg, its K4 (at most
-// C(kNarrowDeg - 1, 2) < 2^24) and its Q (at most 2 kNarrowDeg max-degree,
-// below 2^40 when packing is enabled) share one 64-bit sum PK = K4 << 40 |
-// Q, one atomic per local edge; other edges use K4S and QS.  Local edges
+// edge's lower endpoint has degree at most pk.deg, its K4 (at most
+// C(pk.deg - 1, 2) < 2^(64 - pk.qb)) and its Q (at most 2 pk.deg max-degree
+// < 2^pk.qb) share one 64-bit sum PK = K4 << pk.qb | Q, one atomic per local
+// edge; other edges use K4S and QS.  pk is chosen per graph from its maximum
+// degree (pack_split).  Local edges
 // come in list order, which follows the out-lists, so consecutive lanes
 // touch nearby slots instead of scattered edge ids.
 //
 // When the full k x k bit matrix exceeds the block's mw words, it is held in
 // column blocks of wc words per row: the K4 terms of a local edge are summed
 // over the blocks (popcounts of partial rows), the Q terms are added once.
-constexpr int32_t kNarrowDeg = 5792;
+struct PackSplit {
+    int32_t deg;  // largest lower-end degree of a packed edge (0: no packing)
+    int qb;       // low bits of PK that hold Q
+};
 
-// (narrow: the lower endpoint L[i] of the far edge has degree <= kNarrowDeg,
+// (narrow: the lower endpoint L[i] of the far edge has degree <= pk.deg,
 // kept by the caller in A.Le[i])
 __device__ __forceinline__ void edge_terms_k4q(const LocalArrays& A, int wn, bool first, int i, int j,
                                                uint32_t sij, const uint32_t* __restrict__ TSL,
                                                uint64_t* __restrict__ K4S, uint64_t* __restrict__ QS,
-                                               uint64_t* __restrict__ PK, bool pack) {
+                                               uint64_t* __restrict__ PK, PackSplit pk) {
     uint32_t kij = 0;
     for (int w = 0; w < wn; w++) kij += __popc(A.M[i * wn + w] & A.M[j * wn + w]);
     if (kij) {
@@ -615,8 +620,8 @@ __device__ __forceinline__ void edge_terms_k4q(const LocalArrays& A, int wn, boo
         atomicAdd((unsigned long long*)&A.acc[i], (unsigned long long)(taj + tij));
         atomicAdd((unsigned long long*)&A.acc[j], (unsigned long long)(tai + tij));
     }
-    if (pack && A.Le[i] <= kNarrowDeg) {
-        if (kij | q) atomicAdd((unsigned long long*)&PK[sij], (unsigned long long)(((uint64_t)kij << 40) | q));
+    if (A.Le[i] <= pk.deg) {
+        if (kij | q) atomicAdd((unsigned long long*)&PK[sij], (unsigned long long)(((uint64_t)kij << pk.qb) | q));
     } else {
         if (kij) atomicAdd((unsigned long long*)&K4S[sij], (unsigned long long)kij);
         if (q) atomicAdd((unsigned long long*)&QS[sij], (unsigned long long)q);
@@ -637,7 +642,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                                                   int64_t cnt, int kmax, int64_t mw, char* __restrict__ gscratch,
                                                   uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
                                                   const uint32_t* __restrict__ TSL, uint64_t* __restrict__ K4S,
-                                                  uint64_t* __restrict__ QS, uint64_t* __restrict__ PK, bool pack,
+                                                  uint64_t* __restrict__ QS, uint64_t* __restrict__ PK, PackSplit pk,
                                                   uint32_t* __restrict__ queue, unsigned long long* __restrict__ work) {
     __shared__ int64_t li_sh;
     extern __shared__ char lds[];
@@ -708,7 +713,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                 for (int p = threadIdx.x; p < nl; p += NT) {
                     const uint64_t e = lst[p];
                     edge_terms_k4q(A, wn, first, (int)(e & 0xffff), (int)((e >> 16) & 0xffff), (uint32_t)(e >> 32),
-                                   TSL, K4S, QS, PK, pack);
+                                   TSL, K4S, QS, PK, pk);
                 }
             } else {
                 // no list: wc == W here (the HBM bin holds the full matrix)
@@ -726,7 +731,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
 #pragma unroll
                     for (int u = 0; u < kU; u++)
                         if (ju[u] >= 0)
-                            edge_terms_k4q(A, wn, first, iu[u], ju[u], (uint32_t)su[u], TSL, K4S, QS, PK, pack);
+                            edge_terms_k4q(A, wn, first, iu[u], ju[u], (uint32_t)su[u], TSL, K4S, QS, PK, pk);
                 }
             }
             __syncthreads();
@@ -735,8 +740,8 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
             // each local triangle through i is seen from its two edges at i
             // the edge (a, L[i]): its lower endpoint is a
             const uint64_t k4 = A.cnt[i] >> 1, q = A.acc[i];
-            if (pack && row[a + 1] - row[a] <= kNarrowDeg) {
-                if (k4 | q) atomicAdd((unsigned long long*)&PK[s0 + i], (unsigned long long)((k4 << 40) | q));
+            if (row[a + 1] - row[a] <= pk.deg) {
+                if (k4 | q) atomicAdd((unsigned long long*)&PK[s0 + i], (unsigned long long)((k4 << pk.qb) | q));
             } else {
                 if (k4) atomicAdd((unsigned long long*)&K4S[s0 + i], (unsigned long long)k4);
                 if (q) atomicAdd((unsigned long long*)&QS[s0 + i], (unsigned long long)q);
@@ -873,15 +878,15 @@ __global__ void k_slot_gather_u32(const uint32_t* __restrict__ T, const int32_t*
                                   uint32_t* __restrict__ TSL) {
     GRID_STRIDE(s, nnz) TSL[s] = T[eid[s]];
 }
-// K4[eid[s]] += K4S[s] + PK[s] >> 40, Q[eid[s]] += QS[s] + PK[s] mod 2^40:
+// K4[eid[s]] += K4S[s] + PK[s] >> qb, Q[eid[s]] += QS[s] + PK[s] mod 2^qb:
 // the clique pass adds only at out-slots, one per edge, so the adds need no
 // atomics
 __global__ void k_fold_k4q(const uint64_t* __restrict__ K4S, const uint64_t* __restrict__ QS,
-                           const uint64_t* __restrict__ PK, const int32_t* __restrict__ eid, int64_t nnz,
+                           const uint64_t* __restrict__ PK, int qb, const int32_t* __restrict__ eid, int64_t nnz,
                            uint64_t* __restrict__ K4, uint64_t* __restrict__ Q) {
     GRID_STRIDE(s, nnz) {
         const uint64_t pk = PK[s];
-        const uint64_t k = K4S[s] + (pk >> 40), q = QS[s] + (pk & ((1ULL << 40) - 1));
+        const uint64_t k = K4S[s] + (pk >> qb), q = QS[s] + (pk & ((1ULL << qb) - 1));
         if (k | q) {
             const int32_t e = eid[s];
             K4[e] += k;
@@ -889,6 +894,27 @@ __global__ void k_fold_k4q(const uint64_t* __restrict__ K4S, const uint64_t* __r
         }
     }
 }
+// The packed (K4, Q) split for a graph of maximum degree dmax: with Q in the
+// low qb bits, an edge whose lower end has degree d packs safely when
+// C(d - 1, 2) < 2^(64 - qb) (K4 <= C(T, 2), T <= d - 1) and
+// 2 (d - 1)(dmax - 1) < 2^qb (Q sums at most d - 1 terms of two T's, each
+// below dmax).  The qb that allows the largest d is chosen.
+PackSplit pack_split(int64_t dmax) {
+    PackSplit best{0, 40};
+    for (int qb = 20; qb <= 44; qb++) {
+        const double kcap = std::ldexp(1.0, 64 - qb), qcap = std::ldexp(1.0, qb);
+        // largest d with (d-1)(d-2)/2 < kcap, and with 2 (d-1)(dmax-1) < qcap
+        // (all products below 2^53: exact in double)
+        double dk = std::floor((3.0 + std::sqrt(1.0 + 8.0 * kcap)) / 2.0);
+        while (dk > 2 && (dk - 1) * (dk - 2) / 2 >= kcap) dk -= 1;
+        double dq = dmax > 1 ? std::floor(qcap / (2.0 * (double)(dmax - 1))) : 1e18;
+        while (dq > 1 && 2.0 * (dq - 1) * (double)(dmax - 1) >= qcap) dq -= 1;
+        const double d = std::min(std::min(dk, dq), (double)INT32_MAX);
+        if (d > best.deg) best = PackSplit{(int32_t)d, qb};
+    }
+    return best;
+}
+
 // T[eid[s]] += TS[s] for every slot s with a count
 __global__ void k_fold_u32(const uint32_t* __restrict__ TS, const int32_t* __restrict__ eid, int64_t nnz,
                            uint32_t* __restrict__ T) {
@@ -1122,8 +1148,7 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
     // build)
     uint32_t* TSL = st.TS.get();
     uint64_t *K4S = st.k0.get(), *QS = st.k0.get() + 2 * m, *PK = st.k1.get();
-    // packing needs 2 kNarrowDeg (max degree) < 2^40
-    const bool pack = 2.0 * kNarrowDeg * (double)st.max_deg < (double)(1ULL << 40);
+    const PackSplit pk = pack_split(st.max_deg);
     hipLaunchKernelGGL(k_slot_gather_u32, dim3(gz), dim3(256), 0, s, out.T, st.reid.get(), 2 * m, TSL);
     hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, K4S, 4 * m);
     hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, PK, 2 * m);
@@ -1132,23 +1157,24 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
         hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, st.mw_small, (char*)nullptr,
-                           st.el_small, st.ecap_small, tl, TSL, K4S, QS, PK, pack, (uint32_t*)nullptr, w + 1);
+                           st.el_small, st.ecap_small, tl, TSL, K4S, QS, PK, pk, (uint32_t*)nullptr, w + 1);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, st.mw_mid, (char*)nullptr, st.el_mid,
-                           st.ecap_mid, tl, TSL, K4S, QS, PK, pack, st.lq.get() + 5, w + 1);
+                           st.ecap_mid, tl, TSL, K4S, QS, PK, pk, st.lq.get() + 5, w + 1);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_large.get(), st.n_large, st.kq, st.mw_large, (char*)nullptr,
-                           st.el_large, st.ecap_large, tl, TSL, K4S, QS, PK, pack, st.lq.get() + 1, w + 1);
+                           st.el_large, st.ecap_large, tl, TSL, K4S, QS, PK, pk, st.lq.get() + 1, w + 1);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_k4q<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
-                           st.mw_huge, st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, TSL, K4S, QS, PK, pack,
+                           st.mw_huge, st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, TSL, K4S, QS, PK, pk,
                            st.lq.get() + 3, w + 1);
-    hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, s, K4S, QS, PK, st.reid.get(), 2 * m, out.K4, st.Q.get());
+    hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, s, K4S, QS, PK, pk.qb, st.reid.get(), 2 * m, out.K4,
+                       st.Q.get());
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
 }

@@ -230,12 +230,13 @@ def test_wide_list_slot_sums(ctx):
 
 
 def test_high_degree_local_edge_sums(ctx):
-    """Clique-pass sums of an edge whose ends both have degree above 5792
-    (64-bit K4 and Q sums instead of the packed ones).  Book graph: hubs x, y
+    """Clique-pass sums of an edge whose ends both have degree above the
+    packing limit (separate 64-bit K4 and Q sums instead of the packed one;
+    for maximum degree 70001 the limit is 61356).  Book graph: hubs x, y
     adjacent, N pages each adjacent to both, pages matched in pairs.  Closed
     forms: (x, y) has T = N, K4 = N/2; a hub-page or page-page edge has T = 2,
     K4 = 1; C4 = 0 on every edge (every 4-cycle has a chord)."""
-    N = 6000
+    N = 70000
     x, y = 0, 1
     w = np.arange(2, 2 + N, dtype=np.int32)
     src = np.concatenate([[x], np.full(N, x), np.full(N, y), w[0::2]]).astype(np.int32)
