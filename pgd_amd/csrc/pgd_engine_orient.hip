@@ -140,11 +140,12 @@ __global__ void k_bin_scatter(const int32_t* __restrict__ f, const int64_t* __re
 // local-graph kernels (triangles; 4-cliques + Q).  One block per vertex a of
 // the list; the block owns the local graph on N+(a), k = |N+(a)| <= KMAX.
 // Storage for the per-vertex arrays is either LDS (kGlobal == false) or a
-// per-block slice of HBM scratch (kGlobal == true, for k > KMAX).
+// per-block slice of HBM scratch (kGlobal == true, for k > KMAX).  These are
+// the clique pass's arrays (the triangle pass has its own, TriArrays).
 // ---------------------------------------------------------------------------
 struct LocalArrays {
     int32_t* L;        // N+(a), ascending          [kmax]
-    int32_t* Le;       // edge id of a -> L[i]       [kmax]
+    int32_t* Le;       // edge id of a -> L[i]; then the degree of L[i] [kmax]
     int64_t* obeg;     // first out-slot of L[i]     [kmax]
     int64_t* pref;     // inclusive scan of |N+(L[i])| [kmax]
     uint32_t* cnt;     // per-local-vertex counter   [kmax]
@@ -1104,8 +1105,6 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, st.lq.get(), (int64_t)6);  // bin work queues
     PGD_HIP(hipGetLastError());
     unsigned long long* w = st.work.get();
-    const int32_t *rr = nullptr;
-    (void)rr;
     TriLists tl{st.tlist.get(), st.tcap, (unsigned long long*)st.tcur.get(), st.toff.get(), st.tlen.get()};
     if (st.tcap > 0) {
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, st.tcur.get(), (int64_t)1);
