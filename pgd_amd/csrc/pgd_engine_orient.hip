@@ -959,7 +959,7 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st, bool plan) 
     ensure(st.P, ctx, m);
     ensure(st.Q, ctx, m);
     ensure(st.TS, ctx, nnz);
-    ensure(st.work, ctx, 4);
+    ensure(st.work, ctx, 5);
     ensure(st.lq, ctx, 6);
     int gn = grid_cap(ctx, (n + 255) / 256, 16), gm = grid_cap(ctx, (m + 255) / 256, 16);
     const int vb = bitsfor(n - 1), db = bitsfor(g.max_deg), gb = g.group ? bitsfor(g.ngroups - 1) : 0;
@@ -1101,7 +1101,7 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
     hipLaunchKernelGGL(k_fill_u64, dim3(gm), dim3(256), 0, s, out.K4, m);
     hipLaunchKernelGGL(k_fill_u64, dim3(gm), dim3(256), 0, s, st.P.get(), m);
     hipLaunchKernelGGL(k_fill_u64, dim3(gm), dim3(256), 0, s, st.Q.get(), m);
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)st.work.get(), (int64_t)4);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)st.work.get(), (int64_t)5);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, st.lq.get(), (int64_t)6);  // bin work queues
     PGD_HIP(hipGetLastError());
     unsigned long long* w = st.work.get();
@@ -1201,7 +1201,7 @@ static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, T
 }
 
 void orient_profile(pgd_ctx* ctx, OrientState& st, Timer& timer, Timing* tm) {
-    unsigned long long wk[4] = {0, 0, 0, 0};
+    unsigned long long wk[5] = {0, 0, 0, 0, 0};
     PGD_HIP(hipMemcpyAsync(wk, st.work.get(), sizeof(wk), hipMemcpyDeviceToHost, ctx->stream));
     PGD_HIP(hipStreamSynchronize(ctx->stream));
     // algorithmic HBM bytes per pass (work models):
@@ -1211,9 +1211,11 @@ void orient_profile(pgd_ctx* ctx, OrientState& st, Timer& timer, Timing* tm) {
     //     local edge);
     //   light wedges: the 4 B endpoint id of every wedge, read by the count
     //     and the accumulate pass;
-    //   tiled wedges: the same two 4 B reads plus the 4 B per-slot sum;
+    //   tiled wedges: the 4 B endpoint id read by the count pass, and for
+    //     the wedges of tiles with a repeated endpoint (the only ones the
+    //     accumulate pass visits) a second 4 B read and the 4 B per-slot sum;
     //   combine: 3 x 8 B + 4 B read and 8 B written per edge.
-    const double bytes_of[5] = {4.0 * wk[0], 24.0 * wk[1], 8.0 * wk[2], 12.0 * wk[3], 36.0 * st.m};
+    const double bytes_of[5] = {4.0 * wk[0], 24.0 * wk[1], 8.0 * wk[2], 4.0 * wk[3] + 8.0 * wk[4], 36.0 * st.m};
     const double units_of[5] = {(double)wk[0], (double)wk[1], (double)wk[2], (double)wk[3], (double)st.m};
     double best = 0;
     tm->phases.clear();

@@ -345,10 +345,11 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
                                           const int32_t* __restrict__ col, uint32_t* cnt,
                                           uint64_t* __restrict__ PS, uint32_t* __restrict__ PS32,
                                           int64_t* wbase, int32_t* wslot,
-                                          unsigned long long* wpsum, uint32_t* wflag, int* ovf, int lane, int w,
-                                          unsigned long long& done) {
+                                          unsigned long long* wpsum, uint32_t* wflag, int* ovf, int* rep, int lane,
+                                          int w, unsigned long long* done) {
     const int32_t half = 1 << (log2r - 1);
     const int32_t lo = mode == 2 ? half : 0;
+    bool again = false;  // mode 0, pass 0: some endpoint of this wavefront's wedges was seen before
     // mode 0: the counter of endpoint c is the low (c < half) or high half of
     // word c mod half, so consecutive endpoints use consecutive words (banks)
     const int hb = log2r - 1;
@@ -378,7 +379,9 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
         wpsum[lane] = 0;
         __builtin_amdgcn_wave_barrier();
         const int32_t total = __shfl(x, 63, 64);
-        if (pass == 0 && mode < 2 && lane == 0) done += (unsigned long long)total;
+        // wedges counted (once per tile) and wedges accumulated (tiles with
+        // a repeated endpoint; an overflowing tile once)
+        if (lane == 0 && (pass ? mode < 2 : mode == 0)) done[pass] += (unsigned long long)total;
         // (the owner of position q, the segment holding it, is the last segment
         // starting at or before q: the segment starts inside a round are
         // flagged in LDS and read back as one 64-bit ballot per step, so the
@@ -418,20 +421,22 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
             for (int u = 0; u < kU; u++) c[u] = col[slot[u]] - c0;
             if (mode == 0) {
                 if (pass == 0) {
-                    if (chk) {
-                        uint32_t old[kU];
+                    // the counts before the add tell whether an endpoint repeats
+                    // (a tile without repeats closes no 4-cycle: its
+                    // accumulate pass is skipped) and, with chk, whether a
+                    // count would pass 65535
+                    uint32_t old[kU];
 #pragma unroll
-                        for (int u = 0; u < kU; u++)
-                            old[u] = ok[u] ? atomicAdd(&cnt[c[u] & hm], 1u << ((c[u] >> hb) * 16)) : 0u;
-                        bool hit = false;
+                    for (int u = 0; u < kU; u++)
+                        old[u] = ok[u] ? atomicAdd(&cnt[c[u] & hm], 1u << ((c[u] >> hb) * 16)) : 0u;
+                    bool hit = false;
 #pragma unroll
-                        for (int u = 0; u < kU; u++) hit |= ((old[u] >> ((c[u] >> hb) * 16)) & 0xffffu) == 0xffffu;
-                        if (hit) *ovf = 1;
-                    } else {
-#pragma unroll
-                        for (int u = 0; u < kU; u++)
-                            if (ok[u]) atomicAdd(&cnt[c[u] & hm], 1u << ((c[u] >> hb) * 16));
+                    for (int u = 0; u < kU; u++) {
+                        const uint32_t h = (old[u] >> ((c[u] >> hb) * 16)) & 0xffffu;
+                        hit |= h == 0xffffu;
+                        again |= h != 0;
                     }
+                    if (chk && hit) *ovf = 1;
                     continue;
                 }
                 uint32_t v[kU];
@@ -485,6 +490,7 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
         if (pass == 1 && lane < kc && wpsum[lane]) atomicAdd((unsigned long long*)&PS[wslot[lane]], wpsum[lane]);
         __builtin_amdgcn_wave_barrier();
     }
+    if (pass == 0 && mode == 0 && __ballot(again) && lane == 0) *rep = 1;
 }
 
 // zero n words of LDS (n a multiple of 4, p 16-byte aligned) with 16-byte stores
@@ -517,10 +523,10 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     __shared__ uint32_t wflag[NW][64];  // segment-start flags of one round of a wavefront (4 bytes per lane)
     __shared__ unsigned long long wpsum[NW][64];
     __shared__ int64_t item_sh;
-    __shared__ int ovf;
+    __shared__ int ovf, rep;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int R = 1 << log2r;
-    unsigned long long done = 0;
+    unsigned long long done[2] = {0, 0};  // wedges counted, wedges accumulated
     WedgeSeg* seg = segs + (size_t)blockIdx.x * (size_t)segcap;
     for (;;) {
         if (tid == 0) item_sh = (int64_t)atomicAdd(qctr, 1u);
@@ -579,30 +585,37 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
             if (sb == se) continue;
             const int32_t c0 = (W.t0 + t) << log2r;
             clear_lds<NT>(cnt, R / 2);
-            if (tid == 0) ovf = 0;
+            if (tid == 0) {
+                ovf = 0;
+                rep = 0;
+            }
             __syncthreads();
             tile_pass<NT>(0, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w], wpsum[w], wflag[w],
-                          &ovf, lane, w, done);
+                          &ovf, &rep, lane, w, done);
             __syncthreads();
             if (!ovf) {
-                tile_pass<NT>(1, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w], wpsum[w],
-                              wflag[w], &ovf, lane, w, done);
-                __syncthreads();
+                // every count 1: no 4-cycle closes in this tile
+                if (rep) {
+                    tile_pass<NT>(1, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w],
+                                  wpsum[w], wflag[w], &ovf, &rep, lane, w, done);
+                    __syncthreads();
+                }
             } else {
                 for (int mode = 1; mode <= 2; mode++) {
                     clear_lds<NT>(cnt, R / 2);
                     __syncthreads();
                     tile_pass<NT>(0, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w],
-                                  wpsum[w], wflag[w], &ovf, lane, w, done);
+                                  wpsum[w], wflag[w], &ovf, &rep, lane, w, done);
                     __syncthreads();
                     tile_pass<NT>(1, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w],
-                                  wpsum[w], wflag[w], &ovf, lane, w, done);
+                                  wpsum[w], wflag[w], &ovf, &rep, lane, w, done);
                     __syncthreads();
                 }
             }
         }
     }
-    if (done) atomicAdd(work, done);
+    if (done[0]) atomicAdd(work, done[0]);
+    if (done[1]) atomicAdd(work + 1, done[1]);
 }
 
 // fold the per-slot accumulators into P by edge id
