@@ -493,6 +493,115 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
     if (pass == 0 && mode == 0 && __ballot(again) && lane == 0) *rep = 1;
 }
 
+// A tile of at most 64 segments and few wedges, taken by one wavefront on
+// its own: the endpoints are counted in a private open-addressing table of
+// tw words (entry = endpoint << 16 | count, 0 = empty) instead of the
+// block's counter array, so the block neither clears that array nor
+// synchronises for the tile.  Returns false, having changed nothing, when
+// the tile has more than min(64 kU, tw / 2) wedges (one round per lane, the
+// table at most half full).
+template <int kU>
+__device__ __forceinline__ bool small_tile(uint32_t sb, uint32_t se, int32_t c0, const WedgeSeg* __restrict__ seg,
+                                        const int32_t* __restrict__ col, uint32_t* tab, int tbits,
+                                        uint64_t* __restrict__ PS, uint32_t* __restrict__ PS32, int64_t* wbase,
+                                        int32_t* wslot, unsigned long long* wpsum, uint32_t* wflag, int lane,
+                                        unsigned long long* done) {
+    const int kc = (int)(se - sb);
+    int32_t len = 0, sl = 0;
+    int64_t st = 0;
+    bool wide = false;
+    if (lane < kc) {
+        const WedgeSeg g = seg[sb + lane];
+        len = g.len & 0x7fffffff;
+        wide = g.len < 0;
+        st = g.start;
+        sl = g.sab;
+    }
+    const uint64_t wmask = __ballot(wide);
+    const int32_t x = wave_incl_sum((uint32_t)len);
+    const int32_t total = __shfl(x, 63, 64);
+    if (total > min(64 * kU, 1 << (tbits - 1))) return false;
+    const int tw = 1 << tbits;
+    for (int i = lane; i < tw / 4; i += 64) ((uint4*)tab)[i] = make_uint4(0u, 0u, 0u, 0u);
+    wbase[lane] = st - (int64_t)(x - len);
+    wslot[lane] = sl;
+    wpsum[lane] = 0;
+    wflag[lane] = 0;
+    __builtin_amdgcn_wave_barrier();
+    // owners of the positions: as in the block passes, with one round
+    const int32_t es = lane < kc ? x - len : INT32_MAX;
+    const uint64_t upto = (2ULL << lane) - 1;
+    if (es < 64 * kU) ((uint8_t*)wflag)[(es & 63) * 4 + (es >> 6)] = 1;
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t f4 = wflag[lane];
+    int o[kU];
+    bool ok[kU];
+    int64_t slot[kU];
+    int32_t c[kU];
+    uint64_t starts[kU];
+    int cum = -1;
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+        starts[u] = __ballot(((f4 >> (8 * u)) & 0xffu) != 0);
+        const int32_t q = 64 * u + lane;
+        ok[u] = q < total;
+        o[u] = cum + __popcll(starts[u] & upto);
+        cum += __popcll(starts[u]);
+        slot[u] = wbase[o[u]] + (ok[u] ? q : total - 1);
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) c[u] = col[slot[u]] - c0;
+    // count: claim the endpoint's entry (or find it) and add one
+    const uint32_t tmask = (uint32_t)tw - 1;
+    uint32_t pos[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+        pos[u] = 0;
+        if (ok[u]) {
+            const uint32_t key = (uint32_t)c[u] << 16;
+            uint32_t sp = ((uint32_t)c[u] * 0x9E3779B1u) >> (32 - tbits);
+            for (;;) {
+                const uint32_t e = atomicCAS(&tab[sp], 0u, key | 1u);
+                if (e == 0u) break;
+                if ((e & 0xffff0000u) == key) {
+                    atomicAdd(&tab[sp], 1u);
+                    break;
+                }
+                sp = (sp + 1) & tmask;
+            }
+            pos[u] = sp;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t v[kU];
+    bool again = false;
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+        v[u] = ok[u] ? (tab[pos[u]] & 0xffffu) - 1 : 0u;
+        again |= v[u] != 0;
+    }
+    if (lane == 0) done[0] += (unsigned long long)total;
+    if (!__ballot(again)) return true;  // every count 1: no 4-cycle closes here
+    if (lane == 0) done[1] += (unsigned long long)total;
+#pragma unroll
+    for (int u = 0; u < kU; u++)
+        if (v[u]) slot_add(PS, PS32, wmask, o[u], slot[u], v[u]);
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+        const uint32_t P = wave_incl_sum(v[u]);
+        const uint64_t sm = starts[u] & upto;
+        const int rs = sm ? 63 - __clzll((long long)sm) : 0;
+        const uint32_t before = __shfl(P, rs > 0 ? rs - 1 : 0, 64);
+        const uint32_t run = P - (rs > 0 ? before : 0u);
+        const bool last = lane == 63 || ((starts[u] >> (lane + 1)) & 1);
+        if (last && run) atomicAdd(&wpsum[o[u]], (unsigned long long)run);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < kc && wpsum[lane]) atomicAdd((unsigned long long*)&PS[wslot[lane]], wpsum[lane]);
+    __builtin_amdgcn_wave_barrier();
+    return true;
+}
+
 // zero n words of LDS (n a multiple of 4, p 16-byte aligned) with 16-byte stores
 template <int NT>
 __device__ __forceinline__ void clear_lds(uint32_t* p, int n) {
@@ -526,6 +635,9 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     __shared__ int ovf, rep;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int R = 1 << log2r;
+    // a wavefront's small-tile table: 2^tbits words of the counter array's R/2
+    int tbits = 9;
+    while (tbits > 2 && (NW << tbits) > R / 2) tbits--;
     unsigned long long done[2] = {0, 0};  // wedges counted, wedges accumulated
     WedgeSeg* seg = segs + (size_t)blockIdx.x * (size_t)segcap;
     for (;;) {
@@ -580,9 +692,20 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
         // counters in the same LDS (modes 1 and 2).
         // a count is at most the hub's in-degree: below 65536 no 16-bit counter can overflow
         const bool chk = ie - ib > 65535;
-        for (int t = 0; t < T; t++) {
+        // small tiles first, one per wavefront in the wavefront's slice of
+        // the counter array; cur[t] then marks the tiles left for the block
+        for (int t = w; t < T; t += NW) {
             const uint32_t sb = base[t], se = base[t + 1];
-            if (sb == se) continue;
+            uint32_t left = se - sb > 64u;
+            if (!left && sb < se)
+                left = !small_tile<4>(sb, se, (W.t0 + t) << log2r, seg, col, cnt + w * (1 << tbits), tbits, PS,
+                                      PS32, wbase[w], wslot[w], wpsum[w], wflag[w], lane, done);
+            if (lane == 0) cur[t] = left;
+        }
+        __syncthreads();
+        for (int t = 0; t < T; t++) {
+            if (!cur[t]) continue;
+            const uint32_t sb = base[t], se = base[t + 1];
             const int32_t c0 = (W.t0 + t) << log2r;
             clear_lds<NT>(cnt, R / 2);
             if (tid == 0) {
