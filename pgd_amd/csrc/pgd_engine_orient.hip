@@ -254,7 +254,8 @@ __device__ __forceinline__ void locate(const LocalArrays& A, int k, int64_t tota
 // Record the hits of one round (kU positions per thread) in the block's
 // local edge list: one LDS reservation per wavefront per round.  Entry =
 // (out-slot of the edge (L[i], L[j]) << 32) | (j << 16) | i.
-template <int KU>
+// kFull: every lane of the wavefront is active (the scan is then a DPP scan)
+template <int KU, bool kFull>
 __device__ __forceinline__ void record_hits(uint64_t* __restrict__ el, int* ecount, const int* i, const int* j,
                                             const uint32_t* e) {
     const int lane = threadIdx.x & 63;
@@ -262,18 +263,26 @@ __device__ __forceinline__ void record_hits(uint64_t* __restrict__ el, int* ecou
 #pragma unroll
     for (int u = 0; u < KU; u++) mine += j[u] >= 0;
     // exclusive scan of the lanes' hit counts
-    int x = mine;
+    int x, wave_total, leader;
+    if (kFull) {
+        x = (int)wave_incl_sum((uint32_t)mine);
+        wave_total = __shfl(x, 63, 64);
+        leader = 0;
+    } else {
+        x = mine;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        const uint64_t active = __ballot(1);
+        const int last = 63 - __clzll((long long)active);
+        wave_total = __shfl(x, last, 64);
+        leader = __ffsll((unsigned long long)active) - 1;
     }
-    const uint64_t active = __ballot(1);
-    const int last = 63 - __clzll((long long)active);
-    const int wave_total = __shfl(x, last, 64);
-    const int leader = __ffsll((unsigned long long)active) - 1;
+    if (!wave_total) return;
     int basepos = 0;
-    if (lane == leader && wave_total) basepos = atomicAdd(ecount, wave_total);
+    if (lane == leader) basepos = atomicAdd(ecount, wave_total);
     basepos = __shfl(basepos, leader, 64);
     int pos = basepos + x - mine;
 #pragma unroll
@@ -540,7 +549,7 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
                     e[u] = (uint32_t)slot[u];
                 }
             }
-            if (el) record_hits<kU>(el, &ecount, ii, j, e);
+            if (el) record_hits<kU, true>(el, &ecount, ii, j, e);
             __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
@@ -603,20 +612,30 @@ struct PackSplit {
 };
 
 // (narrow: the lower endpoint L[i] of the far edge has degree <= pk.deg,
-// kept by the caller in A.Le[i])
+// kept by the caller in A.Le[i]; tij: T of the far edge, read by the caller
+// in the first column block)
 __device__ __forceinline__ void edge_terms_k4q(const LocalArrays& A, int wn, bool first, int i, int j,
-                                               uint32_t sij, const uint32_t* __restrict__ TSL,
+                                               uint32_t sij, uint32_t tij,
                                                uint64_t* __restrict__ K4S, uint64_t* __restrict__ QS,
                                                uint64_t* __restrict__ PK, PackSplit pk) {
+    // the rows' words four at a time: the eight LDS reads are in flight together
     uint32_t kij = 0;
-    for (int w = 0; w < wn; w++) kij += __popc(A.M[i * wn + w] & A.M[j * wn + w]);
+    const uint32_t* ri = A.M + i * wn;
+    const uint32_t* rj = A.M + j * wn;
+    int w = 0;
+    for (; w + 4 <= wn; w += 4) {
+        const uint32_t a0 = ri[w], a1 = ri[w + 1], a2 = ri[w + 2], a3 = ri[w + 3];
+        const uint32_t b0 = rj[w], b1 = rj[w + 1], b2 = rj[w + 2], b3 = rj[w + 3];
+        kij += __popc(a0 & b0) + __popc(a1 & b1) + __popc(a2 & b2) + __popc(a3 & b3);
+    }
+    for (; w < wn; w++) kij += __popc(ri[w] & rj[w]);
     if (kij) {
         atomicAdd(&A.cnt[i], kij);
         atomicAdd(&A.cnt[j], kij);
     }
     uint64_t q = 0;
     if (first) {
-        const uint64_t tai = A.ta[i], taj = A.ta[j], tij = TSL[sij];
+        const uint64_t tai = A.ta[i], taj = A.ta[j];
         q = tai + taj;
         atomicAdd((unsigned long long*)&A.acc[i], (unsigned long long)(taj + tij));
         atomicAdd((unsigned long long*)&A.acc[j], (unsigned long long)(tai + tij));
@@ -703,7 +722,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                             e[u] = (uint32_t)slot[u];
                         }
                     }
-                    if (first && el) record_hits<kU>(el, &ecount, i, j, e);
+                    if (first && el) record_hits<kU, false>(el, &ecount, i, j, e);
                 }
             }
             __syncthreads();
@@ -711,10 +730,23 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
             if (lst) {
                 const int nl = gl ? kept : ecount;
                 if (threadIdx.x == 0) done += (unsigned long long)nl;
-                for (int p = threadIdx.x; p < nl; p += NT) {
-                    const uint64_t e = lst[p];
-                    edge_terms_k4q(A, wn, first, (int)(e & 0xffff), (int)((e >> 16) & 0xffff), (uint32_t)(e >> 32),
-                                   TSL, K4S, QS, PK, pk);
+                // kU list entries per thread at once: their loads, and then
+                // their T loads, are in flight together (unconditional:
+                // entries past the end are read as the last one)
+                for (int p = threadIdx.x; p < nl; p += NT * kU) {
+                    uint64_t e[kU];
+                    uint32_t t[kU] = {};
+#pragma unroll
+                    for (int u = 0; u < kU; u++) e[u] = lst[min(p + u * NT, nl - 1)];
+                    if (first) {
+#pragma unroll
+                        for (int u = 0; u < kU; u++) t[u] = TSL[(uint32_t)(e[u] >> 32)];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kU; u++)
+                        if (p + u * NT < nl)
+                            edge_terms_k4q(A, wn, first, (int)(e[u] & 0xffff), (int)((e[u] >> 16) & 0xffff),
+                                           (uint32_t)(e[u] >> 32), t[u], K4S, QS, PK, pk);
                 }
             } else {
                 // no list: wc == W here (the HBM bin holds the full matrix)
@@ -729,10 +761,13 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                         if (iu[u] >= 0) x[u] = col[su[u]];
 #pragma unroll
                     for (int u = 0; u < kU; u++) ju[u] = iu[u] >= 0 ? find_local(A, x[u]) : -1;
+                    uint32_t tu[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; u++) tu[u] = first && ju[u] >= 0 ? TSL[su[u]] : 0u;
 #pragma unroll
                     for (int u = 0; u < kU; u++)
                         if (ju[u] >= 0)
-                            edge_terms_k4q(A, wn, first, iu[u], ju[u], (uint32_t)su[u], TSL, K4S, QS, PK, pk);
+                            edge_terms_k4q(A, wn, first, iu[u], ju[u], (uint32_t)su[u], tu[u], K4S, QS, PK, pk);
                 }
             }
             __syncthreads();

@@ -302,18 +302,6 @@ __global__ void k_item_keys(const uint64_t* __restrict__ key, int64_t n, uint64_
     }
 }
 
-// inclusive sum over the lanes 0..lane of a wavefront: within rows of 16
-// (DPP row shifts), then the row totals carried by DPP row broadcasts
-__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x) {
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);  // row_shr:1
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);  // row_shr:2
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);  // row_shr:4
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);  // row_shr:8
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false); // row_bcast:15 -> rows 1, 3
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false); // row_bcast:31 -> rows 2, 3
-    return x;
-}
-
 // add v to the per-slot accumulator of the slot: 32-bit unless its list
 // (segment o of the group) is wide.  A slot (b, c) gains at most d(c) - 1
 // from each of at most d(b) hubs, below 2^32 unless d(b) * max degree

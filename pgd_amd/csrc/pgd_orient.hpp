@@ -143,6 +143,18 @@ __device__ __forceinline__ int find_in(const int32_t* L, int lo, int hi, int32_t
     return -1;
 }
 
+// inclusive sum over the lanes 0..lane of a wavefront: within rows of 16
+// (DPP row shifts), then the row totals carried by DPP row broadcasts
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);  // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false); // row_bcast:15 -> rows 1, 3
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false); // row_bcast:31 -> rows 2, 3
+    return x;
+}
+
 // first position in col[lo..hi) (ascending) holding a value >= x
 __device__ __forceinline__ int64_t lower_bound_g(const int32_t* __restrict__ col, int64_t lo, int64_t hi, int32_t x) {
     while (lo < hi) {
