@@ -812,7 +812,9 @@ pgd_status pgd_preprocess(pgd_graph* g, const char* method, int32_t s2l) {
         g->prepped = true;
         g->have_order = false;
         g->have_counts = false;
-        if (g->orient) { orient_state_free(ctx, g->orient); g->orient = nullptr; }
+        // (the oriented engine's buffers stay: every count rebuilds their
+        // contents, and releasing and re-mapping its multi-GB lists costs
+        // about 8 ms per call on a 64M-edge graph)
     });
 }
 
