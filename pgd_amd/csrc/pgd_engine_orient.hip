@@ -421,7 +421,23 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
         const int32_t a = list[li];
         const int k = outdeg[a];
         const int ng = (k + 63) / 64;
-        if (tid == 0) ecount = 0;
+        if (tid == 0) {
+            ecount = 0;
+            // LDS bins write the list in place, in the block's reserved part
+            // of G, when the part holds k(k-1)/2 (the most local edges):
+            // parts of twice that are reserved as needed
+            gpos = -1;
+            if (!kGlobal && el) {
+                const int64_t bound = (int64_t)k * (k - 1) / 2;
+                if (chunk_end - chunk_pos < bound) {
+                    const int64_t want = max(2 * bound, (int64_t)kListChunk);
+                    const int64_t c = (int64_t)atomicAdd(tl.cur, (unsigned long long)want);
+                    chunk_pos = c;
+                    chunk_end = c + want <= tl.gcap ? c + want : c;  // an empty part once G is full
+                }
+                if (chunk_end - chunk_pos >= bound) gpos = chunk_pos;
+            }
+        }
         // N+(a), its hash and bitmap, the flattened out-list prefix
         {
             const int64_t s0 = ostart[a];
@@ -475,6 +491,8 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
         const int64_t total = A.pref[k - 1];
         if (tid == 0) done += (unsigned long long)total;
         const int nch = A.gch[ng];
+        // where the hits go: G in place, or the block's list (copied to G below)
+        uint64_t* const ev = (!kGlobal && el && gpos >= 0) ? tl.G + gpos : el;
         for (int cid = w; cid < nch; cid += NW) {
             // the chunk's group: the last g with gch[g] <= cid
             int glo = 0, ghi = ng - 1;
@@ -549,13 +567,22 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
                     e[u] = (uint32_t)slot[u];
                 }
             }
-            if (el) record_hits<kU, true>(el, &ecount, ii, j, e);
+            if (ev) record_hits<kU, true>(ev, &ecount, ii, j, e);
             __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
         for (int i = tid; i < k; i += NT)
             if (A.cnt[i]) atomicAdd(&T[A.Le[i]], A.cnt[i]);
-        if (el) {
+        if (!kGlobal && el) {
+            if (tid == 0) {
+                // written in place, or not kept once G is full (the clique
+                // pass then searches again)
+                const int64_t p = gpos;
+                tl.off[a] = p;
+                tl.len[a] = p < 0 ? -1 : ecount;
+                if (p >= 0) chunk_pos += ecount;
+            }
+        } else if (el) {
             const int ne = ecount;
             if (tid == 0) {
                 // lists are placed in chunks the block reserves (one global
