@@ -161,6 +161,9 @@ typedef struct pgd_tuning {
     int32_t wedge_split; /* work items per block the heavy wedges are cut into (4) */
     int32_t tri_lists;   /* local-edge lists kept from the triangle pass for the clique pass, in
                             entries per edge (0: off; the clique pass searches again) (64) */
+    int32_t tri_search;  /* the triangle pass looks the later members of N+(a) up in the hash table of
+                            N+(b) (out-degree above 32) instead of probing N+(b), when tri_search x
+                            their number is below |N+(b)| (0: always probe) (4) */
 } pgd_tuning;
 pgd_status pgd_tuning_default(pgd_tuning *t);
 /* Parameters of later pgd_decompose calls on g. */

@@ -46,7 +46,8 @@ EXPORTS = [
 class Tuning(C.Structure):
     """pgd_tuning: launch parameters of the oriented engine."""
     _fields_ = [(f, C.c_int32) for f in (
-        "small_k", "mid_k", "grid_small", "grid_mid", "wedge_log2r", "wedge_nt", "wedge_split", "tri_lists")]
+        "small_k", "mid_k", "grid_small", "grid_mid", "wedge_log2r", "wedge_nt", "wedge_split", "tri_lists",
+        "tri_search")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -390,6 +390,71 @@ __device__ __forceinline__ int tri_find(const TriArrays& A, int32_t x, uint32_t 
     }
 }
 
+// Hash tables of the out-lists N+(b) of the vertices of large out-degree,
+// in HBM: entry = (member << 32) | position in N+(b), ~0 empty; off[b] =
+// (first entry << 6) | log2(table size), or -1 without a table.  factor: a
+// local vertex's later members are looked up in its table when factor x
+// their number is below its out-degree (0: never).
+struct OutHash {
+    const uint64_t* tab;
+    const int64_t* off;
+    int factor;
+};
+constexpr int64_t kSearchBit = (int64_t)1 << 62;  // in obeg: the list is searched, not probed
+constexpr int kSearchOwner = 1 << 30;             // in the owner index: likewise
+constexpr int kOutHashMin = 32;                   // out-degree above which a vertex has a table
+
+// position of x in N+(b), or -1 (h = mhash(x))
+__device__ __forceinline__ int32_t out_hash_find(const OutHash& oh, int32_t b, int32_t x, uint32_t h) {
+    const int64_t t = oh.off[b];
+    const int bits = (int)(t & 63);
+    const uint64_t* tab = oh.tab + (t >> 6);
+    const uint32_t mask = (1u << bits) - 1;
+    uint32_t s = h >> (32 - bits);
+    for (;;) {
+        const uint64_t kv = tab[s];
+        if (kv == ~0ull) return -1;
+        if ((int32_t)(kv >> 32) == x) return (int32_t)(uint32_t)kv;
+        s = (s + 1) & mask;
+    }
+}
+
+// table sizes: 2^bits >= 2 out-degree for out-degrees above kOutHashMin
+__global__ void k_out_hash_size(const int32_t* __restrict__ outdeg, int64_t n, int32_t* __restrict__ sz) {
+    GRID_STRIDE(b, n) {
+        const int d = outdeg[b];
+        int bits = 0;
+        if (d > kOutHashMin)
+            while ((1 << bits) < 2 * d) bits++;
+        sz[b] = bits ? 1 << bits : 0;
+    }
+}
+__global__ void k_out_hash_off(const int32_t* __restrict__ sz, const int64_t* __restrict__ ex, int64_t n,
+                               int64_t* __restrict__ off) {
+    GRID_STRIDE(b, n) off[b] = sz[b] ? (ex[b] << 6) | (int64_t)(31 - __clz(sz[b])) : -1;
+}
+// one wavefront per vertex: insert N+(b) into its table (cleared to ~0)
+__global__ void k_out_hash_fill(const int64_t* __restrict__ rrow, const int32_t* __restrict__ rcol,
+                                const int64_t* __restrict__ ostart, const int64_t* __restrict__ off, int64_t n,
+                                uint64_t* __restrict__ tab) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t b = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; b < n;
+         b += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+        const int64_t t = off[b];
+        if (t < 0) continue;
+        const int bits = (int)(t & 63);
+        uint64_t* tb = tab + (t >> 6);
+        const uint32_t mask = (1u << bits) - 1;
+        const int64_t s0 = ostart[b], d = rrow[b + 1] - s0;
+        for (int64_t p = lane; p < d; p += 64) {
+            const int32_t x = rcol[s0 + p];
+            const unsigned long long kv = ((unsigned long long)(uint32_t)x << 32) | (uint32_t)p;
+            uint32_t s = mhash(x) >> (32 - bits);
+            while (atomicCAS((unsigned long long*)&tb[s], ~0ull, kv) != ~0ull) s = (s + 1) & mask;
+        }
+    }
+}
+
 template <int NT, bool kGlobal>
 __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                                                   const int32_t* __restrict__ eid, const int64_t* __restrict__ ostart,
@@ -397,7 +462,8 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
                                                   int64_t cnt, int kmax, size_t stride, char* __restrict__ gscratch,
                                                   uint64_t* __restrict__ elist, int64_t ecap, TriLists tl,
                                                   uint32_t* __restrict__ TS, uint32_t* __restrict__ T,
-                                                  uint32_t* __restrict__ queue, unsigned long long* __restrict__ work) {
+                                                  OutHash oh, uint32_t* __restrict__ queue,
+                                                  unsigned long long* __restrict__ work) {
     constexpr int NW = NT / 64;
     __shared__ int64_t li_sh;
     extern __shared__ char lds[];
@@ -448,8 +514,13 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
                 A.L[i] = b;
                 A.Le[i] = eid[s0 + i];
                 const int64_t ob = ostart[b];
-                A.obeg[i] = ob;
-                A.pref[i] = row[b + 1] - ob;
+                const int64_t od = row[b + 1] - ob;
+                // the later members L[j], j > i, looked up in N+(b)'s table,
+                // when that is cheaper than probing all of N+(b) here
+                const int64_t later = k - 1 - i;
+                const bool srch = oh.factor > 0 && oh.off[b] >= 0 && (int64_t)oh.factor * later < od;
+                A.obeg[i] = srch ? (ob | kSearchBit) : ob;
+                A.pref[i] = srch ? later : od;
                 A.cnt[i] = 0;
             }
             __syncthreads();
@@ -512,12 +583,15 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
                 len = (int32_t)(A.pref[i] - ex);
                 es = (int32_t)(ex - G0);
             }
-            // the group's non-empty lists, ranked
+            // the group's non-empty lists, ranked (a searched list's "slots"
+            // are the local offsets j - i - 1)
             const uint64_t nz = __ballot(len > 0);
             if (len > 0) {
                 const int r = __popcll(nz & (upto >> 1));
-                wbase[w][r] = A.obeg[i] - es;
-                wown[w][r] = i;
+                const int64_t ob = A.obeg[i];
+                const bool srch = (ob & kSearchBit) != 0;
+                wbase[w][r] = (srch ? 0 : ob) - es;
+                wown[w][r] = i | (srch ? kSearchOwner : 0);
             } else {
                 es = INT32_MAX;
             }
@@ -531,7 +605,7 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
             for (int u = 0; u < kU; u++) f[u] = (f4 >> (8 * u)) & 0xffu;
             int cum = __popcll(__ballot(es < R)) - 1;  // last list starting before the chunk
             int ii[kU], j[kU];
-            bool ok[kU];
+            bool ok[kU], sr[kU];
             int64_t slot[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
@@ -542,21 +616,33 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
                 const int o = cum + __popcll(starts & upto);
                 cum += __popcll(starts);
                 slot[u] = wbase[w][o] + (ok[u] ? q : glen - 1);
-                ii[u] = wown[w][o];
+                const int wo = wown[w][o];
+                ii[u] = wo & ~kSearchOwner;
+                sr[u] = (wo & kSearchOwner) != 0;
             }
             int32_t x[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) x[u] = col[slot[u]];
+            for (int u = 0; u < kU; u++) x[u] = sr[u] ? A.L[ii[u] + 1 + (int)slot[u]] : col[slot[u]];
             uint32_t h[kU];
             bool maybe[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
                 h[u] = mhash(x[u]);
                 const uint32_t bw = A.bm[h[u] >> (32 - A.bbits + 5)];
-                maybe[u] = ok[u] && ((bw >> ((h[u] >> (32 - A.bbits)) & 31)) & 1);
+                maybe[u] = ok[u] && !sr[u] && ((bw >> ((h[u] >> (32 - A.bbits)) & 31)) & 1);
             }
 #pragma unroll
-            for (int u = 0; u < kU; u++) j[u] = maybe[u] ? tri_find(A, x[u], h[u]) : -1;
+            for (int u = 0; u < kU; u++) {
+                j[u] = maybe[u] ? tri_find(A, x[u], h[u]) : -1;
+                if (ok[u] && sr[u]) {
+                    // L[j] in N+(L[i])?  Its position there gives the out-slot
+                    const int32_t pos = out_hash_find(oh, A.L[ii[u]], x[u], h[u]);
+                    if (pos >= 0) {
+                        j[u] = ii[u] + 1 + (int)slot[u];
+                        slot[u] = (A.obeg[ii[u]] & ~kSearchBit) + pos;
+                    }
+                }
+            }
             uint32_t e[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
@@ -927,6 +1013,9 @@ int64_t make_bin(pgd_ctx* ctx, const K* key, int64_t n, int64_t lo, int64_t hi, 
     return c;
 }
 
+__global__ void k_fill_ones_u64(uint64_t* __restrict__ p, int64_t n) {
+    GRID_STRIDE(i, n) p[i] = ~0ull;
+}
 __global__ void k_fill_u64(uint64_t* __restrict__ p, int64_t n) {
     GRID_STRIDE(i, n) p[i] = 0;
 }
@@ -1139,6 +1228,46 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st, bool plan) 
             ensure(st.tcur, ctx, 1);
         }
     }
+    {
+        // out-list hash tables of the vertices of large out-degree (the
+        // triangle pass looks members up in them instead of probing the
+        // whole list)
+        st.has_oh = false;
+        if (prm.tri_search > 0 && n > 0) {
+            DBuf<int32_t> sz;
+            DBuf<int64_t> ex;
+            sz.alloc(ctx, n);
+            ex.alloc(ctx, n);
+            const int gn = grid_cap(ctx, (n + 255) / 256, 16);
+            hipLaunchKernelGGL(k_out_hash_size, dim3(gn), dim3(256), 0, s, st.outdeg.get(), n, sz.get());
+            PGD_HIP(hipGetLastError());
+            {
+                size_t bytes = 0;
+                PGD_HIP(rocprim::exclusive_scan(nullptr, bytes, sz.get(), ex.get(), (int64_t)0, (size_t)n,
+                                                rocprim::plus<int64_t>(), s));
+                void* t = tmp_storage(ctx, bytes);
+                PGD_HIP(rocprim::exclusive_scan(t, bytes, sz.get(), ex.get(), (int64_t)0, (size_t)n,
+                                                rocprim::plus<int64_t>(), s));
+            }
+            int64_t last_ex = 0;
+            int32_t last_sz = 0;
+            PGD_HIP(hipMemcpyAsync(&last_ex, ex.get() + (n - 1), sizeof(last_ex), hipMemcpyDeviceToHost, s));
+            PGD_HIP(hipMemcpyAsync(&last_sz, sz.get() + (n - 1), sizeof(last_sz), hipMemcpyDeviceToHost, s));
+            PGD_HIP(hipStreamSynchronize(s));
+            const int64_t entries = last_ex + last_sz;
+            if (entries > 0) {
+                ensure(st.ohoff, ctx, n);
+                ensure(st.ohtab, ctx, (size_t)entries);
+                hipLaunchKernelGGL(k_out_hash_off, dim3(gn), dim3(256), 0, s, sz.get(), ex.get(), n, st.ohoff.get());
+                hipLaunchKernelGGL(k_fill_ones_u64, dim3(grid_cap(ctx, (entries + 255) / 256, 16)), dim3(256), 0, s,
+                                   st.ohtab.get(), entries);
+                hipLaunchKernelGGL(k_out_hash_fill, dim3(grid_cap(ctx, (n * 64 + 255) / 256, 16)), dim3(256), 0, s,
+                                   st.rrow.get(), st.rcol.get(), st.ostart.get(), st.ohoff.get(), n, st.ohtab.get());
+                PGD_HIP(hipGetLastError());
+                st.has_oh = true;
+            }
+        }
+    }
     if (plan && st.n_heavy) wedge_plan_build(ctx, st);
 }
 
@@ -1178,27 +1307,28 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
     }
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_cap(ctx, (2 * m + 255) / 256, 16)), dim3(256), 0, s, st.TS.get(),
                        2 * m);
+    const OutHash oh{st.ohtab.get(), st.ohoff.get(), st.has_oh ? prm.tri_search : 0};
     if (timer) timer->begin("k_local_tri");
     if (st.n_small)
         hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
                            st.small_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (size_t)0, (char*)nullptr,
-                           st.el_small, st.ecap_small, tl, st.TS.get(), out.T, (uint32_t*)nullptr, w);
+                           st.el_small, st.ecap_small, tl, st.TS.get(), out.T, oh, (uint32_t*)nullptr, w);
     if (st.n_mid)
         hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
                            st.mid_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (size_t)0, (char*)nullptr, st.el_mid,
-                           st.ecap_mid, tl, st.TS.get(), out.T, st.lq.get() + 4, w);
+                           st.ecap_mid, tl, st.TS.get(), out.T, oh, st.lq.get() + 4, w);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_tri<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
                            st.outdeg.get(), st.b_large.get(), st.n_large, st.kq, (size_t)0, (char*)nullptr,
-                           st.el_large, st.ecap_large, tl, st.TS.get(), out.T, st.lq.get(), w);
+                           st.el_large, st.ecap_large, tl, st.TS.get(), out.T, oh, st.lq.get(), w);
     if (st.n_huge)
         hipLaunchKernelGGL((k_local_tri<1024, true>), dim3(st.ghuge), dim3(1024), 0, s, st.rrow.get(), st.rcol.get(),
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
                            st.huge_stride, st.hscratch.get(), st.el_huge, st.ecap_huge, tl, st.TS.get(), out.T,
-                           st.lq.get() + 2, w);
+                           oh, st.lq.get() + 2, w);
     const int gz = grid_cap(ctx, (2 * m + 255) / 256, 16);
     hipLaunchKernelGGL(k_fold_u32, dim3(gz), dim3(256), 0, s, st.TS.get(), st.reid.get(), 2 * m, out.T);
     PGD_HIP(hipGetLastError());
@@ -1347,6 +1477,7 @@ const char* tuning_problem(const pgd_tuning& t) {
     if (t.wedge_nt != 256 && t.wedge_nt != 512 && t.wedge_nt != 1024) return "wedge_nt must be 256, 512 or 1024";
     if (t.wedge_split < 1 || t.wedge_split > 64) return "wedge_split must be in 1..64";
     if (t.tri_lists < 0 || t.tri_lists > 1024) return "tri_lists must be in 0..1024";
+    if (t.tri_search < 0 || t.tri_search > 1024) return "tri_search must be in 0..1024";
     return "";
 }
 

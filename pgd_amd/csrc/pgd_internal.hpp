@@ -129,6 +129,7 @@ struct OrientParams {
     int wedge_nt = 512;     // threads per block of the heavy 4-cycle pass (256, 512 or 1024)
     int wedge_split = 4;    // work items per block the heavy wedges are cut into (load balance)
     int tri_lists = 64;     // kept local-edge lists, entries per edge (0: off)
+    int tri_search = 4;     // look members up in N+(b)'s table when this x their number < |N+(b)| (0: off)
 };
 OrientState* orient_state_new();
 OrientParams& orient_params(OrientState* s);
@@ -147,11 +148,12 @@ inline OrientParams params_from(const pgd_tuning& t) {
     p.wedge_nt = t.wedge_nt;
     p.wedge_split = t.wedge_split;
     p.tri_lists = t.tri_lists;
+    p.tri_search = t.tri_search;
     return p;
 }
 inline pgd_tuning tuning_from(const OrientParams& p) {
     return pgd_tuning{p.small_k,     p.mid_k,    p.grid_small,  p.grid_mid,
-                      p.wedge_log2r, p.wedge_nt, p.wedge_split, p.tri_lists};
+                      p.wedge_log2r, p.wedge_nt, p.wedge_split, p.tri_lists, p.tri_search};
 }
 // empty string when t is usable, else what is wrong
 const char* tuning_problem(const pgd_tuning& t);

@@ -258,6 +258,9 @@ struct OrientState {
     DBuf<uint32_t> TS;  // per-slot triangle counts of the triangle pass
     DBuf<unsigned long long> work;
     DBuf<uint32_t> lq;  // work queues of the mid, large and HBM bins of the two local passes
+    DBuf<uint64_t> ohtab;  // out-list hash tables of the vertices of large out-degree
+    DBuf<int64_t> ohoff;   // per vertex rank: (first entry << 6) | log2 size, or -1
+    bool has_oh = false;
     WedgePlan wp;
 };
 
