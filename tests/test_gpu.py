@@ -147,6 +147,8 @@ TUNINGS = [
     dict(wedge_log2r=12, wedge_nt=512, grid_small=4, grid_mid=1, wedge_split=1),
     dict(tri_lists=0),                                       # clique pass searches again
     dict(tri_lists=1),                                       # lists kept until the buffer is full
+    dict(tri_search=0),                                      # every out-list probed
+    dict(tri_search=1),                                      # table lookups whenever fewer than the list
 ]
 
 
@@ -280,6 +282,7 @@ def test_tuning_rejects_bad_values(ctx):
     s, d = pgd_amd.gen_gnp(100, 0.1, 1)
     g = pgd_amd.GraphletCore(s, d, ctx=ctx)
     for bad in (dict(wedge_log2r=17), dict(wedge_nt=128), dict(small_k=65), dict(mid_k=512), dict(wedge_split=0),
+                dict(tri_search=-1),
                 dict(tri_lists=-1)):
         with pytest.raises(pgd_amd.PGDError):
             g.set_tuning(**bad)
