@@ -153,7 +153,7 @@ pgd_status pgd_last_phase(pgd_graph *g, int32_t idx, char *name, int32_t cap, do
  * them, only the time does. */
 typedef struct pgd_tuning {
     int32_t small_k;     /* out-degree up to which one wavefront owns a local graph (64) */
-    int32_t mid_k;       /* out-degree up to which a 256-thread block does, 1024 threads above (192) */
+    int32_t mid_k;       /* out-degree up to which a 256-thread block does, 1024 threads above (256) */
     int32_t grid_small;  /* wavefront blocks per CU launched for the local passes (64) */
     int32_t grid_mid;    /* 256-thread blocks per CU launched (32) */
     int32_t wedge_log2r; /* endpoint tile of the heavy 4-cycle pass, 2^r 16-bit LDS counters, r in 8..16 (15) */

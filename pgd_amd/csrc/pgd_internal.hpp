@@ -122,7 +122,7 @@ struct Timer;
 // tunables of the oriented engine (bin thresholds and per-CU grid factors)
 struct OrientParams {
     int small_k = 64;       // out-degree up to which one wavefront owns a local graph
-    int mid_k = 192;        // up to which a 256-thread block does (1024 threads above)
+    int mid_k = 256;        // up to which a 256-thread block does (1024 threads above)
     int grid_small = 64;    // resident blocks per CU for the wavefront bins
     int grid_mid = 32;      // resident blocks per CU for the 256-thread bin
     int wedge_log2r = 15;   // endpoint tile of the heavy 4-cycle pass: 2^log2r 16-bit LDS counters

@@ -136,7 +136,7 @@ def test_generators_deterministic():
 def test_tuning_defaults():
     """pgd_tuning_default (host only) returns the documented defaults."""
     t = pgd_amd.default_tuning().as_dict()
-    assert t == {"small_k": 64, "mid_k": 192, "grid_small": 64, "grid_mid": 32, "wedge_log2r": 15,
+    assert t == {"small_k": 64, "mid_k": 256, "grid_small": 64, "grid_mid": 32, "wedge_log2r": 15,
                  "wedge_nt": 512, "wedge_split": 4, "tri_lists": 64, "tri_search": 4}
     with pytest.raises(KeyError):
         pgd_amd.default_tuning(no_such_field=1)
