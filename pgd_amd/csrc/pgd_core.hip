@@ -51,6 +51,22 @@ void dfree(pgd_ctx* ctx, void* p, size_t bytes) {
     ctx->mem_used -= (int64_t)bytes;
 }
 
+unsigned long long* pinned_words(pgd_ctx* ctx) {
+    if (!ctx->pin) {
+        void* h = nullptr;
+        PGD_HIP(hipHostMalloc(&h, 8 * sizeof(unsigned long long), hipHostMallocMapped));
+        void* d = nullptr;
+        if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipHostFree(h);
+            throw Error{PGD_ERR_HIP, "hipHostGetDevicePointer failed"};
+        }
+        ctx->pin = (unsigned long long*)h;
+        ctx->pin_dev = (unsigned long long*)d;
+    }
+    return ctx->pin_dev;
+}
+
 void* tmp_storage(pgd_ctx* ctx, size_t bytes) {
     if (bytes > ctx->tmp_bytes) {
         if (ctx->tmp) dfree(ctx, ctx->tmp, ctx->tmp_bytes);
@@ -551,6 +567,7 @@ pgd_status pgd_close(pgd_ctx* ctx) {
     if (!ctx) return PGD_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
     if (ctx->tmp) dfree(ctx, ctx->tmp, ctx->tmp_bytes);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->stream) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);

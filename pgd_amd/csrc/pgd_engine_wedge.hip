@@ -82,8 +82,9 @@ __global__ void k_tt_flags(const int64_t* __restrict__ row, const int32_t* __res
 
 __global__ void k_tt_fill(const int64_t* __restrict__ row, const int32_t* __restrict__ col, int64_t n, int64_t nnz,
                           int log2r, const int32_t* __restrict__ flag, const int64_t* __restrict__ pos,
-                          int64_t total, uint64_t* __restrict__ tt, int64_t* __restrict__ ttrow) {
+                          uint64_t* __restrict__ tt, int64_t* __restrict__ ttrow) {
     const int lane = threadIdx.x & 63;
+    const int64_t total = nnz ? pos[nnz - 1] + flag[nnz - 1] : 0;  // entries
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t b = wave; b < n; b += nw) {
@@ -202,8 +203,10 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
                                                    const int32_t* __restrict__ wcnt, const uint64_t* __restrict__ tt,
                                                    const int64_t* __restrict__ ttrow, const int32_t* __restrict__ hbase,
                                                    const int64_t* __restrict__ htoff, uint32_t* __restrict__ htseg,
-                                                   uint64_t* __restrict__ htw) {
+                                                   uint64_t* __restrict__ htw, unsigned long long* __restrict__ acc,
+                                                   unsigned long long* hout) {
     constexpr int NW = NT / 64;
+    unsigned long long mseg = 0, mw = 0;  // this thread's segments and wedges
     __shared__ uint32_t lseg[kPlanLds];
     __shared__ unsigned long long lw[kPlanLds];
     __shared__ int64_t pj[NW][64], pe[NW][64];
@@ -234,6 +237,8 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
             walk_entries(jb, je, tend, deg, w, pj[wv], pe[wv], pw[wv], pd[wv], flag[wv], tt, lane,
                          [&](int, uint32_t tile, uint32_t, uint32_t len) {
                              const uint32_t t = tile - base;
+                             mseg += 1;
+                             mw += len;
                              if (in_lds) {
                                  atomicAdd(&lseg[t], 1u);
                                  atomicAdd(&lw[t], (unsigned long long)len);
@@ -247,6 +252,19 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
         if (in_lds)
             for (int i = threadIdx.x; i < T; i += NT) { htseg[c0 + i] = lseg[i]; htw[c0 + i] = lw[i]; }
         __syncthreads();
+    }
+    // the totals of all cells (acc[0], acc[1]; acc[2] counts finished
+    // blocks): the last block to finish copies them to host memory, so the
+    // host reads them after a stream synchronisation without a copy
+    if (mseg) atomicAdd(&acc[0], mseg);
+    if (mw) atomicAdd(&acc[1], mw);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(&acc[2], 1ull) == gridDim.x - 1) {
+        __threadfence();
+        hout[0] = atomicAdd(&acc[0], 0ull);
+        hout[1] = atomicAdd(&acc[1], 0ull);
+        __threadfence_system();
     }
 }
 
@@ -801,16 +819,11 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         hipLaunchKernelGGL(k_tt_flags, dim3(gw), dim3(256), 0, s, st.rrow.get(), st.rcol.get(), n, wp.log2r, flag);
         PGD_HIP(hipGetLastError());
         excl_sum(ctx, flag, pos, (size_t)nnz);
-        int64_t last_pos = 0;
-        int32_t last_flag = 0;
-        PGD_HIP(hipMemcpyAsync(&last_pos, pos + (nnz - 1), sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        PGD_HIP(hipMemcpyAsync(&last_flag, flag + (nnz - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        PGD_HIP(hipStreamSynchronize(s));
-        const int64_t total = last_pos + last_flag;
-        ensure(wp.tt, ctx, total);
+        // at most one entry per slot: sized without reading the total back
+        ensure(wp.tt, ctx, nnz);
         ensure(wp.ttrow, ctx, n + 1);
         hipLaunchKernelGGL(k_tt_fill, dim3(gw), dim3(256), 0, s, st.rrow.get(), st.rcol.get(), n, nnz, wp.log2r,
-                           flag, pos, total, wp.tt.get(), wp.ttrow.get());
+                           flag, pos, wp.tt.get(), wp.ttrow.get());
         PGD_HIP(hipGetLastError());
     }
     // (hub, tile) cells
@@ -833,14 +846,21 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         const int g = grid_cap(ctx, (ncell + 255) / 256, 16);
         hipLaunchKernelGGL(w_fill_u32, dim3(g), dim3(256), 0, s, wp.htseg.get(), ncell);
         hipLaunchKernelGGL(w_fill_u64, dim3(g), dim3(256), 0, s, wp.htw.get(), ncell);
+        ensure(wp.pacc, ctx, 3);
+        hipLaunchKernelGGL(w_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)wp.pacc.get(), (int64_t)3);
         hipLaunchKernelGGL(k_plan_cells<256>, dim3(grid_cap(ctx, nh, 8)), dim3(256), 0, s, st.rrow.get(),
                            st.rcol.get(), st.ostart.get(), st.b_heavy.get(), nh, wp.wcnt.get(), wp.tt.get(),
-                           wp.ttrow.get(), wp.hbase.get(), wp.htoff.get(), wp.htseg.get(), wp.htw.get());
+                           wp.ttrow.get(), wp.hbase.get(), wp.htoff.get(), wp.htseg.get(), wp.htw.get(),
+                           wp.pacc.get(), pinned_words(ctx));
         PGD_HIP(hipGetLastError());
     }
-    const uint32_t maxcell = device_max(ctx, wp.htseg.get(), (size_t)ncell);
-    const uint64_t segtot = device_sum<uint64_t>(ctx, wp.htseg.get(), (size_t)ncell);
-    wp.wedges = (int64_t)device_sum<uint64_t>(ctx, wp.htw.get(), (size_t)ncell);
+    PGD_HIP(hipStreamSynchronize(s));
+    const uint64_t segtot = ((volatile unsigned long long*)ctx->pin)[0];
+    wp.wedges = (int64_t)((volatile unsigned long long*)ctx->pin)[1];
+    // a cell holds at most one segment per in-neighbour of its hub: the
+    // maximum degree bounds it (read from the cells when that bound is large)
+    const uint32_t maxcell = st.max_deg <= (1 << 20) ? (uint32_t)st.max_deg
+                                                      : device_max(ctx, wp.htseg.get(), (size_t)ncell);
     // grid from the kernel's occupancy at this tile size
     int per_cu = 0;
     if (prm.wedge_nt == 256) per_cu = tiled_blocks_per_cu<256>(ctx, wp.log2r);

@@ -49,6 +49,10 @@ struct pgd_ctx {
     // rocPRIM temporary storage, grown on demand
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
+    // 8 words of pinned host memory that kernels write and the host reads
+    // after a stream synchronisation (made on first use)
+    unsigned long long* pin = nullptr;
+    unsigned long long* pin_dev = nullptr;
 };
 
 namespace pgd {
@@ -56,6 +60,7 @@ namespace pgd {
 void* dmalloc(pgd_ctx* ctx, size_t bytes);
 void dfree(pgd_ctx* ctx, void* p, size_t bytes);
 void* tmp_storage(pgd_ctx* ctx, size_t bytes);
+unsigned long long* pinned_words(pgd_ctx* ctx);  // device address of ctx->pin
 
 // RAII device buffer charged against the context's cap
 template <typename T>

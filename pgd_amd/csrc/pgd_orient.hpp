@@ -227,6 +227,7 @@ struct WedgePlan {
     DBuf<uint32_t> ps32;            // per-slot P accumulator of the lists that are not wide
     int32_t wide_deg = 0;           // lists of at least this degree are wide
     DBuf<uint32_t> qctr;            // queue head
+    DBuf<unsigned long long> pacc;  // totals of the plan's cells: segments, wedges, finished blocks
 };
 
 // Persistent state of the oriented engine for one graph: the relabelled CSR,
