@@ -576,6 +576,10 @@ pgd_status pgd_close(pgd_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->aux);
         (void)hipStreamDestroy(ctx->aux);
     }
+    if (ctx->side) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamDestroy(ctx->side);
+    }
     delete ctx;
     return PGD_OK;
 }

@@ -1275,15 +1275,33 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st, bool plan) 
 // capturable in a HIP graph).  With a timer, each pass is bracketed by events.
 // Local passes (triangles, 4-cliques, Q), then the 4-cycle passes, which
 // need the plan.
-static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer);
+static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer, hipStream_t side);
 static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer);
 
 void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
-    orient_count_local(ctx, st, out, timer);
+    orient_count_local(ctx, st, out, timer, nullptr);
     orient_count_wedges(ctx, st, out, timer);
 }
 
-static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
+// fork the side stream off the context's stream at this point (event i),
+// and join it back (event i): no-ops without a side stream
+static void fork_begin(pgd_ctx* ctx, OrientState& st, hipStream_t side, int i) {
+    if (!side) return;
+    if (!st.fork[i]) PGD_HIP(hipEventCreateWithFlags(&st.fork[i], hipEventDisableTiming));
+    PGD_HIP(hipEventRecord(st.fork[i], ctx->stream));
+    PGD_HIP(hipStreamWaitEvent(side, st.fork[i], 0));
+}
+static void fork_end(pgd_ctx* ctx, OrientState& st, hipStream_t side, int i) {
+    if (!side) return;
+    if (!st.fork[i]) PGD_HIP(hipEventCreateWithFlags(&st.fork[i], hipEventDisableTiming));
+    PGD_HIP(hipEventRecord(st.fork[i], side));
+    PGD_HIP(hipStreamWaitEvent(ctx->stream, st.fork[i], 0));
+}
+
+// side: a second stream for the wavefront and 256-thread bins, which then
+// run beside the 1024-thread bins (those hold one block per CU and leave
+// wave slots free); nullptr runs every bin on the context's stream
+static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer, hipStream_t side) {
     hipStream_t s = ctx->stream;
     const int64_t m = st.m, n = st.n;
     const OrientParams& prm = st.prm;
@@ -1309,16 +1327,21 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
                        2 * m);
     const OutHash oh{st.ohtab.get(), st.ohoff.get(), st.has_oh ? prm.tri_search : 0};
     if (timer) timer->begin("k_local_tri");
-    if (st.n_small)
-        hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
-                           st.small_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (size_t)0, (char*)nullptr,
-                           st.el_small, st.ecap_small, tl, st.TS.get(), out.T, oh, (uint32_t*)nullptr, w);
-    if (st.n_mid)
-        hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
-                           st.mid_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (size_t)0, (char*)nullptr, st.el_mid,
-                           st.ecap_mid, tl, st.TS.get(), out.T, oh, st.lq.get() + 4, w);
+    auto tri_small_mid = [&](hipStream_t q) {
+        if (st.n_small)
+            hipLaunchKernelGGL((k_local_tri<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
+                               st.small_lds_tri, q, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                               st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, (size_t)0,
+                               (char*)nullptr, st.el_small, st.ecap_small, tl, st.TS.get(), out.T, oh,
+                               (uint32_t*)nullptr, w);
+        if (st.n_mid)
+            hipLaunchKernelGGL((k_local_tri<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
+                               st.mid_lds_tri, q, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                               st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, (size_t)0, (char*)nullptr,
+                               st.el_mid, st.ecap_mid, tl, st.TS.get(), out.T, oh, st.lq.get() + 4, w);
+    };
+    fork_begin(ctx, st, side, 0);
+    if (!side) tri_small_mid(s);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_tri<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds_tri, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
@@ -1329,6 +1352,8 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
                            st.huge_stride, st.hscratch.get(), st.el_huge, st.ecap_huge, tl, st.TS.get(), out.T,
                            oh, st.lq.get() + 2, w);
+    if (side) tri_small_mid(side);
+    fork_end(ctx, st, side, 1);
     const int gz = grid_cap(ctx, (2 * m + 255) / 256, 16);
     hipLaunchKernelGGL(k_fold_u32, dim3(gz), dim3(256), 0, s, st.TS.get(), st.reid.get(), 2 * m, out.T);
     PGD_HIP(hipGetLastError());
@@ -1344,16 +1369,22 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
     hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, K4S, 4 * m);
     hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, PK, 2 * m);
     PGD_HIP(hipGetLastError());
-    if (st.n_small)
-        hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
-                           st.small_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, st.mw_small, (char*)nullptr,
-                           st.el_small, st.ecap_small, tl, TSL, K4S, QS, PK, pk, (uint32_t*)nullptr, w + 1);
-    if (st.n_mid)
-        hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
-                           st.mid_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
-                           st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, st.mw_mid, (char*)nullptr, st.el_mid,
-                           st.ecap_mid, tl, TSL, K4S, QS, PK, pk, st.lq.get() + 5, w + 1);
+    auto k4q_small_mid = [&](hipStream_t q) {
+        if (st.n_small)
+            hipLaunchKernelGGL((k_local_k4q<64, false>), dim3(grid_cap(ctx, st.n_small, prm.grid_small)), dim3(64),
+                               st.small_lds, q, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                               st.outdeg.get(), st.b_small.get(), st.n_small, prm.small_k, st.mw_small,
+                               (char*)nullptr, st.el_small, st.ecap_small, tl, TSL, K4S, QS, PK, pk,
+                               (uint32_t*)nullptr, w + 1);
+        if (st.n_mid)
+            hipLaunchKernelGGL((k_local_k4q<256, false>), dim3(grid_cap(ctx, st.n_mid, prm.grid_mid)), dim3(256),
+                               st.mid_lds, q, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
+                               st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, st.mw_mid, (char*)nullptr,
+                               st.el_mid, st.ecap_mid, tl, TSL, K4S, QS, PK, pk, st.lq.get() + 5, w + 1);
+    };
+    // (the clique pass's bins run one after the other: beside each other
+    // they contend for the per-slot atomics and take longer)
+    k4q_small_mid(s);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
@@ -1364,6 +1395,7 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
                            st.mw_huge, st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, TSL, K4S, QS, PK, pk,
                            st.lq.get() + 3, w + 1);
+
     hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, s, K4S, QS, PK, pk.qb, st.reid.get(), 2 * m, out.K4,
                        st.Q.get());
     PGD_HIP(hipGetLastError());
@@ -1442,7 +1474,8 @@ void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm,
         PGD_HIP(hipEventCreateWithFlags(&planned, hipEventDisableTiming));
         PGD_HIP(hipEventRecord(built, ctx->stream));
     }
-    orient_count_local(ctx, *st, out, &timer);
+    if (!ctx->side) PGD_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    orient_count_local(ctx, *st, out, &timer, ctx->side);
     if (st->n_heavy) {
         hipStream_t main = ctx->stream;
         PGD_HIP(hipStreamWaitEvent(ctx->aux, built, 0));

@@ -40,6 +40,7 @@ struct pgd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;  // second stream of the plain oriented engine (its 4-cycle plan), made on first use
+    hipStream_t side = nullptr;  // third: the small local-graph bins beside the large ones, made on first use
     int num_cus = 256;
     size_t lds_per_block = 65536;  // hipDeviceProp_t::sharedMemPerBlock, read once at open
     int tiled_per_cu[3][9] = {};   // occupancy of the tiled wedge kernel by (block size, tile), 0 = not asked yet
