@@ -1382,9 +1382,8 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
                                st.outdeg.get(), st.b_mid.get(), st.n_mid, prm.mid_k, st.mw_mid, (char*)nullptr,
                                st.el_mid, st.ecap_mid, tl, TSL, K4S, QS, PK, pk, st.lq.get() + 5, w + 1);
     };
-    // (the clique pass's bins run one after the other: beside each other
-    // they contend for the per-slot atomics and take longer)
-    k4q_small_mid(s);
+    fork_begin(ctx, st, side, 2);
+    if (!side) k4q_small_mid(s);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
                            st.large_lds, s, st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(),
@@ -1395,7 +1394,8 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
                            st.reid.get(), st.ostart.get(), st.outdeg.get(), st.b_huge.get(), st.n_huge, st.maxk,
                            st.mw_huge, st.hscratch.get(), (uint64_t*)nullptr, (int64_t)0, tl, TSL, K4S, QS, PK, pk,
                            st.lq.get() + 3, w + 1);
-
+    if (side) k4q_small_mid(side);
+    fork_end(ctx, st, side, 3);
     hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, s, K4S, QS, PK, pk.qb, st.reid.get(), 2 * m, out.K4,
                        st.Q.get());
     PGD_HIP(hipGetLastError());
