@@ -307,17 +307,27 @@ __global__ void k_plan_items(const int64_t* __restrict__ htoff, const int32_t* _
     }
 }
 
-__global__ void k_gather_items(const uint64_t* __restrict__ order, const WedgeItem* __restrict__ in, int64_t n,
-                               WedgeItem* __restrict__ out) {
-    GRID_STRIDE(i, n) out[i] = in[(uint32_t)order[i]];
+// item order by wedge count, largest first to a factor of two: bucket
+// 63 - floor(log2 w) (key = ~w, w >= 1); cnt[64] counts, then cnt[64 + d]
+// is bucket d's next position
+__device__ __forceinline__ int item_bucket(uint64_t key) { return __clzll((long long)~key); }
+__global__ void k_item_buckets(const uint64_t* __restrict__ key, int64_t n, uint32_t* __restrict__ cnt) {
+    GRID_STRIDE(i, n) atomicAdd(&cnt[item_bucket(key[i])], 1u);
 }
-
-__global__ void k_item_keys(const uint64_t* __restrict__ key, int64_t n, uint64_t* __restrict__ k2,
-                            uint64_t* __restrict__ v) {
-    GRID_STRIDE(i, n) {
-        k2[i] = key[i];
-        v[i] = (uint64_t)i;
+__global__ void k_bucket_starts(uint32_t* __restrict__ cnt) {  // one wavefront
+    const int lane = threadIdx.x & 63;
+    const uint32_t c = cnt[lane];
+    uint32_t x = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
     }
+    cnt[64 + lane] = x - c;
+}
+__global__ void k_item_scatter(const uint64_t* __restrict__ key, const WedgeItem* __restrict__ in, int64_t n,
+                               uint32_t* __restrict__ next, WedgeItem* __restrict__ out) {
+    GRID_STRIDE(i, n) out[atomicAdd(&next[item_bucket(key[i])], 1u)] = in[i];
 }
 
 // add v to the per-slot accumulator of the slot: 32-bit unless its list
@@ -891,24 +901,27 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         PGD_HIP(hipStreamSynchronize(s));
         const int64_t ni = wp.nitems;
         DBuf<WedgeItem> raw;
-        DBuf<uint64_t> key, k2, v0, v1;
+        DBuf<uint64_t> key;
+        DBuf<uint32_t> bk;
         raw.alloc(ctx, ni);
         key.alloc(ctx, ni);
-        k2.alloc(ctx, ni);
-        v0.alloc(ctx, ni);
-        v1.alloc(ctx, ni);
+        bk.alloc(ctx, 128);
         hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, s, wp.htoff.get(), wp.hbase.get(), wp.htseg.get(),
                            wp.htw.get(), nh, wp.segcap, wcap, per.get(), ioff.get(), raw.get(), key.get());
         PGD_HIP(hipGetLastError());
         ensure(wp.items, ctx, ni);
         if (ni) {
+            // largest first, by powers of two of the wedges: a counting sort
+            // of 64 buckets with global atomics (no LDS: these kernels find
+            // CU slots beside the local passes)
             const int gi = grid_cap(ctx, (ni + 255) / 256, 16);
-            hipLaunchKernelGGL(k_item_keys, dim3(gi), dim3(256), 0, s, key.get(), ni, k2.get(), v0.get());
-            sort_kv(ctx, k2.get(), key.get(), v0.get(), v1.get(), (size_t)ni, 64u);
-            hipLaunchKernelGGL(k_gather_items, dim3(gi), dim3(256), 0, s, v1.get(), raw.get(), ni, wp.items.get());
+            hipLaunchKernelGGL(w_fill_u32, dim3(1), dim3(64), 0, s, bk.get(), (int64_t)128);
+            hipLaunchKernelGGL(k_item_buckets, dim3(gi), dim3(256), 0, s, key.get(), ni, bk.get());
+            hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(64), 0, s, bk.get());
+            hipLaunchKernelGGL(k_item_scatter, dim3(gi), dim3(256), 0, s, key.get(), raw.get(), ni, bk.get() + 64,
+                               wp.items.get());
             PGD_HIP(hipGetLastError());
         }
-        PGD_HIP(hipStreamSynchronize(s));
     }
     ensure(wp.segs, ctx, (size_t)wp.grid * (size_t)wp.segcap);
     ensure(wp.ps, ctx, nnz);
