@@ -243,16 +243,42 @@ __global__ void k_vertex_feature(const int64_t* __restrict__ row, const int32_t*
 // composite key (owner, key) for the stable per-vertex sort; the large->small
 // order reverses the input first so equal keys end in reverse input order
 // (bucket_sort_neighbors_parallel, graphlet_core.cpp:1846-1877)
+// The feature enters the key as its offset from the smallest (small->large)
+// or from the largest (large->small) feature, in fb bits, so the sort runs
+// over fb + log2 n bits only.
 __global__ void k_neigh_keys(const int32_t* __restrict__ src, const int32_t* __restrict__ col,
-                             const int32_t* __restrict__ feat, int64_t nnz, int s2l,
-                             uint64_t* __restrict__ key, int32_t* __restrict__ val) {
+                             const int32_t* __restrict__ feat, int64_t nnz, int s2l, const int32_t* __restrict__ mm,
+                             int fb, uint64_t* __restrict__ key, int32_t* __restrict__ val) {
+    const int64_t fmin = mm[0], fmax = mm[1];
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
         int64_t p = s2l ? i : nnz - 1 - i;
         int32_t x = col[p];
-        uint32_t k = (uint32_t)feat[x] ^ 0x80000000u;
-        if (!s2l) k = ~k;
-        key[i] = ((uint64_t)(uint32_t)src[p] << 32) | k;
+        const uint64_t k = s2l ? (uint64_t)((int64_t)feat[x] - fmin) : (uint64_t)(fmax - (int64_t)feat[x]);
+        key[i] = ((uint64_t)(uint32_t)src[p] << fb) | k;
         val[i] = x;
+    }
+}
+
+// smallest and largest feature (mm[0], mm[1]; mm set to INT32_MAX, INT32_MIN)
+__global__ void k_feat_minmax(const int32_t* __restrict__ feat, int64_t n, int32_t* __restrict__ mm) {
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        lo = min(lo, feat[x]);
+        hi = max(hi, feat[x]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off, 64));
+        hi = max(hi, __shfl_xor(hi, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&mm[0], lo);
+        atomicMax(&mm[1], hi);
+    }
+}
+__global__ void k_minmax_init(int32_t* mm) {
+    if (threadIdx.x == 0) {
+        mm[0] = INT32_MAX;
+        mm[1] = INT32_MIN;
     }
 }
 
@@ -807,12 +833,21 @@ pgd_status pgd_preprocess(pgd_graph* g, const char* method, int32_t s2l) {
             hipLaunchKernelGGL(k_vertex_feature, dim3(grid), dim3(kBlock), 0, s, g->row.get(), g->col.get(),
                                g->deg.get(), g->have_core ? g->core.get() : nullptr, n, kind, feat.get());
             PGD_HIP(hipGetLastError());
+            DBuf<int32_t> mm;
+            mm.alloc(ctx, 2);
+            hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, s, mm.get());
+            hipLaunchKernelGGL(k_feat_minmax, dim3(grid), dim3(kBlock), 0, s, feat.get(), n, mm.get());
+            PGD_HIP(hipGetLastError());
+            int32_t hmm[2] = {0, 0};
+            PGD_HIP(hipMemcpyAsync(hmm, mm.get(), sizeof(hmm), hipMemcpyDeviceToHost, s));
+            PGD_HIP(hipStreamSynchronize(s));
+            const int fb = bits_for((int64_t)hmm[1] - (int64_t)hmm[0]);
             k0.alloc(ctx, nnz); k1.alloc(ctx, nnz); v0.alloc(ctx, nnz);
             hipLaunchKernelGGL(k_neigh_keys, dim3(grid), dim3(kBlock), 0, s, g->src.get(), g->col.get(), feat.get(),
-                               nnz, s2l ? 1 : 0, k0.get(), v0.get());
+                               nnz, s2l ? 1 : 0, mm.get(), fb, k0.get(), v0.get());
             PGD_HIP(hipGetLastError());
             sort_pairs(ctx, k0.get(), k1.get(), v0.get(), g->col.get(), (size_t)nnz, 0,
-                       32u + (unsigned)bits_for(n - 1));
+                       (unsigned)(fb + bits_for(n - 1)));
         }
         // orientation and edge ids
         g->ev.alloc(ctx, std::max<int64_t>(g->m, 1));
