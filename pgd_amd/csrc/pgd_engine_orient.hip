@@ -320,8 +320,10 @@ struct TriLists {
 
 // ---------------------------------------------------------------------------
 // pass 1: triangles.  Local edge (i,j), i<j <=> triangle {a, L[i], L[j]}.
-// The edge (L[i], L[j]) is counted in TS at its out-slot (folded into T by
-// edge id afterwards); the edges (a, L[i]) through the block's counters.
+// Every edge is counted in TS at its out-slot (the slot in the list of its
+// lower-ranked end): the far edge (L[i], L[j]) per hit, the edges (a, L[i])
+// from the block's counters at the end.  TS is then T by out-slot, as the
+// clique pass reads it, and the fold into T by edge id needs no atomics.
 //
 // The out-lists N+(L[i]) are walked in groups of 64 lists: a wavefront takes
 // chunks of 256 consecutive positions of a group's flattened lists (chunks
@@ -512,7 +514,6 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
             for (int i = tid; i < k; i += NT) {
                 const int32_t b = col[s0 + i];
                 A.L[i] = b;
-                A.Le[i] = eid[s0 + i];
                 const int64_t ob = ostart[b];
                 const int64_t od = row[b + 1] - ob;
                 // the later members L[j], j > i, looked up in N+(b)'s table,
@@ -657,8 +658,11 @@ __global__ __launch_bounds__(NT) void k_local_tri(const int64_t* __restrict__ ro
             __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
-        for (int i = tid; i < k; i += NT)
-            if (A.cnt[i]) atomicAdd(&T[A.Le[i]], A.cnt[i]);
+        {
+            const int64_t s0 = ostart[a];  // the out-slots of a
+            for (int i = tid; i < k; i += NT)
+                if (A.cnt[i]) atomicAdd(&TS[s0 + i], A.cnt[i]);
+        }
         if (!kGlobal && el) {
             if (tid == 0) {
                 // written in place, or not kept once G is full (the clique
@@ -1026,10 +1030,6 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
     GRID_STRIDE(i, n) p[i] = v;
 }
 // TSL[s] = T of the edge at slot s
-__global__ void k_slot_gather_u32(const uint32_t* __restrict__ T, const int32_t* __restrict__ eid, int64_t nnz,
-                                  uint32_t* __restrict__ TSL) {
-    GRID_STRIDE(s, nnz) TSL[s] = T[eid[s]];
-}
 // K4[eid[s]] += K4S[s] + PK[s] >> qb, Q[eid[s]] += QS[s] + PK[s] mod 2^qb:
 // the clique pass adds only at out-slots, one per edge, so the adds need no
 // atomics
@@ -1068,11 +1068,12 @@ PackSplit pack_split(int64_t dmax) {
 }
 
 // T[eid[s]] += TS[s] for every slot s with a count
+// T[eid[s]] = TS[s]: only out-slots hold counts, one per edge (T is zeroed)
 __global__ void k_fold_u32(const uint32_t* __restrict__ TS, const int32_t* __restrict__ eid, int64_t nnz,
                            uint32_t* __restrict__ T) {
     GRID_STRIDE(s, nnz) {
         const uint32_t v = TS[s];
-        if (v) atomicAdd(&T[eid[s]], v);
+        if (v) T[eid[s]] = v;
     }
 }
 
@@ -1359,13 +1360,12 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (timer) timer->begin("k_local_k4q");
-    // the clique pass addresses edges by out-slot: T by slot (in TS, free
-    // after the fold) and per-slot K4/Q accumulators (k0/k1, free after the
-    // build)
+    // the clique pass addresses edges by out-slot: T by out-slot (TS, as the
+    // triangle pass left it) and per-slot K4/Q accumulators (k0/k1, free
+    // after the build)
     uint32_t* TSL = st.TS.get();
     uint64_t *K4S = st.k0.get(), *QS = st.k0.get() + 2 * m, *PK = st.k1.get();
     const PackSplit pk = pack_split(st.max_deg);
-    hipLaunchKernelGGL(k_slot_gather_u32, dim3(gz), dim3(256), 0, s, out.T, st.reid.get(), 2 * m, TSL);
     hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, K4S, 4 * m);
     hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, s, PK, 2 * m);
     PGD_HIP(hipGetLastError());
