@@ -1277,7 +1277,8 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st, bool plan) 
 // Local passes (triangles, 4-cliques, Q), then the 4-cycle passes, which
 // need the plan.
 static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer, hipStream_t side);
-static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer);
+static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer,
+                                hipEvent_t folded = nullptr);
 
 void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     orient_count_local(ctx, st, out, timer, nullptr);
@@ -1396,13 +1397,23 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
                            st.lq.get() + 3, w + 1);
     if (side) k4q_small_mid(side);
     fork_end(ctx, st, side, 3);
-    hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, s, K4S, QS, PK, pk.qb, st.reid.get(), 2 * m, out.K4,
-                       st.Q.get());
+    // with a side stream the fold of the per-slot sums runs there, beside
+    // the 4-cycle passes (which use neither its inputs nor its outputs);
+    // fork[5], recorded after it, is the join before k_combine
+    fork_begin(ctx, st, side, 4);
+    hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, side ? side : s, K4S, QS, PK, pk.qb, st.reid.get(), 2 * m,
+                       out.K4, st.Q.get());
     PGD_HIP(hipGetLastError());
+    if (side) {
+        if (!st.fork[5]) PGD_HIP(hipEventCreateWithFlags(&st.fork[5], hipEventDisableTiming));
+        PGD_HIP(hipEventRecord(st.fork[5], side));
+    }
     if (timer) timer->end();
 }
 
-static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
+// folded (optional): recorded where K4 and Q are complete; waited for
+// before k_combine
+static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer, hipEvent_t folded) {
     hipStream_t s = ctx->stream;
     const int64_t m = st.m;
     const OrientParams& prm = st.prm;
@@ -1418,6 +1429,7 @@ static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, T
     if (timer) timer->begin("k_wedge_tiled");
     if (st.n_heavy) wedge_plan_count(ctx, st, w + 3);
     if (timer) timer->end();
+    if (folded) PGD_HIP(hipStreamWaitEvent(s, folded, 0));
     if (timer) timer->begin("k_combine");
     hipLaunchKernelGGL(k_combine, dim3(gm), dim3(256), 0, s, st.P.get(), st.Q.get(), out.T, out.K4, m, out.C4);
     PGD_HIP(hipGetLastError());
@@ -1495,7 +1507,7 @@ void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm,
         (void)hipEventDestroy(built);
         (void)hipEventDestroy(planned);
     }
-    orient_count_wedges(ctx, *st, out, &timer);
+    orient_count_wedges(ctx, *st, out, &timer, st->fork[5]);
     orient_profile(ctx, *st, timer, tm);
 }
 

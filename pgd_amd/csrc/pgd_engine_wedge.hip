@@ -309,10 +309,26 @@ __global__ void k_plan_items(const int64_t* __restrict__ htoff, const int32_t* _
 
 // item order by wedge count, largest first to a factor of two: bucket
 // 63 - floor(log2 w) (key = ~w, w >= 1); cnt[64] counts, then cnt[64 + d]
-// is bucket d's next position
+// is bucket d's next position.  A wavefront adds once per distinct bucket
+// of its 64 items (the items of one hub mostly share one), so the few
+// bucket counters are not contended item by item.
 __device__ __forceinline__ int item_bucket(uint64_t key) { return __clzll((long long)~key); }
 __global__ void k_item_buckets(const uint64_t* __restrict__ key, int64_t n, uint32_t* __restrict__ cnt) {
-    GRID_STRIDE(i, n) atomicAdd(&cnt[item_bucket(key[i])], 1u);
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); i0 < n; i0 += stride) {
+        const int64_t i = i0 + lane;
+        const bool ok = i < n;
+        const int b = ok ? item_bucket(key[i]) : -1;
+        uint64_t pending = __ballot(ok);
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const int lb = __shfl(b, leader, 64);
+            const uint64_t same = __ballot(ok && b == lb);
+            if (lane == leader) atomicAdd(&cnt[lb], (uint32_t)__popcll(same));
+            pending &= ~same;
+        }
+    }
 }
 __global__ void k_bucket_starts(uint32_t* __restrict__ cnt) {  // one wavefront
     const int lane = threadIdx.x & 63;
@@ -327,7 +343,24 @@ __global__ void k_bucket_starts(uint32_t* __restrict__ cnt) {  // one wavefront
 }
 __global__ void k_item_scatter(const uint64_t* __restrict__ key, const WedgeItem* __restrict__ in, int64_t n,
                                uint32_t* __restrict__ next, WedgeItem* __restrict__ out) {
-    GRID_STRIDE(i, n) out[atomicAdd(&next[item_bucket(key[i])], 1u)] = in[i];
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); i0 < n; i0 += stride) {
+        const int64_t i = i0 + lane;
+        const bool ok = i < n;
+        const int b = ok ? item_bucket(key[i]) : -1;
+        uint64_t pending = __ballot(ok);
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const int lb = __shfl(b, leader, 64);
+            const uint64_t same = __ballot(ok && b == lb);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&next[lb], (uint32_t)__popcll(same));
+            base = __shfl(base, leader, 64);
+            if ((same >> lane) & 1) out[base + __popcll(same & ((1ull << lane) - 1))] = in[i];
+            pending &= ~same;
+        }
+    }
 }
 
 // add v to the per-slot accumulator of the slot: 32-bit unless its list

@@ -262,7 +262,7 @@ struct OrientState {
     DBuf<uint64_t> ohtab;  // out-list hash tables of the vertices of large out-degree
     DBuf<int64_t> ohoff;   // per vertex rank: (first entry << 6) | log2 size, or -1
     bool has_oh = false;
-    hipEvent_t fork[4] = {};  // fork and join of the side stream of the local passes
+    hipEvent_t fork[6] = {};  // fork and join of the side stream of the local passes
     WedgePlan wp;
     ~OrientState() {
         for (hipEvent_t e : fork)
