@@ -1357,8 +1357,6 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
     if (side) tri_small_mid(side);
     fork_end(ctx, st, side, 1);
     const int gz = grid_cap(ctx, (2 * m + 255) / 256, 16);
-    hipLaunchKernelGGL(k_fold_u32, dim3(gz), dim3(256), 0, s, st.TS.get(), st.reid.get(), 2 * m, out.T);
-    PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (timer) timer->begin("k_local_k4q");
     // the clique pass addresses edges by out-slot: T by out-slot (TS, as the
@@ -1384,6 +1382,11 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
                                st.el_mid, st.ecap_mid, tl, TSL, K4S, QS, PK, pk, st.lq.get() + 5, w + 1);
     };
     fork_begin(ctx, st, side, 2);
+    // T by edge id, from TS (the clique pass reads TS only): on the side
+    // stream, ahead of its clique bins
+    hipLaunchKernelGGL(k_fold_u32, dim3(gz), dim3(256), 0, side ? side : s, st.TS.get(), st.reid.get(), 2 * m,
+                       out.T);
+    PGD_HIP(hipGetLastError());
     if (!side) k4q_small_mid(s);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
