@@ -1278,7 +1278,7 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st, bool plan) 
 // need the plan.
 static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer, hipStream_t side);
 static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer,
-                                hipEvent_t folded = nullptr);
+                                hipEvent_t folded = nullptr, hipEvent_t planned = nullptr);
 
 void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer) {
     orient_count_local(ctx, st, out, timer, nullptr);
@@ -1415,8 +1415,10 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
 }
 
 // folded (optional): recorded where K4 and Q are complete; waited for
-// before k_combine
-static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer, hipEvent_t folded) {
+// before k_combine.  planned (optional): recorded where the 4-cycle plan is
+// complete; waited for after the light wedges, before the tiled pass.
+static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer, hipEvent_t folded,
+                                hipEvent_t planned) {
     hipStream_t s = ctx->stream;
     const int64_t m = st.m;
     const OrientParams& prm = st.prm;
@@ -1429,6 +1431,7 @@ static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, T
                            st.P.get(), w + 2);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
+    if (planned) PGD_HIP(hipStreamWaitEvent(s, planned, 0));
     if (timer) timer->begin("k_wedge_tiled");
     if (st.n_heavy) wedge_plan_count(ctx, st, w + 3);
     if (timer) timer->end();
@@ -1505,12 +1508,22 @@ void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm,
             throw;
         }
         ctx->stream = main;
-        PGD_HIP(hipEventRecord(planned, ctx->aux));
-        PGD_HIP(hipStreamWaitEvent(ctx->stream, planned, 0));
         (void)hipEventDestroy(built);
-        (void)hipEventDestroy(planned);
+        try {
+            PGD_HIP(hipEventRecord(planned, ctx->aux));
+        } catch (...) {
+            (void)hipEventDestroy(planned);
+            throw;
+        }
     }
-    orient_count_wedges(ctx, *st, out, &timer, st->fork[5]);
+    // the light wedges run before the wait for the plan
+    try {
+        orient_count_wedges(ctx, *st, out, &timer, st->fork[5], planned);
+    } catch (...) {
+        if (planned) (void)hipEventDestroy(planned);
+        throw;
+    }
+    if (planned) (void)hipEventDestroy(planned);
     orient_profile(ctx, *st, timer, tm);
 }
 

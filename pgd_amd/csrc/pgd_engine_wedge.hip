@@ -799,6 +799,66 @@ __global__ void k_ps_fold(const uint64_t* __restrict__ PS, const uint32_t* __res
     }
 }
 
+// Exclusive prefix sums of n int64 values without LDS, for the plan's scans
+// that run beside the local passes (whose blocks hold the LDS, so an
+// LDS-using scan waits for them): one wavefront per chunk of kScanChunk
+// values sums it, one wavefront scans the chunk sums, and one wavefront per
+// chunk rescans it from its base.
+constexpr int kScanChunk = 4096;
+__device__ __forceinline__ long long wave_incl_sum64(long long x, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+__global__ void k_scan_parts(const int64_t* __restrict__ in, int64_t n, int64_t* __restrict__ part) {
+    const int lane = threadIdx.x;  // one wavefront per block, block b = chunk b
+    const int64_t c0 = (int64_t)blockIdx.x * kScanChunk, c1 = min(n, c0 + kScanChunk);
+    long long x = 0;
+    for (int64_t i = c0 + lane; i < c1; i += 64) x += in[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane == 0) part[blockIdx.x] = x;
+}
+__global__ void k_scan_tops(int64_t* __restrict__ part, int64_t np) {  // one wavefront, in place
+    const int lane = threadIdx.x;
+    long long carry = 0;
+    for (int64_t b = 0; b < np; b += 64) {
+        const int64_t i = b + lane;
+        const long long v = i < np ? part[i] : 0;
+        const long long x = wave_incl_sum64(v, lane);
+        if (i < np) part[i] = carry + x - v;
+        carry += __shfl(x, 63, 64);
+    }
+}
+__global__ void k_scan_apply(const int64_t* __restrict__ in, int64_t n, const int64_t* __restrict__ part,
+                             int64_t* __restrict__ out) {
+    const int lane = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * kScanChunk, c1 = min(n, c0 + kScanChunk);
+    long long carry = part[blockIdx.x];
+    for (int64_t b = c0; b < c1; b += 64) {
+        const int64_t i = b + lane;
+        const long long v = i < c1 ? in[i] : 0;
+        const long long x = wave_incl_sum64(v, lane);
+        if (i < c1) out[i] = carry + x - v;
+        carry += __shfl(x, 63, 64);
+    }
+}
+void excl_sum_nolds(pgd_ctx* ctx, const int64_t* in, int64_t* out, int64_t n) {
+    if (n <= 0) return;
+    hipStream_t s = ctx->stream;
+    const int64_t np = (n + kScanChunk - 1) / kScanChunk;
+    PGD_CHECK(np <= INT32_MAX, PGD_ERR_UNSUPPORTED, "scan too large");
+    DBuf<int64_t> part;
+    part.alloc(ctx, np);
+    hipLaunchKernelGGL(k_scan_parts, dim3((unsigned)np), dim3(64), 0, s, in, n, part.get());
+    hipLaunchKernelGGL(k_scan_tops, dim3(1), dim3(64), 0, s, part.get(), np);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)np), dim3(64), 0, s, in, n, part.get(), out);
+    PGD_HIP(hipGetLastError());
+}
+
 template <int NT>
 void launch_tiled(pgd_ctx* ctx, OrientState& st, unsigned long long* work) {
     WedgePlan& wp = st.wp;
@@ -929,7 +989,7 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
                            wp.htw.get(), nh, wp.segcap, wcap, per.get(), (const int64_t*)nullptr, (WedgeItem*)nullptr,
                            (uint64_t*)nullptr);
         PGD_HIP(hipGetLastError());
-        excl_sum(ctx, per.get(), ioff.get(), (size_t)(nh + 1));
+        excl_sum_nolds(ctx, per.get(), ioff.get(), nh + 1);
         PGD_HIP(hipMemcpyAsync(&wp.nitems, ioff.get() + nh, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         PGD_HIP(hipStreamSynchronize(s));
         const int64_t ni = wp.nitems;
