@@ -59,7 +59,7 @@ typedef struct pgd_graph pgd_graph;
 
 /* decomposition flags */
 #define PGD_MICRO        1u   /* also produce the per-edge columns */
-#define PGD_ENGINE_EDGE  2u   /* force the edge-centric engine */
+/* 2u is not a flag: any bit other than those below is PGD_ERR_INVALID */
 #define PGD_ENGINE_ORIENT 4u  /* force the degree-oriented engine */
 #define PGD_ENGINE_DENSE 8u   /* force the dense (int8 MFMA) engine */
 
@@ -88,7 +88,10 @@ pgd_status pgd_graph_from_coo(pgd_ctx *ctx, int64_t n, int64_t nlines,
  * symmetric simple adjacency held by graphlet_core's `vertices`/`edges`
  * (graphlet_core.h:84-86) after its reader and, if already run, its neighbour
  * ordering.  Neighbour order is kept, so a later pgd_preprocess with a method
- * that orders nothing ("natural") reproduces the object's edge ids. */
+ * that orders nothing ("natural") reproduces the object's edge ids.
+ * Checked before use: row[0] == 0, offsets non-decreasing, ids in [0, n), no
+ * self loop and a symmetric adjacency (PGD_ERR_INVALID otherwise); a repeated
+ * neighbour (the multi-edges read_mtx keeps) is PGD_ERR_UNSUPPORTED. */
 pgd_status pgd_graph_from_csr(pgd_ctx *ctx, int64_t n, const int64_t *row, const int32_t *col,
                               pgd_graph **out);
 pgd_status pgd_graph_free(pgd_graph *g);
@@ -117,7 +120,27 @@ pgd_status pgd_compute_cores(pgd_graph *g, int32_t *core, int32_t *max_core);
  * get_ordering_value's default).  "rand"/"var" are PGD_ERR_UNSUPPORTED
  * (their reference keys are time-seeded / undefined). */
 pgd_status pgd_sort_edges(pgd_graph *g, const char *ordering, int32_t small_to_large);
-/* E_ordered edge ids (m ints). */
+/* sort_edges over a sampled edge set E_s (graphlet_core.cpp:836-851):
+ * E_ordered becomes the `count` edge ids `ids` sorted by the same keys, except
+ * that the "natural" key is the position in E_s.  Every id must be in [0, m)
+ * (PGD_ERR_INVALID otherwise); ids may repeat, as in the reference.  The
+ * decomposition then runs over these count positions only
+ * (graphlet_core.cpp:1380, 1513: loops to E_ordered.size()), with n and m
+ * still the graph's; the micro columns hold m values whose positions past
+ * count stay 0. */
+pgd_status pgd_sort_edges_subset(pgd_graph *g, const char *ordering, int32_t small_to_large,
+                                 const int32_t *ids, int64_t count);
+/* Adopt a caller's E_ordered (graphlet_core.h:207, `vector<Vertex>
+ * E_ordered`, its ids in order): `count` edge ids, each in [0, m).  Replaces
+ * the order pgd_sort_edges would build, so an object whose E_ordered came from
+ * any ordering (a time-seeded "rand", a sampled E_s, or the duplicates a
+ * second sort_edges call appends) is decomposed in exactly that order.  A
+ * later pgd_decompose with PGD_MICRO needs count <= m (the reference's
+ * per-edge vectors hold m positions): PGD_ERR_INVALID otherwise. */
+pgd_status pgd_set_e_ordered(pgd_graph *g, const int32_t *ids, int64_t count);
+/* Number of E_ordered positions (E_ordered.size()). */
+pgd_status pgd_e_ordered_count(pgd_graph *g, int64_t *count);
+/* E_ordered edge ids (pgd_e_ordered_count ints). */
 pgd_status pgd_e_ordered(pgd_graph *g, int32_t *ids);
 
 /* ---- decomposition ----------------------------------------------------
@@ -132,8 +155,7 @@ pgd_status pgd_micro(pgd_graph *g, int32_t column, uint64_t *out);
 /* Raw per-edge counts by edge id: T, K4 (4-cliques on the edge) and C4
  * (induced 4-cycles on the edge); any pointer may be NULL. */
 pgd_status pgd_edge_counts(pgd_graph *g, uint64_t *tri, uint64_t *k4, uint64_t *c4);
-/* Which engine the last pgd_decompose used: 1 edge-centric, 2 oriented,
- * 3 dense. */
+/* Which engine the last pgd_decompose used: 2 oriented, 3 dense. */
 int32_t    pgd_last_engine(pgd_graph *g);
 /* Device time (ms) of the last pgd_decompose's counting kernels, and of its
  * dominant kernel alone, with its launch count (HIP events on the context's

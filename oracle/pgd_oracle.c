@@ -429,8 +429,24 @@ static int cmp_incr(const void *x, const void *y) {   /* incr_bound, graphlet_ve
     return a->id > c->id ? -1 : (a->id < c->id);
 }
 
+/* sort_edges over all m edges (graphlet_core.cpp:854-867) when ids is NULL,
+ * else over the sampled set E_s = ids[0..m) (836-851), whose natural key is
+ * the position in E_s */
+static int sort_edge_list(const or_csr *g, const int32_t *e_v, const int32_t *e_u, const int32_t *ids, int64_t m,
+                          const char *o, int small_to_large, int32_t *e_ordered);
+
 int or_sort_edges(const or_csr *g, const int32_t *e_v, const int32_t *e_u, int64_t m,
                   const char *o, int small_to_large, int32_t *e_ordered) {
+    return sort_edge_list(g, e_v, e_u, NULL, m, o, small_to_large, e_ordered);
+}
+
+int or_sort_edges_subset(const or_csr *g, const int32_t *e_v, const int32_t *e_u, const int32_t *ids,
+                         int64_t count, const char *o, int small_to_large, int32_t *e_ordered) {
+    return sort_edge_list(g, e_v, e_u, ids, count, o, small_to_large, e_ordered);
+}
+
+static int sort_edge_list(const or_csr *g, const int32_t *e_v, const int32_t *e_u, const int32_t *ids, int64_t m,
+                          const char *o, int small_to_large, int32_t *e_ordered) {
     int64_t n = g->n;
     int32_t *deg = (int32_t *)malloc((size_t)(n ? n : 1) * sizeof(int32_t));
     int32_t *core = (int32_t *)malloc((size_t)(n ? n : 1) * sizeof(int32_t));
@@ -464,9 +480,10 @@ int or_sort_edges(const or_csr *g, const int32_t *e_v, const int32_t *e_u, int64
     }
     if (ord == 1 || ord == 8) { free(deg); free(core); return -1; }
     vtx *E = (vtx *)malloc((size_t)(m ? m : 1) * sizeof(vtx));
-    for (int64_t e = 0; e < m; e++) {
+    for (int64_t p = 0; p < m; p++) {
+        const int32_t e = ids ? ids[p] : (int32_t)p;   /* `int edge_id = E_s[e]` (843) */
         int64_t v = e_v[e], u = e_u[e];
-        long long val = e;                  /* get_ordering_value, graphlet_core.cpp:773-834 */
+        long long val = p;                  /* get_ordering_value, graphlet_core.cpp:773-834 */
         switch (ord) {
         case 0: break;
         case 2: val = (long long)deg[u] + deg[v]; break;
@@ -486,8 +503,8 @@ int or_sort_edges(const or_csr *g, const int32_t *e_v, const int32_t *e_u, int64
             break;
         default: val = g->row[u + 1] - g->row[u]; break;
         }
-        E[e].id = (int32_t)e;
-        E[e].b = (int32_t)val;              /* Vertex stores an int bound */
+        E[p].id = e;
+        E[p].b = (int32_t)val;              /* Vertex stores an int bound */
     }
     #undef KC
     qsort(E, (size_t)m, sizeof(vtx), small_to_large ? cmp_incr : cmp_decr);

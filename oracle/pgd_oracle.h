@@ -75,6 +75,10 @@ int64_t or_edge_arrays(const or_csr *g, int32_t *e_v, int32_t *e_u);
  * ("rand", "var").  e_ordered holds m edge ids. */
 int or_sort_edges(const or_csr *g, const int32_t *e_v, const int32_t *e_u, int64_t m,
                   const char *ordering, int small_to_large, int32_t *e_ordered);
+/* sort_edges over a sampled edge set E_s (graphlet_core.cpp:836-851):
+ * E_ordered of the count ids, natural key = position in E_s. */
+int or_sort_edges_subset(const or_csr *g, const int32_t *e_v, const int32_t *e_u, const int32_t *ids,
+                         int64_t count, const char *o, int small_to_large, int32_t *e_ordered);
 
 /* The edge-parallel decomposition over positions [0,count) of `order`
  * (graphlet_core.cpp:1366-1448 macro, 1483-1599 micro, CSR branch).

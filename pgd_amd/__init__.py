@@ -13,7 +13,7 @@ import weakref
 
 import numpy as np
 
-from ._lib import (DEDUP, ENGINE_DENSE, ENGINE_EDGE, ENGINE_ORIENT, MICRO, NMICRO, NSUMS, NTOTALS,
+from ._lib import (DEDUP, ENGINE_DENSE, ENGINE_ORIENT, MICRO, NMICRO, NSUMS, NTOTALS,
                    TOTAL_NAMES, PGDError, Tuning, check, lib, ptr)
 
 __all__ = [
@@ -23,7 +23,7 @@ __all__ = [
 
 MICRO_COLUMNS = ["tri", "local_4_clique", "local_4_chordal_cycle", "local_4_tailed_tris", "local_4_cycle",
                  "local_3_star", "local_4_path", "2_star"]
-_ENGINE = {None: 0, "edge": ENGINE_EDGE, "orient": ENGINE_ORIENT, "dense": ENGINE_DENSE}
+_ENGINE = {None: 0, "orient": ENGINE_ORIENT, "dense": ENGINE_DENSE}
 
 
 def default_tuning(**overrides) -> Tuning:
@@ -228,8 +228,22 @@ class GraphletCore:
               self.ctx.h, "pgd_compute_cores")
         return (K.value, core[: self.n]) if want_cores else K.value
 
-    def sort_edges(self, ordering: str = "deg", small_to_large: bool = False) -> None:
-        check(lib().pgd_sort_edges(self.h, ordering.encode(), int(small_to_large)), self.ctx.h, "pgd_sort_edges")
+    def sort_edges(self, ordering: str = "deg", small_to_large: bool = False, E_s=None) -> None:
+        """sort_edges (graphlet_core.cpp:854-867), or with E_s the overload
+        over a sampled edge set (836-851): E_ordered holds only those ids."""
+        if E_s is None:
+            check(lib().pgd_sort_edges(self.h, ordering.encode(), int(small_to_large)), self.ctx.h,
+                  "pgd_sort_edges")
+        else:
+            ids = np.ascontiguousarray(E_s, dtype=np.int32)
+            check(lib().pgd_sort_edges_subset(self.h, ordering.encode(), int(small_to_large), ptr(ids, C.c_int32),
+                                              len(ids)), self.ctx.h, "pgd_sort_edges_subset")
+        self._ordered = True
+
+    def set_e_ordered(self, ids) -> None:
+        """Adopt a caller's E_ordered (edge ids in order, at most m)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        check(lib().pgd_set_e_ordered(self.h, ptr(ids, C.c_int32), len(ids)), self.ctx.h, "pgd_set_e_ordered")
         self._ordered = True
 
     # -- state getters ----------------------------------------------------
@@ -246,9 +260,11 @@ class GraphletCore:
         return ev[: self.m], eu[: self.m]
 
     def e_ordered(self):
-        ids = np.zeros(max(self.m, 1), dtype=np.int32)
+        k = C.c_int64()
+        check(lib().pgd_e_ordered_count(self.h, C.byref(k)), self.ctx.h)
+        ids = np.zeros(max(k.value, 1), dtype=np.int32)
         check(lib().pgd_e_ordered(self.h, ptr(ids, C.c_int32)), self.ctx.h)
-        return ids[: self.m]
+        return ids[: k.value]
 
     # -- decomposition ----------------------------------------------------
     def decompose(self, micro: bool = False, engine: str | None = None) -> list[int]:
@@ -302,7 +318,7 @@ class GraphletCore:
         return t
 
     def last_engine(self) -> str:
-        return {1: "edge", 2: "orient", 3: "dense"}.get(lib().pgd_last_engine(self.h), "none")
+        return {2: "orient", 3: "dense"}.get(lib().pgd_last_engine(self.h), "none")
 
     def last_phases(self) -> list[dict]:
         """Per-pass device time and work model of the last decomposition."""
@@ -327,12 +343,13 @@ class GraphletCore:
     def micro_text(self) -> str:
         ev, eu = self.edge_arrays()
         order = self.e_ordered()
-        cols = [self.micro(c) for c in ("tri", "2_star", "local_4_clique", "local_4_chordal_cycle",
-                                        "local_4_tailed_tris", "local_4_cycle", "local_3_star", "local_4_path")]
+        k = len(order)  # one row per E_ordered position
+        cols = [self.micro(c)[:k] for c in ("tri", "2_star", "local_4_clique", "local_4_chordal_cycle",
+                                            "local_4_tailed_tris", "local_4_cycle", "local_3_star", "local_4_path")]
         v1 = ev[order].astype(np.uint64) + np.uint64(1)
         u1 = eu[order].astype(np.uint64) + np.uint64(1)
         head = "% src,dst,triangle,2-star,4-clique,4-chordal-cycle,4-tailed-triangle,4-cycle,3-star,4-path\n"
-        rows = np.stack([v1, u1] + [c.astype(np.uint64) for c in cols], axis=1) if self.m else None
+        rows = np.stack([v1, u1] + [c.astype(np.uint64) for c in cols], axis=1) if k else None
         if rows is None:
             return head
         return head + "\n".join(",".join(str(int(x)) for x in r) for r in rows) + "\n"

@@ -19,7 +19,6 @@ NMICRO = 8
 MICRO_TRI, MICRO_CLIQUE, MICRO_CHORDAL, MICRO_TAILED, MICRO_CYCLE, MICRO_STAR3, MICRO_PATH, MICRO_STAR2 = range(8)
 DEDUP = 1
 MICRO = 1
-ENGINE_EDGE = 2
 ENGINE_ORIENT = 4
 ENGINE_DENSE = 8
 
@@ -33,7 +32,8 @@ TOTAL_NAMES = [
 EXPORTS = [
     "pgd_open", "pgd_close", "pgd_status_str", "pgd_set_mem_cap", "pgd_mem_in_use", "pgd_last_error",
     "pgd_graph_from_coo", "pgd_graph_from_csr", "pgd_graph_free", "pgd_graph_info", "pgd_graph_csr", "pgd_preprocess",
-    "pgd_edge_arrays", "pgd_compute_cores", "pgd_sort_edges", "pgd_e_ordered", "pgd_decompose", "pgd_micro",
+    "pgd_edge_arrays", "pgd_compute_cores", "pgd_sort_edges", "pgd_sort_edges_subset",
+    "pgd_set_e_ordered", "pgd_e_ordered_count", "pgd_e_ordered", "pgd_decompose", "pgd_micro",
     "pgd_edge_counts", "pgd_last_engine", "pgd_last_timing", "pgd_last_phase", "pgd_derive", "pgd_batch_create", "pgd_batch_run",
     "pgd_batch_micro", "pgd_batch_info", "pgd_batch_free", "pgd_batch_set_tuning", "pgd_batch_get_tuning",
     "pgd_batch_autotune", "pgd_batch_last_ms", "pgd_tuning_default", "pgd_graph_set_tuning", "pgd_graph_get_tuning",
@@ -89,6 +89,9 @@ def lib() -> C.CDLL:
         "pgd_edge_arrays": (I32, [P, PI32, PI32]),
         "pgd_compute_cores": (I32, [P, PI32, PI32]),
         "pgd_sort_edges": (I32, [P, C.c_char_p, I32]),
+        "pgd_sort_edges_subset": (I32, [P, C.c_char_p, I32, PI32, I64]),
+        "pgd_set_e_ordered": (I32, [P, PI32, I64]),
+        "pgd_e_ordered_count": (I32, [P, PI64]),
         "pgd_e_ordered": (I32, [P, PI32]),
         "pgd_decompose": (I32, [P, U32, PU64]),
         "pgd_micro": (I32, [P, I32, PU64]),

@@ -37,7 +37,6 @@ HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", f"-I{INC}", f"-I{CSRC}",
 
 HIP_SOURCES = [
     "pgd_core.hip",
-    "pgd_engine_edge.hip",
     "pgd_engine_orient.hip",
     "pgd_engine_wedge.hip",
     "pgd_engine_dense.hip",

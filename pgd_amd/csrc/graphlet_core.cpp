@@ -148,7 +148,6 @@ void graphlet_core::read_graph(const std::string& filename) {
 void graphlet_core::preprocess(std::string neigh, bool s2l, int /*adj_limit*/, std::string /*rep*/) {
     check(pgd_preprocess(g_, neigh.c_str(), s2l ? 1 : 0), "preprocess");
     fetch_edges();
-    ordered_ = false;
 }
 
 void graphlet_core::fetch_edges() {
@@ -181,11 +180,26 @@ void graphlet_core::compute_assortativity() {
     if (m == 0) r = 0.0;
 }
 
+// the device's new order appended to E_ordered
+void graphlet_core::append_order() {
+    int64_t k = 0;
+    check(pgd_e_ordered_count(g_, &k), "pgd_e_ordered_count");
+    const size_t at = E_ordered.size();
+    E_ordered.resize(at + (size_t)k);
+    if (k) check(pgd_e_ordered(g_, E_ordered.data() + at), "pgd_e_ordered");
+}
+
 void graphlet_core::sort_edges(std::string ordering, bool s2l) {
     check(pgd_sort_edges(g_, ordering.c_str(), s2l ? 1 : 0), "sort_edges");
-    E_ordered.resize((size_t)m_);
-    if (m_) check(pgd_e_ordered(g_, E_ordered.data()), "pgd_e_ordered");
-    ordered_ = true;
+    append_order();
+}
+
+void graphlet_core::sort_edges(std::string ordering, bool s2l, std::vector<unsigned long>& E_s) {
+    std::vector<int32_t> ids(E_s.size());
+    for (size_t i = 0; i < E_s.size(); i++) ids[i] = (int32_t)E_s[i];  // `int edge_id = E_s[e]` (843)
+    check(pgd_sort_edges_subset(g_, ordering.c_str(), s2l ? 1 : 0, ids.data(), (int64_t)ids.size()),
+          "sort_edges");
+    append_order();
 }
 
 void graphlet_core::set_totals(const uint64_t* t) {
@@ -197,7 +211,8 @@ void graphlet_core::set_totals(const uint64_t* t) {
 }
 
 void graphlet_core::decompose(bool micro) {
-    if (!ordered_) sort_edges();  // graphlet_core.cpp:1376-1378
+    if (E_ordered.empty()) sort_edges();  // graphlet_core.cpp:1376-1378
+    check(pgd_set_e_ordered(g_, E_ordered.data(), (int64_t)E_ordered.size()), "E_ordered");
     uint64_t sums[PGD_NSUMS], tot[PGD_NTOTALS];
     check(pgd_decompose(g_, micro ? PGD_MICRO : 0u, sums), "graphlet_decomposition");
     check(pgd_derive(n_, m_, sums, tot), "pgd_derive");
@@ -263,15 +278,15 @@ std::string graphlet_core::get_micro_stats(bool output_id, std::string delim) {
            "4-tailed-triangle" + delim + "4-cycle" + delim + "3-star" + delim + "4-path" + "\n";
     std::vector<char> buf(256);
     out.reserve(out.size() + (size_t)m_ * 40);
-    for (int64_t e = 0; e < m_ && (size_t)e < tri.size(); e++) {
+    for (size_t e = 0; e < E_ordered.size() && e < tri.size(); e++) {
         char* p = buf.data();
-        const int32_t id = E_ordered[(size_t)e];
+        const int32_t id = E_ordered[e];
         const unsigned long long vals[10] = {(unsigned long long)e_v[(size_t)id] + 1,
                                              (unsigned long long)e_u[(size_t)id] + 1,
-                                             tri[(size_t)e], two_star[(size_t)e], local_4_clique[(size_t)e],
-                                             local_4_chordal_cycle[(size_t)e], local_4_tailed_tris[(size_t)e],
-                                             local_4_cycle[(size_t)e], local_3_star[(size_t)e],
-                                             local_4_path[(size_t)e]};
+                                             tri[e], two_star[e], local_4_clique[e],
+                                             local_4_chordal_cycle[e], local_4_tailed_tris[e],
+                                             local_4_cycle[e], local_3_star[e],
+                                             local_4_path[e]};
         for (int k = output_id ? 0 : 2; k < 10; k++) {
             p = put_u64(p, vals[k]);
             if (k < 9) for (char c : delim) *p++ = c;

@@ -46,7 +46,10 @@ public:
                     int adj_limit = 10000, std::string rep = "auto");
     void compute_cores();
     void compute_assortativity();
+    // sort_edges appends to E_ordered, as the reference does (graphlet_core.cpp:854-867)
     void sort_edges(std::string edge_ordering = "degree", bool is_small_to_large = false);
+    // the overload over a sampled edge set E_s (graphlet_core.cpp:836-851)
+    void sort_edges(std::string ordering_technique, bool is_small_to_large, std::vector<unsigned long>& E_s);
     void graphlet_decomposition(int max_num_workers = 1);
     void graphlet_decomposition_micro(int max_num_workers = 1);
 
@@ -72,7 +75,9 @@ public:
     void print_connected_GFD();
     void print_disconnected_GFD();
 
-    // results (graphlet_core.h:310-343)
+    // results (graphlet_core.h:310-343); E_ordered is public as in the
+    // reference (graphlet_core.h:207): the decomposition runs over whatever
+    // edge ids it holds
     unsigned long long total_2_1edge = 0, total_2_indep = 0, total_3_tris = 0, total_2_star = 0, total_3_1edge = 0,
                        total_3_indep = 0, total_4_clique = 0, total_4_chordcycle = 0, total_4_tailed_tris = 0,
                        total_4_cycle = 0, total_3_star = 0, total_4_path = 0, total_4_2edge = 0, total_4_1edge = 0,
@@ -92,8 +97,8 @@ private:
     void fetch_edges();
     void set_totals(const uint64_t* t);
     pgd_graph* g_ = nullptr;
+    void append_order();
     int64_t n_ = 0, m_ = 0;
-    bool ordered_ = false;
 };
 
 // pgd_ctx shared by every graphlet_core of the process (device 0)

@@ -73,9 +73,8 @@ __global__ __launch_bounds__(256) void k_epilogue_batch(
     const int32_t* __restrict__ ev, const int32_t* __restrict__ eu, const int64_t* __restrict__ row,
     const uint32_t* __restrict__ Tc, const uint64_t* __restrict__ K4c, const uint64_t* __restrict__ C4c,
     const int64_t* __restrict__ gn, const int64_t* __restrict__ gm, uint64_t* __restrict__ sums,
-    uint64_t* __restrict__ micro, int64_t mstride, uint32_t* __restrict__ inexact_out) {
+    uint64_t* __restrict__ micro, int64_t mstride) {
     __shared__ int64_t it_sh;
-    uint32_t inexact = 0;
     for (;;) {
         if (threadIdx.x == 0) it_sh = (int64_t)atomicAdd(qctr, 1u);
         __syncthreads();
@@ -90,12 +89,11 @@ __global__ __launch_bounds__(256) void k_epilogue_batch(
         for (int64_t p = I.p0 + threadIdx.x; p < I.p1; p += blockDim.x) {
             const int32_t id = order[p];
             const int32_t v = ev[id], u = eu[id];
-            inexact += edge_terms(row[v + 1] - row[v], row[u + 1] - row[u], Tc[id], K4c[id], C4c[id], n, m, acc,
-                                  micro, mstride, p);
+            edge_terms(row[v + 1] - row[v], row[u + 1] - row[u], Tc[id], K4c[id], C4c[id], n, m, acc, micro,
+                       mstride, p);
         }
         block_add_sums(acc, sums + (size_t)I.graph * PGD_NSUMS);
     }
-    if (inexact) atomicAdd(inexact_out, inexact);
 }
 
 }  // namespace
@@ -112,7 +110,7 @@ struct pgd_batch {
     DBuf<EpiItem> items;
     int64_t nitems = 0;
     DBuf<uint64_t> sums;
-    DBuf<uint32_t> ctl;  // [0] inexact micro values, [1] epilogue queue head
+    DBuf<uint32_t> ctl;  // [0] epilogue queue head
     OrientState* st = nullptr;
     OrientParams prm;
     hipGraph_t graph = nullptr;
@@ -168,13 +166,13 @@ void batch_count(pgd_batch* b) {
     if (b->M) orient_count(ctx, *b->st, EdgeCounts{U->T.get(), U->K4.get(), U->C4.get()}, nullptr);
     const int64_t ns = (int64_t)b->ng * PGD_NSUMS;
     hipLaunchKernelGGL(b_fill_u64, dim3(grid_cap(ctx, (ns + 255) / 256, 4)), dim3(256), 0, s, b->sums.get(), ns);
-    hipLaunchKernelGGL(b_fill_u32, dim3(1), dim3(64), 0, s, b->ctl.get(), (int64_t)2);
+    hipLaunchKernelGGL(b_fill_u32, dim3(1), dim3(64), 0, s, b->ctl.get(), (int64_t)1);
     PGD_HIP(hipGetLastError());
     if (b->nitems) {
         hipLaunchKernelGGL(k_epilogue_batch, dim3(grid_cap(ctx, b->nitems, 8)), dim3(256), 0, s, b->items.get(),
-                           b->nitems, b->ctl.get() + 1, U->order.get(), U->ev.get(), U->eu.get(), U->row.get(),
+                           b->nitems, b->ctl.get(), U->order.get(), U->ev.get(), U->eu.get(), U->row.get(),
                            U->T.get(), U->K4.get(), U->C4.get(), b->gn.get(), b->gm.get(), b->sums.get(),
-                           (b->flags & PGD_MICRO) ? U->micro.get() : nullptr, b->M, b->ctl.get());
+                           (b->flags & PGD_MICRO) ? U->micro.get() : nullptr, b->M);
         PGD_HIP(hipGetLastError());
     }
 }
@@ -214,18 +212,14 @@ double batch_replay(pgd_batch* b) {
     PGD_HIP(hipGraphLaunch(b->exec, s));
     PGD_HIP(hipEventRecord(e1, s));
     b->host_sums.resize((size_t)b->ng * PGD_NSUMS);
-    uint32_t ctl[2] = {0, 0};
     PGD_HIP(hipMemcpyAsync(b->host_sums.data(), b->sums.get(), b->host_sums.size() * sizeof(uint64_t),
                            hipMemcpyDeviceToHost, s));
-    PGD_HIP(hipMemcpyAsync(ctl, b->ctl.get(), sizeof(ctl), hipMemcpyDeviceToHost, s));
     PGD_HIP(hipStreamSynchronize(s));
     float ms = 0;
     PGD_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->last_ms = ms;
-    PGD_CHECK(!(b->flags & PGD_MICRO) || ctl[0] == 0, PGD_ERR_UNSUPPORTED,
-              "a per-edge micro value exceeds 2^53; the exact double emulation for it is not implemented");
     return ms;
 }
 
@@ -322,7 +316,7 @@ void batch_build(pgd_batch* b, int32_t ngraphs, const int32_t* n, const int64_t*
     PGD_HIP(hipMemcpyAsync(b->gn.get(), b->n.data(), ng * sizeof(int64_t), hipMemcpyHostToDevice, s));
     PGD_HIP(hipMemcpyAsync(b->gm.get(), b->m.data(), ng * sizeof(int64_t), hipMemcpyHostToDevice, s));
     b->sums.alloc(ctx, (size_t)ng * PGD_NSUMS);
-    b->ctl.alloc(ctx, 2);
+    b->ctl.alloc(ctx, 1);
     const int64_t M1 = std::max<int64_t>(b->M, 1);
     U->T.alloc(ctx, M1);
     U->K4.alloc(ctx, M1);
@@ -505,13 +499,9 @@ void batch_collect(pgd_batch* b, std::vector<Request*>& reqs) {
     pgd_ctx* ctx = b->ctx;
     hipStream_t s = ctx->stream;
     b->host_sums.resize((size_t)b->ng * PGD_NSUMS);
-    uint32_t ctl[2] = {0, 0};
     PGD_HIP(hipMemcpyAsync(b->host_sums.data(), b->sums.get(), b->host_sums.size() * sizeof(uint64_t),
                            hipMemcpyDeviceToHost, s));
-    PGD_HIP(hipMemcpyAsync(ctl, b->ctl.get(), sizeof(ctl), hipMemcpyDeviceToHost, s));
     PGD_HIP(hipStreamSynchronize(s));
-    PGD_CHECK(!(b->flags & PGD_MICRO) || ctl[0] == 0, PGD_ERR_UNSUPPORTED,
-              "a per-edge micro value exceeds 2^53; the exact double emulation for it is not implemented");
     std::shared_ptr<DBuf<uint64_t>> block;
     if ((b->flags & PGD_MICRO) && b->M) {
         block = std::make_shared<DBuf<uint64_t>>();

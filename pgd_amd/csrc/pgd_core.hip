@@ -169,6 +169,25 @@ __global__ void k_count_repeats(const uint64_t* __restrict__ key, int64_t L, uin
         if (j > 0 && key[j] != ~0ULL && key[j] == key[j - 1]) atomicAdd(reps, 1u);
 }
 
+// A caller's CSR, checked after k_csr_owner has passed its ids: the directed
+// pairs (v, u) sorted, and the transposed pairs (u, v) sorted, must be the same
+// sequence (a symmetric adjacency) with no pair twice (no repeated neighbour).
+__global__ void k_dir_keys(const int32_t* __restrict__ src, const int32_t* __restrict__ col, int64_t nnz,
+                           uint64_t* __restrict__ kf, uint64_t* __restrict__ kt) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t v = (uint32_t)src[j], u = (uint32_t)col[j];
+        kf[j] = v << 32 | u;
+        kt[j] = u << 32 | v;
+    }
+}
+__global__ void k_csr_sym_check(const uint64_t* __restrict__ kf, const uint64_t* __restrict__ kt, int64_t nnz,
+                                uint32_t* __restrict__ bad) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * blockDim.x) {
+        if (kf[j] != kt[j]) atomicOr(bad, 2u);
+        if (j > 0 && kf[j] == kf[j - 1]) atomicOr(bad, 4u);
+    }
+}
+
 // line i (kept, compacted index o) -> slots 2o (src->dst) and 2o+1 (dst->src)
 __global__ void k_make_slots(const int32_t* __restrict__ s, const int32_t* __restrict__ d,
                              const int32_t* __restrict__ keep, const int64_t* __restrict__ off, int64_t L,
@@ -346,13 +365,18 @@ __global__ void k_core_update(const int64_t* __restrict__ row, const int32_t* __
 // ---- edge order keys (get_ordering_value, graphlet_core.cpp:773-834) -------
 enum OrderKind { OK_NATURAL = 0, OK_DEG, OK_KCORE, OK_KCORE_DEG, OK_DEG_VOL, OK_KCORE_VOL, OK_DEG_KCORE_VOL, OK_DEFAULT };
 
+// Keys of the positions p < cnt of the edge list: edge p itself (sort_edges
+// over all edges, graphlet_core.cpp:854-867) or, with ids, the edge ids[p] of
+// a sampled set E_s (837-851), whose natural key is the position p.
 __global__ void k_order_keys(const int32_t* __restrict__ ev, const int32_t* __restrict__ eu,
                              const int32_t* __restrict__ deg, const int32_t* __restrict__ core,
-                             const int64_t* __restrict__ row, const int32_t* __restrict__ col, int64_t m,
-                             int kind, int s2l, uint64_t* __restrict__ key) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+                             const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                             const int32_t* __restrict__ ids, int64_t cnt, int kind, int s2l,
+                             uint64_t* __restrict__ key) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < cnt; p += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t e = ids ? ids[p] : (int32_t)p;
         int32_t v = ev[e], u = eu[e];
-        long long val = e;
+        long long val = p;
         switch (kind) {
             case OK_NATURAL: break;
             case OK_DEG: val = (long long)deg[u] + deg[v]; break;
@@ -378,7 +402,7 @@ __global__ void k_order_keys(const int32_t* __restrict__ ev, const int32_t* __re
         }
         uint32_t k = (uint32_t)(int32_t)val ^ 0x80000000u;  // Vertex keeps an int bound
         if (!s2l) k = ~k;                                   // decr_bound: larger bound first
-        key[e] = ((uint64_t)k << 32) | (uint32_t)~(uint32_t)e;  // ties: larger id first
+        key[p] = ((uint64_t)k << 32) | (uint32_t)~(uint32_t)e;  // ties: larger id first
     }
 }
 
@@ -396,32 +420,39 @@ __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ or
                                                   const int32_t* __restrict__ eu, const int64_t* __restrict__ row,
                                                   const uint32_t* __restrict__ Tc, const uint64_t* __restrict__ K4c,
                                                   const uint64_t* __restrict__ C4c, int64_t n, int64_t m,
-                                                  uint64_t* __restrict__ sums, uint64_t* __restrict__ micro,
-                                                  uint32_t* __restrict__ flags) {
+                                                  int64_t nord, uint64_t* __restrict__ sums,
+                                                  uint64_t* __restrict__ micro) {
     uint64_t acc[PGD_NSUMS];
 #pragma unroll
     for (int k = 0; k < PGD_NSUMS; k++) acc[k] = 0;
-    uint32_t inexact = 0;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+    // the loop runs over the E_ordered positions (all m edges, a sampled set
+    // E_s, or any list of edge ids); m stays the graph's edge count in the
+    // equations.  With micro, nord <= m.
+    const int64_t np = nord > m ? nord : m;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < np; e += (int64_t)gridDim.x * blockDim.x) {
+        if (e >= nord) {
+            // positions past |E_ordered| keep the 0 of the reference's
+            // freshly resized per-edge vectors
+            if (micro)
+                for (int k = 0; k < PGD_NMICRO; k++) micro[(size_t)k * m + e] = 0;
+            continue;
+        }
         int32_t id = order[e];
         int32_t v = ev[id], u = eu[id];
-        inexact += edge_terms(row[v + 1] - row[v], row[u + 1] - row[u], Tc[id], K4c[id], C4c[id], n, m, acc, micro,
-                              m, e);
+        edge_terms(row[v + 1] - row[v], row[u + 1] - row[u], Tc[id], K4c[id], C4c[id], n, m, acc, micro, m, e);
     }
     block_add_sums(acc, sums);
-    if (inexact) atomicAdd(flags, inexact);
 }
 
 }  // namespace
 
 namespace pgd {
-void epilogue(pgd_ctx* ctx, const GraphView& g, EdgeCounts c, uint64_t* d_sums, uint64_t* d_micro, uint32_t* d_flags) {
+void epilogue(pgd_ctx* ctx, const GraphView& g, EdgeCounts c, uint64_t* d_sums, uint64_t* d_micro) {
     PGD_HIP(hipMemsetAsync(d_sums, 0, PGD_NSUMS * sizeof(uint64_t), ctx->stream));
-    PGD_HIP(hipMemsetAsync(d_flags, 0, sizeof(uint32_t), ctx->stream));
-    if (g.m == 0) return;
-    int grid = std::min(grid_for(g.m), ctx->num_cus * 8);
+    if (g.m == 0 && g.nord == 0) return;
+    int grid = std::min(grid_for(std::max(g.m, g.nord)), ctx->num_cus * 8);
     hipLaunchKernelGGL(k_epilogue, dim3(grid), dim3(256), 0, ctx->stream, g.order, g.ev, g.eu, g.row, c.T, c.K4,
-                       c.C4, g.n, g.m, d_sums, d_micro, d_flags);
+                       c.C4, g.n, g.m, g.nord, d_sums, d_micro);
     PGD_HIP(hipGetLastError());
 }
 }  // namespace pgd
@@ -740,6 +771,12 @@ pgd_status pgd_graph_from_csr(pgd_ctx* ctx, int64_t n, const int64_t* row, const
     if (n >= INT32_MAX) return PGD_ERR_UNSUPPORTED;
     const int64_t nnz = row[n];
     if (row[0] != 0 || nnz < 0 || (nnz & 1) || (nnz > 0 && !col)) return PGD_ERR_INVALID;
+    // offsets non-decreasing (so every list lies inside [0, nnz)) before any upload
+    for (int64_t v = 0; v < n; v++)
+        if (row[v + 1] < row[v]) {
+            ctx->last_err = "CSR offsets decrease";
+            return PGD_ERR_INVALID;
+        }
     *out = nullptr;
     pgd_graph* g = new (std::nothrow) pgd_graph();
     if (!g) return PGD_ERR_OOM;
@@ -773,7 +810,25 @@ pgd_status pgd_graph_from_csr(pgd_ctx* ctx, int64_t n, const int64_t* row, const
         uint32_t hbad = 0;
         PGD_HIP(hipMemcpyAsync(&hbad, bad.get(), sizeof(hbad), hipMemcpyDeviceToHost, s));
         sync(ctx);
-        PGD_CHECK(hbad == 0, PGD_ERR_INVALID, "CSR with a decreasing offset, an id outside [0, n) or a self loop");
+        PGD_CHECK(hbad == 0, PGD_ERR_INVALID, "CSR with an id outside [0, n) or a self loop");
+        if (nnz) {
+            DBuf<uint64_t> kf, kt, sf, st;
+            kf.alloc(ctx, nnz); kt.alloc(ctx, nnz); sf.alloc(ctx, nnz); st.alloc(ctx, nnz);
+            int grid = std::min(grid_for(nnz), ctx->num_cus * 16);
+            hipLaunchKernelGGL(k_dir_keys, dim3(grid), dim3(kBlock), 0, s, g->src.get(), g->col.get(), nnz, kf.get(),
+                               kt.get());
+            PGD_HIP(hipGetLastError());
+            const unsigned kb = 32u + (unsigned)bits_for(n - 1);
+            sort_keys(ctx, kf.get(), sf.get(), (size_t)nnz, 0, kb);
+            sort_keys(ctx, kt.get(), st.get(), (size_t)nnz, 0, kb);
+            hipLaunchKernelGGL(k_csr_sym_check, dim3(grid), dim3(kBlock), 0, s, sf.get(), st.get(), nnz, bad.get());
+            PGD_HIP(hipGetLastError());
+            PGD_HIP(hipMemcpyAsync(&hbad, bad.get(), sizeof(hbad), hipMemcpyDeviceToHost, s));
+            sync(ctx);
+            PGD_CHECK(!(hbad & 2u), PGD_ERR_INVALID, "CSR is not symmetric");
+            PGD_CHECK(!(hbad & 4u), PGD_ERR_UNSUPPORTED,
+                      "CSR repeats a neighbour (a multi-edge); only simple graphs are supported");
+        }
         g->max_deg = md;
         g->avg_deg = n > 0 ? (int32_t)((double)nnz / (double)n) : 0;
     });
@@ -899,6 +954,50 @@ pgd_status pgd_compute_cores(pgd_graph* g, int32_t* core, int32_t* max_core) {
     });
 }
 
+namespace {
+// E_ordered from the keys of positions [0, cnt): all edges (ids == nullptr)
+// or the device copy of a sampled set E_s
+void build_order(pgd_graph* g, int kind, int32_t s2l, const int32_t* ids, int64_t cnt) {
+    pgd_ctx* ctx = g->ctx;
+    PGD_HIP(hipSetDevice(ctx->device));
+    if (kind == OK_KCORE || kind == OK_KCORE_DEG || kind == OK_KCORE_VOL || kind == OK_DEG_KCORE_VOL)
+        compute_cores(g);
+    g->have_order = false;
+    g->have_counts = false;
+    g->have_micro = false;
+    const int64_t need = std::max<int64_t>(std::max(g->m, cnt), 1);
+    if (g->order.n < (size_t)need) g->order.alloc(ctx, need);
+    if (cnt) {
+        DBuf<uint64_t> k0, k1;
+        k0.alloc(ctx, cnt);
+        k1.alloc(ctx, cnt);
+        int grid = std::min(grid_for(cnt), ctx->num_cus * 16);
+        hipLaunchKernelGGL(k_order_keys, dim3(grid), dim3(kBlock), 0, ctx->stream, g->ev.get(), g->eu.get(),
+                           g->deg.get(), g->have_core ? g->core.get() : nullptr, g->row.get(), g->col.get(), ids,
+                           cnt, kind, s2l ? 1 : 0, k0.get());
+        PGD_HIP(hipGetLastError());
+        sort_keys(ctx, k0.get(), k1.get(), (size_t)cnt, 0, 64);
+        hipLaunchKernelGGL(k_order_from_keys, dim3(grid), dim3(kBlock), 0, ctx->stream, k1.get(), cnt,
+                           g->order.get());
+        PGD_HIP(hipGetLastError());
+    }
+    sync(ctx);
+    g->nord = cnt;
+    g->have_order = true;
+}
+
+// an edge list given by the caller: ids in [0, m), repeats allowed
+pgd_status check_edge_ids(pgd_graph* g, const int32_t* ids, int64_t count) {
+    if (count < 0 || count >= ((int64_t)1 << 31) || (count > 0 && !ids)) return PGD_ERR_INVALID;
+    for (int64_t i = 0; i < count; i++)
+        if (ids[i] < 0 || ids[i] >= g->m) {
+            g->ctx->last_err = "edge id outside [0, m)";
+            return PGD_ERR_INVALID;
+        }
+    return PGD_OK;
+}
+}  // namespace
+
 pgd_status pgd_sort_edges(pgd_graph* g, const char* ordering, int32_t s2l) {
     if (!g) return PGD_ERR_INVALID;
     if (!g->prepped) return PGD_ERR_STATE;
@@ -907,58 +1006,92 @@ pgd_status pgd_sort_edges(pgd_graph* g, const char* ordering, int32_t s2l) {
         g->ctx->last_err = "ordering has no reproducible key in the reference (rand/var)";
         return PGD_ERR_UNSUPPORTED;
     }
+    return guarded(g->ctx, [&] { build_order(g, kind, s2l, nullptr, g->m); });
+}
+
+pgd_status pgd_sort_edges_subset(pgd_graph* g, const char* ordering, int32_t s2l, const int32_t* ids,
+                                 int64_t count) {
+    if (!g) return PGD_ERR_INVALID;
+    if (!g->prepped) return PGD_ERR_STATE;
+    pgd_status st = check_edge_ids(g, ids, count);
+    if (st != PGD_OK) return st;
+    int kind = order_kind(ordering);
+    if (kind < 0) {
+        g->ctx->last_err = "ordering has no reproducible key in the reference (rand/var)";
+        return PGD_ERR_UNSUPPORTED;
+    }
+    return guarded(g->ctx, [&] {
+        DBuf<int32_t> d;
+        d.alloc(g->ctx, std::max<int64_t>(count, 1));
+        if (count)
+            PGD_HIP(hipMemcpyAsync(d.get(), ids, count * sizeof(int32_t), hipMemcpyHostToDevice, g->ctx->stream));
+        build_order(g, kind, s2l, d.get(), count);
+    });
+}
+
+pgd_status pgd_set_e_ordered(pgd_graph* g, const int32_t* ids, int64_t count) {
+    if (!g) return PGD_ERR_INVALID;
+    if (!g->prepped) return PGD_ERR_STATE;
+    pgd_status st = check_edge_ids(g, ids, count);
+    if (st != PGD_OK) return st;
     return guarded(g->ctx, [&] {
         pgd_ctx* ctx = g->ctx;
         PGD_HIP(hipSetDevice(ctx->device));
-        const int64_t m = g->m;
-        if (kind == OK_KCORE || kind == OK_KCORE_DEG || kind == OK_KCORE_VOL || kind == OK_DEG_KCORE_VOL)
-            compute_cores(g);
-        g->order.alloc(ctx, std::max<int64_t>(m, 1));
-        if (m) {
-            DBuf<uint64_t> k0, k1;
-            k0.alloc(ctx, m);
-            k1.alloc(ctx, m);
-            int grid = std::min(grid_for(m), ctx->num_cus * 16);
-            hipLaunchKernelGGL(k_order_keys, dim3(grid), dim3(kBlock), 0, ctx->stream, g->ev.get(), g->eu.get(),
-                               g->deg.get(), g->have_core ? g->core.get() : nullptr, g->row.get(), g->col.get(), m,
-                               kind, s2l ? 1 : 0, k0.get());
-            PGD_HIP(hipGetLastError());
-            sort_keys(ctx, k0.get(), k1.get(), (size_t)m, 0, 64);
-            hipLaunchKernelGGL(k_order_from_keys, dim3(grid), dim3(kBlock), 0, ctx->stream, k1.get(), m,
-                               g->order.get());
-            PGD_HIP(hipGetLastError());
-        }
+        g->have_order = false;
+        g->have_counts = false;
+        g->have_micro = false;
+        const int64_t need = std::max<int64_t>(std::max(g->m, count), 1);
+        if (g->order.n < (size_t)need) g->order.alloc(ctx, need);
+        if (count)
+            PGD_HIP(hipMemcpyAsync(g->order.get(), ids, count * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
         sync(ctx);
+        g->nord = count;
         g->have_order = true;
     });
 }
 
+pgd_status pgd_e_ordered_count(pgd_graph* g, int64_t* count) {
+    if (!g || !count) return PGD_ERR_INVALID;
+    if (!g->have_order) return PGD_ERR_STATE;
+    *count = g->nord;
+    return PGD_OK;
+}
+
 pgd_status pgd_e_ordered(pgd_graph* g, int32_t* ids) {
-    if (!g || !ids) return PGD_ERR_INVALID;
+    if (!g || (!ids && g->nord)) return PGD_ERR_INVALID;
     if (!g->have_order) return PGD_ERR_STATE;
     return guarded(g->ctx, [&] {
-        if (g->m)
-            PGD_HIP(hipMemcpyAsync(ids, g->order.get(), g->m * sizeof(int32_t), hipMemcpyDeviceToHost, g->ctx->stream));
+        if (g->nord)
+            PGD_HIP(hipMemcpyAsync(ids, g->order.get(), g->nord * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                   g->ctx->stream));
         sync(g->ctx);
     });
 }
 
 pgd_status pgd_decompose(pgd_graph* g, uint32_t flags, uint64_t* sums) {
-    if (!g) return PGD_ERR_INVALID;
+    if (!g || !sums) return PGD_ERR_INVALID;
+    if (flags & ~(PGD_MICRO | PGD_ENGINE_ORIENT | PGD_ENGINE_DENSE)) {
+        g->ctx->last_err = "unknown decomposition flag";
+        return PGD_ERR_INVALID;
+    }
     if (!g->prepped || !g->have_order) return PGD_ERR_STATE;
+    if ((flags & PGD_MICRO) && g->nord > g->m) {
+        // the reference's micro vectors hold m positions (graphlet_core.cpp:1516-1522)
+        g->ctx->last_err = "E_ordered has more than m positions; per-edge columns hold m";
+        return PGD_ERR_INVALID;
+    }
     return guarded(g->ctx, [&] {
         pgd_ctx* ctx = g->ctx;
         PGD_HIP(hipSetDevice(ctx->device));
+        g->have_counts = false;
+        g->have_micro = false;
         const int64_t m = std::max<int64_t>(g->m, 1);
         if (g->T.n != (size_t)m) {
             g->T.alloc(ctx, m);
             g->K4.alloc(ctx, m);
             g->C4.alloc(ctx, m);
         }
-        if (!g->sums.get()) {
-            g->sums.alloc(ctx, PGD_NSUMS);
-            g->flags.alloc(ctx, 1);
-        }
+        if (!g->sums.get()) g->sums.alloc(ctx, PGD_NSUMS);
         const bool micro = (flags & PGD_MICRO) != 0;
         if (micro && g->micro.n != (size_t)PGD_NMICRO * m) g->micro.alloc(ctx, (size_t)PGD_NMICRO * m);
         GraphView v = g->view();
@@ -969,27 +1102,20 @@ pgd_status pgd_decompose(pgd_graph* g, uint32_t flags, uint64_t* sums) {
         PGD_HIP(hipEventRecord(e0, ctx->stream));
         Timing t;
         int engine;
-        if (flags & PGD_ENGINE_EDGE) engine = 1;
-        else if (flags & PGD_ENGINE_ORIENT) engine = 2;
+        if (flags & PGD_ENGINE_ORIENT) engine = 2;
         else if (flags & PGD_ENGINE_DENSE) engine = 3;
         else engine = engine_dense_ok(v) ? 3 : 2;
-        // the edge-centric engine hung on its first device run and stays
-        // disabled until the cause is found
-        PGD_CHECK(engine != 1, PGD_ERR_UNSUPPORTED, "edge-centric engine disabled");
         if (g->m > 0) {
-            if (engine == 1) engine_edge(ctx, v, c, &t);
-            else if (engine == 2) engine_orient(ctx, v, c, &t, &g->orient, g->prm);
+            if (engine == 2) engine_orient(ctx, v, c, &t, &g->orient, g->prm);
             else {
                 PGD_CHECK(engine_dense_fits(v), PGD_ERR_UNSUPPORTED, "graph too large for the dense engine");
                 engine_dense(ctx, v, c, &t);
             }
         }
-        epilogue(ctx, v, c, g->sums.get(), micro ? g->micro.get() : nullptr, g->flags.get());
+        epilogue(ctx, v, c, g->sums.get(), micro ? g->micro.get() : nullptr);
         PGD_HIP(hipEventRecord(e1, ctx->stream));
-        uint32_t inexact = 0;
         PGD_HIP(hipMemcpyAsync(sums, g->sums.get(), PGD_NSUMS * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                ctx->stream));
-        PGD_HIP(hipMemcpyAsync(&inexact, g->flags.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
         sync(ctx);
         float ms = 0;
         PGD_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -1000,8 +1126,6 @@ pgd_status pgd_decompose(pgd_graph* g, uint32_t flags, uint64_t* sums) {
         g->last_engine = engine;
         g->have_counts = true;
         g->have_micro = micro;
-        PGD_CHECK(!micro || inexact == 0, PGD_ERR_UNSUPPORTED,
-                  "a per-edge micro value exceeds 2^53; the exact double emulation for it is not implemented");
     });
 }
 

@@ -27,13 +27,34 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 
 constexpr uint64_t kExact = 1ULL << 53;
 
+// The binary64 value nearest p (ties to even), as an integer: what the
+// reference's (double)p holds.  For p < 2^63 the result fits 64 bits.
+__device__ __forceinline__ uint64_t rne53(uint64_t p) {
+    if (p < kExact) return p;
+    const int L = 64 - __clzll((long long)p);
+    const int sh = L - 53;
+    uint64_t q = p >> sh;
+    const uint64_t rem = p & ((1ULL << sh) - 1), hf = 1ULL << (sh - 1);
+    if (rem > hf || (rem == hf && (q & 1))) q++;
+    return q << sh;
+}
+
 // One edge at E_ordered position e of a graph with n vertices and m edges:
 // adds its terms to acc[15]; with micro != nullptr writes its 8 columns at
-// micro[col * mstride + e].  Returns 1 when a micro value would need the
-// double emulation beyond 2^53 (not implemented; the caller reports it).
-__device__ __forceinline__ uint32_t edge_terms(int64_t dv, int64_t du, uint64_t T, uint64_t K, uint64_t C, int64_t n,
-                                               int64_t m, uint64_t* acc, uint64_t* __restrict__ micro,
-                                               int64_t mstride, int64_t e) {
+// micro[col * mstride + e].
+//
+// The micro columns follow graphlet_core.cpp:1566-1575 value for value,
+// including the binary64 steps of the reference above 2^53 (vertex ids are
+// int32, so T, Sv, Su < 2^31 and every product below is < 2^62):
+//   chordal = (double)(T(T-1)) / 2.0 - (double)K4, stored as uint64: both
+//     operands are integers, so the difference is rounded once and exact
+//     below 2^53; C(T,2) >= K4 keeps it non-negative;
+//   3-star  = (uint64)((double)(Sv(Sv-1)) / 2.0), then that value plus
+//     (double)(Su(Su-1)) / 2.0, rounded once (the first term has at most 53
+//     significant bits, so its conversion back to double is exact).
+__device__ __forceinline__ void edge_terms(int64_t dv, int64_t du, uint64_t T, uint64_t K, uint64_t C, int64_t n,
+                                           int64_t m, uint64_t* acc, uint64_t* __restrict__ micro, int64_t mstride,
+                                           int64_t e) {
     uint64_t sv = (uint64_t)dv - T - 1, su = (uint64_t)du - T - 1;
     uint64_t S = sv + su;
     uint64_t D = (uint64_t)n - (S + T) - 2;
@@ -52,24 +73,19 @@ __device__ __forceinline__ uint32_t edge_terms(int64_t dv, int64_t du, uint64_t 
     acc[12] += half_rne(D * (D - 1));
     acc[13] += S * D;
     acc[14] += T * D;
-    uint32_t inexact = 0;
     if (micro) {
-        uint64_t tt = T * (T - 1);
-        int64_t svs = (int64_t)sv, sus = (int64_t)su;
-        uint64_t a3 = (uint64_t)(svs * (svs - 1)), b3 = (uint64_t)(sus * (sus - 1));
-        // exact below 2^53; larger values would need the double emulation
-        if (tt >= kExact || K >= kExact || a3 >= kExact || b3 >= kExact || (a3 >> 1) + (b3 >> 1) >= kExact)
-            inexact = 1;
+        const uint64_t tt = T * (T - 1);
+        const int64_t svs = (int64_t)sv, sus = (int64_t)su;
+        const uint64_t a3 = (uint64_t)(svs * (svs - 1)), b3 = (uint64_t)(sus * (sus - 1));
         micro[(size_t)PGD_MICRO_TRI * mstride + e] = T;
         micro[(size_t)PGD_MICRO_CLIQUE * mstride + e] = K;
-        micro[(size_t)PGD_MICRO_CHORDAL * mstride + e] = (tt >> 1) - K;
+        micro[(size_t)PGD_MICRO_CHORDAL * mstride + e] = rne53(half_rne(tt) - rne53(K));
         micro[(size_t)PGD_MICRO_TAILED * mstride + e] = 0;
         micro[(size_t)PGD_MICRO_CYCLE * mstride + e] = C;
-        micro[(size_t)PGD_MICRO_STAR3 * mstride + e] = (a3 >> 1) + (b3 >> 1);
+        micro[(size_t)PGD_MICRO_STAR3 * mstride + e] = rne53(half_rne(a3) + half_rne(b3));
         micro[(size_t)PGD_MICRO_PATH * mstride + e] = (uint64_t)(svs * sus) - C;
         micro[(size_t)PGD_MICRO_STAR2 * mstride + e] = S;
     }
-    return inexact;
 }
 
 // block-wide sum of acc[15] (blockDim.x a multiple of 64, at most 1024);

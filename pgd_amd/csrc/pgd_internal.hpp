@@ -117,9 +117,10 @@ struct GraphView {
     const int32_t* deg;   // n
     const int32_t* ev;    // m
     const int32_t* eu;    // m
-    const int32_t* order; // m, E_ordered
+    const int32_t* order; // nord, E_ordered
     const int32_t* group = nullptr;  // n, graph of each vertex (batched union), or null
     int64_t ngroups = 0;
+    int64_t nord = 0;     // E_ordered positions: m, or |E_s| <= m for a sampled edge set
 };
 
 // engines (each writes EdgeCounts for every edge id)
@@ -141,7 +142,6 @@ OrientState* orient_state_new();
 OrientParams& orient_params(OrientState* s);
 void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st, bool plan = true);
 void orient_count(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Timer* timer);
-void engine_edge(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t);
 void engine_orient(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t, OrientState** cache,
                    const OrientParams& prm);
 inline OrientParams params_from(const pgd_tuning& t) {
@@ -169,9 +169,8 @@ bool engine_dense_fits(const GraphView& g);  // may be requested (small enough)
 void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t);
 
 // epilogue: the 15 sums (and the micro columns when micro != nullptr),
-// processed in E_ordered order.
-void epilogue(pgd_ctx* ctx, const GraphView& g, EdgeCounts c, uint64_t* d_sums, uint64_t* d_micro,
-              uint32_t* d_flags);
+// processed in E_ordered order over its nord positions.
+void epilogue(pgd_ctx* ctx, const GraphView& g, EdgeCounts c, uint64_t* d_sums, uint64_t* d_micro);
 
 // launch-geometry helpers
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
@@ -190,17 +189,19 @@ struct pgd_graph {
     int32_t K = 0;
     pgd::DBuf<int32_t> core;
     bool have_order = false;
+    int64_t nord = 0;  // E_ordered positions (m after pgd_sort_edges; |E_s| for a sampled set)
     pgd::DBuf<int32_t> order;
     bool have_counts = false, have_micro = false;
     pgd::DBuf<uint32_t> T;
     pgd::DBuf<uint64_t> K4, C4, micro, sums;
-    pgd::DBuf<uint32_t> flags;
     int last_engine = 0;
     pgd::Timing timing;
     pgd::OrientState* orient = nullptr;
     pgd::OrientParams prm;
     pgd::GraphView view() const {
-        return pgd::GraphView{n, m, nnz, max_deg, row.get(), col.get(), src.get(), deg.get(),
-                              ev.get(), eu.get(), order.get()};
+        pgd::GraphView v{n, m, nnz, max_deg, row.get(), col.get(), src.get(), deg.get(), ev.get(), eu.get(),
+                         order.get()};
+        v.nord = nord;
+        return v;
     }
 };

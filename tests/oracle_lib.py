@@ -45,6 +45,8 @@ def lib():
         L.or_edge_arrays.restype = C.c_int64
         L.or_sort_edges.argtypes = [C.POINTER(CSR), P32, P32, C.c_int64, C.c_char_p, C.c_int, P32]
         L.or_sort_edges.restype = C.c_int
+        L.or_sort_edges_subset.argtypes = [C.POINTER(CSR), P32, P32, P32, C.c_int64, C.c_char_p, C.c_int, P32]
+        L.or_sort_edges_subset.restype = C.c_int
         L.or_decompose.argtypes = [C.POINTER(CSR), P32, P32, P32, C.c_int64, C.c_int64, C.c_int, PU,
                                    PU, PU, PU, PU, PU, PU]
         L.or_derive.argtypes = [C.c_int64, C.c_int64, PU, PU]
@@ -142,6 +144,16 @@ class OracleGraph:
         if rc != 0:
             raise ValueError(f"ordering {ordering} has no reproducible key")
         return out[:m]
+
+    def sort_edges_subset(self, ev, eu, es, ordering: str = "deg", s2l: bool = False):
+        """sort_edges over a sampled edge set E_s (graphlet_core.cpp:836-851)."""
+        es = np.ascontiguousarray(es, dtype=np.int32)
+        out = np.zeros(max(len(es), 1), dtype=np.int32)
+        rc = lib().or_sort_edges_subset(C.byref(self.c), _p(ev, C.c_int32), _p(eu, C.c_int32), _p(es, C.c_int32),
+                                        len(es), ordering.encode(), int(s2l), _p(out, C.c_int32))
+        if rc != 0:
+            raise ValueError(f"ordering {ordering} has no reproducible key")
+        return out[: len(es)]
 
     def decompose(self, ev, eu, order, micro: bool = True, threads: int = 0, count: int | None = None):
         cnt = len(order) if count is None else count

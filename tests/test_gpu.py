@@ -21,8 +21,7 @@ from tests.test_oracle import GOLDENS, GOLD, flags_to_args, golden_graph_path
 
 pytestmark = pytest.mark.gpu
 
-# the edge-centric engine is disabled (it hung on its first device run;
-# cause not yet identified), so only the oriented engine is exercised here
+# the two engines the library has: degree-oriented (sparse) and dense int8 MFMA
 ENGINES = ["orient", "dense"]
 DENSE_MAX_N = 8192
 
@@ -53,9 +52,10 @@ def micro_matrix(g) -> np.ndarray:
     ev, eu = g.edge_arrays()
     order = g.e_ordered()
     cols = [ev[order].astype(np.uint64) + np.uint64(1), eu[order].astype(np.uint64) + np.uint64(1)]
-    cols += [g.micro(c) for c in ("tri", "2_star", "local_4_clique", "local_4_chordal_cycle", "local_4_tailed_tris",
-                                  "local_4_cycle", "local_3_star", "local_4_path")]
-    return np.stack(cols, axis=1) if g.m else np.zeros((0, 10), np.uint64)
+    k = len(order)  # one row per E_ordered position
+    cols += [g.micro(c)[:k] for c in ("tri", "2_star", "local_4_clique", "local_4_chordal_cycle",
+                                      "local_4_tailed_tris", "local_4_cycle", "local_3_star", "local_4_path")]
+    return np.stack(cols, axis=1) if k else np.zeros((0, 10), np.uint64)
 
 
 def micro_sha(mat: np.ndarray) -> str:

@@ -164,3 +164,89 @@ def test_batched_workload_is_seeded():
     assert 200 < np.median(ns) < 500
     for n, s, d in a:
         assert np.all(s != d) and (len(s) == 0 or (s.max() < n and d.max() < n))
+
+
+REF_ARITH = os.path.join(ROOT, "oracle", "_ref", "ref_arith")
+
+
+def _ref_arith(lines: list[str]) -> list[list[int]]:
+    import subprocess
+    r = subprocess.run([REF_ARITH], input="\n".join(lines) + "\n", capture_output=True, text=True, timeout=120,
+                       check=True)
+    return [[int(x) for x in ln.split()] for ln in r.stdout.splitlines()]
+
+
+def _edge_sums(rng, n_inputs: int):
+    vals = []
+    for b in (52, 53, 54, 55, 60, 62, 63, 64):
+        for d in (-3, -2, -1, 0, 1, 2, 3, 5, 7):
+            vals.append(((1 << b) + d) % (1 << 64))
+    vals += [3 * (1 << 53) + 1, 3 * (1 << 53) + 2, 6 * (1 << 55) - 1, (1 << 64) - 1, (1 << 64) - 2048]
+    cases = []
+    for a in vals:
+        for n in (5, 2**21 + 7, 2**24, 2**31 - 1):
+            cases.append((n, 1000, [a] * 15))
+            cases.append((n, 3, [a if k % 2 else (a >> 3) for k in range(15)]))
+    for i in range(n_inputs):
+        n = int(rng.choice([int(rng.integers(0, 5000)), int(rng.integers(0, 2**22)), int(rng.integers(0, 2**31))]))
+        cases.append((n, int(rng.integers(0, max(1, n * 20))), _rand_sums(rng, big=i % 3 == 0)))
+    return cases
+
+
+@pytest.mark.skipif(not os.path.exists(REF_ARITH), reason="needs the reference compiler harness (build container)")
+def test_derive_matches_reference_build_flags():
+    """pgd_derive == the reference's own derivation expressions compiled with
+    the reference's -ffast-math flags (oracle/ref_arith.cpp), including the
+    values above 2^53 where x/3.0 and x*fl(1/3) could differ."""
+    cases = _edge_sums(np.random.default_rng(11), 5000)
+    ref = _ref_arith([f"D {n} {m} " + " ".join(str(x) for x in s) for n, m, s in cases])
+    for (n, m, s), want in zip(cases, ref):
+        assert pgd_amd.derive(n, m, s) == want, (n, m, s)
+
+
+def _rne53(p: int) -> int:
+    """Nearest binary64 value to p (ties to even), as an integer."""
+    if p < 1 << 53:
+        return p
+    sh = p.bit_length() - 53
+    q, rem = p >> sh, p & ((1 << sh) - 1)
+    hf = 1 << (sh - 1)
+    if rem > hf or (rem == hf and q & 1):
+        q += 1
+    return q << sh
+
+
+def _micro_emulation(T: int, K: int, C: int, dv: int, du: int):
+    """The device epilogue's integer steps for the chordal, 3-star and 4-path
+    columns (pgd_epilogue.hpp), restated for this check."""
+    M = (1 << 64) - 1
+    sv, su = dv - T - 1, du - T - 1
+    chordal = _rne53((_rne53(T * (T - 1)) >> 1) - _rne53(K))
+    star3 = _rne53((_rne53(sv * (sv - 1)) >> 1) + (_rne53(su * (su - 1)) >> 1))
+    path4 = (sv * su - C) & M
+    return [chordal, star3, path4]
+
+
+@pytest.mark.skipif(not os.path.exists(REF_ARITH), reason="needs the reference compiler harness (build container)")
+def test_micro_emulation_matches_reference_build_flags():
+    """The epilogue's emulation of the micro columns' binary64 steps ==
+    graphlet_core.cpp:1566-1575 compiled with the reference's flags, on
+    per-edge values up to and past 2^53 (T, Sv, Su up to 2^31)."""
+    rng = np.random.default_rng(5)
+    cases = []
+    for T in (0, 1, 2, 94906265, 94906266, 94906267, 94906268, 134217729, 2**30 + 3, 2**31 - 2):
+        for extra in (0, 1, 5, 94906267, 2**30):
+            dv = min(T + 1 + extra, 2**31 - 1)
+            du = T + 1 + int(rng.integers(0, max(1, 2**31 - T - 2)))
+            kmax = T * (T - 1) // 2
+            for K in sorted({0, min(1, kmax), kmax // 3, max(kmax - 7, 0), kmax}):  # K4 <= C(T, 2)
+                cases.append((T, K, int(rng.integers(0, 2**40)), dv, du))
+    for _ in range(3000):
+        T = int(rng.integers(0, 2**31 - 2))
+        dv = T + 1 + int(rng.integers(0, max(1, 2**31 - T - 2)))
+        du = T + 1 + int(rng.integers(0, max(1, 2**31 - T - 2)))
+        K = int(rng.integers(0, T * (T - 1) // 2 + 1))
+        cases.append((T, K, int(rng.integers(0, 2**50)), dv, du))
+    ref = _ref_arith([f"M {T} {K} {C} {dv} {du}" for T, K, C, dv, du in cases])
+    for c, want in zip(cases, ref):
+        assert _micro_emulation(*c) == want, c

@@ -256,3 +256,41 @@ def test_derive_wrap_cases_against_reference_golden(tmpdir_mod):
     res = run_pipeline(g, micro=False)
     assert res["totals"] == ent["totals"]
     assert res["sums"][12] < res["m"] * (res["n"] ** 2) // 2  # N8 wrapped
+
+
+# ---------------------------------------------------------------------------
+# sort_edges over a sampled edge set E_s (graphlet_core.cpp:836-851): the
+# oracle against goldens from the reference library (oracle/ref_subset.cpp)
+SUBSET = json.load(open(os.path.join(GOLD, "goldens_subset.json"))) if os.path.exists(
+    os.path.join(GOLD, "goldens_subset.json")) else {}
+
+
+def oracle_subset_run(path: str, flags: list[str], es: np.ndarray):
+    kw = flags_to_args(flags)
+    g = OracleGraph.from_file(path)
+    g.order_neighbors(kw["neigh"], kw["s2l_neigh"])
+    ev, eu = g.edge_arrays()
+    order = g.sort_edges_subset(ev, eu, es, kw["ordering"], kw["s2l"])
+    res = g.decompose(ev, eu, order, micro=True)
+    res.update(ev=ev, eu=eu, order=order)
+    res["totals"] = derive(g.n, g.m, res["sums"])
+    return res
+
+
+@pytest.mark.parametrize("key", sorted(SUBSET))
+def test_oracle_subset_matches_reference(key, tmp_path):
+    ent = SUBSET[key]
+    path = golden_graph_path(ent["graph"], GOLDENS[ent["graph"]], str(tmp_path))
+    data = np.load(os.path.join(GOLD, ent["file"]))
+    res = oracle_subset_run(path, ent["flags"], data["E_s"])
+    assert res["totals"] == ent["totals"]
+    want = data["micro"]
+    assert len(res["order"]) == ent["count"] == want.shape[0]
+    ev, eu, order = res["ev"], res["eu"], res["order"]
+    assert np.array_equal(ev[order] + 1, want[:, 0]) and np.array_equal(eu[order] + 1, want[:, 1])
+    assert np.array_equal(res["tri"], want[:, 2])
+    assert np.array_equal(res["k4"], want[:, 4])
+    assert np.array_equal(res["chordal"], want[:, 5])
+    assert np.array_equal(res["c4"], want[:, 7])
+    assert np.array_equal(res["star3"], want[:, 8])
+    assert np.array_equal(res["path4"], want[:, 9])
