@@ -150,6 +150,30 @@ pgd_status pgd_e_ordered(pgd_graph *g, int32_t *ids);
  * sums receives PGD_NSUMS values.  With PGD_MICRO the per-edge columns are
  * kept on the device (fetch with pgd_micro). */
 pgd_status pgd_decompose(pgd_graph *g, uint32_t flags, uint64_t *sums);
+/* Allocation-free decomposition of a small graph.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same result as pgd_decompose(g, flags | PGD_ENGINE_DENSE, ...) -- the
+ * dense int8-MFMA engine and the epilogue -- with every intermediate array
+ * (adjacency, products, per-edge counts) in one static device array of
+ * PGD_NOALLOC_SCRATCH_BYTES per GPU, whose address is taken once per device
+ * at pgd_open.  It calls no allocating routine.
+ *   g:       after pgd_preprocess and an edge order (pgd_sort_edges,
+ *            pgd_sort_edges_subset or pgd_set_e_ordered); at most
+ *            PGD_NOALLOC_MAX_N vertices, else PGD_ERR_UNSUPPORTED
+ *   flags:   0 or PGD_MICRO (anything else is PGD_ERR_INVALID)
+ *   d_sums:  device memory for PGD_NSUMS uint64 (cleared, then written)
+ *   d_micro: with PGD_MICRO, device memory for PGD_NMICRO x m uint64, column
+ *            c at d_micro + c*m, in E_ordered order as pgd_micro returns it
+ * Returns after the results are in place (the context's stream is
+ * synchronised).  Every call on one GPU shares that GPU's scratch array,
+ * which keeps the last call's contents: calls that overlap on one GPU (from
+ * several threads or contexts) race and must be serialised by the caller.
+ * The graph's own per-edge buffers (pgd_micro, pgd_edge_counts) are not
+ * touched. */
+#define PGD_NOALLOC_MAX_N 1024
+#define PGD_NOALLOC_SCRATCH_BYTES 36820992  /* 35.1 MiB per GPU, held until the process ends */
+pgd_status pgd_decompose_noalloc(pgd_graph *g, uint32_t flags, uint64_t *d_sums, uint64_t *d_micro);
 /* Download one micro column (m uint64 values in E_ordered order). */
 pgd_status pgd_micro(pgd_graph *g, int32_t column, uint64_t *out);
 /* Raw per-edge counts by edge id: T, K4 (4-cliques on the edge) and C4

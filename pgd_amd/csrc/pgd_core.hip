@@ -444,11 +444,18 @@ __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ or
     block_add_sums(acc, sums);
 }
 
+__global__ void k_zero_sums(uint64_t* __restrict__ sums) {
+    if (threadIdx.x < PGD_NSUMS) sums[threadIdx.x] = 0;
+}
+
 }  // namespace
 
 namespace pgd {
 void epilogue(pgd_ctx* ctx, const GraphView& g, EdgeCounts c, uint64_t* d_sums, uint64_t* d_micro) {
-    PGD_HIP(hipMemsetAsync(d_sums, 0, PGD_NSUMS * sizeof(uint64_t), ctx->stream));
+    // the sums start at zero (a fill kernel: the allocation-free entry point
+    // hands in a caller's buffer)
+    hipLaunchKernelGGL(k_zero_sums, dim3(1), dim3(64), 0, ctx->stream, d_sums);
+    PGD_HIP(hipGetLastError());
     if (g.m == 0 && g.nord == 0) return;
     int grid = std::min(grid_for(std::max(g.m, g.nord)), ctx->num_cus * 8);
     hipLaunchKernelGGL(k_epilogue, dim3(grid), dim3(256), 0, ctx->stream, g.order, g.ev, g.eu, g.row, c.T, c.K4,
@@ -611,6 +618,7 @@ pgd_status pgd_open(int32_t device, pgd_ctx** out) {
         PGD_HIP(hipDeviceGetDefaultMemPool(&pool, device));
         uint64_t keep = UINT64_MAX;
         PGD_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+        noalloc_scratch_init(device);
     });
     if (st != PGD_OK) {
         delete ctx;
@@ -1126,6 +1134,24 @@ pgd_status pgd_decompose(pgd_graph* g, uint32_t flags, uint64_t* sums) {
         g->last_engine = engine;
         g->have_counts = true;
         g->have_micro = micro;
+    });
+}
+
+pgd_status pgd_decompose_noalloc(pgd_graph* g, uint32_t flags, uint64_t* d_sums, uint64_t* d_micro) {
+    if (!g || !d_sums) return PGD_ERR_INVALID;
+    if (flags & ~PGD_MICRO) {
+        g->ctx->last_err = "unknown decomposition flag";
+        return PGD_ERR_INVALID;
+    }
+    if ((flags & PGD_MICRO) && !d_micro) return PGD_ERR_INVALID;
+    if (!g->prepped || !g->have_order) return PGD_ERR_STATE;
+    if (g->nord > g->m && (flags & PGD_MICRO)) {
+        g->ctx->last_err = "E_ordered has more than m positions; per-edge columns hold m";
+        return PGD_ERR_INVALID;
+    }
+    return guarded(g->ctx, [&] {
+        PGD_HIP(hipSetDevice(g->ctx->device));
+        decompose_noalloc(g->ctx, g->view(), d_sums, (flags & PGD_MICRO) ? d_micro : nullptr);
     });
 }
 

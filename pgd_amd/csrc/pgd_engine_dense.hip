@@ -15,6 +15,8 @@
 // base-128 digits (each an int8 in [0, 127]); every int32 accumulator stays
 // below n * 127 * 1 < 2^31 and the digits recombine exactly in int64.  All
 // operands are symmetric, so both MFMA operands are read along rows.
+#include <mutex>
+
 #include "pgd_orient.hpp"
 
 namespace pgd {
@@ -149,6 +151,73 @@ __global__ __launch_bounds__(256) void k_dense_edges(
     }
 }
 
+// The engine's arrays for one graph: A, its bit rows, the products S, P (two
+// digits), Q (two digits), the digits of S and T, and the per-edge counts
+// (the caller's, or scratch)
+struct DenseBufs {
+    int8_t *A, *s0, *s1, *t0, *t1;
+    int32_t *S, *P0, *P1, *Q0, *Q1;
+    uint32_t* bits;
+};
+
+// bytes of DenseBufs for padded size N, each array 256-byte aligned
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t dense_bytes(int64_t N) {
+    return 5 * align256((size_t)N * N) + 5 * align256(4 * (size_t)N * N) + align256((size_t)N * N / 8);
+}
+DenseBufs carve(unsigned char* base, int64_t N) {
+    const size_t b1 = align256((size_t)N * N), b4 = align256(4 * (size_t)N * N);
+    DenseBufs d;
+    unsigned char* p = base;
+    d.A = (int8_t*)p; p += b1;
+    d.s0 = (int8_t*)p; p += b1;
+    d.s1 = (int8_t*)p; p += b1;
+    d.t0 = (int8_t*)p; p += b1;
+    d.t1 = (int8_t*)p; p += b1;
+    d.S = (int32_t*)p; p += b4;
+    d.P0 = (int32_t*)p; p += b4;
+    d.P1 = (int32_t*)p; p += b4;
+    d.Q0 = (int32_t*)p; p += b4;
+    d.Q1 = (int32_t*)p; p += b4;
+    d.bits = (uint32_t*)p;
+    return d;
+}
+
+// the launch sequence; every array of d is written before it is read (A and
+// the bit rows are cleared first), so no state carries over between calls
+void dense_launch(pgd_ctx* ctx, const GraphView& g, const DenseBufs& d, EdgeCounts out, Timer* timer) {
+    hipStream_t s = ctx->stream;
+    const int64_t n = g.n, m = g.m;
+    const int64_t N = (n + kTile - 1) / kTile * kTile, NN = N * N;
+    const int Wd = (int)(N / 32);
+    if (timer) timer->begin("k_dense_adj");
+    const int gf = grid_cap(ctx, (NN / 4 + 255) / 256, 16);
+    hipLaunchKernelGGL(d_fill_u32, dim3(gf), dim3(256), 0, s, (uint32_t*)d.A, NN / 4);
+    hipLaunchKernelGGL(d_fill_u32, dim3(grid_cap(ctx, (N * Wd + 255) / 256, 16)), dim3(256), 0, s, d.bits, N * Wd);
+    hipLaunchKernelGGL(k_dense_adj, dim3(grid_cap(ctx, (n * 64 + 255) / 256, 16)), dim3(256), 0, s, g.row, g.col, n,
+                       N, Wd, d.A, d.bits);
+    PGD_HIP(hipGetLastError());
+    if (timer) timer->end();
+    const dim3 gg((unsigned)(N / kTile), (unsigned)(N / kTile));
+    if (timer) timer->begin("k_gemm_i8");
+    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, d.A, d.A, d.S, N);
+    PGD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_dense_digits, dim3(grid_cap(ctx, (NN + 255) / 256, 16)), dim3(256), 0, s, d.S, d.A, NN,
+                       d.s0, d.s1, d.t0, d.t1);
+    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, d.A, d.s0, d.P0, N);
+    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, d.A, d.s1, d.P1, N);
+    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, d.A, d.t0, d.Q0, N);
+    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, d.A, d.t1, d.Q1, N);
+    PGD_HIP(hipGetLastError());
+    if (timer) timer->end();
+    if (timer) timer->begin("k_dense_edges");
+    if (m)
+        hipLaunchKernelGGL(k_dense_edges, dim3(grid_cap(ctx, (m * 64 + 255) / 256, 16)), dim3(256), 0, s, g.ev, g.eu,
+                           g.row, m, N, Wd, d.S, d.P0, d.P1, d.Q0, d.Q1, d.bits, out.T, out.K4, out.C4);
+    PGD_HIP(hipGetLastError());
+    if (timer) timer->end();
+}
+
 }  // namespace
 
 bool engine_dense_fits(const GraphView& g) { return g.n <= kDenseMaxN; }
@@ -163,54 +232,13 @@ bool engine_dense_ok(const GraphView& g) {
 
 void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm) {
     PGD_CHECK(g.n <= kDenseMaxN, PGD_ERR_UNSUPPORTED, "graph too large for the dense engine");
-    hipStream_t s = ctx->stream;
     const int64_t n = g.n, m = g.m;
     const int64_t N = (n + kTile - 1) / kTile * kTile, NN = N * N;
-    const int Wd = (int)(N / 32);
-    DBuf<int8_t> A, s0, s1, t0, t1;
-    DBuf<int32_t> S, P0, P1, Q0, Q1;
-    DBuf<uint32_t> bits;
-    A.alloc(ctx, NN);
-    bits.alloc(ctx, N * Wd);
-    S.alloc(ctx, NN);
+    DBuf<unsigned char> buf;
+    buf.alloc(ctx, dense_bytes(N));
     Timer timer{ctx};
-    timer.begin("k_dense_adj");
-    const int gf = grid_cap(ctx, (NN / 4 + 255) / 256, 16);
-    hipLaunchKernelGGL(d_fill_u32, dim3(gf), dim3(256), 0, s, (uint32_t*)A.get(), NN / 4);
-    hipLaunchKernelGGL(d_fill_u32, dim3(grid_cap(ctx, (N * Wd + 255) / 256, 16)), dim3(256), 0, s, bits.get(),
-                       N * Wd);
-    hipLaunchKernelGGL(k_dense_adj, dim3(grid_cap(ctx, (n * 64 + 255) / 256, 16)), dim3(256), 0, s, g.row, g.col, n,
-                       N, Wd, A.get(), bits.get());
-    PGD_HIP(hipGetLastError());
-    timer.end();
-    const dim3 gg((unsigned)(N / kTile), (unsigned)(N / kTile));
-    timer.begin("k_gemm_i8");
-    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, A.get(), A.get(), S.get(), N);
-    PGD_HIP(hipGetLastError());
-    s0.alloc(ctx, NN);
-    s1.alloc(ctx, NN);
-    t0.alloc(ctx, NN);
-    t1.alloc(ctx, NN);
-    hipLaunchKernelGGL(k_dense_digits, dim3(grid_cap(ctx, (NN + 255) / 256, 16)), dim3(256), 0, s, S.get(), A.get(),
-                       NN, s0.get(), s1.get(), t0.get(), t1.get());
-    P0.alloc(ctx, NN);
-    P1.alloc(ctx, NN);
-    Q0.alloc(ctx, NN);
-    Q1.alloc(ctx, NN);
-    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, A.get(), s0.get(), P0.get(), N);
-    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, A.get(), s1.get(), P1.get(), N);
-    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, A.get(), t0.get(), Q0.get(), N);
-    hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, A.get(), t1.get(), Q1.get(), N);
-    PGD_HIP(hipGetLastError());
-    timer.end();
-    timer.begin("k_dense_edges");
-    if (m)
-        hipLaunchKernelGGL(k_dense_edges, dim3(grid_cap(ctx, (m * 64 + 255) / 256, 16)), dim3(256), 0, s, g.ev, g.eu,
-                           g.row, m, N, Wd, S.get(), P0.get(), P1.get(), Q0.get(), Q1.get(), bits.get(), out.T,
-                           out.K4, out.C4);
-    PGD_HIP(hipGetLastError());
-    timer.end();
-    PGD_HIP(hipStreamSynchronize(s));
+    dense_launch(ctx, g, carve(buf.get(), N), out, &timer);
+    PGD_HIP(hipStreamSynchronize(ctx->stream));
     // work models: the MFMA products are 5 x 2 N^3 int8 operations; the adjacency
     // pass writes N^2 + N^2/8 bytes; the edge pass reads its five int32 entries
     // and, per common neighbour, three bit rows
@@ -229,6 +257,69 @@ void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm) 
     }
     tm->dominant_ms = best;
     tm->dominant_launches = 1;
+}
+
+}  // namespace pgd
+
+// ---------------------------------------------------------------------------
+// Allocation-free decomposition of small graphs (pgd_decompose_noalloc).
+// Every intermediate array of the dense engine and the per-edge counts live
+// in one static device array per GPU, sized for n <= PGD_NOALLOC_MAX_N.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int64_t kNoAllocMaxN = PGD_NOALLOC_MAX_N;
+constexpr int64_t kNoAllocMaxM = kNoAllocMaxN * (kNoAllocMaxN - 1) / 2;
+constexpr size_t kNoAllocCounts =
+    pgd::align256(4 * (size_t)kNoAllocMaxM) + 2 * pgd::align256(8 * (size_t)kNoAllocMaxM);
+constexpr size_t kNoAllocBytes = pgd::dense_bytes(kNoAllocMaxN) + kNoAllocCounts;
+static_assert(kNoAllocBytes == PGD_NOALLOC_SCRATCH_BYTES, "pgd_hip.h states the scratch size");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};  // device address of `scratch` on each GPU
+}  // namespace
+
+namespace pgd {
+
+// once per device, with that device current: the address of this GPU's copy
+// of `scratch` (host code never passes the symbol itself to a kernel)
+void noalloc_scratch_init(int device) {
+    PGD_CHECK(device >= 0 && device < kMaxDevices, PGD_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return;
+    int cur = 0;
+    PGD_HIP(hipGetDevice(&cur));
+    PGD_HIP(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(cur);
+    PGD_HIP(e);
+    PGD_CHECK(ptr != nullptr, PGD_ERR_HIP, "no device address for the scratch array");
+    g_scratch[device] = (unsigned char*)ptr;
+}
+
+void decompose_noalloc(pgd_ctx* ctx, const GraphView& g, uint64_t* d_sums, uint64_t* d_micro) {
+    PGD_CHECK(g.n <= kNoAllocMaxN, PGD_ERR_UNSUPPORTED,
+              "pgd_decompose_noalloc: more than PGD_NOALLOC_MAX_N vertices");
+    PGD_CHECK(g.m <= kNoAllocMaxM, PGD_ERR_UNSUPPORTED, "pgd_decompose_noalloc: too many edges");
+    unsigned char* base;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = g_scratch[ctx->device];
+    }
+    PGD_CHECK(base != nullptr, PGD_ERR_STATE, "scratch not initialised on this device");
+    const int64_t N = (g.n + kTile - 1) / kTile * kTile;
+    const DenseBufs d = carve(base, N);
+    unsigned char* cp = base + dense_bytes(kNoAllocMaxN);
+    EdgeCounts c{(uint32_t*)cp, (uint64_t*)(cp + align256(4 * (size_t)kNoAllocMaxM)),
+                 (uint64_t*)(cp + align256(4 * (size_t)kNoAllocMaxM) + align256(8 * (size_t)kNoAllocMaxM))};
+    if (g.m > 0) dense_launch(ctx, g, d, c, nullptr);
+    epilogue(ctx, g, c, d_sums, d_micro);
+    PGD_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 }  // namespace pgd

@@ -167,6 +167,10 @@ void orient_state_free(pgd_ctx* ctx, OrientState* s);
 bool engine_dense_ok(const GraphView& g);    // chosen automatically (above 80% density, small enough)
 bool engine_dense_fits(const GraphView& g);  // may be requested (small enough)
 void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t);
+// allocation-free path (pgd_decompose_noalloc): the dense engine and the
+// epilogue over the static scratch array of this GPU
+void noalloc_scratch_init(int device);
+void decompose_noalloc(pgd_ctx* ctx, const GraphView& g, uint64_t* d_sums, uint64_t* d_micro);
 
 // epilogue: the 15 sums (and the micro columns when micro != nullptr),
 // processed in E_ordered order over its nord positions.
