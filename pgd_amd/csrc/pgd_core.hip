@@ -29,17 +29,48 @@ using namespace pgd;
 // ===========================================================================
 namespace pgd {
 
+// Size class of a request: the next value m * 2^k with m in [8, 15] (at most
+// 12.5% above the request), at least 256 bytes.
+size_t size_class(size_t bytes) {
+    if (bytes <= 256) return 256;
+    int k = 0;
+    while ((bytes - 1) >> (k + 3) >= 8) k++;  // mantissa of bytes-1 in [8, 15] at scale 2^k
+    return (((bytes - 1) >> k) + 1) << k;
+}
+
+// cached blocks beyond this many bytes go back to the runtime pool
+constexpr size_t kCacheLimit = (size_t)48 << 30;
+
 void* dmalloc(pgd_ctx* ctx, size_t bytes) {
     if (ctx->mem_cap > 0 && ctx->mem_used.load() + (int64_t)bytes > ctx->mem_cap)
         throw Error{PGD_ERR_OOM, "device memory cap exceeded (" + std::to_string(ctx->mem_used.load() + bytes) +
                                      " > " + std::to_string(ctx->mem_cap) + " bytes)"};
-    // stream-ordered: allocation and release follow the context's stream, so
-    // freeing a buffer never waits for work on other streams
+    const size_t cls = size_class(bytes);
     void* p = nullptr;
-    hipError_t e = hipMallocAsync(&p, bytes, ctx->stream);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error{PGD_ERR_OOM, "hipMallocAsync(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e)};
+    {
+        // a cached block of this class: released earlier in the order of
+        // this context's stream, so work enqueued on it from now on follows
+        // every use of the block's previous owner
+        std::lock_guard<std::mutex> lk(ctx->cache_mu);
+        auto it = ctx->cache.find(cls);
+        if (it != ctx->cache.end()) {
+            p = it->second;
+            ctx->cache.erase(it);
+            ctx->cached_bytes -= cls;
+        }
+    }
+    if (!p) {
+        hipError_t e = hipMallocAsync(&p, cls, ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            // out of memory with blocks parked in the cache: give them back and retry
+            release_cached(ctx);
+            e = hipMallocAsync(&p, cls, ctx->stream);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error{PGD_ERR_OOM, "hipMallocAsync(" + std::to_string(cls) + ") failed: " + hipGetErrorString(e)};
+        }
     }
     ctx->mem_used += (int64_t)bytes;
     return p;
@@ -47,8 +78,34 @@ void* dmalloc(pgd_ctx* ctx, size_t bytes) {
 
 void dfree(pgd_ctx* ctx, void* p, size_t bytes) {
     if (!p) return;
-    (void)hipFreeAsync(p, ctx->stream);
+    const size_t cls = size_class(bytes);
     ctx->mem_used -= (int64_t)bytes;
+    std::vector<void*> spill;
+    {
+        std::lock_guard<std::mutex> lk(ctx->cache_mu);
+        ctx->cache.emplace(cls, p);
+        ctx->cached_bytes += cls;
+        // over the limit: the largest blocks go back first
+        while (ctx->cached_bytes > kCacheLimit && !ctx->cache.empty()) {
+            auto last = std::prev(ctx->cache.end());
+            ctx->cached_bytes -= last->first;
+            spill.push_back(last->second);
+            ctx->cache.erase(last);
+        }
+    }
+    for (void* q : spill) (void)hipFreeAsync(q, ctx->stream);
+}
+
+void release_cached(pgd_ctx* ctx) {
+    std::vector<void*> all;
+    {
+        std::lock_guard<std::mutex> lk(ctx->cache_mu);
+        for (auto& kv : ctx->cache) all.push_back(kv.second);
+        ctx->cache.clear();
+        ctx->cached_bytes = 0;
+    }
+    for (void* q : all) (void)hipFreeAsync(q, ctx->stream);
+    if (!all.empty()) (void)hipStreamSynchronize(ctx->stream);
 }
 
 unsigned long long* pinned_words(pgd_ctx* ctx) {
@@ -632,6 +689,7 @@ pgd_status pgd_close(pgd_ctx* ctx) {
     if (!ctx) return PGD_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
     if (ctx->tmp) dfree(ctx, ctx->tmp, ctx->tmp_bytes);
+    release_cached(ctx);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->stream) {
         (void)hipStreamSynchronize(ctx->stream);

@@ -6,6 +6,8 @@
 
 #include <atomic>
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -46,6 +48,13 @@ struct pgd_ctx {
     int tiled_per_cu[3][9] = {};   // occupancy of the tiled wedge kernel by (block size, tile), 0 = not asked yet
     int64_t mem_cap = 0;   // 0: no cap
     std::atomic<int64_t> mem_used{0};  // also released from other threads (served results)
+    // released device blocks kept for reuse by size class (see dmalloc):
+    // reuse follows the context's stream as the runtime pool's would, without
+    // a runtime call, whose pool lock stalls concurrent contexts for
+    // milliseconds at a time under the served form's allocation rate
+    std::mutex cache_mu;
+    std::multimap<size_t, void*> cache;
+    size_t cached_bytes = 0;
     std::string last_err;
     // rocPRIM temporary storage, grown on demand
     void* tmp = nullptr;
@@ -60,6 +69,7 @@ namespace pgd {
 
 void* dmalloc(pgd_ctx* ctx, size_t bytes);
 void dfree(pgd_ctx* ctx, void* p, size_t bytes);
+void release_cached(pgd_ctx* ctx);  // return every cached block to the runtime pool
 void* tmp_storage(pgd_ctx* ctx, size_t bytes);
 unsigned long long* pinned_words(pgd_ctx* ctx);  // device address of ctx->pin
 
