@@ -134,6 +134,39 @@ def cpu_baseline(g, target_s: float, threads: int) -> dict:
     }
 
 
+def bench_micro_writer(g) -> dict:
+    """The .micro table of the last decomposition, formatted on the device
+    (pgd_write_micro): to /dev/null (device formatting, pinned copies to the
+    host, write calls) and to a file in the temporary directory."""
+    import tempfile
+    out = {}
+    fd = os.open(os.devnull, os.O_WRONLY)
+    try:
+        g.write_micro_fd(fd)  # warm-up (buffers)
+        t0 = time.perf_counter()
+        nb = g.write_micro_fd(fd)
+        dt = time.perf_counter() - t0
+    finally:
+        os.close(fd)
+    out["bytes"] = nb
+    out["rows"] = g.m
+    out["devnull_s"] = dt
+    out["devnull_GB_per_s"] = nb / dt / 1e9
+    path = os.path.join(tempfile.gettempdir(), f"pgd_bench_{os.getpid()}.micro")
+    try:
+        t0 = time.perf_counter()
+        g.write_micro_stats(path)
+        dt = time.perf_counter() - t0
+        out["file_s"] = dt
+        out["file_GB_per_s"] = nb / dt / 1e9
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+    log(f"[bench] micro writer: {nb / 1e9:.2f} GB, {out['devnull_GB_per_s']:.1f} GB/s to /dev/null, "
+        f"{out['file_GB_per_s']:.1f} GB/s to a file")
+    return out
+
+
 def bench_dense(pgd_amd, ctx) -> dict:
     """Dense tier on G(500, 0.9) and G(2000, 0.5): device time of the int8
     MFMA engine and of the sparse oriented engine on the same graph, and the
@@ -199,13 +232,16 @@ def bench_batched(pgd_amd, ctx, count: int = 8192) -> dict:
     out = {"graphs": count, "edges": M, "vertices": b.n_total,
            "workload": "batched: BA(m in [2,8]) / G(n,p) avg degree [4,32], n log-uniform [20,5000], seed 1",
            "create_s": t_create, "first_run_s": t_first, "replay_ms": dev, "replay_wall_s": wall,
-           "graphs_per_s": count / wall, "edges_per_s": M / wall,
+           "graphs_per_s": count / (t_create + t_first), "edges_per_s": M / (t_create + t_first),
+           "graphs_per_s_basis": "end to end: batch creation (upload, union build, preprocessing, plan, capture) "
+                                 "plus its first run",
            "one_graph_per_launch_graphs_per_s": plain_gps, "one_graph_per_launch_sample": len(idx),
-           "speedup_vs_one_graph_per_launch": (count / wall) / plain_gps,
-           "end_to_end_graphs_per_s": count / (t_create + t_first),
+           "speedup_vs_one_graph_per_launch": (count / (t_create + t_first)) / plain_gps,
+           "replay_graphs_per_s": count / wall, "replay_speedup_vs_one_graph_per_launch": (count / wall) / plain_gps,
            "tuning": best.as_dict(), "autotuned_ms": tune_ms}
     b.free()
-    log(f"[bench] batched: {out['graphs_per_s']:.0f} graphs/s vs {plain_gps:.0f} one-graph-per-launch")
+    log(f"[bench] batched: {out['graphs_per_s']:.0f} graphs/s end to end ({out['replay_graphs_per_s']:.0f} replayed) "
+        f"vs {plain_gps:.0f} one-graph-per-launch")
     return out
 
 
@@ -248,6 +284,50 @@ def bench_served(pgd_amd, ctx, rate: float = 2000.0, seconds: float = 4.0, large
            "edges_per_s": st["edges"] / st["elapsed_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
            "batches": st["batches"], "elapsed_s": st["elapsed_s"]}
     log(f"[bench] served: {out['graphs_per_s']:.0f} graphs/s p50 {out['p50_ms']:.1f} ms p99 {out['p99_ms']:.1f} ms")
+    # capacity: the same request mix submitted as fast as the host can, so the
+    # queue never runs dry; graphs/s is then what the server can finish
+    srv = pgd_amd.Server(ctx=ctx, hbm_budget=96 << 30)
+    rng = np.random.default_rng(3)
+    ncap, nl2 = 16000, 0
+    for i in range(ncap):
+        if rng.random() < large_frac:
+            g = large[nl2 % 2]
+            nl2 += 1
+        else:
+            g = pool[i % len(pool)]
+        srv.submit(*g, micro=True)
+        if i % 256 == 255:
+            for rid in srv.poll():
+                srv.release(rid)
+    srv.drain(600000)
+    for rid in srv.poll():
+        srv.release(rid)
+    st = srv.stats()
+    srv.free()
+    out["capacity"] = {"submitted": ncap, "large": nl2, "graphs_per_s": st["finished"] / st["elapsed_s"],
+                       "edges_per_s": st["edges"] / st["elapsed_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                       "batches": st["batches"], "elapsed_s": st["elapsed_s"]}
+    log(f"[bench] served capacity: {out['capacity']['graphs_per_s']:.0f} graphs/s")
+    # one graph per launch on the same mix: the plain path per request (build,
+    # preprocessing, E_ordered, counting with micro columns), timed on 256 pool
+    # graphs and on each large graph twice, weighted 99:1
+    def plain(n, s, d):
+        t0 = time.perf_counter()
+        g = pgd_amd.GraphletCore(s, d, n=n, ctx=ctx)
+        g.sort_edges("deg")
+        g.decompose(micro=True)
+        g.free()
+        return time.perf_counter() - t0
+    plain(*pool[0])
+    t_small = float(np.mean([plain(*pool[i]) for i in range(0, len(pool), max(1, len(pool) // 256))]))
+    t_large = float(np.mean([plain(*large[j % 2]) for j in range(4)]))
+    mix = 1.0 / ((1 - large_frac) * t_small + large_frac * t_large)
+    out["one_graph_per_launch_same_mix_graphs_per_s"] = mix
+    out["one_graph_per_launch_small_ms"] = t_small * 1e3
+    out["one_graph_per_launch_large_ms"] = t_large * 1e3
+    out["capacity_vs_one_graph_per_launch_same_mix"] = out["capacity"]["graphs_per_s"] / mix
+    log(f"[bench] one graph per launch on the served mix: {mix:.0f} graphs/s "
+        f"(small {t_small * 1e3:.2f} ms, R-MAT-18 {t_large * 1e3:.1f} ms)")
     return out
 
 
@@ -277,8 +357,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sections", default="dense,batched,served",
-                    help="extra measurements after the headline: any of dense,batched,served (or none)")
+    ap.add_argument("--sections", default="io,dense,batched,served",
+                    help="extra measurements after the headline: any of io,dense,batched,served (or none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (scale for the PMC byte counters)")
     ap.add_argument("--served-only", action="store_true",
@@ -371,6 +451,11 @@ def main() -> int:
         "survey_model": {"bytes": B, "achieved_GBs": B / dt / 1e9, "frac_of_hbm_peak": B / dt / 1e9 / HBM_PEAK_GBS},
         "totals": dict(zip(pgd_amd.TOTAL_NAMES, totals)),
     }
+    if "io" in args.sections.split(","):
+        try:
+            out["micro_writer"] = bench_micro_writer(g)
+        except Exception as e:  # a section must not hide the headline
+            out["micro_writer"] = {"error": repr(e)}
     for sec, fn in (("dense", bench_dense), ("batched", bench_batched)):
         if sec in args.sections.split(","):
             try:
@@ -379,6 +464,9 @@ def main() -> int:
                 out[sec] = {"error": repr(e)}
     if served is not None:
         out["served"] = served
+        plain = out.get("batched", {}).get("one_graph_per_launch_graphs_per_s")
+        if plain and "capacity" in served:
+            served["capacity_vs_one_graph_per_launch"] = served["capacity"]["graphs_per_s"] / plain
     if not args.no_cpu_baseline:
         try:
             threads = min(os.cpu_count() or 1, 16)

@@ -176,6 +176,17 @@ pgd_status pgd_decompose(pgd_graph *g, uint32_t flags, uint64_t *sums);
 pgd_status pgd_decompose_noalloc(pgd_graph *g, uint32_t flags, uint64_t *d_sums, uint64_t *d_micro);
 /* Download one micro column (m uint64 values in E_ordered order). */
 pgd_status pgd_micro(pgd_graph *g, int32_t column, uint64_t *out);
+/* The per-edge table as text, formatted on the device and written to the
+ * file descriptor fd (write_micro_stats / get_micro_stats,
+ * graphlet_core.cpp:1668-1761, 1780-1827): `prefix` (the file's "% ", or
+ * "" for the stdout dump), the column header, then one row per E_ordered
+ * position:  [e_v+1,e_u+1,]tri,2-star,4-clique,4-chordal-cycle,
+ * 4-tailed-triangle,4-cycle,3-star,4-path  (the ids with output_id).
+ * Needs the micro columns of the last pgd_decompose (PGD_ERR_STATE
+ * otherwise).  *bytes (may be NULL) receives the bytes written; a failed
+ * write(2) is PGD_ERR_INVALID.  Device memory: about 450 MB of chunk
+ * buffers; host: two pinned chunk buffers of about 220 MB. */
+pgd_status pgd_write_micro(pgd_graph *g, int32_t fd, int32_t output_id, const char *prefix, int64_t *bytes);
 /* Raw per-edge counts by edge id: T, K4 (4-cliques on the edge) and C4
  * (induced 4-cycles on the edge); any pointer may be NULL. */
 pgd_status pgd_edge_counts(pgd_graph *g, uint64_t *tri, uint64_t *k4, uint64_t *c4);

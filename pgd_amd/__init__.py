@@ -336,6 +336,23 @@ class GraphletCore:
         lib().pgd_last_timing(self.h, C.byref(a), C.byref(b), C.byref(k))
         return a.value, b.value, k.value
 
+    def write_micro_fd(self, fd: int, output_id: bool = True, prefix: str = "% ") -> int:
+        """The micro table as text, formatted on the device, to a file
+        descriptor (pgd_write_micro); returns the bytes written."""
+        n = C.c_int64()
+        check(lib().pgd_write_micro(self.h, fd, int(output_id), prefix.encode(), C.byref(n)), self.ctx.h,
+              "pgd_write_micro")
+        return n.value
+
+    def write_micro_stats(self, filename: str, output_id: bool = True) -> int:
+        """write_micro_stats (graphlet_core.cpp:1668-1761), device formatted."""
+        import os
+        fd = os.open(filename, os.O_WRONLY | os.O_CREAT | os.O_TRUNC, 0o644)
+        try:
+            return self.write_micro_fd(fd, output_id, "% ")
+        finally:
+            os.close(fd)
+
     # -- output formats (graphlet_core.cpp:2061-2084, 2138-2162, 1668-1761) --
     def macro_text(self) -> str:
         return "".join(f"{k} = {v}\n" for k, v in self.totals.items())

@@ -33,7 +33,7 @@ EXPORTS = [
     "pgd_open", "pgd_close", "pgd_status_str", "pgd_set_mem_cap", "pgd_mem_in_use", "pgd_last_error",
     "pgd_graph_from_coo", "pgd_graph_from_csr", "pgd_graph_free", "pgd_graph_info", "pgd_graph_csr", "pgd_preprocess",
     "pgd_edge_arrays", "pgd_compute_cores", "pgd_sort_edges", "pgd_sort_edges_subset",
-    "pgd_set_e_ordered", "pgd_e_ordered_count", "pgd_e_ordered", "pgd_decompose", "pgd_decompose_noalloc", "pgd_micro",
+    "pgd_set_e_ordered", "pgd_e_ordered_count", "pgd_e_ordered", "pgd_decompose", "pgd_decompose_noalloc", "pgd_micro", "pgd_write_micro",
     "pgd_edge_counts", "pgd_last_engine", "pgd_last_timing", "pgd_last_phase", "pgd_derive", "pgd_batch_create", "pgd_batch_run",
     "pgd_batch_micro", "pgd_batch_info", "pgd_batch_free", "pgd_batch_set_tuning", "pgd_batch_get_tuning",
     "pgd_batch_autotune", "pgd_batch_last_ms", "pgd_tuning_default", "pgd_graph_set_tuning", "pgd_graph_get_tuning",
@@ -96,6 +96,7 @@ def lib() -> C.CDLL:
         "pgd_decompose": (I32, [P, U32, PU64]),
         "pgd_decompose_noalloc": (I32, [P, U32, C.c_void_p, C.c_void_p]),
         "pgd_micro": (I32, [P, I32, PU64]),
+        "pgd_write_micro": (I32, [P, I32, I32, C.c_char_p, PI64]),
         "pgd_edge_counts": (I32, [P, PU64, PU64, PU64]),
         "pgd_last_engine": (I32, [P]),
         "pgd_last_timing": (I32, [P, C.POINTER(D), C.POINTER(D), PI32]),

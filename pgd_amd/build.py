@@ -41,6 +41,7 @@ HIP_SOURCES = [
     "pgd_engine_wedge.hip",
     "pgd_engine_dense.hip",
     "pgd_batch.hip",
+    "pgd_writer.hip",
 ]
 HOST_SOURCES = ["pgd_reader.cpp", "pgd_gen.cpp", "pgd_derive.cpp"]
 DRIVER_SOURCES = ["graphlet_core.cpp", "pgd_main.cpp"]

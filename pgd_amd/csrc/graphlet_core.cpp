@@ -9,6 +9,9 @@
 #include "graphlet_core.hpp"
 
 #include <algorithm>
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -301,12 +304,23 @@ void graphlet_core::print_micro_stats(bool output_id, std::string delim) {
     print_line(80);
     std::cout << "\n\nMicro graphlet statistics" << std::endl;
     print_line(80);
-    std::cout << get_micro_stats(output_id, delim) << std::endl;
+    if (delim == ",") {
+        // formatted on the device and written straight to stdout
+        std::cout.flush();
+        std::fflush(stdout);
+        check(pgd_write_micro(g_, 1, output_id ? 1 : 0, "", nullptr), "print_micro_stats");
+        std::cout << std::endl;
+    } else {
+        std::cout << get_micro_stats(output_id, delim) << std::endl;
+    }
 }
 
 void graphlet_core::write_micro_stats(std::string& filename, bool output_id) {
-    std::ofstream f(filename.c_str(), std::ios::binary);
-    f << "% " << get_micro_stats(output_id, ",");
+    const int fd = ::open(filename.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (fd < 0) return;  // as an ofstream that failed to open: nothing is written
+    const pgd_status st = pgd_write_micro(g_, fd, output_id ? 1 : 0, "% ", nullptr);
+    ::close(fd);
+    check(st, "write_micro_stats");
 }
 
 namespace {

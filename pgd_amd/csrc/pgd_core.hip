@@ -1224,6 +1224,15 @@ pgd_status pgd_micro(pgd_graph* g, int32_t column, uint64_t* out) {
     });
 }
 
+pgd_status pgd_write_micro(pgd_graph* g, int32_t fd, int32_t output_id, const char* prefix, int64_t* bytes) {
+    if (!g || fd < 0) return PGD_ERR_INVALID;
+    if (!g->have_micro) return PGD_ERR_STATE;
+    return guarded(g->ctx, [&] {
+        PGD_HIP(hipSetDevice(g->ctx->device));
+        write_micro_text(g->ctx, g->view(), g->micro.get(), fd, output_id != 0, prefix, bytes);
+    });
+}
+
 pgd_status pgd_edge_counts(pgd_graph* g, uint64_t* tri, uint64_t* k4, uint64_t* c4) {
     if (!g) return PGD_ERR_INVALID;
     if (!g->have_counts) return PGD_ERR_STATE;

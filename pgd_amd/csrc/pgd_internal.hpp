@@ -186,6 +186,10 @@ void decompose_noalloc(pgd_ctx* ctx, const GraphView& g, uint64_t* d_sums, uint6
 // processed in E_ordered order over its nord positions.
 void epilogue(pgd_ctx* ctx, const GraphView& g, EdgeCounts c, uint64_t* d_sums, uint64_t* d_micro);
 
+// the .micro text of a decomposition's micro columns, written to fd
+void write_micro_text(pgd_ctx* ctx, const GraphView& g, const uint64_t* micro, int fd, bool output_id,
+                      const char* prefix, int64_t* written);
+
 // launch-geometry helpers
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
