@@ -167,6 +167,52 @@ def bench_micro_writer(g) -> dict:
     return out
 
 
+def bench_reader(pgd_amd, ctx, n, s, d, g) -> dict:
+    """read_graph straight to a device CSR (pgd_graph_from_file: file text to
+    CSR in reader order, as the reference's reader leaves it before
+    preprocessing) on the R-MAT-22 lines written as a .mtx file, and as an
+    edge list (the same text; its header line reads as a self loop), which
+    takes the device renumbering by first appearance.  All host threads, and
+    one thread for comparison; the file is in the page cache (just written)."""
+    import shutil
+    import tempfile
+    out = {"threads": os.cpu_count()}
+    base = os.path.join(tempfile.gettempdir(), f"pgd_bench_{os.getpid()}")
+    mtx, txt = base + ".mtx", base + ".txt"
+    try:
+        pgd_amd.write_mtx(mtx, n, s, d)
+        shutil.copyfile(mtx, txt)
+        out["file_bytes"] = os.path.getsize(mtx)
+        ref = pgd_amd.GraphletCore(s, d, n=n, ctx=ctx, preprocess=False)  # the host-COO route, reader order
+        row0, col0 = ref.csr()
+        deg0 = np.sort(np.diff(row0))
+        ref.free()
+        for name, path in (("mtx", mtx), ("edge_list", txt)):
+            res = {}
+            for label, threads in (("all_threads", 0), ("one_thread", 1)):
+                t0 = time.perf_counter()
+                h = pgd_amd.GraphletCore.from_file(path, ctx=ctx, threads=threads, preprocess=False)
+                dt = time.perf_counter() - t0
+                if label == "all_threads":
+                    r, c = h.csr()
+                    res["edges"] = h.m
+                    if name == "mtx":  # the same ids and lines: the same CSR as the host route
+                        res["same_csr_as_host_route"] = bool(np.array_equal(r, row0) and np.array_equal(c, col0))
+                    else:  # renumbered ids: the same degree sequence
+                        res["same_degrees_as_host_route"] = bool(np.array_equal(np.sort(np.diff(r)), deg0))
+                h.free()
+                res[label + "_s"] = dt
+                res[label + "_lines_per_s"] = len(s) / dt
+            out[name] = res
+            log(f"[bench] reader {name}: {res['all_threads_s']:.2f} s with all threads, "
+                f"{res['one_thread_s']:.2f} s with one")
+    finally:
+        for p in (mtx, txt):
+            if os.path.exists(p):
+                os.remove(p)
+    return out
+
+
 def bench_dense(pgd_amd, ctx) -> dict:
     """Dense tier on G(500, 0.9) and G(2000, 0.5): device time of the int8
     MFMA engine and of the sparse oriented engine on the same graph, and the
@@ -456,6 +502,10 @@ def main() -> int:
             out["micro_writer"] = bench_micro_writer(g)
         except Exception as e:  # a section must not hide the headline
             out["micro_writer"] = {"error": repr(e)}
+        try:
+            out["reader"] = bench_reader(pgd_amd, ctx, n, s, d, g)
+        except Exception as e:  # a section must not hide the headline
+            out["reader"] = {"error": repr(e)}
     for sec, fn in (("dense", bench_dense), ("batched", bench_batched)):
         if sec in args.sections.split(","):
             try:

@@ -94,6 +94,15 @@ pgd_status pgd_graph_from_coo(pgd_ctx *ctx, int64_t n, int64_t nlines,
  * neighbour (the multi-edges read_mtx keeps) is PGD_ERR_UNSUPPORTED. */
 pgd_status pgd_graph_from_csr(pgd_ctx *ctx, int64_t n, const int64_t *row, const int32_t *col,
                               pgd_graph **out);
+/* read_graph (graphlet_core.cpp:259-283) straight to a device graph: the
+ * same result as pgd_read_graph followed by pgd_graph_from_coo (PGD_DEDUP
+ * for edge lists), with the text parsed by `threads` host threads (0: all
+ * hardware threads) and, for edge lists, the renumbering by first appearance
+ * (graphlet_core.cpp:413-428) done on the device.  self_loops (may be NULL)
+ * receives the number of self-loop lines dropped.  PGD_ERR_INVALID for a
+ * missing file or an id the reference reader cannot handle. */
+pgd_status pgd_graph_from_file(pgd_ctx *ctx, const char *path, int32_t threads, pgd_graph **out,
+                               int64_t *self_loops);
 pgd_status pgd_graph_free(pgd_graph *g);
 /* n = num_vertices(), m = num_edges(), max degree, avg degree
  * (graphlet_core.h:220-221, 262, 276; vertex_degrees graphlet_core.cpp:877-891) */

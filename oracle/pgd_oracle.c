@@ -247,10 +247,13 @@ static int read_edge_list(const char *path, or_csr *out, int64_t *self_loops) {
         if (fix) { v--; u--; }
         if (v == u) { self++; continue; }
         if (v < 0 || u < 0 || v > maxv || u > maxv) { bad = 1; break; }   /* reference: out-of-bounds */
+        /* v and u take their new ids in place (graphlet_core.cpp:417-424), so a
+         * field the next line leaves unset carries the new id */
         if (lookup[v] == -1) lookup[v] = vid++;
-        int32_t a = lookup[v];
+        v = lookup[v];
         if (lookup[u] == -1) lookup[u] = vid++;
-        int32_t b = lookup[u];
+        u = lookup[u];
+        int32_t a = v, b = u;
         if (vid > cap) {
             int64_t nc = cap ? cap * 2 : 1024;
             lists = (ivec *)realloc(lists, (size_t)nc * sizeof(ivec));

@@ -200,9 +200,28 @@ class GraphletCore:
         return self
 
     @classmethod
-    def from_file(cls, path: str, **kw) -> "GraphletCore":
-        n, src, dst, dedup, _ = read_graph(path)
-        return cls(src, dst, n=n, dedup=dedup, **kw)
+    def from_file(cls, path: str, ctx: Context | None = None, neigh_ordering: str = "deg",
+                  neigh_small_to_large: bool = False, preprocess: bool = True, threads: int = 0) -> "GraphletCore":
+        """read_graph straight to the device (pgd_graph_from_file): the text
+        parsed by `threads` host threads, renumbering and CSR on the device."""
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        loops = C.c_int64()
+        check(lib().pgd_graph_from_file(self.ctx.h, path.encode(), threads, C.byref(h), C.byref(loops)),
+              self.ctx.h, f"pgd_graph_from_file({path})")
+        self.h = h
+        self.ctx._graphs.add(self)
+        nn, mm, md, ad = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        lib().pgd_graph_info(h, C.byref(nn), C.byref(mm), C.byref(md), C.byref(ad))
+        self.n, self.m, self.max_degree, self.avg_degree = nn.value, mm.value, md.value, ad.value
+        self.self_loops = loops.value
+        self.sums = None
+        self.totals = None
+        self._ordered = False
+        if preprocess:
+            self.preprocess(neigh_ordering, neigh_small_to_large)
+        return self
 
     def free(self) -> None:
         if getattr(self, "h", None) and getattr(self.ctx, "h", None):

@@ -129,13 +129,9 @@ void graphlet_core::read_graph(const std::string& filename) {
             if (rows != cols) std::cout << "[pgd]  error: this is not a square matrix, attempting to proceed." << std::endl;
         }
     }
-    int64_t n = 0, nl = 0, loops = 0;
-    int32_t *s = nullptr, *d = nullptr, dedup = 0;
-    check(pgd_read_graph(filename.c_str(), &n, &nl, &s, &d, &dedup, &loops), "read_graph");
-    pgd_status st = pgd_graph_from_coo(default_context(), n, nl, s, d, dedup ? PGD_DEDUP : 0u, &g_);
-    pgd_free(s);
-    pgd_free(d);
-    check(st, "pgd_graph_from_coo");
+    // the text parsed by all host threads; renumbering, dedup and CSR on the device
+    int64_t loops = 0;
+    check(pgd_graph_from_file(default_context(), filename.c_str(), 0, &g_, &loops), "read_graph");
     int32_t md = 0, ad = 0;
     check(pgd_graph_info(g_, &n_, &m_, &md, &ad), "pgd_graph_info");
     max_degree = md;

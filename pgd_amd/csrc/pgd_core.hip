@@ -21,6 +21,7 @@
 
 #include "pgd_epilogue.hpp"
 #include "pgd_internal.hpp"
+#include "pgd_reader.hpp"
 
 using namespace pgd;
 
@@ -193,13 +194,59 @@ inline int grid_for(int64_t n, int block = kBlock) {
 // ===========================================================================
 // CSR construction kernels
 // ===========================================================================
+// keep[i]: line i is an edge (not a self loop).  With drop_neg, a line whose
+// first id is negative was removed by the reader (its self loops) and is not
+// an error.
 __global__ void k_line_keep(const int32_t* __restrict__ s, const int32_t* __restrict__ d, int64_t L, int64_t n,
-                            int32_t* __restrict__ keep, uint32_t* __restrict__ bad) {
+                            int drop_neg, int32_t* __restrict__ keep, uint32_t* __restrict__ bad) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L; i += (int64_t)gridDim.x * blockDim.x) {
         int32_t a = s[i], b = d[i];
+        if (drop_neg && a < 0) { keep[i] = 0; continue; }
         if (a < 0 || b < 0 || a >= n || b >= n) { atomicOr(bad, 1u); keep[i] = 0; continue; }
         keep[i] = a != b;
     }
+}
+
+// ---- renumbering of an edge list by first appearance (read_edge_list,
+// graphlet_core.cpp:413-428): position p = 2 * line + (0 for v, 1 for u);
+// a raw id's new id is the number of raw ids first seen before it.
+// first[x] = the first position of raw id x over the non-loop lines
+__global__ void k_first_pos(const int32_t* __restrict__ v, const int32_t* __restrict__ u, int64_t L, int64_t maxv,
+                            uint32_t* __restrict__ first, unsigned long long* __restrict__ stat) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t a = v[i], b = u[i];
+        if (a == b) { atomicAdd(&stat[0], 1ull); continue; }  // a self loop: no id
+        if (a < 0 || b < 0 || a > maxv || b > maxv) { atomicOr(&stat[1], 1ull); continue; }
+        atomicMin(&first[a], (uint32_t)(2 * i));
+        atomicMin(&first[b], (uint32_t)(2 * i + 1));
+    }
+}
+// mark[p] = 1 when position p is the first appearance of its raw id
+__global__ void k_first_mark(const int32_t* __restrict__ v, const int32_t* __restrict__ u, int64_t L,
+                             const uint32_t* __restrict__ first, uint32_t* __restrict__ mark) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < 2 * L; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = p >> 1;
+        const int32_t a = v[i], b = u[i];
+        mark[p] = a != b && first[(p & 1) ? b : a] == (uint32_t)p;
+    }
+}
+// the new ids in place; self-loop lines become (-1, -1), dropped later
+__global__ void k_renumber(int32_t* __restrict__ v, int32_t* __restrict__ u, int64_t L,
+                           const uint32_t* __restrict__ first, const uint32_t* __restrict__ rank) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t a = v[i], b = u[i];
+        if (a == b) {
+            v[i] = -1;
+            u[i] = -1;
+            continue;
+        }
+        v[i] = (int32_t)rank[first[a]];
+        u[i] = (int32_t)rank[first[b]];
+    }
+}
+__global__ void k_fill_u32_max(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0xffffffffu;
 }
 
 __global__ void k_pair_keys(const int32_t* __restrict__ s, const int32_t* __restrict__ d, const int32_t* __restrict__ keep,
@@ -721,34 +768,26 @@ pgd_status pgd_mem_in_use(pgd_ctx* ctx, int64_t* bytes) {
 
 const char* pgd_last_error(pgd_ctx* ctx) { return ctx ? ctx->last_err.c_str() : "no context"; }
 
-pgd_status pgd_graph_from_coo(pgd_ctx* ctx, int64_t n, int64_t L, const int32_t* src, const int32_t* dst,
-                              uint32_t flags, pgd_graph** out) {
-    if (!ctx || !out || n < 0 || L < 0 || (L > 0 && (!src || !dst))) return PGD_ERR_INVALID;
-    if (n >= INT32_MAX || L >= ((int64_t)1 << 31)) return PGD_ERR_UNSUPPORTED;
-    *out = nullptr;
-    pgd_graph* g = new (std::nothrow) pgd_graph();
-    if (!g) return PGD_ERR_OOM;
-    g->ctx = ctx;
-    pgd_status st = guarded(ctx, [&] {
+namespace {
+// The CSR of a graph from its COO in device memory (ds, dd: L lines in file
+// order, ids in [0, n); with drop_neg, lines with a negative first id are
+// skipped): self loops dropped, repeated pairs dropped keeping the first
+// (PGD_DEDUP) or refused, each adjacency list in line order, degrees.
+void coo_build(pgd_ctx* ctx, pgd_graph* g, int64_t n, int64_t L, DBuf<int32_t>& ds, DBuf<int32_t>& dd,
+               uint32_t flags, bool drop_neg) {
         PGD_HIP(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
-        DBuf<int32_t> ds, dd, keep;
+        DBuf<int32_t> keep;
         DBuf<int64_t> off;
         DBuf<uint32_t> bad;
-        ds.alloc(ctx, L);
-        dd.alloc(ctx, L);
         keep.alloc(ctx, L);
         off.alloc(ctx, L + 1);
         bad.alloc(ctx, 1);
-        if (L) {
-            PGD_HIP(hipMemcpyAsync(ds.get(), src, L * sizeof(int32_t), hipMemcpyHostToDevice, s));
-            PGD_HIP(hipMemcpyAsync(dd.get(), dst, L * sizeof(int32_t), hipMemcpyHostToDevice, s));
-        }
         PGD_HIP(hipMemsetAsync(bad.get(), 0, sizeof(uint32_t), s));
         int grid = std::min(grid_for(L), ctx->num_cus * 16);
         if (L) {
-            hipLaunchKernelGGL(k_line_keep, dim3(grid), dim3(kBlock), 0, s, ds.get(), dd.get(), L, n, keep.get(),
-                               bad.get());
+            hipLaunchKernelGGL(k_line_keep, dim3(grid), dim3(kBlock), 0, s, ds.get(), dd.get(), L, n,
+                               drop_neg ? 1 : 0, keep.get(), bad.get());
             PGD_HIP(hipGetLastError());
         }
         uint32_t hbad = 0;
@@ -823,11 +862,128 @@ pgd_status pgd_graph_from_coo(pgd_ctx* ctx, int64_t n, int64_t L, const int32_t*
         // vertex_degrees: max_degree starts from vertex 0's degree; avg is an int
         g->max_deg = md;
         g->avg_deg = n > 0 ? (int32_t)((double)nnz / (double)n) : 0;
+}
+}  // namespace
+
+pgd_status pgd_graph_from_coo(pgd_ctx* ctx, int64_t n, int64_t L, const int32_t* src, const int32_t* dst,
+                              uint32_t flags, pgd_graph** out) {
+    if (!ctx || !out || n < 0 || L < 0 || (L > 0 && (!src || !dst))) return PGD_ERR_INVALID;
+    if (n >= INT32_MAX || L >= ((int64_t)1 << 31)) return PGD_ERR_UNSUPPORTED;
+    *out = nullptr;
+    pgd_graph* g = new (std::nothrow) pgd_graph();
+    if (!g) return PGD_ERR_OOM;
+    g->ctx = ctx;
+    pgd_status st = guarded(ctx, [&] {
+        PGD_HIP(hipSetDevice(ctx->device));
+        DBuf<int32_t> ds, dd;
+        ds.alloc(ctx, L);
+        dd.alloc(ctx, L);
+        if (L) {
+            PGD_HIP(hipMemcpyAsync(ds.get(), src, L * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+            PGD_HIP(hipMemcpyAsync(dd.get(), dst, L * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+        }
+        coo_build(ctx, g, n, L, ds, dd, flags, false);
     });
     if (st != PGD_OK) {
         delete g;
         return st;
     }
+    *out = g;
+    return PGD_OK;
+}
+
+pgd_status pgd_graph_from_file(pgd_ctx* ctx, const char* path, int32_t threads, pgd_graph** out,
+                               int64_t* self_loops) {
+    if (!ctx || !out || !path) return PGD_ERR_INVALID;
+    *out = nullptr;
+    RawLines R;
+    pgd_status st = read_raw_lines(path, threads, R);
+    if (st != PGD_OK) {
+        ctx->last_err = std::string("cannot read ") + path;
+        return st;
+    }
+    const int64_t L = (int64_t)R.v.size();
+    if (L >= ((int64_t)1 << 31)) return PGD_ERR_UNSUPPORTED;
+    pgd_graph* g = new (std::nothrow) pgd_graph();
+    if (!g) return PGD_ERR_OOM;
+    g->ctx = ctx;
+    int64_t loops = R.loops;
+    st = guarded(ctx, [&] {
+        PGD_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        const uint32_t flags = R.mtx ? 0u : PGD_DEDUP;
+        DBuf<int32_t> ds, dd;
+        ds.alloc(ctx, L);
+        dd.alloc(ctx, L);
+        if (L) {
+            PGD_HIP(hipMemcpyAsync(ds.get(), R.v.data(), L * sizeof(int32_t), hipMemcpyHostToDevice, s));
+            PGD_HIP(hipMemcpyAsync(dd.get(), R.u.data(), L * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        }
+        if (R.final_ids) {  // replayed on the host: final ids, self loops removed
+            PGD_CHECK(R.n < INT32_MAX, PGD_ERR_UNSUPPORTED, "more than 2^31 - 1 vertices");
+            coo_build(ctx, g, R.n, L, ds, dd, flags, false);
+            return;
+        }
+        const int grid = std::min(grid_for(L), ctx->num_cus * 16);
+        if (R.mtx) {
+            // n = 1 + the largest endpoint of a non-loop line; a negative id
+            // is an error (the reference never terminates on it)
+            int64_t maxid = -1;
+            bool neg = false;
+            for (int64_t i = 0; i < L; i++) {
+                const int32_t a = R.v[(size_t)i], b = R.u[(size_t)i];
+                if (a == b) { loops++; continue; }
+                if (a < 0 || b < 0) neg = true;
+                maxid = std::max<int64_t>(maxid, std::max(a, b));
+            }
+            PGD_CHECK(!neg, PGD_ERR_INVALID, "negative vertex id");
+            PGD_CHECK(maxid + 1 < INT32_MAX, PGD_ERR_UNSUPPORTED, "more than 2^31 - 1 vertices");
+            coo_build(ctx, g, maxid + 1, L, ds, dd, flags, false);
+            return;
+        }
+        // edge list: renumbering by first appearance on the device
+        const int64_t T = R.maxv + 1;
+        DBuf<uint32_t> first, mark, rank;
+        DBuf<unsigned long long> stat;
+        first.alloc(ctx, T);
+        stat.alloc(ctx, 2);
+        PGD_HIP(hipMemsetAsync(stat.get(), 0, 2 * sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(k_fill_u32_max, dim3(std::min(grid_for(T), ctx->num_cus * 16)), dim3(kBlock), 0, s,
+                           first.get(), T);
+        unsigned long long hs[2] = {0, 0};
+        if (L) {
+            hipLaunchKernelGGL(k_first_pos, dim3(grid), dim3(kBlock), 0, s, ds.get(), dd.get(), L, R.maxv,
+                               first.get(), stat.get());
+            PGD_HIP(hipGetLastError());
+        }
+        PGD_HIP(hipMemcpyAsync(hs, stat.get(), sizeof(hs), hipMemcpyDeviceToHost, s));
+        sync(ctx);
+        PGD_CHECK(hs[1] == 0, PGD_ERR_INVALID, "vertex id outside the reader's table");
+        loops = (int64_t)hs[0];
+        int64_t nv = 0;
+        if (L) {
+            mark.alloc(ctx, 2 * L);
+            rank.alloc(ctx, 2 * L);
+            hipLaunchKernelGGL(k_first_mark, dim3(std::min(grid_for(2 * L), ctx->num_cus * 16)), dim3(kBlock), 0,
+                               s, ds.get(), dd.get(), L, first.get(), mark.get());
+            PGD_HIP(hipGetLastError());
+            exclusive_sum(ctx, mark.get(), rank.get(), (size_t)(2 * L));
+            uint32_t tail[2] = {0, 0};
+            PGD_HIP(hipMemcpyAsync(&tail[0], rank.get() + (2 * L - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            PGD_HIP(hipMemcpyAsync(&tail[1], mark.get() + (2 * L - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            hipLaunchKernelGGL(k_renumber, dim3(grid), dim3(kBlock), 0, s, ds.get(), dd.get(), L, first.get(),
+                               rank.get());
+            PGD_HIP(hipGetLastError());
+            sync(ctx);
+            nv = (int64_t)tail[0] + tail[1];
+        }
+        coo_build(ctx, g, nv, L, ds, dd, flags, true);
+    });
+    if (st != PGD_OK) {
+        delete g;
+        return st;
+    }
+    if (self_loops) *self_loops = loops;
     *out = g;
     return PGD_OK;
 }

@@ -49,6 +49,11 @@ READER_CASES = {
     "rd_gap_ids.mtx": "%%MatrixMarket matrix coordinate pattern symmetric\n20 20 7\n2 5\n5 9\n9 2\n2 12\n12 5\n9 12\n12 14\n",
     "rd_mtx_dups.mtx": "%%MatrixMarket matrix coordinate pattern symmetric\n6 6 8\n1 2\n2 3\n3 1\n2 1\n3 4\n4 4\n4 5\n5 3\n",
     "rd_mtx_comments.mtx": "%%MatrixMarket matrix coordinate pattern symmetric\n% c\n% d\n5 5 6\n1 2\n2 3\n% inner\n3 1\n3 4\n4 5\n5 3\n",
+    # fields a line leaves unset keep the previous line's values
+    "rd_carry_tokens.txt": "1 2\n3 \n4 5\n 6\n7 8 9\n3\n5 6\n2 3\n6 1\n 8\n9 \n",
+    "rd_carry_csv.csv": "a,b\n1,2\n,3\n4,\n5,6\n2,,3\n6,1\n,7\n8,\n",
+    "rd_carry_mtx.mtx": "%%MatrixMarket matrix coordinate pattern symmetric\n6 6 5\n1 2\n3\n4 5\n6\n2 6\n",
+    "rd_blank_first.txt": "\n# not a comment block any more\n1 2\n2 3\n3 1\n3 4\n",
 }
 
 # reference fixtures copied (compressed) as inputs
@@ -123,7 +128,8 @@ def main():
         name = os.path.basename(rel)
         src = os.path.join(REF_SRC, rel)
         gz = os.path.join(HERE, "inputs", name + ".gz")
-        with open(src, "rb") as f, gzip.open(gz, "wb", compresslevel=9) as g:
+        with open(src, "rb") as f, open(gz, "wb") as raw, \
+                gzip.GzipFile(fileobj=raw, mode="wb", compresslevel=9, mtime=0) as g:  # byte-stable output
             g.write(f.read())
         local = os.path.join(work, name)
         shutil.copy(src, local)

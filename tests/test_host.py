@@ -250,3 +250,24 @@ def test_micro_emulation_matches_reference_build_flags():
     ref = _ref_arith([f"M {T} {K} {C} {dv} {du}" for T, K, C, dv, du in cases])
     for c, want in zip(cases, ref):
         assert _micro_emulation(*c) == want, c
+
+
+@pytest.mark.parametrize("chunk", ["1", "7", "64"])
+def test_parallel_reader_with_tiny_chunks(chunk, tmp, monkeypatch):
+    """The reader cut into chunks of a few bytes, so that carried values,
+    comments and the first-line rules meet chunk boundaries: the same COO as
+    one chunk, and as the oracle reader (pinned against the reference)."""
+    names = [k for k in GOLDENS if "@" not in k and GOLDENS[k].get("m", 0) <= 20000 and "input" in GOLDENS[k]]
+    assert any(k.startswith("rd_carry") for k in names)
+    for name in names:
+        path = golden_graph_path(name, GOLDENS[name], tmp)
+        monkeypatch.delenv("PGD_READER_CHUNK", raising=False)
+        one = pgd_amd.read_graph(path)
+        monkeypatch.setenv("PGD_READER_CHUNK", chunk)
+        many = pgd_amd.read_graph(path)
+        assert one[0] == many[0] and one[3:] == many[3:], name
+        assert np.array_equal(one[1], many[1]) and np.array_equal(one[2], many[2]), name
+        n, src, dst, dedup, _ = many
+        row, col = coo_to_csr(n, src, dst, dedup)
+        orow, ocol = OracleGraph.from_file(path).csr()
+        assert np.array_equal(row, orow) and np.array_equal(col, ocol), name
