@@ -49,6 +49,15 @@ def cpu_info() -> dict:
     return {"model": model, "nproc": os.cpu_count()}
 
 
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: every core this process may use
+    (a one-GPU box grants 16, and os.cpu_count() reports that share)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
 def survey_bytes(row: np.ndarray, col: np.ndarray, ev: np.ndarray, eu: np.ndarray, micro: bool) -> float:
     """B(G) = 4 sum_e [d_v + d_u + min(s(u)-d_v, s(v)-d_u)] + 8(n+1) + 8m + 64m[micro] + 136."""
     deg = np.diff(row).astype(np.int64)
@@ -128,6 +137,7 @@ def cpu_baseline(g, target_s: float, threads: int) -> dict:
     est_total = dt * total_work / swork
     return {
         "value": len(order) / est_total, "unit": "edges/s", "cores": threads, "kind": "port",
+        "estimated_total_s": est_total,
         "sample": f"every {stride}th edge of E_ordered ({len(samp)} edges, {swork:.3e} of {total_work:.3e} "
                   f"probes, {dt:.1f}s); total extrapolated by the probe model: {est_total:.0f}s",
         "cpu": cpu_info(),
@@ -393,6 +403,40 @@ def served_fresh_process() -> dict:
         return {"error": repr(e)}
 
 
+def validate_cpu_baseline(pgd_amd, ctx, scales, target_s: float, threads: int) -> list:
+    """The sampled CPU baseline's extrapolation against full runs of the
+    oracle over every edge of E_ordered, on smaller R-MAT graphs."""
+    from tests.oracle_lib import OracleGraph
+    res = []
+    for scale in scales:
+        s, d = pgd_amd.gen_rmat(scale, 16, 0.57, 0.19, 0.19, 1)
+        n = int(max(s.max(), d.max())) + 1
+        g = pgd_amd.GraphletCore(s, d, n=n, ctx=ctx)
+        g.sort_edges("deg")
+        est = cpu_baseline(g, target_s, threads)
+        row, col = g.csr()
+        ev, eu = g.edge_arrays()
+        order = g.e_ordered()
+        g.free()
+        og = OracleGraph.from_csr(row, col)
+        log(f"[bench] full oracle run on R-MAT-{scale} ({len(order)} edges, {threads} threads)")
+        # in 32 consecutive ranges of E_ordered, each timed (the edges are
+        # independent: the sum is the full run), with a line after each
+        full = 0.0
+        cuts = np.linspace(0, len(order), 33).astype(np.int64)
+        for k in range(32):
+            part = np.ascontiguousarray(order[cuts[k]:cuts[k + 1]])
+            t0 = time.perf_counter()
+            og.decompose(ev, eu, part, micro=False, threads=threads)
+            full += time.perf_counter() - t0
+            log(f"[bench]   range {k + 1}/32 done, {full:.1f} s so far")
+        res.append({"scale": scale, "edges": int(len(order)), "full_s": full,
+                    "estimated_s": est["estimated_total_s"], "sample": est["sample"],
+                    "relative_error": est["estimated_total_s"] / full - 1.0})
+        log(f"[bench] R-MAT-{scale}: full {full:.1f} s, estimated {est['estimated_total_s']:.1f} s")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -409,10 +453,21 @@ def main() -> int:
                     help="only run a plain 1 GiB device copy three times (scale for the PMC byte counters)")
     ap.add_argument("--served-only", action="store_true",
                     help="only the served measurement, in this (fresh) process; prints its JSON")
+    ap.add_argument("--validate-cpu-baseline", default="",
+                    help="comma-separated R-MAT scales: compare the sampled CPU baseline's extrapolation with full "
+                         "oracle runs there, print that JSON and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: run with --gpus 1"}))
         return 2
+    if args.validate_cpu_baseline:
+        import pgd_amd
+        ctx = pgd_amd.Context(0)
+        scales = [int(x) for x in args.validate_cpu_baseline.split(",")]
+        print(json.dumps({"cpu_baseline_validation": validate_cpu_baseline(
+            pgd_amd, ctx, scales, args.cpu_seconds, cpu_threads())}), flush=True)
+        ctx.close()
+        return 0
     if args.served_only:
         import pgd_amd
         ctx = pgd_amd.Context(0)
@@ -519,8 +574,7 @@ def main() -> int:
             served["capacity_vs_one_graph_per_launch"] = served["capacity"]["graphs_per_s"] / plain
     if not args.no_cpu_baseline:
         try:
-            threads = min(os.cpu_count() or 1, 16)
-            out["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds, threads)
+            out["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds, cpu_threads())
             out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
         except Exception as e:  # the baseline must not hide the GPU number
             out["cpu_baseline"] = {"error": repr(e)}
