@@ -731,32 +731,39 @@ struct PackSplit {
 // (narrow: the lower endpoint L[i] of the far edge has degree <= pk.deg,
 // kept by the caller in A.Le[i]; tij: T of the far edge, read by the caller
 // in the first column block)
-__device__ __forceinline__ void edge_terms_k4q(const LocalArrays& A, int wn, bool first, int i, int j,
-                                               uint32_t sij, uint32_t tij,
-                                               uint64_t* __restrict__ K4S, uint64_t* __restrict__ QS,
-                                               uint64_t* __restrict__ PK, PackSplit pk) {
-    // the rows' words four at a time: the eight LDS reads are in flight together
+__device__ __forceinline__ void edge_terms_k4q(const LocalArrays& A, int wn, int ws, bool first, bool act, int i,
+                                               int j, uint32_t sij, uint32_t tij, uint64_t* __restrict__ K4S,
+                                               uint64_t* __restrict__ QS, uint64_t* __restrict__ PK, PackSplit pk) {
+    // called by every lane of the wavefront (act: the lane holds a local
+    // edge); rows are ws words apart, wn of them held
+    const int lane = threadIdx.x & 63;
     uint32_t kij = 0;
-    const uint32_t* ri = A.M + i * wn;
-    const uint32_t* rj = A.M + j * wn;
-    int w = 0;
-    for (; w + 4 <= wn; w += 4) {
-        const uint32_t a0 = ri[w], a1 = ri[w + 1], a2 = ri[w + 2], a3 = ri[w + 3];
-        const uint32_t b0 = rj[w], b1 = rj[w + 1], b2 = rj[w + 2], b3 = rj[w + 3];
-        kij += __popc(a0 & b0) + __popc(a1 & b1) + __popc(a2 & b2) + __popc(a3 & b3);
+    if (act) {
+        // the rows' words four at a time: the eight LDS reads are in flight together
+        const uint32_t* ri = A.M + i * ws;
+        const uint32_t* rj = A.M + j * ws;
+        int w = 0;
+        for (; w + 4 <= wn; w += 4) {
+            const uint32_t a0 = ri[w], a1 = ri[w + 1], a2 = ri[w + 2], a3 = ri[w + 3];
+            const uint32_t b0 = rj[w], b1 = rj[w + 1], b2 = rj[w + 2], b3 = rj[w + 3];
+            kij += __popc(a0 & b0) + __popc(a1 & b1) + __popc(a2 & b2) + __popc(a3 & b3);
+        }
+        for (; w < wn; w++) kij += __popc(ri[w] & rj[w]);
     }
-    for (; w < wn; w++) kij += __popc(ri[w] & rj[w]);
-    if (kij) {
-        atomicAdd(&A.cnt[i], kij);
-        atomicAdd(&A.cnt[j], kij);
-    }
+    // lanes in list order share the local vertex i: one atomic per run of i
+    const LaneRun r = lane_run(act ? (uint32_t)i : ~(uint32_t)lane);
+    const uint32_t ksum = run_sum32(kij, r.start);
+    if (act && r.last && ksum) atomicAdd(&A.cnt[i], ksum);
+    if (act && kij) atomicAdd(&A.cnt[j], kij);
     uint64_t q = 0;
     if (first) {
-        const uint64_t tai = A.ta[i], taj = A.ta[j];
+        const uint64_t tai = act ? A.ta[i] : 0, taj = act ? A.ta[j] : 0;
         q = tai + taj;
-        atomicAdd((unsigned long long*)&A.acc[i], (unsigned long long)(taj + tij));
-        atomicAdd((unsigned long long*)&A.acc[j], (unsigned long long)(tai + tij));
+        const uint64_t asum = run_sum64(act ? taj + tij : 0, r.start);
+        if (act && r.last && asum) atomicAdd((unsigned long long*)&A.acc[i], (unsigned long long)asum);
+        if (act) atomicAdd((unsigned long long*)&A.acc[j], (unsigned long long)(tai + tij));
     }
+    if (!act) return;
     if (A.Le[i] <= pk.deg) {
         if (kij | q) atomicAdd((unsigned long long*)&PK[sij], (unsigned long long)(((uint64_t)kij << pk.qb) | q));
     } else {
@@ -766,10 +773,24 @@ __device__ __forceinline__ void edge_terms_k4q(const LocalArrays& A, int wn, boo
 }
 
 // set the bits of the local edge (i, j) that fall in columns [jlo, jlo + 32 wn)
-__device__ __forceinline__ void set_pair(uint32_t* M, int wn, int jlo, int i, int j) {
+// (rows ws words apart)
+__device__ __forceinline__ void set_pair(uint32_t* M, int wn, int ws, int jlo, int i, int j) {
     const int ji = j - jlo, ii = i - jlo;
-    if (ji >= 0 && ji < 32 * wn) atomicOr(&M[i * wn + (ji >> 5)], 1u << (ji & 31));
-    if (ii >= 0 && ii < 32 * wn) atomicOr(&M[j * wn + (ii >> 5)], 1u << (ii & 31));
+    if (ji >= 0 && ji < 32 * wn) atomicOr(&M[i * ws + (ji >> 5)], 1u << (ji & 31));
+    if (ii >= 0 && ii < 32 * wn) atomicOr(&M[j * ws + (ii >> 5)], 1u << (ii & 31));
+}
+// the same, called by every lane of the wavefront (act: the lane holds a local
+// edge): lanes in list order share i and often the word of row i, so the
+// bits of a run of equal words are or-ed across lanes and set by one atomic
+__device__ __forceinline__ void set_pair_wave(uint32_t* M, int wn, int ws, int jlo, int i, int j, bool act) {
+    const int lane = threadIdx.x & 63;
+    const int ji = j - jlo, ii = i - jlo;
+    const bool a1 = act && ji >= 0 && ji < 32 * wn;
+    const uint32_t key = a1 ? (uint32_t)(i * ws + (ji >> 5)) : ~(uint32_t)lane;
+    const LaneRun r = lane_run(key);
+    const uint32_t bits = run_or(a1 ? 1u << (ji & 31) : 0u, r.start);
+    if (a1 && r.last) atomicOr(&M[key], bits);
+    if (act && ii >= 0 && ii < 32 * wn) atomicOr(&M[j * ws + (ii >> 5)], 1u << (ii & 31));
 }
 
 template <int NT, bool kGlobal>
@@ -808,17 +829,22 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
         const uint64_t* gl = kept >= 0 ? tl.G + tl.off[a] : nullptr;
         for (int cw = 0; cw < W; cw += wc) {
             const int wn = min(wc, W - cw), jlo = 32 * cw;
+            // an odd row stride when it fits: the rows read together by a
+            // wavefront start in different banks
+            const int ws = (int64_t)k * (wn | 1) <= mw ? (wn | 1) : wn;
             const bool first = cw == 0;
-            for (int w = threadIdx.x; w < k * wn; w += NT) A.M[w] = 0;
+            for (int w = threadIdx.x; w < k * ws; w += NT) A.M[w] = 0;
             __syncthreads();
             // local edges: the kept list, this block's list (recorded while
             // the first column block is built), or a new search
             const uint64_t* src = gl ? gl : (first ? nullptr : el);
             const int ne = gl ? kept : ecount;
             if (src) {
-                for (int p = threadIdx.x; p < ne; p += NT) {
-                    const uint64_t e = src[p];
-                    set_pair(A.M, wn, jlo, (int)(e & 0xffff), (int)((e >> 16) & 0xffff));
+                const int lane = threadIdx.x & 63;
+                for (int pb = threadIdx.x - lane; pb < ne; pb += NT) {  // uniform per wavefront
+                    const int p = pb + lane;
+                    const uint64_t e = src[min(p, ne - 1)];
+                    set_pair_wave(A.M, wn, ws, jlo, (int)(e & 0xffff), (int)((e >> 16) & 0xffff), p < ne);
                 }
             } else {
                 for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
@@ -835,7 +861,7 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
 #pragma unroll
                     for (int u = 0; u < kU; u++) {
                         if (j[u] >= 0) {
-                            set_pair(A.M, wn, jlo, i[u], j[u]);
+                            set_pair(A.M, wn, ws, jlo, i[u], j[u]);
                             e[u] = (uint32_t)slot[u];
                         }
                     }
@@ -850,7 +876,9 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                 // kU list entries per thread at once: their loads, and then
                 // their T loads, are in flight together (unconditional:
                 // entries past the end are read as the last one)
-                for (int p = threadIdx.x; p < nl; p += NT * kU) {
+                const int lane = threadIdx.x & 63;
+                for (int pb = threadIdx.x - lane; pb < nl; pb += NT * kU) {  // uniform per wavefront
+                    const int p = pb + lane;
                     uint64_t e[kU];
                     uint32_t t[kU] = {};
 #pragma unroll
@@ -861,14 +889,15 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                     }
 #pragma unroll
                     for (int u = 0; u < kU; u++)
-                        if (p + u * NT < nl)
-                            edge_terms_k4q(A, wn, first, (int)(e[u] & 0xffff), (int)((e[u] >> 16) & 0xffff),
-                                           (uint32_t)(e[u] >> 32), t[u], K4S, QS, PK, pk);
+                        edge_terms_k4q(A, wn, ws, first, p + u * NT < nl, (int)(e[u] & 0xffff),
+                                       (int)((e[u] >> 16) & 0xffff), (uint32_t)(e[u] >> 32), t[u], K4S, QS, PK, pk);
                 }
             } else {
                 // no list: wc == W here (the HBM bin holds the full matrix)
                 if (threadIdx.x == 0) done += (unsigned long long)total;
-                for (int64_t t0 = threadIdx.x; t0 < total; t0 += NT * kU) {
+                const int64_t lane = threadIdx.x & 63;
+                for (int64_t tb = threadIdx.x - lane; tb < total; tb += NT * kU) {  // uniform per wavefront
+                    const int64_t t0 = tb + lane;
                     int iu[kU], ju[kU];
                     int64_t su[kU];
                     int32_t x[kU];
@@ -883,8 +912,8 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
                     for (int u = 0; u < kU; u++) tu[u] = first && ju[u] >= 0 ? TSL[su[u]] : 0u;
 #pragma unroll
                     for (int u = 0; u < kU; u++)
-                        if (ju[u] >= 0)
-                            edge_terms_k4q(A, wn, first, iu[u], ju[u], (uint32_t)su[u], tu[u], K4S, QS, PK, pk);
+                        edge_terms_k4q(A, wn, ws, first, ju[u] >= 0, iu[u], max(ju[u], 0), (uint32_t)su[u], tu[u],
+                                       K4S, QS, PK, pk);
                 }
             }
             __syncthreads();

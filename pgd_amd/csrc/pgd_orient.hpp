@@ -155,6 +155,56 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x) {
     return x;
 }
 
+// Runs of equal keys in lane order (the whole wavefront takes part; a lane
+// with nothing to contribute passes a key of its own, e.g. ~lane): `start` is
+// the first lane of this lane's run, `last` marks the run's last lane.  Used
+// to turn the same-address LDS atomics of a run into one.  The run starts
+// come from one ballot, and the sums below are DPP scans (row shifts, then
+// the row broadcasts of lanes 15 and 31), so no LDS instruction is spent.
+struct LaneRun {
+    int start;
+    bool last;
+};
+__device__ __forceinline__ LaneRun lane_run(uint32_t key) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t kp = (uint32_t)__shfl_up((int)key, 1, 64);
+    const uint64_t heads = __ballot(lane == 0 || kp != key);
+    const int start = 63 - __clzll((long long)(heads & ((2ULL << lane) - 1)));
+    return LaneRun{start, lane == 63 || ((heads >> (lane + 1)) & 1) != 0};
+}
+// one step of a segmented inclusive scan: the value `off` lanes below
+// (within the row), or a row broadcast, when it lies in this lane's run
+#define PGD_RUN_STEPS(OP, T, MOV)                                                                              \
+    {                                                                                                          \
+        const int lane_ = threadIdx.x & 63, r_ = lane_ & 15;                                                   \
+        T y_;                                                                                                  \
+        y_ = MOV(x, 0x111, 0xf); if (r_ >= 1 && lane_ - 1 >= start) x = x OP y_;                               \
+        y_ = MOV(x, 0x112, 0xf); if (r_ >= 2 && lane_ - 2 >= start) x = x OP y_;                               \
+        y_ = MOV(x, 0x114, 0xf); if (r_ >= 4 && lane_ - 4 >= start) x = x OP y_;                               \
+        y_ = MOV(x, 0x118, 0xf); if (r_ >= 8 && lane_ - 8 >= start) x = x OP y_;                               \
+        y_ = MOV(x, 0x142, 0xa); if (((lane_ >> 4) & 1) && start <= (lane_ & ~15) - 1) x = x OP y_;            \
+        y_ = MOV(x, 0x143, 0xc); if ((lane_ >> 5) && start <= 31) x = x OP y_;                                 \
+    }
+#define PGD_DPP32(v, ctrl, rmask) (uint32_t) __builtin_amdgcn_update_dpp(0, (int)(v), ctrl, rmask, 0xf, true)
+#define PGD_DPP64(v, ctrl, rmask)                                                                          \
+    ((uint64_t)(uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)((v) >> 32), ctrl, rmask, 0xf, true) \
+         << 32 |                                                                                           \
+     (uint64_t)(uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v), ctrl, rmask, 0xf, true))
+// inclusive sum / or of x over the lanes of this lane's run up to it (the
+// run's last lane holds the run's total)
+__device__ __forceinline__ uint32_t run_sum32(uint32_t x, int start) {
+    PGD_RUN_STEPS(+, uint32_t, PGD_DPP32)
+    return x;
+}
+__device__ __forceinline__ uint64_t run_sum64(uint64_t x, int start) {
+    PGD_RUN_STEPS(+, uint64_t, PGD_DPP64)
+    return x;
+}
+__device__ __forceinline__ uint32_t run_or(uint32_t x, int start) {
+    PGD_RUN_STEPS(|, uint32_t, PGD_DPP32)
+    return x;
+}
+
 // first position in col[lo..hi) (ascending) holding a value >= x
 __device__ __forceinline__ int64_t lower_bound_g(const int32_t* __restrict__ col, int64_t lo, int64_t hi, int32_t x) {
     while (lo < hi) {
