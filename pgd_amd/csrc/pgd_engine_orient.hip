@@ -104,20 +104,19 @@ __global__ void k_out_start(const int64_t* __restrict__ row, const int32_t* __re
     }
 }
 
-// upper bound of the wedges a-b-c (b,c < a): sum of d(b) over in-neighbours b
+// upper bound of the wedges c-b-a with a > max(b, c) anchored at c: the sum
+// of |N+(b)| over the neighbours b of c (for b < c only the part of N+(b)
+// above c counts, which |N+(b)| bounds)
 __global__ void k_wedge_bound(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
-                              const int64_t* __restrict__ ostart, int64_t n, int64_t* __restrict__ ub) {
+                              const int32_t* __restrict__ outdeg, int64_t n, int64_t* __restrict__ ub) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t a = wave; a < n; a += nw) {
+    for (int64_t c = wave; c < n; c += nw) {
         int64_t s = 0;
-        for (int64_t p = row[a] + lane; p < ostart[a]; p += 64) {
-            int32_t b = col[p];
-            s += row[b + 1] - row[b];
-        }
+        for (int64_t p = row[c] + lane; p < row[c + 1]; p += 64) s += outdeg[col[p]];
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0) ub[a] = s;
+        if (lane == 0) ub[c] = s;
     }
 }
 
@@ -935,18 +934,20 @@ __global__ __launch_bounds__(NT) void k_local_k4q(const int64_t* __restrict__ ro
 }
 
 // ---------------------------------------------------------------------------
-// P: wedges a-b-c with b, c < a.  In-neighbours of a are the prefix
-// [row[a], ostart[a]) of its ascending list; for each b the c < a are the
-// prefix of N(b) below a.
+// P: every 4-cycle is found once, from the vertex c opposite its
+// highest-ranked vertex a, as two wedges c-b-a and c-b'-a with
+// a > max(b, c) and a > max(b', c).  For the anchor c the endpoints a of
+// the wedges through its neighbour b are the suffix of the ascending N(b)
+// above max(b, c).
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ uint32_t hslot(int32_t c) { return ((uint32_t)c * 2654435761u) >> 23; }  // 9 bits
 
-// light vertices: one wavefront (block of 64) per vertex, an LDS hash of
-// (c -> wedge count); distinct c <= bound <= 256 < 512 slots
+// light anchors: one wavefront (block of 64) per anchor c, an LDS hash of
+// (endpoint a -> wedge count); distinct a <= bound <= 256 < 512 slots.  The
+// wedge c-b-a adds cnt[a]-1 to the edges (b, a) and (c, b).
 __global__ __launch_bounds__(64) void k_wedge_light(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                                                     const int32_t* __restrict__ eid,
-                                                    const int64_t* __restrict__ ostart,
                                                     const int32_t* __restrict__ list, int64_t cnt,
                                                     uint64_t* __restrict__ P, unsigned long long* __restrict__ work) {
     __shared__ int32_t hkey[kHashSlots];
@@ -958,18 +959,19 @@ __global__ __launch_bounds__(64) void k_wedge_light(const int64_t* __restrict__ 
     const int tid = threadIdx.x;
     unsigned long long done = 0;
     for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
-        const int32_t a = list[li];
-        const int64_t ib = row[a], ie = ostart[a];
+        const int32_t c = list[li];
+        const int64_t ib = row[c], ie = row[c + 1];
         for (int s = tid; s < kHashSlots; s += 64) { hkey[s] = -1; hcnt[s] = 0; }
         __syncthreads();
         for (int pass = 0; pass < 2; pass++) {
             for (int64_t cb = ib; cb < ie; cb += 64) {
                 const int kc = (int)min<int64_t>(64, ie - cb);
                 if (tid < kc) {
-                    int32_t b = col[cb + tid];
-                    int64_t bb = row[b];
-                    bbeg[tid] = bb;
-                    pref[tid] = lower_bound_g(col, bb, row[b + 1], a) - bb;
+                    const int32_t b = col[cb + tid];
+                    const int64_t be = row[b + 1];
+                    const int64_t q = lower_bound_g(col, row[b], be, max(b, c) + 1);  // first endpoint above max(b, c)
+                    bbeg[tid] = q;
+                    pref[tid] = be - q;
                     psum[tid] = 0;
                 }
                 __syncthreads();
@@ -980,20 +982,20 @@ __global__ __launch_bounds__(64) void k_wedge_light(const int64_t* __restrict__ 
                     const int i = owner_of(pref, kc, t);
                     const int64_t ex = i ? pref[i - 1] : 0;
                     const int64_t slot = bbeg[i] + (t - ex);
-                    const int32_t c = col[slot];
-                    uint32_t h = hslot(c);
+                    const int32_t a = col[slot];
+                    uint32_t h = hslot(a);
                     if (pass == 0) {
                         for (;;) {
-                            int32_t old = atomicCAS(&hkey[h], -1, c);
-                            if (old == -1 || old == c) { atomicAdd(&hcnt[h], 1u); break; }
+                            int32_t old = atomicCAS(&hkey[h], -1, a);
+                            if (old == -1 || old == a) { atomicAdd(&hcnt[h], 1u); break; }
                             h = (h + 1) & (kHashSlots - 1);
                         }
                     } else {
-                        while (hkey[h] != c) h = (h + 1) & (kHashSlots - 1);
-                        const uint32_t s = hcnt[h] - 1;
-                        if (s) {
-                            atomicAdd((unsigned long long*)&P[eid[slot]], (unsigned long long)s);
-                            atomicAdd((unsigned long long*)&psum[i], (unsigned long long)s);
+                        while (hkey[h] != a) h = (h + 1) & (kHashSlots - 1);
+                        const uint32_t v = hcnt[h] - 1;
+                        if (v) {
+                            atomicAdd((unsigned long long*)&P[eid[slot]], (unsigned long long)v);
+                            atomicAdd((unsigned long long*)&psum[i], (unsigned long long)v);
                         }
                     }
                 }
@@ -1184,7 +1186,7 @@ void orient_build(pgd_ctx* ctx, const GraphView& g, OrientState& st, bool plan) 
     st.n_huge = make_bin(ctx, st.outdeg.get(), n, st.kmax_lds, INT32_MAX, st.b_huge, st.flags, st.off);
     {
         int gw = grid_cap(ctx, (n * 64 + 255) / 256, 16);
-        hipLaunchKernelGGL(k_wedge_bound, dim3(gw), dim3(256), 0, s, st.rrow.get(), st.rcol.get(), st.ostart.get(),
+        hipLaunchKernelGGL(k_wedge_bound, dim3(gw), dim3(256), 0, s, st.rrow.get(), st.rcol.get(), st.outdeg.get(),
                            n, st.ub.get());
         PGD_HIP(hipGetLastError());
     }
@@ -1456,8 +1458,8 @@ static void orient_count_wedges(pgd_ctx* ctx, OrientState& st, EdgeCounts out, T
     if (timer) timer->begin("k_wedge_light");
     if (st.n_light)
         hipLaunchKernelGGL(k_wedge_light, dim3(grid_cap(ctx, st.n_light, prm.grid_small)), dim3(64), 0, s,
-                           st.rrow.get(), st.rcol.get(), st.reid.get(), st.ostart.get(), st.b_light.get(), st.n_light,
-                           st.P.get(), w + 2);
+                           st.rrow.get(), st.rcol.get(), st.reid.get(), st.b_light.get(), st.n_light, st.P.get(),
+                           w + 2);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     if (planned) PGD_HIP(hipStreamWaitEvent(s, planned, 0));

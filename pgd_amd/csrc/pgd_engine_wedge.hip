@@ -1,20 +1,25 @@
 // pgd_engine_wedge.hip -- the 4-cycle pass (P) of the oriented engine for
-// heavy vertices, with the wedge endpoints counted in LDS tiles.
+// heavy anchors, with the wedge endpoints counted in LDS tiles.
 //
-// A 4-cycle is found once, from its highest-ranked vertex a, as two wedges
-// a-b-c and a-b'-c with b, b', c < a.  For the hub a, cnt_a[c] = number of
-// wedges a-.-c; the wedge a-b-c adds cnt_a[c]-1 to the edges (a,b) and (b,c).
-// cnt_a spans all ranks below a (millions for a hub), so it is cut into
-// endpoint tiles of 2^log2r ranks whose counters live in LDS.  For one hub,
-// the wedges of one tile are a set of segments: runs of the ascending lists
-// N(b) (b an in-neighbour of a) whose endpoints fall in the tile.  A per-vertex
-// tile table (one entry per run of equal tiles in N(b)) lets a block bucket a
-// hub's segments by tile with a counting sort, then count (LDS atomics) and
-// accumulate tile by tile.  The update of the edge (b,c) is added to a
-// per-slot accumulator PS (consecutive lanes hit consecutive slots of N(b)),
+// A 4-cycle is found once, from the vertex c opposite its highest-ranked
+// vertex a, as two wedges c-b-a and c-b'-a with a > max(b, c) and
+// a > max(b', c).  For the anchor c, cnt_c[a] = number of wedges c-.-a; the
+// wedge c-b-a adds cnt_c[a]-1 to the edges (b, a) and (c, b).  The endpoints
+// of the wedges through b are the suffix of the ascending list N(b) above
+// max(b, c); on a power-law graph b is mostly a hub and the suffix its
+// out-list, so these runs are long (about 190 wedges on R-MAT scale 22,
+// against 20 for the prefixes below a of the same wedges counted from a).
+// cnt_c spans the ranks above c, so it is cut into endpoint tiles of
+// 2^log2r ranks whose counters live in LDS.  For one anchor, the wedges of
+// one tile are a set of segments: runs of the suffixes whose endpoints fall
+// in the tile.  A per-vertex tile table (one entry per run of equal tiles in
+// N(b)) lets a block bucket an anchor's segments by tile with a counting
+// sort, then count (LDS atomics) and accumulate tile by tile.  The update of
+// the edge (b, a) is added to a per-slot accumulator PS (consecutive lanes
+// hit consecutive slots of N(b)), that of (c, b) once per segment, and PS is
 // folded into P by edge id at the end.
 //
-// The work is cut into items (hub, tile range) bounded in segments (the
+// The work is cut into items (anchor, tile range) bounded in segments (the
 // per-block buffer) and in wedges (load balance), and a persistent grid pulls
 // them largest first from a device queue.
 #include "pgd_orient.hpp"
@@ -43,27 +48,41 @@ __global__ void k_out_slots(const int64_t* __restrict__ row, const int64_t* __re
         for (int64_t s = ostart[v] + lane; s < row[v + 1]; s += 64) outslot[eid[s]] = (int32_t)s;
 }
 
-// wcnt[s] = |N(b) below a| for every in-slot s = (a, b) of a heavy hub a --
-// the offset, within N(b), of the twin slot b -> a -- and the hub's first
-// endpoint tile (the lowest tile any N(b) reaches); one wavefront per hub
+// sfx[s] = the offset in N(b) of the first endpoint above max(b, c), for
+// every slot s = (c, b) of a heavy anchor c: |N+(b)| from its end when
+// b > c, the twin slot b -> c plus one when b < c.  Also the anchor's tile
+// range: from the lowest tile of a suffix's first endpoint to the highest
+// tile of a suffix's last one (nt = 0 when every suffix is empty); one
+// wavefront per anchor
 __global__ void k_pair_wedges(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                               const int64_t* __restrict__ ostart, const int32_t* __restrict__ eid,
                               const int32_t* __restrict__ outslot, const int32_t* __restrict__ heavy, int64_t nheavy,
-                              int log2r, int32_t* __restrict__ wcnt, int32_t* __restrict__ hbase) {
+                              int log2r, int32_t* __restrict__ sfx, int32_t* __restrict__ hbase,
+                              int64_t* __restrict__ nt) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t h = wave; h < nheavy; h += nw) {
-        const int32_t a = heavy[h];
-        int32_t lo = a;
-        for (int64_t s = row[a] + lane; s < ostart[a]; s += 64) {
+        const int32_t c = heavy[h];
+        int32_t lo = INT32_MAX, hi = -1;
+        for (int64_t s = row[c] + lane; s < row[c + 1]; s += 64) {
             const int32_t b = col[s];
-            const int64_t rb = row[b];
-            wcnt[s] = (int32_t)(outslot[eid[s]] - rb);
-            lo = min(lo, col[rb]);
+            const int64_t rb = row[b], re = row[b + 1];
+            const int64_t q = b > c ? ostart[b] : (int64_t)outslot[eid[s]] + 1;
+            sfx[s] = (int32_t)(q - rb);
+            if (q < re) {
+                lo = min(lo, col[q]);
+                hi = max(hi, col[re - 1]);
+            }
         }
-        for (int off = 32; off > 0; off >>= 1) lo = min(lo, __shfl_xor(lo, off, 64));
-        if (lane == 0) hbase[h] = lo >> log2r;
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = min(lo, __shfl_xor(lo, off, 64));
+            hi = max(hi, __shfl_xor(hi, off, 64));
+        }
+        if (lane == 0) {
+            hbase[h] = hi >= 0 ? lo >> log2r : 0;
+            nt[h] = hi >= 0 ? (int64_t)(hi >> log2r) - (lo >> log2r) + 1 : 0;
+        }
     }
 }
 
@@ -96,13 +115,6 @@ __global__ void k_tt_fill(const int64_t* __restrict__ row, const int32_t* __rest
     if (blockIdx.x == 0 && threadIdx.x == 0) ttrow[n] = total;
 }
 
-// number of endpoint tiles of each heavy hub a: from its first tile to the
-// tile of rank a-1
-__global__ void k_hub_tiles(const int32_t* __restrict__ heavy, const int32_t* __restrict__ hbase, int64_t nheavy,
-                            int log2r, int64_t* __restrict__ nt) {
-    GRID_STRIDE(h, nheavy) nt[h] = ((int64_t)(heavy[h] - 1) >> log2r) + 1 - hbase[h];
-}
-
 // first tile-table entry of b with tile >= t (entries ascending by tile)
 __device__ __forceinline__ int64_t tt_seek(const uint64_t* __restrict__ tt, int64_t lo, int64_t hi, uint32_t t) {
     while (lo < hi) {
@@ -124,16 +136,16 @@ __device__ __forceinline__ int64_t tt_seek_off(const uint64_t* __restrict__ tt, 
     return lo;
 }
 
-// The tile-table entries of 64 pairs (a, b), flattened across a wavefront.
-// Lane i describes pair i: entries [jb, je) of b's table (possibly empty),
-// the end of b's table, |N(b)| and the pair's bound w.  fn(pair, tile,
-// offset, len) is called once per entry, consecutive lanes on consecutive
-// entries of a pair; pair is the rank of the lane among the lanes with
-// entries.  flag: 64 words of the wavefront's LDS (the start flag of position
+// The tile-table entries of 64 pairs (c, b), flattened across a wavefront.
+// Lane i describes pair i: entries [jb, je) of b's table (possibly empty;
+// entry jb holds the suffix start q), the end of b's table, |N(b)| and q.
+// fn(pair, tile, offset, len) is called once per entry with the part of the
+// entry's run at or above q, consecutive lanes on consecutive entries of a
+// pair; pair is the rank of the lane among the lanes with entries.  flag: 64 words of the wavefront's LDS (the start flag of position
 // p of a round is byte p >> 6 of word p & 63, so each lane clears and reads
 // its own word); pj, pe, pw, pd: 64 entries each, indexed by that rank.
 template <class F>
-__device__ __forceinline__ void walk_entries(int64_t jb, int64_t je, int64_t tend, uint32_t deg, uint32_t w,
+__device__ __forceinline__ void walk_entries(int64_t jb, int64_t je, int64_t tend, uint32_t deg, uint32_t q0,
                                              int64_t* pj, int64_t* pe, uint32_t* pw, uint32_t* pd, uint32_t* flag,
                                              const uint64_t* __restrict__ tt, int lane, F&& fn) {
     const int32_t cnt = (int32_t)(je > jb ? je - jb : 0);
@@ -150,7 +162,7 @@ __device__ __forceinline__ void walk_entries(int64_t jb, int64_t je, int64_t ten
         const int r = __popcll(nz & ((1ULL << lane) - 1));
         pj[r] = jb - (x - cnt);  // entry index = pj[owner] + flattened position
         pe[r] = tend;
-        pw[r] = w;
+        pw[r] = q0;
         pd[r] = deg;
     }
     const int32_t es = cnt ? x - cnt : INT32_MAX;
@@ -186,21 +198,19 @@ __device__ __forceinline__ void walk_entries(int64_t jb, int64_t je, int64_t ten
 #pragma unroll
         for (int u = 0; u < kU; u++)
             if (o[u] >= 0) {
-                const uint32_t off = (uint32_t)e[u];
-                const uint32_t len = min((uint32_t)en[u], pw[o[u]]) - off;
-                fn(o[u], (uint32_t)(e[u] >> 32), off, len);
+                const uint32_t off = max((uint32_t)e[u], pw[o[u]]);
+                fn(o[u], (uint32_t)(e[u] >> 32), off, (uint32_t)en[u] - off);
             }
         __builtin_amdgcn_wave_barrier();
     }
 }
 
-// segments and wedges per (hub, tile) cell; one block per hub, its pairs
-// 64 per wavefront
+// segments and wedges per (anchor, tile) cell; one block per anchor, its
+// pairs 64 per wavefront
 template <int NT>
 __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
-                                                   const int64_t* __restrict__ ostart,
                                                    const int32_t* __restrict__ heavy, int64_t nheavy,
-                                                   const int32_t* __restrict__ wcnt, const uint64_t* __restrict__ tt,
+                                                   const int32_t* __restrict__ sfx, const uint64_t* __restrict__ tt,
                                                    const int64_t* __restrict__ ttrow, const int32_t* __restrict__ hbase,
                                                    const int64_t* __restrict__ htoff, uint32_t* __restrict__ htseg,
                                                    uint64_t* __restrict__ htw, unsigned long long* __restrict__ acc,
@@ -214,27 +224,29 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
     __shared__ uint32_t flag[NW][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
-        const int32_t a = heavy[h];
+        const int32_t c = heavy[h];
         const int64_t c0 = htoff[h], T = htoff[h + 1] - c0;
         const uint32_t base = (uint32_t)hbase[h];
         const bool in_lds = T <= kPlanLds;
         if (in_lds)
             for (int i = threadIdx.x; i < T; i += NT) { lseg[i] = 0; lw[i] = 0; }
         __syncthreads();
-        const int64_t ib = row[a], ie = ostart[a];
+        const int64_t ib = row[c], ie = row[c + 1];
         for (int64_t g0 = ib + (int64_t)wv * 64; g0 < ie; g0 += NT) {
             const int64_t s = g0 + lane;
             int64_t jb = 0, je = 0, tend = 0;
-            uint32_t w = 0, deg = 0;
+            uint32_t q = 0, deg = 0;
             if (s < ie) {
                 const int32_t b = col[s];
-                w = (uint32_t)wcnt[s];
-                jb = ttrow[b];
-                tend = ttrow[b + 1];
+                q = (uint32_t)sfx[s];
                 deg = (uint32_t)(row[b + 1] - row[b]);
-                je = tt_seek_off(tt, jb, tend, w);  // entries starting below the bound
+                if (q < deg) {
+                    tend = ttrow[b + 1];
+                    jb = tt_seek_off(tt, ttrow[b], tend, q + 1) - 1;  // the entry holding q
+                    je = tend;
+                }
             }
-            walk_entries(jb, je, tend, deg, w, pj[wv], pe[wv], pw[wv], pd[wv], flag[wv], tt, lane,
+            walk_entries(jb, je, tend, deg, q, pj[wv], pe[wv], pw[wv], pd[wv], flag[wv], tt, lane,
                          [&](int, uint32_t tile, uint32_t, uint32_t len) {
                              const uint32_t t = tile - base;
                              mseg += 1;
@@ -373,29 +385,34 @@ __device__ __forceinline__ void slot_add(uint64_t* __restrict__ PS, uint32_t* __
     else atomicAdd(&PS32[slot], v);
 }
 
-// One pass of one wavefront over its share of the segments of a tile.
-// pass 0 counts endpoints, pass 1 adds cnt-1 per wedge to the slot's
-// accumulator (PS32, or PS for a wide list) and, per segment, to the edge
-// (a, b) (PS).  mode 0: 16-bit counters
-// over the whole tile; with chk, a count reaching 65536 sets *ovf (without
-// chk the hub has fewer than 65536 in-neighbours, so no count can); mode
-// 1/2: 32-bit counters over the lower/upper half of the tile, other
-// endpoints skipped.
+// One pass of the block over the segments of a tile, 64 segments (a group)
+// at a time.  pass 0 counts endpoints, pass 1 adds cnt-1 per wedge to the
+// slot's accumulator (PS32, or PS for a wide list) and, per segment, to the
+// anchor's edge (c, b) (PS).  mode 0: 16-bit counters over the whole tile;
+// with chk, a count reaching 65536 sets *ovf (without chk the anchor has
+// fewer than 65536 neighbours, so no count can); mode 1/2: 32-bit counters
+// over the lower/upper half of the tile, other endpoints skipped.
 //
-// Each wavefront takes groups of 64 segments, scans their lengths in
-// registers and walks the flattened wedges; every lane computes kU positions
-// per round.  Positions past the end are clamped to the last one (same
-// owner, same slot), so the owner lookups, the endpoint loads and the
-// counter reads are issued for all lanes without branches and overlap; only
-// the updates are predicated.
+// Every wavefront scans the group's lengths in registers; the group's
+// flattened wedges are walked in rounds of 64 kU positions, round r by
+// wavefront r mod NW (a group's segments are few and long: the rounds, not
+// the segments, are shared out).  Every lane computes kU positions per
+// round.  Positions past the end are clamped to the last one (same owner,
+// same slot), so the owner lookups, the endpoint loads and the counter reads
+// are issued for all lanes without branches and overlap; only the updates
+// are predicated.  gbase/gslot (the group's slot bases and anchor slots) are
+// written by wavefront 0; gpsum collects the per-segment sums of pass 1,
+// flushed by wavefront 0 after the group's closing barrier.
 template <int NT>
 __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t sb, uint32_t se, int32_t c0,
                                           int log2r, const WedgeSeg* __restrict__ seg,
                                           const int32_t* __restrict__ col, uint32_t* cnt,
                                           uint64_t* __restrict__ PS, uint32_t* __restrict__ PS32,
-                                          int64_t* wbase, int32_t* wslot,
-                                          unsigned long long* wpsum, uint32_t* wflag, int* ovf, int* rep, int lane,
-                                          int w, unsigned long long* done) {
+                                          int64_t* gbase, int32_t* gslot, unsigned long long* gpsum,
+                                          uint32_t* wflag, int* ovf, int* rep, int lane, int w,
+                                          unsigned long long* done) {
+    constexpr int NW = NT / 64;
+    constexpr int kU = 4;
     const int32_t half = 1 << (log2r - 1);
     const int32_t lo = mode == 2 ? half : 0;
     bool again = false;  // mode 0, pass 0: some endpoint of this wavefront's wedges was seen before
@@ -403,7 +420,7 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
     // word c mod half, so consecutive endpoints use consecutive words (banks)
     const int hb = log2r - 1;
     const int32_t hm = half - 1;
-    for (uint32_t gb = sb + (uint32_t)w * 64; gb < se; gb += NT) {
+    for (uint32_t gb = sb; gb < se; gb += 64) {
         const int kc = (int)min<uint32_t>(64, se - gb);
         int32_t len = 0, sl = 0;
         int64_t st = 0;
@@ -416,30 +433,26 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
             sl = g.sab;
         }
         const uint64_t wmask = __ballot(wide);  // segments of wide lists
-        int32_t x = len;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int32_t y = __shfl_up(x, off, 64);
-            if (lane >= off) x += y;
+        const int32_t x = (int32_t)wave_incl_sum((uint32_t)len);
+        if (w == 0) {
+            // slot of flattened position q in segment i: gbase[i] + q
+            gbase[lane] = st - (int64_t)(x - len);
+            gslot[lane] = sl;
+            gpsum[lane] = 0;
         }
-        // slot of flattened position q in segment i: wbase[i] + q
-        wbase[lane] = st - (int64_t)(x - len);
-        wslot[lane] = sl;
-        wpsum[lane] = 0;
-        __builtin_amdgcn_wave_barrier();
+        __syncthreads();
         const int32_t total = __shfl(x, 63, 64);
         // wedges counted (once per tile) and wedges accumulated (tiles with
         // a repeated endpoint; an overflowing tile once)
-        if (lane == 0 && (pass ? mode < 2 : mode == 0)) done[pass] += (unsigned long long)total;
+        if (w == 0 && lane == 0 && (pass ? mode < 2 : mode == 0)) done[pass] += (unsigned long long)total;
         // (the owner of position q, the segment holding it, is the last segment
         // starting at or before q: the segment starts inside a round are
         // flagged in LDS and read back as one 64-bit ballot per step, so the
         // owner is a popcount, not a search -- segments are non-empty, so their
         // starts are distinct)
-        constexpr int kU = 4;
         const int32_t es = lane < kc ? x - len : INT32_MAX;  // exclusive start of segment `lane`
         const uint64_t upto = (2ULL << lane) - 1;             // lanes 0..lane
-        for (int32_t rb0 = 0; rb0 < total; rb0 += 64 * kU) {
+        for (int32_t rb0 = w * 64 * kU; rb0 < total; rb0 += NW * 64 * kU) {
             int o[kU];
             bool ok[kU];
             int64_t slot[kU];
@@ -464,7 +477,7 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
                 // past the end: the owner is the last segment; clamp to its last position
                 o[u] = cum + __popcll(starts[u] & upto);
                 cum += __popcll(starts[u]);
-                slot[u] = wbase[o[u]] + (ok[u] ? q : total - 1);
+                slot[u] = gbase[o[u]] + (ok[u] ? q : total - 1);
             }
 #pragma unroll
             for (int u = 0; u < kU; u++) c[u] = col[slot[u]] - c0;
@@ -494,7 +507,7 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
 #pragma unroll
                 for (int u = 0; u < kU; u++)
                     if (ok[u] && v[u]) slot_add(PS, PS32, wmask, o[u], slot[u], v[u]);
-                // the edge (a, b) of each segment: the lanes of one segment are
+                // the edge (c, b) of each segment: the lanes of one segment are
                 // a run (delimited by segment starts), summed by a wavefront
                 // scan and added once by the run's last lane (v < 2^16, so a
                 // step's sums fit 32 bits)
@@ -507,7 +520,7 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
                     const uint32_t before = __shfl(P, rs > 0 ? rs - 1 : 0, 64);
                     const uint32_t run = P - (rs > 0 ? before : 0u);
                     const bool last = lane == 63 || ((starts[u] >> (lane + 1)) & 1);
-                    if (last && run) atomicAdd(&wpsum[o[u]], (unsigned long long)run);
+                    if (last && run) atomicAdd(&gpsum[o[u]], (unsigned long long)run);
                 }
             } else {
                 bool in[kU];
@@ -531,13 +544,15 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
                 for (int u = 0; u < kU; u++)
                     if (in[u] && v[u]) {
                         slot_add(PS, PS32, wmask, o[u], slot[u], v[u]);
-                        atomicAdd(&wpsum[o[u]], (unsigned long long)v[u]);
+                        atomicAdd(&gpsum[o[u]], (unsigned long long)v[u]);
                     }
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        if (pass == 1 && lane < kc && wpsum[lane]) atomicAdd((unsigned long long*)&PS[wslot[lane]], wpsum[lane]);
-        __builtin_amdgcn_wave_barrier();
+        // the group's closing barrier: every wavefront is done with gbase,
+        // gslot and gpsum before wavefront 0 flushes and rewrites them
+        __syncthreads();
+        if (pass == 1 && w == 0 && lane < kc && gpsum[lane])
+            atomicAdd((unsigned long long*)&PS[gslot[lane]], gpsum[lane]);
     }
     if (pass == 0 && mode == 0 && __ballot(again) && lane == 0) *rep = 1;
 }
@@ -665,8 +680,8 @@ __device__ __forceinline__ void clear_lds(uint32_t* p, int n) {
 // synchronises between the count and the accumulate pass of a tile.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_wedge_tiled(
-    const int64_t* __restrict__ row, const int32_t* __restrict__ col, const int64_t* __restrict__ ostart,
-    const int32_t* __restrict__ heavy, const int32_t* __restrict__ wcnt, const uint64_t* __restrict__ tt,
+    const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+    const int32_t* __restrict__ heavy, const int32_t* __restrict__ sfx, const uint64_t* __restrict__ tt,
     const int64_t* __restrict__ ttrow, const int32_t* __restrict__ hbase, const int64_t* __restrict__ htoff,
     const uint32_t* __restrict__ htseg, const WedgeItem* __restrict__ items, int64_t nitems, int log2r,
     WedgeSeg* __restrict__ segs, int64_t segcap, uint32_t* __restrict__ qctr, uint64_t* __restrict__ PS,
@@ -680,6 +695,9 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     __shared__ int32_t wslot[NW][64];
     __shared__ uint32_t wflag[NW][64];  // segment-start flags of one round of a wavefront (4 bytes per lane)
     __shared__ unsigned long long wpsum[NW][64];
+    __shared__ int64_t gbase[64];  // the block passes' current group of segments
+    __shared__ int32_t gslot[64];
+    __shared__ unsigned long long gpsum[64];
     __shared__ int64_t item_sh;
     __shared__ int ovf, rep;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -696,11 +714,10 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
         __syncthreads();
         if (it >= nitems) break;
         const WedgeItem W = items[it];
-        const int32_t a = heavy[W.hub];
-        const int32_t hb0 = hbase[W.hub];
-        const int64_t ib = row[a], ie = ostart[a];
+        const int32_t a = heavy[W.hub];  // the anchor
+        const int64_t ib = row[a], ie = row[a + 1];
         const int T = W.t1 - W.t0;
-        // bucket the hub's segments in [t0, t1) by tile: offsets from the plan's
+        // bucket the anchor's segments in [t0, t1) by tile: offsets from the plan's
         // per-cell counts, then one walk scatters
         if (tid == 0) {
             const uint32_t* cells = htseg + htoff[W.hub] + (W.t0 - hbase[W.hub]);
@@ -714,32 +731,31 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
         }
         __syncthreads();
         for (int64_t s = ib + tid; s < ie; s += NT) {
-            const uint32_t wc = (uint32_t)wcnt[s];
-            if (!wc) continue;
             const int32_t b = col[s];
-            const int64_t j1 = ttrow[b + 1];
-            // every list's first entry is at or above the hub's first tile, so
-            // an item that starts there needs no search
-            int64_t j = W.t0 > hb0 ? tt_seek(tt, ttrow[b], j1, (uint32_t)W.t0) : ttrow[b];
+            const uint32_t q = (uint32_t)sfx[s];  // the suffix of N(b) above max(b, a)
             const uint32_t deg = (uint32_t)(row[b + 1] - row[b]);
+            if (q >= deg) continue;
+            const int64_t j1 = ttrow[b + 1];
+            int64_t j = tt_seek_off(tt, ttrow[b], j1, q + 1) - 1;  // the entry holding q
+            if ((int32_t)(tt[j] >> 32) < W.t0) j = tt_seek(tt, j, j1, (uint32_t)W.t0);
             for (; j < j1; j++) {
                 const uint64_t e = tt[j];
-                const uint32_t off = (uint32_t)e;
+                const uint32_t off = max((uint32_t)e, q);
                 const int t = (int)(e >> 32) - W.t0;
-                if (off >= wc || t >= T) break;
+                if (t >= T) break;
                 const uint32_t nxt = j + 1 < j1 ? (uint32_t)tt[j + 1] : deg;
                 const uint32_t p = atomicAdd(&cur[t], 1u);
-                const uint32_t lw = (min(nxt, wc) - off) | (deg >= (uint32_t)wide_deg ? 0x80000000u : 0u);
+                const uint32_t lw = (nxt - off) | (deg >= (uint32_t)wide_deg ? 0x80000000u : 0u);
                 seg[p] = WedgeSeg{(uint32_t)(row[b] + off), (int32_t)lw, (int32_t)s};
             }
         }
         __syncthreads();
         // tile by tile: count the endpoints, then accumulate cnt-1 per wedge.
         // Counters are 16-bit, two per word (mode 0); a count that would pass
-        // 65535 (a hub with that many in-neighbours on one endpoint) flags the
-        // tile, which is then recounted as two half tiles with 32-bit
+        // 65535 (an anchor with that many neighbours on one endpoint) flags
+        // the tile, which is then recounted as two half tiles with 32-bit
         // counters in the same LDS (modes 1 and 2).
-        // a count is at most the hub's in-degree: below 65536 no 16-bit counter can overflow
+        // a count is at most the anchor's degree: below 65536 no 16-bit counter can overflow
         const bool chk = ie - ib > 65535;
         // small tiles first, one per wavefront in the wavefront's slice of
         // the counter array; cur[t] then marks the tiles left for the block
@@ -762,25 +778,25 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
                 rep = 0;
             }
             __syncthreads();
-            tile_pass<NT>(0, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w], wpsum[w], wflag[w],
-                          &ovf, &rep, lane, w, done);
+            tile_pass<NT>(0, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gslot, gpsum, wflag[w], &ovf, &rep,
+                          lane, w, done);
             __syncthreads();
             if (!ovf) {
                 // every count 1: no 4-cycle closes in this tile
                 if (rep) {
-                    tile_pass<NT>(1, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w],
-                                  wpsum[w], wflag[w], &ovf, &rep, lane, w, done);
+                    tile_pass<NT>(1, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gslot, gpsum,
+                                  wflag[w], &ovf, &rep, lane, w, done);
                     __syncthreads();
                 }
             } else {
                 for (int mode = 1; mode <= 2; mode++) {
                     clear_lds<NT>(cnt, R / 2);
                     __syncthreads();
-                    tile_pass<NT>(0, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w],
-                                  wpsum[w], wflag[w], &ovf, &rep, lane, w, done);
+                    tile_pass<NT>(0, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gslot, gpsum,
+                                  wflag[w], &ovf, &rep, lane, w, done);
                     __syncthreads();
-                    tile_pass<NT>(1, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, wbase[w], wslot[w],
-                                  wpsum[w], wflag[w], &ovf, &rep, lane, w, done);
+                    tile_pass<NT>(1, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gslot, gpsum,
+                                  wflag[w], &ovf, &rep, lane, w, done);
                     __syncthreads();
                 }
             }
@@ -864,7 +880,7 @@ void launch_tiled(pgd_ctx* ctx, OrientState& st, unsigned long long* work) {
     WedgePlan& wp = st.wp;
     const size_t dyn = (size_t)2 << wp.log2r;
     hipLaunchKernelGGL(k_wedge_tiled<NT>, dim3(wp.grid), dim3(NT), dyn, ctx->stream, st.rrow.get(), st.rcol.get(),
-                       st.ostart.get(), st.b_heavy.get(), wp.wcnt.get(), wp.tt.get(), wp.ttrow.get(), wp.hbase.get(),
+                       st.b_heavy.get(), wp.sfx.get(), wp.tt.get(), wp.ttrow.get(), wp.hbase.get(),
                        wp.htoff.get(), wp.htseg.get(), wp.items.get(), wp.nitems, wp.log2r, wp.segs.get(), wp.segcap,
                        wp.qctr.get(), wp.ps.get(), wp.ps32.get(), wp.wide_deg, work);
 }
@@ -884,8 +900,9 @@ int tiled_blocks_per_cu(pgd_ctx* ctx, int log2r) {
 
 }  // namespace
 
-// plan of the heavy 4-cycle pass: pair wedge counts, tile table, per-cell
-// sizes, items sorted largest first, per-block buffers (host synchronisations)
+// plan of the heavy 4-cycle pass: suffix starts and tile ranges of the
+// anchors' pairs, tile table, per-cell sizes, items sorted largest first,
+// per-block buffers (host synchronisations)
 void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     hipStream_t s = ctx->stream;
     WedgePlan& wp = st.wp;
@@ -896,9 +913,11 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
               "wedge block size must be 256, 512 or 1024");
     PGD_CHECK(nnz < INT32_MAX, PGD_ERR_UNSUPPORTED, "2m >= 2^31 slots: the wedge plan indexes slots as int32");
     wp.log2r = prm.wedge_log2r;
-    // pair wedge counts
-    ensure(wp.wcnt, ctx, nnz);
+    // suffix starts of the anchors' pairs and the anchors' tile ranges
+    ensure(wp.sfx, ctx, nnz);
     ensure(wp.hbase, ctx, nh);
+    DBuf<int64_t> nt;
+    nt.alloc(ctx, nh + 1);
     {
         // st.v0 (2m int32, free after the relabel) holds the out-slot of every edge
         int32_t* outslot = st.v0.get();
@@ -906,7 +925,7 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
                            st.rrow.get(), st.ostart.get(), st.reid.get(), n, outslot);
         hipLaunchKernelGGL(k_pair_wedges, dim3(grid_cap(ctx, (nh * 64 + 255) / 256, 16)), dim3(256), 0, s,
                            st.rrow.get(), st.rcol.get(), st.ostart.get(), st.reid.get(), outslot, st.b_heavy.get(),
-                           nh, wp.log2r, wp.wcnt.get(), wp.hbase.get());
+                           nh, wp.log2r, wp.sfx.get(), wp.hbase.get(), nt.get());
         PGD_HIP(hipGetLastError());
     }
     PGD_HIP(hipGetLastError());
@@ -929,14 +948,10 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
                            flag, pos, wp.tt.get(), wp.ttrow.get());
         PGD_HIP(hipGetLastError());
     }
-    // (hub, tile) cells
+    // (anchor, tile) cells
     ensure(wp.htoff, ctx, nh + 1);
     int64_t ncell = 0;
     {
-        DBuf<int64_t> nt;
-        nt.alloc(ctx, nh + 1);
-        hipLaunchKernelGGL(k_hub_tiles, dim3(grid_cap(ctx, (nh + 255) / 256, 16)), dim3(256), 0, s,
-                           st.b_heavy.get(), wp.hbase.get(), nh, wp.log2r, nt.get());
         hipLaunchKernelGGL(w_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)nt.get() + nh, (int64_t)1);
         PGD_HIP(hipGetLastError());
         excl_sum(ctx, nt.get(), wp.htoff.get(), (size_t)(nh + 1));
@@ -952,7 +967,7 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         ensure(wp.pacc, ctx, 3);
         hipLaunchKernelGGL(w_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)wp.pacc.get(), (int64_t)3);
         hipLaunchKernelGGL(k_plan_cells<256>, dim3(grid_cap(ctx, nh, 8)), dim3(256), 0, s, st.rrow.get(),
-                           st.rcol.get(), st.ostart.get(), st.b_heavy.get(), nh, wp.wcnt.get(), wp.tt.get(),
+                           st.rcol.get(), st.b_heavy.get(), nh, wp.sfx.get(), wp.tt.get(),
                            wp.ttrow.get(), wp.hbase.get(), wp.htoff.get(), wp.htseg.get(), wp.htw.get(),
                            wp.pacc.get(), pinned_words(ctx));
         PGD_HIP(hipGetLastError());
@@ -960,7 +975,7 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     PGD_HIP(hipStreamSynchronize(s));
     const uint64_t segtot = ((volatile unsigned long long*)ctx->pin)[0];
     wp.wedges = (int64_t)((volatile unsigned long long*)ctx->pin)[1];
-    // a cell holds at most one segment per in-neighbour of its hub: the
+    // a cell holds at most one segment per neighbour of its anchor: the
     // maximum degree bounds it (read from the cells when that bound is large)
     const uint32_t maxcell = st.max_deg <= (1 << 20) ? (uint32_t)st.max_deg
                                                       : device_max(ctx, wp.htseg.get(), (size_t)ncell);

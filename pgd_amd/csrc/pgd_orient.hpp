@@ -240,11 +240,11 @@ struct Timer {
 };
 
 // ---------------------------------------------------------------------------
-// tiled wedge plan (heavy vertices of the 4-cycle pass)
+// tiled wedge plan (heavy anchors of the 4-cycle pass)
 // ---------------------------------------------------------------------------
 // One segment: the run of N(b) (ascending CSR slots [start, start+len)) whose
-// endpoints c fall in one endpoint tile, for the in-neighbour b of a hub a;
-// sab is the slot of b in N(a) (the edge (a, b)).  The top bit of len marks
+// endpoints a fall in one endpoint tile, above max(b, c), for the neighbour
+// b of an anchor c; sab is the slot of b in N(c) (the edge (c, b)).  The top bit of len marks
 // a wide list (d(b) * max degree >= 2^32: its slots take 64-bit sums).
 // Slots are below 2^31 (the plan requires 2m < 2^31): 12 bytes a segment.
 struct WedgeSeg {
@@ -253,8 +253,8 @@ struct WedgeSeg {
     int32_t sab;
 };
 
-// one unit of work of the tiled kernel: hub (index into the heavy list) and
-// its endpoint tiles [t0, t1)
+// one unit of work of the tiled kernel: anchor (index into the heavy list)
+// and its endpoint tiles [t0, t1)
 struct WedgeItem {
     int32_t hub, t0, t1, pad;
 };
@@ -264,13 +264,13 @@ struct WedgePlan {
     int tmax = 128;                 // tiles per item
     int64_t nitems = 0, segcap = 0, wedges = 0;
     int grid = 0;
-    DBuf<int32_t> wcnt;             // per in-slot: |N(b) below a| for the pair (a,b)
+    DBuf<int32_t> sfx;              // per slot (c, b) of a heavy anchor: offset in N(b) of the first endpoint above max(b, c)
     DBuf<uint64_t> tt;              // tile table entries: (tile << 32) | offset in N(b)
     DBuf<int64_t> ttrow;            // first tile-table entry of each vertex (n+1)
-    DBuf<int32_t> hbase;            // first endpoint tile of each heavy hub
-    DBuf<int64_t> htoff;            // first (hub, tile) cell of each heavy hub (+1)
-    DBuf<uint32_t> htseg;           // segments per (hub, tile) cell
-    DBuf<uint64_t> htw;             // wedges per (hub, tile) cell
+    DBuf<int32_t> hbase;            // first endpoint tile of each heavy anchor
+    DBuf<int64_t> htoff;            // first (anchor, tile) cell of each heavy anchor (+1)
+    DBuf<uint32_t> htseg;           // segments per (anchor, tile) cell
+    DBuf<uint64_t> htw;             // wedges per (anchor, tile) cell
     DBuf<WedgeItem> items;          // work queue, largest first
     DBuf<WedgeSeg> segs;            // per-block segment buffers, segcap each
     DBuf<uint64_t> ps;              // per-slot P accumulator (ascending CSR slots), 64-bit sums
