@@ -529,7 +529,14 @@ void batch_collect(pgd_batch* b, std::vector<Request*>& reqs) {
 // kLargeLines lines or more), so a large graph never holds small requests
 // back; each worker alternates two private streams, building one batch while
 // the previous one counts.  Both share the HBM budget and the statistics.
+// A large graph nearly fills the device by itself, so large requests are not
+// gathered into one big union: a batch of them stops at kLargeBatchLines
+// lines (at least one request), which keeps the two streams alternating.
+// On R-MAT scale-18 requests (2^22 lines each) a bound of two per batch
+// served 5620 graphs/s of the 1% mix, against 4681 with one, 4764 with four
+// and 2063 with no bound.
 constexpr int64_t kLargeLines = (int64_t)1 << 18;
+constexpr int64_t kLargeBatchLines = (int64_t)1 << 23;
 
 struct ServerWorker {
     pgd_ctx* lane[2] = {nullptr, nullptr};  // private contexts: one stream each
@@ -571,15 +578,17 @@ bool pgd_server::head_admissible(int c) const {
 std::vector<Request*> pgd_server::admit(int c) {
     std::deque<Request*>& q = cls[c].pending;
     std::vector<Request*> out;
-    int64_t bytes = 0;
+    int64_t bytes = 0, lines = 0;
     while (!q.empty()) {
         Request* r = q.front();
         const bool idle = out.empty() && inflight_reqs == 0;  // the head alone always fits an empty device
         if (!out.empty() && r->flags != out[0]->flags) break;
+        if (c == 1 && !out.empty() && lines + (int64_t)r->src.size() > kLargeBatchLines) break;
         if ((int64_t)out.size() + inflight_reqs >= max_inflight && !idle) break;
         if (inflight_bytes + bytes + r->footprint > budget && !idle) break;
         out.push_back(r);
         bytes += r->footprint;
+        lines += (int64_t)r->src.size();
         q.pop_front();
     }
     inflight_bytes += bytes;

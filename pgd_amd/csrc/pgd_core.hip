@@ -295,7 +295,7 @@ __global__ void k_csr_sym_check(const uint64_t* __restrict__ kf, const uint64_t*
 // line i (kept, compacted index o) -> slots 2o (src->dst) and 2o+1 (dst->src)
 __global__ void k_make_slots(const int32_t* __restrict__ s, const int32_t* __restrict__ d,
                              const int32_t* __restrict__ keep, const int64_t* __restrict__ off, int64_t L,
-                             uint32_t* __restrict__ owner, int32_t* __restrict__ nb, int32_t* __restrict__ deg) {
+                             uint32_t* __restrict__ owner, int32_t* __restrict__ nb) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L; i += (int64_t)gridDim.x * blockDim.x) {
         if (!keep[i]) continue;
         int64_t o = off[i];
@@ -304,9 +304,27 @@ __global__ void k_make_slots(const int32_t* __restrict__ s, const int32_t* __res
         nb[2 * o] = b;
         owner[2 * o + 1] = (uint32_t)b;
         nb[2 * o + 1] = a;
-        atomicAdd(&deg[a], 1);
-        atomicAdd(&deg[b], 1);
     }
+}
+
+// CSR offsets from the slots' owners sorted ascending: row[v] = the first
+// slot whose owner is >= v (row[n] = nnz), by binary search; no atomics, so
+// a hub's many slots do not all update one degree counter
+__global__ void k_row_from_owners(const int32_t* __restrict__ owner, int64_t nnz, int64_t n,
+                                  int64_t* __restrict__ row) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v <= n; v += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = nnz;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (owner[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        row[v] = lo;
+    }
+}
+__global__ void k_deg_from_row(const int64_t* __restrict__ row, int64_t n, int32_t* __restrict__ deg) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+        deg[v] = (int32_t)(row[v + 1] - row[v]);
 }
 
 // owner of every CSR slot and the degrees, from the offsets (one wavefront per vertex)
@@ -327,10 +345,6 @@ __global__ void k_csr_owner(const int64_t* __restrict__ row, const int32_t* __re
             if (u < 0 || u >= n || u == v) atomicOr(bad, 1u);
         }
     }
-}
-
-__global__ void k_fill_row_tail(int64_t* row, int64_t n, int64_t nnz) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) row[n] = nnz;
 }
 
 // ===========================================================================
@@ -835,22 +849,25 @@ void coo_build(pgd_ctx* ctx, pgd_graph* g, int64_t n, int64_t L, DBuf<int32_t>& 
         g->col.alloc(ctx, nnz);
         g->src.alloc(ctx, nnz);
         g->deg.alloc(ctx, std::max<int64_t>(n, 1));
-        PGD_HIP(hipMemsetAsync(g->deg.get(), 0, std::max<int64_t>(n, 1) * sizeof(int32_t), s));
         if (nnz) {
             DBuf<uint32_t> own;
             DBuf<int32_t> nb;
             own.alloc(ctx, nnz);
             nb.alloc(ctx, nnz);
             hipLaunchKernelGGL(k_make_slots, dim3(grid), dim3(kBlock), 0, s, ds.get(), dd.get(), keep.get(),
-                               off.get(), L, own.get(), nb.get(), g->deg.get());
+                               off.get(), L, own.get(), nb.get());
             PGD_HIP(hipGetLastError());
             // stable by owner: every adjacency list stays in line order
             sort_pairs(ctx, own.get(), (uint32_t*)g->src.get(), nb.get(), g->col.get(), (size_t)nnz, 0,
                        (unsigned)bits_for(n - 1));
         }
-        if (n) exclusive_sum(ctx, g->deg.get(), g->row.get(), (size_t)n);
-        hipLaunchKernelGGL(k_fill_row_tail, dim3(1), dim3(1), 0, s, g->row.get(), n, nnz);
-        PGD_HIP(hipGetLastError());
+        {
+            // owners ascending (src, n <= INT32_MAX): offsets by search, degrees from them
+            const int gv = std::min(grid_for(n + 1), ctx->num_cus * 16);
+            hipLaunchKernelGGL(k_row_from_owners, dim3(gv), dim3(kBlock), 0, s, g->src.get(), nnz, n, g->row.get());
+            if (n) hipLaunchKernelGGL(k_deg_from_row, dim3(gv), dim3(kBlock), 0, s, g->row.get(), n, g->deg.get());
+            PGD_HIP(hipGetLastError());
+        }
         int32_t md = 0;
         if (n) {
             DBuf<int32_t> mx;
