@@ -90,66 +90,122 @@ __global__ void k_dense_digits(const int32_t* __restrict__ S, const int8_t* __re
     }
 }
 
-// per-edge counts; one wavefront per edge.  The 4-cliques on e = (v,u) are
-// the edges among C = N(v) & N(u): lane l keeps words l, l+64, ... of C in
-// registers, the wavefront walks the members w of C in order (a uniform
-// loop), and each lane adds popcount(N(w) & C) over its words: every edge of
-// C is seen from both ends.
+// per-edge counts.  K4(e) for e = (v,u) is the number of edges among
+// C = N(v) & N(u), half of the sum over the members w of C of
+// popcount(N(w) & C).  A block keeps a slab of SW words of every bit row in
+// LDS (column words [q SW, q SW + SW) of rows 0..N-1) and works through groups
+// of kK4Waves x 64 edges, each group once per slab: a wavefront takes 64
+// edges of the group one after another, lists the members of C in LDS
+// (16-bit ids, N <= 8192), and every lane adds, for the members lane,
+// lane + 64, ..., the popcounts of the member's slab row AND the slab of C.
+// Lane i keeps the running sum of the wavefront's i-th edge across the slabs
+// and, after the last one, finishes that edge's T, K4 and C4 from the
+// products.  Bit rows are re-read from L2 once per edge and slab; the member
+// rows come from LDS.
 constexpr int kMaxWordsPerLane = kDenseMaxN / 32 / 64;
+constexpr int kK4Waves = 8;
+constexpr int kK4Threads = 64 * kK4Waves;
+constexpr size_t kLdsBytes = 160 * 1024;
 
-__global__ __launch_bounds__(256) void k_dense_edges(
+template <int SW>
+__global__ __launch_bounds__(kK4Threads) void k_dense_k4(
     const int32_t* __restrict__ ev, const int32_t* __restrict__ eu, const int64_t* __restrict__ row, int64_t m,
     int64_t N, int Wd, const int32_t* __restrict__ S, const int32_t* __restrict__ P0, const int32_t* __restrict__ P1,
     const int32_t* __restrict__ Q0, const int32_t* __restrict__ Q1, const uint32_t* __restrict__ bits,
     uint32_t* __restrict__ T, uint64_t* __restrict__ K4, uint64_t* __restrict__ C4) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    const int per = (Wd + 63) / 64;  // words of C per lane (<= kMaxWordsPerLane)
-    for (int64_t e = wave; e < m; e += nw) {
-        const int64_t v = ev[e], u = eu[e];
-        const uint32_t* bv = bits + v * Wd;
-        const uint32_t* bu = bits + u * Wd;
-        uint32_t c[kMaxWordsPerLane];
+    extern __shared__ __align__(16) uint32_t lds[];
+    uint32_t* slab = lds;  // N x SW words
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint16_t* list = (uint16_t*)(lds + N * SW) + (size_t)wv * N;
+    const int nslab = (Wd + SW - 1) / SW;
+    const int nwl = (Wd + 63) / 64;  // words of C per lane (<= kMaxWordsPerLane)
+    const int64_t ngroups = (m + kK4Threads - 1) / kK4Threads;
+    for (int64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        const int64_t eb = gi * kK4Threads + (int64_t)wv * 64;  // this wavefront's 64 edges
+        const int ne = (int)max<int64_t>(0, min<int64_t>(64, m - eb));
+        uint32_t mine = 0;  // twice K4 of edge eb + lane, summed over the slabs
+        for (int q = 0; q < nslab; q++) {
+            __syncthreads();  // no wavefront still reads the previous slab
+            for (int64_t i = threadIdx.x; i < N * SW; i += kK4Threads) {
+                const int64_t r = i / SW;
+                const int wj = q * SW + (int)(i % SW);
+                slab[i] = wj < Wd ? bits[r * Wd + wj] : 0u;
+            }
+            __syncthreads();
+            for (int i = 0; i < ne; i++) {
+                const int64_t e = eb + i;
+                const int64_t v = ev[e], u = eu[e];
+                uint32_t cw[kMaxWordsPerLane];
+                uint32_t cnt = 0;
 #pragma unroll
-        for (int q = 0; q < kMaxWordsPerLane; q++) {
-            const int x = lane + 64 * q;
-            c[q] = (q < per && x < Wd) ? (bv[x] & bu[x]) : 0u;
-        }
-        uint32_t twice = 0;
-        for (int j = 0; j < Wd; j++) {
-            // word j of C, broadcast from its lane (uniform across the wavefront)
-            uint32_t cj = 0;
-#pragma unroll
-            for (int q = 0; q < kMaxWordsPerLane; q++)
-                if (q == j / 64) cj = __shfl(c[q], j % 64, 64);
-            while (cj) {
-                const int b = __ffs(cj) - 1;
-                cj &= cj - 1;
-                const uint32_t* bw = bits + ((int64_t)j * 32 + b) * Wd;
-#pragma unroll
-                for (int q = 0; q < kMaxWordsPerLane; q++) {
-                    const int x = lane + 64 * q;
-                    if (q < per && x < Wd) twice += __popc(bw[x] & c[q]);
+                for (int k = 0; k < kMaxWordsPerLane; k++) {
+                    const int x = lane + 64 * k;
+                    cw[k] = (k < nwl && x < Wd) ? (bits[v * Wd + x] & bits[u * Wd + x]) : 0u;
+                    cnt += __popc(cw[k]);
                 }
+                const uint32_t incl = wave_incl_sum(cnt);
+                const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+                uint32_t pos = incl - cnt;
+#pragma unroll
+                for (int k = 0; k < kMaxWordsPerLane; k++) {
+                    uint32_t x = cw[k];
+                    const uint32_t base = (uint32_t)(lane + 64 * k) * 32u;
+                    while (x) {
+                        list[pos++] = (uint16_t)(base + (uint32_t)(__ffs(x) - 1));
+                        x &= x - 1;
+                    }
+                }
+                // the slab's words of C, from the lanes that hold them
+                uint32_t cs[SW];
+#pragma unroll
+                for (int j = 0; j < SW; j++) {
+                    const int wj = q * SW + j;
+                    uint32_t src = 0;
+#pragma unroll
+                    for (int k = 0; k < kMaxWordsPerLane; k++)
+                        if (k == (wj >> 6)) src = cw[k];
+                    const uint32_t got = (uint32_t)__shfl((int)src, wj & 63, 64);
+                    cs[j] = wj < Wd ? got : 0u;
+                }
+                __builtin_amdgcn_wave_barrier();
+                uint32_t acc = 0;
+                for (uint32_t r = lane; r < total; r += 64) {
+                    const uint32_t* rw = slab + (size_t)list[r] * SW;
+#pragma unroll
+                    for (int j = 0; j < SW; j++) acc += __popc(rw[j] & cs[j]);
+                }
+                __builtin_amdgcn_wave_barrier();  // the list is read before the next edge rewrites it
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) acc += (uint32_t)__shfl_xor((int)acc, off, 64);
+                if (lane == i) mine += acc;
             }
         }
-        uint64_t tw = twice;
-        for (int off = 32; off > 0; off >>= 1) tw += __shfl_xor(tw, off, 64);
-        if (lane == 0) {
+        if (lane < ne) {
+            const int64_t e = eb + lane;
+            const int64_t v = ev[e], u = eu[e];
             const int64_t vu = v * N + u, uv = u * N + v;
             const uint64_t t = (uint64_t)S[vu];
             const uint64_t a3 = (uint64_t)P0[vu] + 128 * (uint64_t)P1[vu];
             const uint64_t dv = (uint64_t)(row[v + 1] - row[v]), du = (uint64_t)(row[u + 1] - row[u]);
             const uint64_t p = a3 - dv - du + 1;
-            const uint64_t q = (uint64_t)Q0[vu] + 128 * (uint64_t)Q1[vu] + (uint64_t)Q0[uv] + 128 * (uint64_t)Q1[uv];
-            const uint64_t k4 = tw >> 1;
+            const uint64_t qq = (uint64_t)Q0[vu] + 128 * (uint64_t)Q1[vu] + (uint64_t)Q0[uv] + 128 * (uint64_t)Q1[uv];
+            const uint64_t k4 = (uint64_t)(mine >> 1);
             T[e] = (uint32_t)t;
             K4[e] = k4;
-            C4[e] = p - q + 2 * t + 2 * k4;
+            C4[e] = p - qq + 2 * t + 2 * k4;
         }
     }
 }
+
+// slab width (words) of k_dense_k4 for padded size N: the largest power of
+// two up to 16 whose slab and member lists fit the LDS
+int k4_slab_words(int64_t N) {
+    const int64_t lists = (int64_t)kK4Waves * N * 2;
+    int sw = 16;
+    while (sw > 1 && N * sw * 4 + lists > (int64_t)kLdsBytes) sw >>= 1;
+    return sw;
+}
+size_t k4_lds(int64_t N, int sw) { return (size_t)N * sw * 4 + (size_t)kK4Waves * N * 2; }
 
 // The engine's arrays for one graph: A, its bit rows, the products S, P (two
 // digits), Q (two digits), the digits of S and T, and the per-edge counts
@@ -183,6 +239,22 @@ DenseBufs carve(unsigned char* base, int64_t N) {
     return d;
 }
 
+// k_dense_k4 with slab width SW: dynamic LDS above 64 KiB is allowed for the
+// kernel (an attribute of the current device), and the grid fills every CU
+// with as many blocks as the occupancy at this LDS size allows
+template <int SW>
+void launch_k4(pgd_ctx* ctx, const GraphView& g, int64_t N, int Wd, const DenseBufs& d, EdgeCounts out) {
+    const size_t lds = k4_lds(N, SW);
+    PGD_HIP(hipFuncSetAttribute((const void*)k_dense_k4<SW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kLdsBytes));
+    int per_cu = 0;
+    PGD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dense_k4<SW>, kK4Threads, lds));
+    PGD_CHECK(per_cu > 0, PGD_ERR_UNSUPPORTED, "dense 4-clique pass: slab and lists exceed the LDS of a CU");
+    const int64_t ngroups = (g.m + kK4Threads - 1) / kK4Threads;
+    hipLaunchKernelGGL(k_dense_k4<SW>, dim3(grid_cap(ctx, ngroups, per_cu)), dim3(kK4Threads), lds, ctx->stream,
+                       g.ev, g.eu, g.row, g.m, N, Wd, d.S, d.P0, d.P1, d.Q0, d.Q1, d.bits, out.T, out.K4, out.C4);
+}
+
 // the launch sequence; every array of d is written before it is read (A and
 // the bit rows are cleared first), so no state carries over between calls
 void dense_launch(pgd_ctx* ctx, const GraphView& g, const DenseBufs& d, EdgeCounts out, Timer* timer) {
@@ -210,10 +282,16 @@ void dense_launch(pgd_ctx* ctx, const GraphView& g, const DenseBufs& d, EdgeCoun
     hipLaunchKernelGGL(k_gemm_i8, gg, dim3(256), 0, s, d.A, d.t1, d.Q1, N);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
-    if (timer) timer->begin("k_dense_edges");
-    if (m)
-        hipLaunchKernelGGL(k_dense_edges, dim3(grid_cap(ctx, (m * 64 + 255) / 256, 16)), dim3(256), 0, s, g.ev, g.eu,
-                           g.row, m, N, Wd, d.S, d.P0, d.P1, d.Q0, d.Q1, d.bits, out.T, out.K4, out.C4);
+    if (timer) timer->begin("k_dense_k4");
+    if (m) {
+        switch (k4_slab_words(N)) {
+            case 16: launch_k4<16>(ctx, g, N, Wd, d, out); break;
+            case 8: launch_k4<8>(ctx, g, N, Wd, d, out); break;
+            case 4: launch_k4<4>(ctx, g, N, Wd, d, out); break;
+            case 2: launch_k4<2>(ctx, g, N, Wd, d, out); break;
+            default: launch_k4<1>(ctx, g, N, Wd, d, out); break;
+        }
+    }
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
 }
