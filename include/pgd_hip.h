@@ -181,7 +181,7 @@ pgd_status pgd_decompose(pgd_graph *g, uint32_t flags, uint64_t *sums);
  * The graph's own per-edge buffers (pgd_micro, pgd_edge_counts) are not
  * touched. */
 #define PGD_NOALLOC_MAX_N 1024
-#define PGD_NOALLOC_SCRATCH_BYTES 36820992  /* 35.1 MiB per GPU, held until the process ends */
+#define PGD_NOALLOC_SCRATCH_BYTES 29480960  /* 28.1 MiB per GPU, held until the process ends */
 pgd_status pgd_decompose_noalloc(pgd_graph *g, uint32_t flags, uint64_t *d_sums, uint64_t *d_micro);
 /* Download one micro column (m uint64 values in E_ordered order). */
 pgd_status pgd_micro(pgd_graph *g, int32_t column, uint64_t *out);
