@@ -84,6 +84,31 @@ def measured_traffic(kernel: str, workload: str) -> float | None:
     return None
 
 
+def measured_atomics(kernel: str, workload: str) -> float | None:
+    """Memory-side atomic requests per launch of `kernel` on `workload`
+    (rocprofv3 TCC_ATOMIC_sum, from the summaries committed under profiles/)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 4 and p[0] == "atomic_requests" and p[1] == kernel and p[2] == workload:
+                    return float(p[3])
+    return None
+
+
+def atomic_ceiling(ctx, words: int = 1 << 27, reps: int = 10) -> dict:
+    """The device's rate of global atomic adds (pgd_atomic_rate): one atomic
+    add per 32-bit counter and launch, consecutive lanes on consecutive
+    counters, so a wavefront's adds form 64-byte requests of 16 counters.
+    Those requests are performed past the L2 (a profile of them counts every
+    one as an L2 miss and 64 bytes written per request), so this rate, not
+    the HBM bandwidth, bounds a pass made of per-slot sums."""
+    r, ms = ctx.atomic_rate(words, reps)
+    return {"words": words, "launches": reps, "ms_per_launch": ms, "requests_per_s": r,
+            "atomic_adds_per_s": r * 16.0, "method": "pgd_atomic_rate (one atomic add per int32 counter per launch)"}
+
+
 def roofline(phases: list[list[dict]], workload: str = "") -> dict | None:
     """Roofline of the dominant counting kernel: its algorithmic HBM bytes
     (the engine's work model for that pass, per launch) over its average
@@ -231,7 +256,8 @@ def bench_reader(pgd_amd, ctx, n, s, d, g) -> dict:
 def bench_dense(pgd_amd, ctx) -> dict:
     """Dense tier on G(500, 0.9) and G(2000, 0.5): device time of the int8
     MFMA engine and of the sparse oriented engine on the same graph, and the
-    MFMA products' achieved int8 rate (5 products of 2 N^3 operations each)."""
+    MFMA launches' achieved int8 rate (S = A.A, 2 N^3 operations, and the
+    fused [A3 | AT], 8 N^3; the digit kernel between them is timed apart)."""
     out = {}
     for n, p in ((500, 0.9), (2000, 0.5)):
         s, d = pgd_amd.gen_gnp(n, p, 1)
@@ -244,9 +270,9 @@ def bench_dense(pgd_amd, ctx) -> dict:
             for _ in range(3):
                 g.decompose(micro=True, engine=eng)
                 ms.append(g.last_timing()[0])
-                ph = {q["name"]: q for q in g.last_phases()}
-                if "k_gemm_i8" in ph:
-                    gemm.append((ph["k_gemm_i8"]["ms"], ph["k_gemm_i8"]["bytes"]))
+                mm = [q for q in g.last_phases() if q["name"].startswith("k_gemm_i8")]
+                if mm:  # the MFMA launches (S, then [A3 | AT]); bytes holds their operations
+                    gemm.append((sum(q["ms"] for q in mm), sum(q["bytes"] for q in mm)))
             res[eng] = float(np.median(ms))
             if gemm:
                 gms, ops = sorted(gemm)[len(gemm) // 2]
@@ -456,6 +482,8 @@ def main() -> int:
                     help="extra measurements after the headline: any of io,dense,batched,served (or none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (scale for the PMC byte counters)")
+    ap.add_argument("--atomic-ceiling", action="store_true",
+                    help="only measure the device's global atomic-add request rate (pgd_atomic_rate) and print it")
     ap.add_argument("--served-only", action="store_true",
                     help="only the served measurement, in this (fresh) process; prints its JSON")
     ap.add_argument("--validate-cpu-baseline", default="",
@@ -494,6 +522,12 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration": {"bytes_read": a.numel() * 4, "bytes_written": a.numel() * 4,
                                                "launches": 3}}))
+        return 0
+    if args.atomic_ceiling:
+        import pgd_amd
+        ctx = pgd_amd.Context(0)
+        print(json.dumps({"atomic_ceiling": atomic_ceiling(ctx)}))
+        ctx.close()
         return 0
     import pgd_amd
     import torch
@@ -557,6 +591,18 @@ def main() -> int:
         "survey_model": {"bytes": B, "achieved_GBs": B / dt / 1e9, "frac_of_hbm_peak": B / dt / 1e9 / HBM_PEAK_GBS},
         "totals": dict(zip(pgd_amd.TOTAL_NAMES, totals)),
     }
+    # the bound the dominant kernel meets first: its per-slot sums are global
+    # atomics, performed past the L2 at the device's atomic-request rate
+    rl = out["roofline"]
+    if rl:
+        try:
+            ceil = atomic_ceiling(ctx)
+            req = measured_atomics(rl["kernel"], out["config"]["workload"])
+            rate = req / (rl["ms"] / 1e3) if req else None
+            out["atomics"] = {"kernel": rl["kernel"], "requests_per_launch": req, "requests_per_s": rate,
+                              "ceiling": ceil, "frac_of_ceiling": rate / ceil["requests_per_s"] if rate else None}
+        except Exception as e:  # a section must not hide the headline
+            out["atomics"] = {"error": repr(e)}
     if "io" in args.sections.split(","):
         try:
             out["micro_writer"] = bench_micro_writer(g)

@@ -72,6 +72,17 @@ pgd_status  pgd_set_mem_cap(pgd_ctx *ctx, int64_t bytes);
 pgd_status  pgd_mem_in_use(pgd_ctx *ctx, int64_t *bytes);
 /* Last error detail for this context (static storage owned by the context). */
 const char *pgd_last_error(pgd_ctx *ctx);
+/* The device's rate of global atomic adds, for the roofline of the passes
+ * whose per-slot sums are atomics (no counterpart in the reference: a
+ * measurement helper).  `words` 32-bit counters in device memory each get
+ * one atomic add per launch (consecutive lanes on consecutive words, so a
+ * wavefront's adds form 64-byte requests of 16 words), `reps` launches are
+ * timed on the context's stream after one untimed launch, and every counter
+ * is checked to equal reps + 1; *requests_per_s = reps * words / 16 /
+ * seconds.  words in
+ * [2^20, 2^31), reps in [1, 100]. */
+pgd_status  pgd_atomic_rate(pgd_ctx *ctx, int64_t words, int32_t reps, double *requests_per_s,
+                            double *ms_per_launch);
 
 /* ---- graph construction -----------------------------------------------
  * Replaces the CSR assembly of read_mtx (graphlet_core.cpp:509-517) and

@@ -63,6 +63,12 @@ class Context:
         check(lib().pgd_mem_in_use(self.h, C.byref(v)), self.h)
         return v.value
 
+    def atomic_rate(self, words: int = 1 << 27, reps: int = 10) -> tuple[float, float]:
+        """(requests/s, ms per launch) of global atomic adds (pgd_atomic_rate)."""
+        r, ms = C.c_double(), C.c_double()
+        check(lib().pgd_atomic_rate(self.h, words, reps, C.byref(r), C.byref(ms)), self.h, "pgd_atomic_rate")
+        return r.value, ms.value
+
     def __enter__(self):
         return self
 

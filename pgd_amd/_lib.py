@@ -31,7 +31,7 @@ TOTAL_NAMES = [
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = [
     "pgd_open", "pgd_close", "pgd_status_str", "pgd_set_mem_cap", "pgd_mem_in_use", "pgd_last_error",
-    "pgd_graph_from_coo", "pgd_graph_from_csr", "pgd_graph_from_file", "pgd_graph_free", "pgd_graph_info", "pgd_graph_csr", "pgd_preprocess",
+    "pgd_atomic_rate", "pgd_graph_from_coo", "pgd_graph_from_csr", "pgd_graph_from_file", "pgd_graph_free", "pgd_graph_info", "pgd_graph_csr", "pgd_preprocess",
     "pgd_edge_arrays", "pgd_compute_cores", "pgd_sort_edges", "pgd_sort_edges_subset",
     "pgd_set_e_ordered", "pgd_e_ordered_count", "pgd_e_ordered", "pgd_decompose", "pgd_decompose_noalloc", "pgd_micro", "pgd_write_micro",
     "pgd_edge_counts", "pgd_last_engine", "pgd_last_timing", "pgd_last_phase", "pgd_derive", "pgd_batch_create", "pgd_batch_run",
@@ -80,6 +80,7 @@ def lib() -> C.CDLL:
         "pgd_set_mem_cap": (I32, [P, I64]),
         "pgd_mem_in_use": (I32, [P, PI64]),
         "pgd_last_error": (C.c_char_p, [P]),
+        "pgd_atomic_rate": (I32, [P, I64, I32, C.POINTER(D), C.POINTER(D)]),
         "pgd_graph_from_coo": (I32, [P, I64, I64, PI32, PI32, U32, C.POINTER(P)]),
         "pgd_graph_from_csr": (I32, [P, I64, PI64, PI32, C.POINTER(P)]),
         "pgd_graph_from_file": (I32, [P, C.c_char_p, I32, C.POINTER(P), PI64]),

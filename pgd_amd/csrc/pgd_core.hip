@@ -774,6 +774,61 @@ pgd_status pgd_set_mem_cap(pgd_ctx* ctx, int64_t bytes) {
     return PGD_OK;
 }
 
+namespace {
+__global__ void k_atomic_probe(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&p[i], 1u);
+}
+__global__ void k_atomic_check(const uint32_t* __restrict__ p, int64_t n, uint32_t want, uint32_t* __restrict__ bad) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (p[i] != want) atomicOr(bad, 1u);
+}
+}  // namespace
+
+pgd_status pgd_atomic_rate(pgd_ctx* ctx, int64_t words, int32_t reps, double* requests_per_s, double* ms_per_launch) {
+    if (!ctx || !requests_per_s || words < ((int64_t)1 << 20) || words >= ((int64_t)1 << 31) || reps < 1 ||
+        reps > 100)
+        return PGD_ERR_INVALID;
+    return guarded(ctx, [&] {
+        PGD_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        DBuf<uint32_t> cnt, bad;
+        cnt.alloc(ctx, words);
+        bad.alloc(ctx, 1);
+        PGD_HIP(hipMemsetAsync(cnt.get(), 0, (size_t)words * 4, s));
+        PGD_HIP(hipMemsetAsync(bad.get(), 0, 4, s));
+        const int grid = std::min(grid_for(words), ctx->num_cus * 16);
+        // one untimed launch first: the first launch of a kernel of the
+        // library also loads its code
+        hipLaunchKernelGGL(k_atomic_probe, dim3(grid), dim3(kBlock), 0, s, cnt.get(), words);
+        PGD_HIP(hipGetLastError());
+        hipEvent_t e0, e1;
+        PGD_HIP(hipEventCreate(&e0));
+        PGD_HIP(hipEventCreate(&e1));
+        float ms = 0;
+        hipError_t err = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && err == hipSuccess; r++) {
+            hipLaunchKernelGGL(k_atomic_probe, dim3(grid), dim3(kBlock), 0, s, cnt.get(), words);
+            err = hipGetLastError();
+        }
+        if (err == hipSuccess) err = hipEventRecord(e1, s);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        PGD_HIP(err);
+        hipLaunchKernelGGL(k_atomic_check, dim3(grid), dim3(kBlock), 0, s, cnt.get(), words, (uint32_t)reps + 1u,
+                           bad.get());
+        PGD_HIP(hipGetLastError());
+        uint32_t hbad = 0;
+        PGD_HIP(hipMemcpyAsync(&hbad, bad.get(), 4, hipMemcpyDeviceToHost, s));
+        sync(ctx);
+        PGD_CHECK(hbad == 0, PGD_ERR_STATE, "atomic probe: a counter does not hold the number of launches (timed ones and the warm-up)");
+        *requests_per_s = (double)reps * (double)words / 16.0 / ((double)ms / 1e3);
+        if (ms_per_launch) *ms_per_launch = (double)ms / reps;
+    });
+}
+
 pgd_status pgd_mem_in_use(pgd_ctx* ctx, int64_t* bytes) {
     if (!ctx || !bytes) return PGD_ERR_INVALID;
     *bytes = ctx->mem_used.load();

@@ -358,11 +358,17 @@ void dense_launch(pgd_ctx* ctx, const GraphView& g, const DenseBufs& d, EdgeCoun
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
     const unsigned nt = (unsigned)(N / kTile);
-    if (timer) timer->begin("k_gemm_i8");
     // S = A . A (the first N columns of [A | 127 A]); then [A3 | AT] over K = 2N
+    if (timer) timer->begin("k_gemm_i8");
     hipLaunchKernelGGL(k_gemm_i8, dim3(nt, nt), dim3(256), 0, s, d.A2, 2 * N, d.A2, 2 * N, d.S, N, N);
+    PGD_HIP(hipGetLastError());
+    if (timer) timer->end();
+    if (timer) timer->begin("k_dense_digits");
     hipLaunchKernelGGL(k_dense_digits, dim3(grid_cap(ctx, (NN + 255) / 256, 16)), dim3(256), 0, s, d.S, d.A2, N,
                        d.X);
+    PGD_HIP(hipGetLastError());
+    if (timer) timer->end();
+    if (timer) timer->begin("k_gemm_i8_pq");
     hipLaunchKernelGGL(k_gemm_i8, dim3(2 * nt, nt), dim3(256), 0, s, d.A2, 2 * N, d.X, 2 * N, d.PQ, 2 * N, 2 * N);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
@@ -401,19 +407,22 @@ void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm) 
     Timer timer{ctx};
     dense_launch(ctx, g, carve(buf.get(), N), out, &timer);
     PGD_HIP(hipStreamSynchronize(ctx->stream));
-    // work models: the MFMA products are 2 N^3 (S) + 8 N^3 ([A3 | AT] over
+    // work models: the MFMA products are 2 N^3 (S) and 8 N^3 ([A3 | AT] over
     // K = 2N) int8 operations; the adjacency pass writes 2 N^2 + N^2/8 bytes;
     // the edge pass reads four int32 products and writes T, K4 and C4 per edge
-    const double flops = 10.0 * (double)N * N * N;
+    const double n3 = (double)N * N * N;
     tm->phases.clear();
     double best = 0;
     for (size_t i = 0; i < timer.ev.size(); i++) {
         float ms = 0;
         PGD_HIP(hipEventElapsedTime(&ms, timer.ev[i].first, timer.ev[i].second));
         double bytes = 0, units = 0;
+        // for the MFMA passes, bytes holds operations
         if (i == 0) { bytes = 2.0 * NN + NN / 8.0; units = (double)NN; }
-        if (i == 1) { bytes = flops; units = 2; }  // for the MFMA pass: operations, not bytes
-        if (i == 2) { bytes = 36.0 * m; units = (double)m; }
+        if (i == 1) { bytes = 2.0 * n3; units = 1; }
+        if (i == 2) { bytes = 9.0 * NN; units = (double)NN; }  // S and A read, four digit bytes written
+        if (i == 3) { bytes = 8.0 * n3; units = 1; }
+        if (i == 4) { bytes = 36.0 * m; units = (double)m; }
         tm->phases.push_back(Phase{timer.names[i], ms, bytes, units});
         if (ms > best) best = ms;
     }

@@ -1486,9 +1486,10 @@ void orient_profile(pgd_ctx* ctx, OrientState& st, Timer& timer, Timing* tm) {
     //     and the accumulate pass;
     //   tiled wedges: the 4 B endpoint id read by the count pass, and for
     //     the wedges of tiles with a repeated endpoint (the only ones the
-    //     accumulate pass visits) a second 4 B read and the 4 B per-slot sum;
+    //     accumulate pass visits) a second 4 B read and the read-modify-write
+    //     of the 4 B per-slot sum (4 B read and 4 B written);
     //   combine: 3 x 8 B + 4 B read and 8 B written per edge.
-    const double bytes_of[5] = {4.0 * wk[0], 24.0 * wk[1], 8.0 * wk[2], 4.0 * wk[3] + 8.0 * wk[4], 36.0 * st.m};
+    const double bytes_of[5] = {4.0 * wk[0], 24.0 * wk[1], 8.0 * wk[2], 4.0 * wk[3] + 12.0 * wk[4], 36.0 * st.m};
     const double units_of[5] = {(double)wk[0], (double)wk[1], (double)wk[2], (double)wk[3], (double)st.m};
     double best = 0;
     tm->phases.clear();

@@ -349,3 +349,14 @@ def test_adopt_reference_csr(ctx, tmp):
     assert [totals[k] for k in pgd_amd.TOTAL_NAMES] == ent["totals"]
     assert micro_sha(micro_matrix(g)) == ent["micro_sha256"]
     g.free()
+
+
+def test_atomic_rate_probe(ctx):
+    """pgd_atomic_rate: every counter checked to equal the launch count, and
+    a positive rate; out-of-range arguments are refused before any launch."""
+    r, ms = ctx.atomic_rate(1 << 22, 3)
+    assert r > 0 and ms > 0
+    for words, reps in ((1 << 10, 1), (1 << 22, 0), (1 << 22, 101), (1 << 31, 1)):
+        with pytest.raises(pgd_amd.PGDError) as e:
+            ctx.atomic_rate(words, reps)
+        assert e.value.code == -1
