@@ -1413,11 +1413,6 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
                                st.el_mid, st.ecap_mid, tl, TSL, K4S, QS, PK, pk, st.lq.get() + 5, w + 1);
     };
     fork_begin(ctx, st, side, 2);
-    // T by edge id, from TS (the clique pass reads TS only): on the side
-    // stream, ahead of its clique bins
-    hipLaunchKernelGGL(k_fold_u32, dim3(gz), dim3(256), 0, side ? side : s, st.TS.get(), st.reid.get(), 2 * m,
-                       out.T);
-    PGD_HIP(hipGetLastError());
     if (!side) k4q_small_mid(s);
     if (st.n_large)
         hipLaunchKernelGGL((k_local_k4q<1024, false>), dim3(grid_cap(ctx, st.n_large, 1)), dim3(1024),
@@ -1431,10 +1426,16 @@ static void orient_count_local(pgd_ctx* ctx, OrientState& st, EdgeCounts out, Ti
                            st.lq.get() + 3, w + 1);
     if (side) k4q_small_mid(side);
     fork_end(ctx, st, side, 3);
-    // with a side stream the fold of the per-slot sums runs there, beside
-    // the 4-cycle passes (which use neither its inputs nor its outputs);
-    // fork[5], recorded after it, is the join before k_combine
+    // with a side stream the folds of the per-slot sums run there, beside
+    // the 4-cycle passes (which use neither their inputs nor their outputs);
+    // fork[5], recorded after them, is the join before k_combine.  The
+    // triangle fold (T by edge id, from TS; the clique pass reads TS only)
+    // comes here too: launched ahead of the clique bins it raced the
+    // 1024-thread bin for the CUs, and when it lost, the side stream's bins
+    // waited behind it (the clique pass then ended about 18 ms later)
     fork_begin(ctx, st, side, 4);
+    hipLaunchKernelGGL(k_fold_u32, dim3(gz), dim3(256), 0, side ? side : s, st.TS.get(), st.reid.get(), 2 * m,
+                       out.T);
     hipLaunchKernelGGL(k_fold_k4q, dim3(gz), dim3(256), 0, side ? side : s, K4S, QS, PK, pk.qb, st.reid.get(), 2 * m,
                        out.K4, st.Q.get());
     PGD_HIP(hipGetLastError());
