@@ -280,14 +280,23 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
     }
 }
 
-// cut each hub's tiles into items: at most segcap segments, kTmax tiles and
-// (unless a single tile exceeds it) wcap wedges; empty items are dropped.
-// With out == nullptr only the number of items per hub is written.
+// cut each anchor's tiles into items: at most segcap segments, kTmax tiles
+// and (unless a single tile exceeds it) wcap wedges; empty items are dropped.
+// With out == nullptr only the number of items per anchor is written, and
+// nitems[nheavy] is cleared (the scan's last term).  zero128, when given, is
+// cleared by the first block (the item sort's bucket counters): these two
+// words would otherwise each take a launch, and a launch of the plan waits
+// for a free wave slot beside the persistent local passes.
 __global__ void k_plan_items(const int64_t* __restrict__ htoff, const int32_t* __restrict__ hbase,
                              const uint32_t* __restrict__ htseg, const uint64_t* __restrict__ htw, int64_t nheavy,
                              int64_t segcap, uint64_t wcap,
                              int64_t* __restrict__ nitems, const int64_t* __restrict__ ioff,
-                             WedgeItem* __restrict__ out, uint64_t* __restrict__ key) {
+                             WedgeItem* __restrict__ out, uint64_t* __restrict__ key, uint32_t* __restrict__ zero128) {
+    if (blockIdx.x == 0) {
+        if (!out && threadIdx.x == 0) nitems[nheavy] = 0;
+        if (zero128)
+            for (int i = threadIdx.x; i < 128; i += blockDim.x) zero128[i] = 0;
+    }
     GRID_STRIDE(h, nheavy) {
         const int64_t c0 = htoff[h], T = htoff[h + 1] - c0;
         int64_t cnt = 0, t0 = 0, segs = 0;
@@ -999,10 +1008,9 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         per.alloc(ctx, nh + 1);
         ioff.alloc(ctx, nh + 1);
         const int g = grid_cap(ctx, (nh + 255) / 256, 16);
-        hipLaunchKernelGGL(w_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)per.get() + nh, (int64_t)1);
         hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, s, wp.htoff.get(), wp.hbase.get(), wp.htseg.get(),
                            wp.htw.get(), nh, wp.segcap, wcap, per.get(), (const int64_t*)nullptr, (WedgeItem*)nullptr,
-                           (uint64_t*)nullptr);
+                           (uint64_t*)nullptr, (uint32_t*)nullptr);
         PGD_HIP(hipGetLastError());
         excl_sum_nolds(ctx, per.get(), ioff.get(), nh + 1);
         PGD_HIP(hipMemcpyAsync(&wp.nitems, ioff.get() + nh, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -1015,7 +1023,7 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         key.alloc(ctx, ni);
         bk.alloc(ctx, 128);
         hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, s, wp.htoff.get(), wp.hbase.get(), wp.htseg.get(),
-                           wp.htw.get(), nh, wp.segcap, wcap, per.get(), ioff.get(), raw.get(), key.get());
+                           wp.htw.get(), nh, wp.segcap, wcap, per.get(), ioff.get(), raw.get(), key.get(), bk.get());
         PGD_HIP(hipGetLastError());
         ensure(wp.items, ctx, ni);
         if (ni) {
@@ -1023,7 +1031,6 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
             // of 64 buckets with global atomics (no LDS: these kernels find
             // CU slots beside the local passes)
             const int gi = grid_cap(ctx, (ni + 255) / 256, 16);
-            hipLaunchKernelGGL(w_fill_u32, dim3(1), dim3(64), 0, s, bk.get(), (int64_t)128);
             hipLaunchKernelGGL(k_item_buckets, dim3(gi), dim3(256), 0, s, key.get(), ni, bk.get());
             hipLaunchKernelGGL(k_bucket_starts, dim3(1), dim3(64), 0, s, bk.get());
             hipLaunchKernelGGL(k_item_scatter, dim3(gi), dim3(256), 0, s, key.get(), raw.get(), ni, bk.get() + 64,
