@@ -204,9 +204,37 @@ __global__ __launch_bounds__(kK4Threads) void k_dense_k4(
 #pragma unroll
                 for (int k = 0; k < kMaxWordsPerLane; k++) cw[k] = rv[k] & ru[k];
                 if (i + 1 < ne) load_rows(i + 1);
+                // the slab's words of C: each lane stores its words in this
+                // wavefront's list area, and every lane reads the slab's words
+                // back (the same address for all lanes); the member list then
+                // overwrites the area
+                uint32_t* cwl = (uint32_t*)list;
+#pragma unroll
+                for (int k = 0; k < kMaxWordsPerLane; k++)
+                    if (k < nwl && lane + 64 * k < Wd) cwl[lane + 64 * k] = cw[k];
+                __builtin_amdgcn_wave_barrier();
+                uint32_t cs[SW];
+                if constexpr (SW >= 4) {
+#pragma unroll
+                    for (int j = 0; j < SW; j += 4) {
+                        const uint4 t = *(const uint4*)(cwl + q0 + j);
+                        cs[j] = t.x;
+                        cs[j + 1] = t.y;
+                        cs[j + 2] = t.z;
+                        cs[j + 3] = t.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < SW; j++) cs[j] = cwl[q0 + j];
+                }
+#pragma unroll
+                for (int j = 0; j < SW; j++)
+                    if (q0 + j >= Wd) cs[j] = 0u;
+                __builtin_amdgcn_wave_barrier();
                 // members in ascending order: word 64k + lane's members after
                 // those of every lower word; nlo / nhi = members below the
-                // slab's first / past its last word
+                // slab's first / past its last word (members past the slab
+                // are never read, so only those below it are written)
                 uint32_t base = 0, nlo = 0, nhi = 0;
 #pragma unroll
                 for (int k = 0; k < kMaxWordsPerLane; k++) {
@@ -214,7 +242,7 @@ __global__ __launch_bounds__(kK4Threads) void k_dense_k4(
                     const uint32_t cnt = (uint32_t)__popc(cw[k]);
                     const uint32_t incl = wave_incl_sum(cnt);
                     uint32_t pos = base + incl - cnt;
-                    uint32_t x = cw[k];
+                    uint32_t x = lane + 64 * k < qe ? cw[k] : 0u;
                     const uint32_t wbase = (uint32_t)(lane + 64 * k) * 32u;
                     while (x) {
                         list[pos++] = (uint16_t)(wbase + (uint32_t)(__ffs(x) - 1));
@@ -226,18 +254,6 @@ __global__ __launch_bounds__(kK4Threads) void k_dense_k4(
                     if (rl > 0) nlo = base + (rl >= 64 ? (uint32_t)__shfl((int)incl, 63, 64) : at_lo);
                     if (rh > 0) nhi = base + (rh >= 64 ? (uint32_t)__shfl((int)incl, 63, 64) : at_hi);
                     base += (uint32_t)__shfl((int)incl, 63, 64);
-                }
-                // the slab's words of C, from the lanes that hold them
-                uint32_t cs[SW];
-#pragma unroll
-                for (int j = 0; j < SW; j++) {
-                    const int wj = q0 + j;
-                    uint32_t src = 0;
-#pragma unroll
-                    for (int k = 0; k < kMaxWordsPerLane; k++)
-                        if (k == (wj >> 6)) src = cw[k];
-                    const uint32_t got = (uint32_t)__shfl((int)src, wj & 63, 64);
-                    cs[j] = wj < Wd ? got : 0u;
                 }
                 __builtin_amdgcn_wave_barrier();
                 // twice the pairs {w, x} of C with x in the slab: a member below
