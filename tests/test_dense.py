@@ -65,7 +65,10 @@ def complete(n):
     return i.astype(np.int32), j.astype(np.int32)
 
 
-@pytest.mark.parametrize("case", ["k64", "k100", "gnp65", "gnp130_0.7", "gnp777_0.3", "gnp4000_0.3", "sparse_ba"])
+# padded sizes N = 128, 256, 896, 4096, 6528 and 8064: the 4-clique pass holds
+# slabs of 16, 16, 16, 4, 2 and 1 words of every bit row in LDS
+@pytest.mark.parametrize("case", ["k64", "k100", "gnp65", "gnp130_0.7", "gnp777_0.3", "gnp4000_0.3", "gnp6500_0.05",
+                                  "gnp8000_0.02", "sparse_ba"])
 def test_dense_matches_sparse(case, ctx):
     say(f"[gpu] dense vs sparse {case}")
     if case == "k64":
@@ -79,7 +82,7 @@ def test_dense_matches_sparse(case, ctx):
     else:
         _, nn, pp = case.split("_")[0][:3], int(case.split("_")[0][3:]), float(case.split("_")[1])
         n, (s, d) = nn, pgd_amd.gen_gnp(nn, pp, 5)
-    g, out = both_engines(ctx, s, d, n, micro=(case != "gnp4000_0.3"))
+    g, out = both_engines(ctx, s, d, n, micro=(n < 4000))
     assert_same(out)
     if case.startswith("k"):
         # closed forms of the complete graph K_n, per edge
