@@ -50,6 +50,32 @@ __global__ void b_fill_u32(uint32_t* __restrict__ p, int64_t n) {
     GRID_STRIDE(i, n) p[i] = 0;
 }
 
+// union ids of the packed edge lists, in place: line i belongs to the graph g
+// with line_off[g] <= i < line_off[g + 1] (the last such g, so graphs without
+// lines are passed over) and its ids are shifted by voff[g]; an id outside
+// [0, n_g) sets *bad and the line gets -1, -1
+__global__ void k_union_ids(int32_t* __restrict__ s, int32_t* __restrict__ d, int64_t L,
+                            const int64_t* __restrict__ line_off, const int64_t* __restrict__ voff, int32_t ng,
+                            uint32_t* __restrict__ bad) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t lo = 0, hi = ng - 1;  // line_off[0] = 0 <= i < line_off[ng] = L
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (line_off[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        const int64_t off = voff[lo], n = voff[lo + 1] - off;
+        const int32_t x = s[i], y = d[i];
+        if (x < 0 || y < 0 || x >= n || y >= n) {
+            atomicOr(bad, 1u);
+            s[i] = -1;
+            d[i] = -1;
+        } else {
+            s[i] = (int32_t)(x + off);
+            d[i] = (int32_t)(y + off);
+        }
+    }
+}
+
 __global__ void k_vertex_group(const int64_t* __restrict__ voff, int32_t ng, int32_t* __restrict__ group) {
     for (int64_t g = blockIdx.x; g < ng; g += gridDim.x)
         for (int64_t v = voff[g] + threadIdx.x; v < voff[g + 1]; v += blockDim.x) group[v] = (int32_t)g;
@@ -245,25 +271,37 @@ void batch_build(pgd_batch* b, int32_t ngraphs, const int32_t* n, const int64_t*
     PGD_CHECK(L == 0 || (src && dst), PGD_ERR_INVALID, "missing edge lists");
     b->N = b->voff[ng];
     PGD_CHECK(b->N < INT32_MAX, PGD_ERR_UNSUPPORTED, "batch has 2^31 vertices or more");
-    // union COO: graph-local ids shifted by voff (checked against n_g)
-    std::vector<int32_t> us(std::max<int64_t>(L, 1)), ud(std::max<int64_t>(L, 1));
-    int64_t bad = 0;
-    for (int32_t g = 0; g < ng; g++) {
-        const int64_t off = b->voff[g], ng_ = n[g];
-        for (int64_t i = line_off[g]; i < line_off[g + 1]; i++) {
-            const int32_t x = src[i], y = dst[i];
-            if (x < 0 || y < 0 || x >= ng_ || y >= ng_) bad++;
-            us[i] = (int32_t)(x + off);
-            ud[i] = (int32_t)(y + off);
+    PGD_CHECK(L < ((int64_t)1 << 31), PGD_ERR_UNSUPPORTED, "batch has 2^31 lines or more");
+    // union COO on the device: graph-local ids shifted by voff, checked against n_g
+    {
+        hipStream_t s = ctx->stream;
+        DBuf<int32_t> ds, dd;
+        DBuf<int64_t> dlo, dvo;
+        DBuf<uint32_t> dbad;
+        ds.alloc(ctx, L);
+        dd.alloc(ctx, L);
+        dlo.alloc(ctx, ng + 1);
+        dvo.alloc(ctx, ng + 1);
+        dbad.alloc(ctx, 1);
+        uint32_t bad = 0;
+        if (L) {
+            PGD_HIP(hipMemcpyAsync(ds.get(), src, L * sizeof(int32_t), hipMemcpyHostToDevice, s));
+            PGD_HIP(hipMemcpyAsync(dd.get(), dst, L * sizeof(int32_t), hipMemcpyHostToDevice, s));
+            PGD_HIP(hipMemcpyAsync(dlo.get(), line_off, (ng + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+            PGD_HIP(hipMemcpyAsync(dvo.get(), b->voff.data(), (ng + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+            PGD_HIP(hipMemsetAsync(dbad.get(), 0, sizeof(uint32_t), s));
+            hipLaunchKernelGGL(k_union_ids, dim3(grid_cap(ctx, (L + 255) / 256, 16)), dim3(256), 0, s, ds.get(),
+                               dd.get(), L, dlo.get(), dvo.get(), ng, dbad.get());
+            PGD_HIP(hipGetLastError());
+            PGD_HIP(hipMemcpyAsync(&bad, dbad.get(), sizeof(bad), hipMemcpyDeviceToHost, s));
+            PGD_HIP(hipStreamSynchronize(s));
         }
+        PGD_CHECK(bad == 0, PGD_ERR_INVALID, "vertex id outside [0, n) of its graph");
+        pgd_graph* U = new pgd_graph();
+        U->ctx = ctx;
+        b->U = U;
+        graph_from_device_coo(ctx, U, b->N, L, ds, dd, b->flags & PGD_DEDUP);
     }
-    PGD_CHECK(bad == 0, PGD_ERR_INVALID, "vertex id outside [0, n) of its graph");
-    check_status(ctx, pgd_graph_from_coo(ctx, b->N, L, us.data(), ud.data(), b->flags & PGD_DEDUP, &b->U),
-                 "union CSR");
-    us.clear();
-    us.shrink_to_fit();
-    ud.clear();
-    ud.shrink_to_fit();
     check_status(ctx, pgd_preprocess(b->U, "deg", 0), "preprocess");
     check_status(ctx, pgd_sort_edges(b->U, "deg", 0), "sort_edges");
     pgd_graph* U = b->U;

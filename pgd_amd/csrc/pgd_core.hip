@@ -937,6 +937,13 @@ void coo_build(pgd_ctx* ctx, pgd_graph* g, int64_t n, int64_t L, DBuf<int32_t>& 
 }
 }  // namespace
 
+extern "C++" namespace pgd {
+void graph_from_device_coo(pgd_ctx* ctx, pgd_graph* g, int64_t n, int64_t L, DBuf<int32_t>& ds,
+                           DBuf<int32_t>& dd, uint32_t flags) {
+    coo_build(ctx, g, n, L, ds, dd, flags, false);
+}
+}  // namespace pgd
+
 pgd_status pgd_graph_from_coo(pgd_ctx* ctx, int64_t n, int64_t L, const int32_t* src, const int32_t* dst,
                               uint32_t flags, pgd_graph** out) {
     if (!ctx || !out || n < 0 || L < 0 || (L > 0 && (!src || !dst))) return PGD_ERR_INVALID;

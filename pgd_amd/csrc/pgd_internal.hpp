@@ -190,6 +190,12 @@ void epilogue(pgd_ctx* ctx, const GraphView& g, EdgeCounts c, uint64_t* d_sums, 
 void write_micro_text(pgd_ctx* ctx, const GraphView& g, const uint64_t* micro, int fd, bool output_id,
                       const char* prefix, int64_t* written);
 
+// the CSR of g (a new, empty pgd_graph) from a COO already in device memory
+// (L lines, ids in [0, n)), as pgd_graph_from_coo builds it from host lines;
+// ds and dd are only read
+void graph_from_device_coo(pgd_ctx* ctx, pgd_graph* g, int64_t n, int64_t L, DBuf<int32_t>& ds,
+                           DBuf<int32_t>& dd, uint32_t flags);
+
 // launch-geometry helpers
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
