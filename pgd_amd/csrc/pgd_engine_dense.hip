@@ -139,182 +139,301 @@ __global__ void k_dense_digits(const int32_t* __restrict__ S, const int8_t* __re
     }
 }
 
-// per-edge counts.  K4(e) for e = (v,u) is the number of edges among
-// C = N(v) & N(u): half the sum over the members w of C of
-// popcount(N(w) & C).  A block keeps a slab of SW words of every bit row in
-// LDS (column words [q SW, q SW + SW) of rows 0..N-1) and works through
-// groups of kK4Waves x 64 edges, each group once per slab.  A wavefront takes
-// its 64 edges one after another (their ids held one per lane, the next
-// edge's bit rows loaded while the current one is counted), lists the
-// members of C in LDS in ascending order (16-bit ids, N <= 8192), and the
-// lanes take the members below the slab's end, four at a time: each adds the
-// popcounts of the member's slab row AND the slab of C, twice for a member
-// below the slab (whose pairs with the slab's members are met once) and once
-// for a member inside it (whose pairs inside the slab are met from both
-// ends); members above the slab add nothing, their pairs being met from the
-// lower end.  Lane i keeps the sum of the wavefront's i-th edge across the
-// slabs and, after the last one, finishes that edge's T, K4 and C4 from the
-// products.
-constexpr int kMaxWordsPerLane = kDenseMaxN / 32 / 64;
-constexpr int kK4Waves = 8;
-constexpr int kK4Threads = 64 * kK4Waves;
-constexpr size_t kLdsBytes = 160 * 1024;
+// ---- 4-cliques per edge on the matrix cores.  For a vertex v with the
+// ascending neighbour list L (d = |L|), B = A[L, L] is its local graph, and
+// for u = L[i]
+//   2 K4(v,u) = sum_j B[i][j] (B.B)[i][j]
+// (ordered pairs of adjacent members of N(v) & N(u)).  Each edge is taken at
+// its lower end: only the rows i >= i0 (L[i] > v) are formed.  The products
+// run per vertex on compressed bit rows (column x of A becomes column j of B
+// when x = L[j]), which cuts the operations by (N/d)^2 against masking A.
+//
+// k_dense_compress writes, for each vertex of a chunk, B's rows as bits,
+// words past d zero (row stride `stride` words, stride = dpad / 32 with dpad
+// the largest degree rounded up to 128), L, (v, d, i0), and appends the
+// vertex's row tiles (slot << 6 | tile, 128 rows each) to the tile list.
+// Bits are moved with the compress of Hacker's Delight (7-4): five masked
+// shift steps whose masks depend only on the mask word, so they are computed
+// once per vertex and word and shared by every row.
+constexpr int kK4MaxW = kDenseMaxN / 32;  // words of a bit row at most
+constexpr int kK4Rows = 128;              // rows (and columns) of one product tile
 
-template <int SW>
-__global__ __launch_bounds__(kK4Threads) void k_dense_k4(
-    const int32_t* __restrict__ ev, const int32_t* __restrict__ eu, const int64_t* __restrict__ row, int64_t m,
-    int64_t N, int Wd, const int32_t* __restrict__ S, const int32_t* __restrict__ PQ, const uint32_t* __restrict__ bits,
-    uint32_t* __restrict__ T, uint64_t* __restrict__ K4, uint64_t* __restrict__ C4) {
-    extern __shared__ __align__(16) uint32_t lds[];
-    uint32_t* slab = lds;  // N x SW words
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint16_t* list = (uint16_t*)(lds + N * SW) + (size_t)wv * N;
-    const int nslab = (Wd + SW - 1) / SW;
-    const int nwl = (Wd + 63) / 64;  // words of C per lane (<= kMaxWordsPerLane)
-    const int64_t ngroups = (m + kK4Threads - 1) / kK4Threads;
-    for (int64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
-        const int64_t eb = gi * kK4Threads + (int64_t)wv * 64;  // this wavefront's 64 edges
-        const int ne = (int)max<int64_t>(0, min<int64_t>(64, m - eb));
-        const int32_t myv = lane < ne ? ev[eb + lane] : 0, myu = lane < ne ? eu[eb + lane] : 0;
-        uint32_t mine = 0;  // twice K4 of edge eb + lane, summed over the slabs
-        for (int q = 0; q < nslab; q++) {
-            __syncthreads();  // no wavefront still reads the previous slab
-            for (int64_t i = threadIdx.x; i < N * SW; i += kK4Threads) {
-                const int64_t r = i / SW;
-                const int wj = q * SW + (int)(i % SW);
-                slab[i] = wj < Wd ? bits[r * Wd + wj] : 0u;
-            }
-            __syncthreads();
-            const int q0 = q * SW, qe = min(q0 + SW, Wd);  // the slab's words of C
-            // the bit rows of N(v) and N(u), one edge ahead
-            uint32_t rv[kMaxWordsPerLane], ru[kMaxWordsPerLane];
-            auto load_rows = [&](int i) {
-                const int64_t v = __shfl(myv, i, 64), u = __shfl(myu, i, 64);
+struct K4Info {
+    int32_t v, d, i0, pad;
+};
+
+__global__ __launch_bounds__(256) void k_dense_compress(const uint32_t* __restrict__ bits, int Wd, int64_t v0,
+                                                        int64_t v1, int64_t stride, int64_t vwords, int64_t dpad,
+                                                        uint32_t* __restrict__ G, uint16_t* __restrict__ Lg,
+                                                        K4Info* __restrict__ info, uint32_t* __restrict__ tiles,
+                                                        uint32_t* __restrict__ count,
+                                                        unsigned long long* __restrict__ ops) {
+    __shared__ uint32_t msk[kK4MaxW], pre[kK4MaxW], mv[5][kK4MaxW];
+    __shared__ uint32_t rowbuf[4][kK4MaxW];
+    __shared__ uint32_t wsum[4];
+    __shared__ uint16_t L[kDenseMaxN];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int64_t v = v0 + blockIdx.x; v < v1; v += gridDim.x) {
+        const int64_t slot = v - v0;
+        __syncthreads();  // the previous vertex is done with the LDS
+        uint32_t m = 0, c = 0;
+        if (tid < Wd) {
+            m = bits[v * Wd + tid];
+            c = (uint32_t)__popc(m);
+            msk[tid] = m;
+            uint32_t mm = m, mk = ~m << 1;
 #pragma unroll
-                for (int k = 0; k < kMaxWordsPerLane; k++) {
-                    const int x = lane + 64 * k;
-                    const bool ok = k < nwl && x < Wd;
-                    rv[k] = ok ? bits[v * Wd + x] : 0u;
-                    ru[k] = ok ? bits[u * Wd + x] : 0u;
-                }
-            };
-            if (ne > 0) load_rows(0);
-            for (int i = 0; i < ne; i++) {
-                uint32_t cw[kMaxWordsPerLane];
-#pragma unroll
-                for (int k = 0; k < kMaxWordsPerLane; k++) cw[k] = rv[k] & ru[k];
-                if (i + 1 < ne) load_rows(i + 1);
-                // the slab's words of C: each lane stores its words in this
-                // wavefront's list area, and every lane reads the slab's words
-                // back (the same address for all lanes); the member list then
-                // overwrites the area
-                uint32_t* cwl = (uint32_t*)list;
-#pragma unroll
-                for (int k = 0; k < kMaxWordsPerLane; k++)
-                    if (k < nwl && lane + 64 * k < Wd) cwl[lane + 64 * k] = cw[k];
-                __builtin_amdgcn_wave_barrier();
-                uint32_t cs[SW];
-                if constexpr (SW >= 4) {
-#pragma unroll
-                    for (int j = 0; j < SW; j += 4) {
-                        const uint4 t = *(const uint4*)(cwl + q0 + j);
-                        cs[j] = t.x;
-                        cs[j + 1] = t.y;
-                        cs[j + 2] = t.z;
-                        cs[j + 3] = t.w;
-                    }
-                } else {
-#pragma unroll
-                    for (int j = 0; j < SW; j++) cs[j] = cwl[q0 + j];
-                }
-#pragma unroll
-                for (int j = 0; j < SW; j++)
-                    if (q0 + j >= Wd) cs[j] = 0u;
-                __builtin_amdgcn_wave_barrier();
-                // members in ascending order: word 64k + lane's members after
-                // those of every lower word; nlo / nhi = members below the
-                // slab's first / past its last word (members past the slab
-                // are never read, so only those below it are written)
-                uint32_t base = 0, nlo = 0, nhi = 0;
-#pragma unroll
-                for (int k = 0; k < kMaxWordsPerLane; k++) {
-                    if (k >= nwl) break;
-                    const uint32_t cnt = (uint32_t)__popc(cw[k]);
-                    const uint32_t incl = wave_incl_sum(cnt);
-                    uint32_t pos = base + incl - cnt;
-                    uint32_t x = lane + 64 * k < qe ? cw[k] : 0u;
-                    const uint32_t wbase = (uint32_t)(lane + 64 * k) * 32u;
-                    while (x) {
-                        list[pos++] = (uint16_t)(wbase + (uint32_t)(__ffs(x) - 1));
-                        x &= x - 1;
-                    }
-                    const int rl = q0 - 64 * k, rh = qe - 64 * k;
-                    const uint32_t at_lo = (uint32_t)__shfl((int)incl, max(rl - 1, 0), 64);
-                    const uint32_t at_hi = (uint32_t)__shfl((int)incl, max(rh - 1, 0), 64);
-                    if (rl > 0) nlo = base + (rl >= 64 ? (uint32_t)__shfl((int)incl, 63, 64) : at_lo);
-                    if (rh > 0) nhi = base + (rh >= 64 ? (uint32_t)__shfl((int)incl, 63, 64) : at_hi);
-                    base += (uint32_t)__shfl((int)incl, 63, 64);
-                }
-                __builtin_amdgcn_wave_barrier();
-                // twice the pairs {w, x} of C with x in the slab: a member below
-                // the slab meets each such x above it (weight 2); the members
-                // inside meet each pair inside twice (weight 1).  A lane takes
-                // four consecutive members per step (one 8-byte list read, the
-                // four rows' loads issued together); ids past nhi weigh 0
-                uint32_t acc = 0;
-                for (uint32_t r0 = 0; r0 < nhi; r0 += 256) {
-                    const uint32_t r = r0 + 4u * (uint32_t)lane;
-                    uint2 ids = make_uint2(0u, 0u);
-                    if (r < nhi) ids = *(const uint2*)(list + r);
-                    uint32_t w[4], wt[4];
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        const uint32_t id = ((t < 2 ? ids.x : ids.y) >> (16 * (t & 1))) & 0xffffu;
-                        const bool ok = r + t < nhi;
-                        w[t] = ok ? id : 0u;
-                        wt[t] = ok ? (r + t < nlo ? 2u : 1u) : 0u;
-                    }
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        const uint32_t* rw = slab + (size_t)w[t] * SW;
-                        uint32_t pc = 0;
-#pragma unroll
-                        for (int j = 0; j < SW; j++) pc += __popc(rw[j] & cs[j]);
-                        acc += wt[t] * pc;
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();  // the list is read before the next edge rewrites it
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) acc += (uint32_t)__shfl_xor((int)acc, off, 64);
-                if (lane == i) mine += acc;
+            for (int s = 0; s < 5; s++) {
+                uint32_t mp = mk ^ (mk << 1);
+                mp ^= mp << 2;
+                mp ^= mp << 4;
+                mp ^= mp << 8;
+                mp ^= mp << 16;
+                const uint32_t t = mp & mm;
+                mv[s][tid] = t;
+                mm = (mm ^ t) | (t >> (1 << s));
+                mk &= ~mp;
             }
         }
-        if (lane < ne) {
-            const int64_t e = eb + lane;
-            const int64_t v = myv, u = myu;
-            const uint64_t t = (uint64_t)S[v * N + u];
-            // [A3 | AT]: row stride 2N, A3 in columns [0, N), AT in [N, 2N)
-            const uint64_t a3 = (uint64_t)PQ[v * 2 * N + u];
-            const uint64_t dv = (uint64_t)(row[v + 1] - row[v]), du = (uint64_t)(row[u + 1] - row[u]);
-            const uint64_t p = a3 - dv - du + 1;
-            const uint64_t qq = (uint64_t)PQ[v * 2 * N + N + u] + (uint64_t)PQ[u * 2 * N + N + v];
-            const uint64_t k4 = (uint64_t)(mine >> 1);
-            T[e] = (uint32_t)t;
-            K4[e] = k4;
-            C4[e] = p - qq + 2 * t + 2 * k4;
+        const uint32_t inc = wave_incl_sum(c);
+        if (lane == 63) wsum[wv] = inc;
+        for (int i = tid; i < 4 * kK4MaxW; i += 256) (&rowbuf[0][0])[i] = 0u;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int k = 0; k < wv; k++) before += wsum[k];
+        const uint32_t d = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (tid < Wd) {
+            uint32_t pos = before + inc - c;
+            pre[tid] = pos;
+            uint32_t x = m;
+            while (x) {
+                L[pos++] = (uint16_t)(32 * tid + __ffs(x) - 1);
+                x &= x - 1;
+            }
+        }
+        __syncthreads();
+        const int vw = (int)(v >> 5);
+        const uint32_t i0 = pre[vw] + (uint32_t)__popc(msk[vw] & ((1u << (v & 31)) - 1u));
+        for (uint32_t i = tid; i < d; i += 256) Lg[slot * dpad + i] = L[i];
+        if (tid == 0) {
+            info[slot] = K4Info{(int32_t)v, (int32_t)d, (int32_t)i0, 0};
+            const uint32_t nt = (d - i0 + kK4Rows - 1) / kK4Rows;
+            if (nt) {
+                const uint32_t base = atomicAdd(count, nt);
+                for (uint32_t t = 0; t < nt; t++) tiles[base + t] = (uint32_t)slot << 6 | t;
+                const unsigned long long dp = (d + kK4Rows - 1) / kK4Rows * kK4Rows;
+                atomicAdd(ops, 2ull * kK4Rows * dp * dp * nt);
+            }
+        }
+        // B's rows: wavefront wv takes rows wv, wv + 4, ...; lane w compresses
+        // word w of row L[i] under the mask and ors it in at bit pre[w]
+        uint32_t* rb = rowbuf[wv];
+        uint32_t* gv = G + slot * vwords;
+        for (uint32_t i = wv; i < d; i += 4) {
+            const int64_t u = L[i];
+            for (int w = lane; w < Wd; w += 64) {
+                const uint32_t mw = msk[w];
+                if (!mw) continue;
+                uint32_t x = bits[u * Wd + w] & mw;
+#pragma unroll
+                for (int s = 0; s < 5; s++) {
+                    const uint32_t t = x & mv[s][w];
+                    x = (x ^ t) | (t >> (1 << s));
+                }
+                const uint32_t off = pre[w], sh = off & 31;
+                atomicOr(&rb[off >> 5], x << sh);
+                if (sh && sh + (uint32_t)__popc(mw) > 32) atomicOr(&rb[(off >> 5) + 1], x >> (32 - sh));
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int q = lane; q < stride; q += 64) {
+                gv[(int64_t)i * stride + q] = rb[q];
+                rb[q] = 0u;
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
 
-// slab width (words) of k_dense_k4 for padded size N: the largest power of
-// two up to 16 whose slab and member lists fit the LDS
-int k4_slab_words(int64_t N) {
-    const int64_t lists = (int64_t)kK4Waves * N * 2;
-    int sw = 16;
-    while (sw > 1 && N * sw * 4 + lists > (int64_t)kLdsBytes) sw >>= 1;
-    return sw;
+// 32 bits as 32 bytes of 0/1 (bit k -> byte k): each nibble spread by a 24-bit
+// multiply, whose four shifted copies of the nibble do not overlap
+__device__ __forceinline__ void bits_to_bytes(int8_t* __restrict__ dst, uint32_t x) {
+    uint4 lo, hi;
+    lo.x = __umul24(x & 0xfu, 0x204081u) & 0x01010101u;
+    lo.y = __umul24((x >> 4) & 0xfu, 0x204081u) & 0x01010101u;
+    lo.z = __umul24((x >> 8) & 0xfu, 0x204081u) & 0x01010101u;
+    lo.w = __umul24((x >> 12) & 0xfu, 0x204081u) & 0x01010101u;
+    hi.x = __umul24((x >> 16) & 0xfu, 0x204081u) & 0x01010101u;
+    hi.y = __umul24((x >> 20) & 0xfu, 0x204081u) & 0x01010101u;
+    hi.z = __umul24((x >> 24) & 0xfu, 0x204081u) & 0x01010101u;
+    hi.w = __umul24(x >> 28, 0x204081u) & 0x01010101u;
+    *(uint4*)dst = lo;
+    *(uint4*)(dst + 16) = hi;
 }
-size_t k4_lds(int64_t N, int sw) { return (size_t)N * sw * 4 + (size_t)kK4Waves * N * 2; }
+
+// One block per row tile of one vertex's local graph (tiles from the list
+// k_dense_compress built): for 128 rows i of B (from i0 + 128 t) and every
+// 128-column tile of B, the int8 product (B.B) over the depth dp (d rounded up
+// to 128), staged 64 deep in LDS as k_gemm_i8 stages it but expanded from the
+// bit rows; the epilogue adds the products where B[i][j] = 1 to per-row sums.
+// Twice K4(v, L[i]) goes to K4M[v N + L[i]].
+__global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ G, const uint16_t* __restrict__ Lg,
+                                                   const K4Info* __restrict__ info, const uint32_t* __restrict__ tiles,
+                                                   const uint32_t* __restrict__ count, int64_t stride,
+                                                   int64_t vwords, int64_t dpad, int64_t N,
+                                                   int32_t* __restrict__ K4M) {
+    __shared__ __align__(16) int8_t As[2][kTile * kGStride];
+    __shared__ __align__(16) int8_t Xs[2][kTile * kGStride];
+    __shared__ uint32_t msk[kK4Rows * 4];
+    __shared__ int32_t rs[kK4Rows];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wm = w >> 1, wn = w & 1;
+    const int r0 = tid >> 1, h = tid & 1;  // staging: 32 bits of row r0 (word h of each 64-deep step)
+    const int fo = (lane & 31) * kGStride + 16 * (lane >> 5);
+    const uint32_t ntiles = *count;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t tl = tiles[t];
+        const int64_t slot = tl >> 6;
+        const K4Info inf = info[slot];
+        const int d = inf.d, rbase = inf.i0 + (int)(tl & 63) * kK4Rows;
+        const int dp = (d + kK4Rows - 1) / kK4Rows * kK4Rows;
+        const uint32_t* gv = G + slot * vwords;
+        const int ia = rbase + r0;
+        const uint32_t* pa = gv + (int64_t)min(ia, d - 1) * stride + h;
+        if (tid < kK4Rows) rs[tid] = 0;
+        int32_t part[2][16];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) part[a][r] = 0;
+        for (int ct = 0; ct < dp; ct += kK4Rows) {
+            const int jx = ct + r0;
+            const uint32_t* px = gv + (int64_t)min(jx, d - 1) * stride + h;
+            uint32_t wa = ia < d ? pa[0] : 0u, wx = jx < d ? px[0] : 0u;
+            v16i acc[2][2];
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) acc[a][b] = v16i{};
+            int buf = 0;
+            for (int ks = 0; ks < dp; ks += kGK) {
+                bits_to_bytes(As[buf] + r0 * kGStride + 32 * h, wa);
+                bits_to_bytes(Xs[buf] + r0 * kGStride + 32 * h, wx);
+                __syncthreads();
+                if (ks + kGK < dp) {
+                    const int nw = (ks + kGK) >> 5;
+                    wa = ia < d ? pa[nw] : 0u;
+                    wx = jx < d ? px[nw] : 0u;
+                }
+                const int8_t* as = As[buf] + (wm * 64) * kGStride + fo;
+                const int8_t* xs = Xs[buf] + (wn * 64) * kGStride + fo;
+#pragma unroll
+                for (int kk = 0; kk < kGK; kk += 32) {
+                    const v4i a0 = *(const v4i*)(as + kk), a1 = *(const v4i*)(as + 32 * kGStride + kk);
+                    const v4i x0 = *(const v4i*)(xs + kk), x1 = *(const v4i*)(xs + 32 * kGStride + kk);
+                    acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, x0, acc[0][0], 0, 0, 0);
+                    acc[0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, x1, acc[0][1], 0, 0, 0);
+                    acc[1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, x0, acc[1][0], 0, 0, 0);
+                    acc[1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, x1, acc[1][1], 0, 0, 0);
+                }
+                buf ^= 1;
+            }
+            // B's bits of this tile (rows rbase.., words ct / 32 .. + 4)
+            {
+                const int rr = tid >> 1, q = 2 * (tid & 1), i = rbase + rr;
+                uint2 mw = make_uint2(0u, 0u);
+                if (i < d) mw = *(const uint2*)(gv + (int64_t)i * stride + (ct >> 5) + q);
+                msk[rr * 4 + q] = mw.x;
+                msk[rr * 4 + q + 1] = mw.y;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+#pragma unroll
+                    for (int r = 0; r < 16; r++) {
+                        const int rl = wm * 64 + a * 32 + (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
+                        const int cl = wn * 64 + b * 32 + (lane & 31);
+                        const uint32_t bit = (msk[rl * 4 + (cl >> 5)] >> (cl & 31)) & 1u;
+                        part[a][r] += bit ? acc[a][b][r] : 0;
+                    }
+            __syncthreads();  // msk and the staging buffers are rewritten by the next column tile
+        }
+        // each row's sum over the 32 lanes of its half wavefront, then over the
+        // two wavefronts that hold its two column halves
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                int32_t x = part[a][r];
+#pragma unroll
+                for (int off = 16; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+                if ((lane & 31) == 0) atomicAdd(&rs[wm * 64 + a * 32 + (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5)], x);
+            }
+        __syncthreads();
+        if (tid < kK4Rows) {
+            const int i = rbase + tid;
+            if (i < d) K4M[(int64_t)inf.v * N + Lg[slot * dpad + i]] = rs[tid];
+        }
+        __syncthreads();  // rs is cleared for the next tile
+    }
+}
+
+// per-edge T, K4 and C4 from the products (S, [A3 | AT], K4M at (min, max))
+__global__ void k_dense_finish(const int32_t* __restrict__ ev, const int32_t* __restrict__ eu,
+                               const int64_t* __restrict__ row, int64_t m, int64_t N, const int32_t* __restrict__ S,
+                               const int32_t* __restrict__ PQ, const int32_t* __restrict__ K4M,
+                               uint32_t* __restrict__ T, uint64_t* __restrict__ K4, uint64_t* __restrict__ C4) {
+    GRID_STRIDE(e, m) {
+        const int64_t a = ev[e], b = eu[e];
+        const int64_t v = min(a, b), u = max(a, b);
+        const uint64_t t = (uint64_t)S[v * N + u];
+        // [A3 | AT]: row stride 2N, A3 in columns [0, N), AT in [N, 2N)
+        const uint64_t a3 = (uint64_t)PQ[v * 2 * N + u];
+        const uint64_t dv = (uint64_t)(row[v + 1] - row[v]), du = (uint64_t)(row[u + 1] - row[u]);
+        const uint64_t p = a3 - dv - du + 1;
+        const uint64_t qq = (uint64_t)PQ[v * 2 * N + N + u] + (uint64_t)PQ[u * 2 * N + N + v];
+        const uint64_t k4 = (uint64_t)(K4M[v * N + u] >> 1);
+        T[e] = (uint32_t)t;
+        K4[e] = k4;
+        C4[e] = p - qq + 2 * t + 2 * k4;
+    }
+}
+
+// The per-vertex work area of the 4-clique products for graphs of largest
+// degree dmax: slots of compressed rows, lists, (v, d, i0) and row tiles for
+// `chunk` vertices at a time, then the tile counter and the operation count
+struct K4Plan {
+    int64_t dpad, stride, vwords, rtmax, chunk;
+    size_t oL, oInfo, oTiles, oCount;
+};
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+inline size_t k4_per_vertex(int64_t dpad) {
+    return (size_t)dpad * (dpad / 32) * 4 + (size_t)dpad * 2 + sizeof(K4Info) + (size_t)(dpad / kK4Rows) * 4;
+}
+K4Plan k4_plan(int64_t dmax, size_t wbytes) {
+    K4Plan p;
+    p.dpad = std::max<int64_t>(1, (dmax + kK4Rows - 1) / kK4Rows) * kK4Rows;
+    p.stride = p.dpad / 32;
+    p.vwords = p.dpad * p.stride;
+    p.rtmax = p.dpad / kK4Rows;
+    const size_t slack = 5 * 256 + 256;  // alignment of the four arrays, the counters
+    PGD_CHECK(wbytes > slack + k4_per_vertex(p.dpad), PGD_ERR_STATE, "dense 4-clique work area too small");
+    p.chunk = (int64_t)((wbytes - slack) / k4_per_vertex(p.dpad));
+    p.oL = align256((size_t)p.chunk * p.vwords * 4);
+    p.oInfo = p.oL + align256((size_t)p.chunk * p.dpad * 2);
+    p.oTiles = p.oInfo + align256((size_t)p.chunk * sizeof(K4Info));
+    p.oCount = p.oTiles + align256((size_t)p.chunk * p.rtmax * 4);
+    PGD_CHECK(p.oCount + 256 <= wbytes, PGD_ERR_STATE, "dense 4-clique work area layout");
+    return p;
+}
+// the work area the engine allocates: every vertex in one chunk, within
+// max(16 N^2, 512 MiB) bytes
+size_t k4_work_bytes(int64_t n, int64_t dmax, int64_t N) {
+    const int64_t dpad = std::max<int64_t>(1, (dmax + kK4Rows - 1) / kK4Rows) * kK4Rows;
+    const size_t per = k4_per_vertex(dpad), slack = 5 * 256 + 256;
+    const size_t cap = std::max<size_t>(16 * (size_t)N * N, (size_t)512 << 20);
+    return std::max(per + slack + 1, std::min(per * (size_t)std::max<int64_t>(n, 1) + slack, cap));
+}
 
 // The engine's arrays for one graph: [A | 127 A], its bit rows, S, the
 // digits operand, and the products [A3 | AT]
@@ -322,10 +441,11 @@ struct DenseBufs {
     int8_t *A2, *X;
     int32_t *S, *PQ;
     uint32_t* bits;
+    unsigned char* W = nullptr;  // work area of the 4-clique products (wbytes)
+    size_t wbytes = 0;
 };
 
 // bytes of DenseBufs for padded size N, each array 256-byte aligned
-constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr size_t dense_bytes(int64_t N) {
     return align256(2 * (size_t)N * N) + align256(4 * (size_t)N * N) + align256(4 * (size_t)N * N) +
            align256(8 * (size_t)N * N) + align256((size_t)N * N / 8);
@@ -340,22 +460,6 @@ DenseBufs carve(unsigned char* base, int64_t N) {
     d.PQ = (int32_t*)p; p += align256(8 * nn);
     d.bits = (uint32_t*)p;
     return d;
-}
-
-// k_dense_k4 with slab width SW: dynamic LDS above 64 KiB is allowed for the
-// kernel (an attribute of the current device), and the grid fills every CU
-// with as many blocks as the occupancy at this LDS size allows
-template <int SW>
-void launch_k4(pgd_ctx* ctx, const GraphView& g, int64_t N, int Wd, const DenseBufs& d, EdgeCounts out) {
-    const size_t lds = k4_lds(N, SW);
-    PGD_HIP(hipFuncSetAttribute((const void*)k_dense_k4<SW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)kLdsBytes));
-    int per_cu = 0;
-    PGD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dense_k4<SW>, kK4Threads, lds));
-    PGD_CHECK(per_cu > 0, PGD_ERR_UNSUPPORTED, "dense 4-clique pass: slab and lists exceed the LDS of a CU");
-    const int64_t ngroups = (g.m + kK4Threads - 1) / kK4Threads;
-    hipLaunchKernelGGL(k_dense_k4<SW>, dim3(grid_cap(ctx, ngroups, per_cu)), dim3(kK4Threads), lds, ctx->stream,
-                       g.ev, g.eu, g.row, g.m, N, Wd, d.S, d.PQ, d.bits, out.T, out.K4, out.C4);
 }
 
 // the launch sequence; every array of d is written before it is read (A and
@@ -388,18 +492,42 @@ void dense_launch(pgd_ctx* ctx, const GraphView& g, const DenseBufs& d, EdgeCoun
     hipLaunchKernelGGL(k_gemm_i8, dim3(2 * nt, nt), dim3(256), 0, s, d.A2, 2 * N, d.X, 2 * N, d.PQ, 2 * N, 2 * N);
     PGD_HIP(hipGetLastError());
     if (timer) timer->end();
-    if (timer) timer->begin("k_dense_k4");
     if (m) {
-        switch (k4_slab_words(N)) {
-            case 16: launch_k4<16>(ctx, g, N, Wd, d, out); break;
-            case 8: launch_k4<8>(ctx, g, N, Wd, d, out); break;
-            case 4: launch_k4<4>(ctx, g, N, Wd, d, out); break;
-            case 2: launch_k4<2>(ctx, g, N, Wd, d, out); break;
-            default: launch_k4<1>(ctx, g, N, Wd, d, out); break;
+        // twice K4 per edge at (min, max) of K4M, which takes X's place (X is
+        // read only by the second product)
+        int32_t* K4M = (int32_t*)d.X;
+        const K4Plan kp = k4_plan(g.max_deg, d.wbytes);
+        uint32_t* G = (uint32_t*)d.W;
+        uint16_t* Lg = (uint16_t*)(d.W + kp.oL);
+        K4Info* info = (K4Info*)(d.W + kp.oInfo);
+        uint32_t* tiles = (uint32_t*)(d.W + kp.oTiles);
+        uint32_t* count = (uint32_t*)(d.W + kp.oCount);
+        unsigned long long* ops = (unsigned long long*)(d.W + kp.oCount + 8);
+        hipLaunchKernelGGL(d_fill_u32, dim3(1), dim3(64), 0, s, count, (int64_t)4);
+        PGD_HIP(hipGetLastError());
+        int per_cu = 0;
+        PGD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dense_k4m, 256, 0));
+        per_cu = std::max(per_cu, 1);
+        for (int64_t v0 = 0; v0 < n; v0 += kp.chunk) {
+            const int64_t v1 = std::min<int64_t>(n, v0 + kp.chunk);
+            if (v0) hipLaunchKernelGGL(d_fill_u32, dim3(1), dim3(64), 0, s, count, (int64_t)1);
+            if (timer) timer->begin("k_dense_compress");
+            hipLaunchKernelGGL(k_dense_compress, dim3(grid_cap(ctx, v1 - v0, 8)), dim3(256), 0, s, d.bits, Wd, v0,
+                               v1, kp.stride, kp.vwords, kp.dpad, G, Lg, info, tiles, count, ops);
+            PGD_HIP(hipGetLastError());
+            if (timer) timer->end();
+            if (timer) timer->begin("k_dense_k4m");
+            hipLaunchKernelGGL(k_dense_k4m, dim3(grid_cap(ctx, (v1 - v0) * kp.rtmax, per_cu)), dim3(256), 0, s, G,
+                               Lg, info, tiles, count, kp.stride, kp.vwords, kp.dpad, N, K4M);
+            PGD_HIP(hipGetLastError());
+            if (timer) timer->end();
         }
+        if (timer) timer->begin("k_dense_finish");
+        hipLaunchKernelGGL(k_dense_finish, dim3(grid_cap(ctx, (m + 255) / 256, 16)), dim3(256), 0, s, g.ev, g.eu,
+                           g.row, m, N, d.S, d.PQ, K4M, out.T, out.K4, out.C4);
+        PGD_HIP(hipGetLastError());
+        if (timer) timer->end();
     }
-    PGD_HIP(hipGetLastError());
-    if (timer) timer->end();
 }
 
 }  // namespace
@@ -418,30 +546,53 @@ void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm) 
     PGD_CHECK(g.n <= kDenseMaxN, PGD_ERR_UNSUPPORTED, "graph too large for the dense engine");
     const int64_t n = g.n, m = g.m;
     const int64_t N = (n + kTile - 1) / kTile * kTile, NN = N * N;
-    DBuf<unsigned char> buf;
+    DBuf<unsigned char> buf, work;
     buf.alloc(ctx, dense_bytes(N));
+    DenseBufs d = carve(buf.get(), N);
+    if (m) {
+        d.wbytes = k4_work_bytes(n, g.max_deg, N);
+        work.alloc(ctx, d.wbytes);
+        d.W = work.get();
+    }
     Timer timer{ctx};
-    dense_launch(ctx, g, carve(buf.get(), N), out, &timer);
+    dense_launch(ctx, g, d, out, &timer);
+    unsigned long long k4_ops = 0;
+    if (m) {
+        const K4Plan kp = k4_plan(g.max_deg, d.wbytes);
+        PGD_HIP(hipMemcpyAsync(&k4_ops, d.W + kp.oCount + 8, sizeof(k4_ops), hipMemcpyDeviceToHost, ctx->stream));
+    }
     PGD_HIP(hipStreamSynchronize(ctx->stream));
-    // work models: the MFMA products are 2 N^3 (S) and 8 N^3 ([A3 | AT] over
-    // K = 2N) int8 operations; the adjacency pass writes 2 N^2 + N^2/8 bytes;
-    // the edge pass reads four int32 products and writes T, K4 and C4 per edge
+    // work models: the MFMA products are 2 N^3 (S), 8 N^3 ([A3 | AT] over
+    // K = 2N) and, for the 4-clique products, 2 x 128 x dp^2 per row tile of
+    // a vertex (dp its degree rounded up to 128; counted on the device) int8
+    // operations; the adjacency pass writes 2 N^2 + N^2/8 bytes; the edge pass
+    // reads four int32 products and writes T, K4 and C4 per edge.  A pass run
+    // once per chunk of vertices is one phase (times summed).
     const double n3 = (double)N * N * N;
     tm->phases.clear();
-    double best = 0;
     for (size_t i = 0; i < timer.ev.size(); i++) {
         float ms = 0;
         PGD_HIP(hipEventElapsedTime(&ms, timer.ev[i].first, timer.ev[i].second));
-        double bytes = 0, units = 0;
-        // for the MFMA passes, bytes holds operations
-        if (i == 0) { bytes = 2.0 * NN + NN / 8.0; units = (double)NN; }
-        if (i == 1) { bytes = 2.0 * n3; units = 1; }
-        if (i == 2) { bytes = 9.0 * NN; units = (double)NN; }  // S and A read, four digit bytes written
-        if (i == 3) { bytes = 8.0 * n3; units = 1; }
-        if (i == 4) { bytes = 36.0 * m; units = (double)m; }
-        tm->phases.push_back(Phase{timer.names[i], ms, bytes, units});
-        if (ms > best) best = ms;
+        const std::string name = timer.names[i];
+        Phase* ph = nullptr;
+        for (auto& q : tm->phases)
+            if (q.name == name) ph = &q;
+        if (ph) {
+            ph->ms += ms;
+            continue;
+        }
+        double bytes = 0, units = 0;  // for the MFMA passes, bytes holds operations
+        if (name == "k_dense_adj") { bytes = 2.0 * NN + NN / 8.0; units = (double)NN; }
+        if (name == "k_gemm_i8") { bytes = 2.0 * n3; units = 1; }
+        if (name == "k_dense_digits") { bytes = 9.0 * NN; units = (double)NN; }  // S and A read, four digit bytes
+        if (name == "k_gemm_i8_pq") { bytes = 8.0 * n3; units = 1; }
+        if (name == "k_dense_compress") { units = (double)n; }
+        if (name == "k_dense_k4m") { bytes = (double)k4_ops; units = 1; }
+        if (name == "k_dense_finish") { bytes = 36.0 * m; units = (double)m; }
+        tm->phases.push_back(Phase{name, ms, bytes, units});
     }
+    double best = 0;
+    for (auto& q : tm->phases) best = std::max(best, q.ms);
     tm->dominant_ms = best;
     tm->dominant_launches = 1;
 }
@@ -500,7 +651,11 @@ void decompose_noalloc(pgd_ctx* ctx, const GraphView& g, uint64_t* d_sums, uint6
     }
     PGD_CHECK(base != nullptr, PGD_ERR_STATE, "scratch not initialised on this device");
     const int64_t N = (g.n + kTile - 1) / kTile * kTile;
-    const DenseBufs d = carve(base, N);
+    DenseBufs d = carve(base, N);
+    // the 4-clique products' work area takes [A | 127 A]'s place (read only by
+    // the products before them), in chunks of vertices
+    d.W = (unsigned char*)d.A2;
+    d.wbytes = align256(2 * (size_t)N * N);
     unsigned char* cp = base + dense_bytes(kNoAllocMaxN);
     EdgeCounts c{(uint32_t*)cp, (uint64_t*)(cp + align256(4 * (size_t)kNoAllocMaxM)),
                  (uint64_t*)(cp + align256(4 * (size_t)kNoAllocMaxM) + align256(8 * (size_t)kNoAllocMaxM))};
