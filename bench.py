@@ -255,32 +255,46 @@ def bench_reader(pgd_amd, ctx, n, s, d, g) -> dict:
 
 def bench_dense(pgd_amd, ctx) -> dict:
     """Dense tier on G(500, 0.9) and G(2000, 0.5): device time of the int8
-    MFMA engine and of the sparse oriented engine on the same graph, and the
-    MFMA launches' achieved int8 rate (S = A.A, 2 N^3 operations, and the
-    fused [A3 | AT], 8 N^3; the digit kernel between them is timed apart)."""
+    MFMA engine and of the sparse oriented engine on the same graph, the
+    product launches' achieved int8 rate (S = A.A, 2 N^3 operations, and the
+    fused [A3 | AT], 8 N^3; the digit kernel between them is timed apart), the
+    rate of the 4-clique products of the local graphs, and the tier's phases."""
     out = {}
     for n, p in ((500, 0.9), (2000, 0.5)):
         s, d = pgd_amd.gen_gnp(n, p, 1)
         g = pgd_amd.GraphletCore(s, d, n=n, ctx=ctx)
         g.sort_edges("deg")
         res = {}
+
+        def mfma(ms_ops):
+            gms, ops = sorted(ms_ops)[len(ms_ops) // 2]
+            return {"bound": "mfma", "achieved": ops / (gms / 1e3) / 1e12, "peak": INT8_PEAK_TOPS,
+                    "unit": "TOP/s", "frac": ops / (gms / 1e3) / 1e12 / INT8_PEAK_TOPS, "ops": ops, "ms": gms}
+
         for eng in ("dense", "orient"):
             g.decompose(micro=True, engine=eng)  # warm-up (allocations, plan)
-            ms, gemm = [], []
+            ms, gemm, k4m, phases = [], [], [], []
             for _ in range(3):
                 g.decompose(micro=True, engine=eng)
                 ms.append(g.last_timing()[0])
-                mm = [q for q in g.last_phases() if q["name"].startswith("k_gemm_i8")]
+                ph = g.last_phases()
+                phases.append({q["name"]: q["ms"] for q in ph})
+                mm = [q for q in ph if q["name"].startswith("k_gemm_i8")]
                 if mm:  # the MFMA launches (S, then [A3 | AT]); bytes holds their operations
                     gemm.append((sum(q["ms"] for q in mm), sum(q["bytes"] for q in mm)))
+                kk = [q for q in ph if q["name"] == "k_dense_k4m"]
+                if kk:  # the 4-clique products of the local graphs (operations counted on the device)
+                    k4m.append((kk[0]["ms"], kk[0]["bytes"]))
             res[eng] = float(np.median(ms))
             if gemm:
-                gms, ops = sorted(gemm)[len(gemm) // 2]
-                res["mfma"] = {"bound": "mfma", "achieved": ops / (gms / 1e3) / 1e12, "peak": INT8_PEAK_TOPS,
-                               "unit": "TOP/s", "frac": ops / (gms / 1e3) / 1e12 / INT8_PEAK_TOPS,
-                               "ops": ops, "ms": gms}
+                res["mfma"] = mfma(gemm)
+            if k4m:
+                res["mfma_k4"] = mfma(k4m)
+            if eng == "dense":
+                res["phases_ms"] = {k: float(np.median([p_[k] for p_ in phases])) for k in phases[0]}
         out[f"gnp_{n}_{p}_seed1"] = {"m": g.m, "dense_ms": res["dense"], "orient_ms": res["orient"],
-                                     "edges_per_s": g.m / (res["dense"] / 1e3), "mfma": res.get("mfma")}
+                                     "edges_per_s": g.m / (res["dense"] / 1e3), "mfma": res.get("mfma"),
+                                     "mfma_k4": res.get("mfma_k4"), "phases_ms": res.get("phases_ms")}
         g.free()
         log(f"[bench] dense G({n},{p}): dense {res['dense']:.2f} ms, sparse {res['orient']:.2f} ms")
     return out

@@ -161,15 +161,26 @@ constexpr int kK4Rows = 128;              // rows (and columns) of one product t
 struct K4Info {
     int32_t v, d, i0, pad;
 };
+constexpr int kCB = 8;  // rows per lane in flight in k_dense_compress
+// lanes of k_dense_compress per row: the mask words Wd rounded up to a power
+// of two, at most 64 (a wavefront then takes 64 / that rows at once)
+__host__ __device__ inline int k4_lanes_per_row(int Wd) {
+    int l = 1;
+    while (l < Wd && l < 64) l <<= 1;
+    return l;
+}
+// row-buffer words per wavefront of k_dense_compress: kCB batches of rows of
+// `stride` words (stride <= Wd <= lanes per row when Wd <= 64)
+inline int k4_rowbuf_words(int Wd, int64_t stride) { return kCB * (64 / k4_lanes_per_row(Wd)) * (int)stride; }
 
 __global__ __launch_bounds__(256) void k_dense_compress(const uint32_t* __restrict__ bits, int Wd, int64_t v0,
                                                         int64_t v1, int64_t stride, int64_t vwords, int64_t dpad,
                                                         uint32_t* __restrict__ G, uint16_t* __restrict__ Lg,
                                                         K4Info* __restrict__ info, uint32_t* __restrict__ tiles,
                                                         uint32_t* __restrict__ count,
-                                                        unsigned long long* __restrict__ ops) {
+                                                        unsigned long long* __restrict__ ops, int rbw) {
     __shared__ uint32_t msk[kK4MaxW], pre[kK4MaxW], mv[5][kK4MaxW];
-    __shared__ uint32_t rowbuf[4][kK4MaxW];
+    extern __shared__ uint32_t rowbuf[];  // per wavefront: kCB rpp rows of `stride` words (k4_rowbuf_words)
     __shared__ uint32_t wsum[4];
     __shared__ uint16_t L[kDenseMaxN];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -197,7 +208,7 @@ __global__ __launch_bounds__(256) void k_dense_compress(const uint32_t* __restri
         }
         const uint32_t inc = wave_incl_sum(c);
         if (lane == 63) wsum[wv] = inc;
-        for (int i = tid; i < 4 * kK4MaxW; i += 256) (&rowbuf[0][0])[i] = 0u;
+        for (int i = tid; i < 4 * rbw; i += 256) rowbuf[i] = 0u;
         __syncthreads();
         uint32_t before = 0;
         for (int k = 0; k < wv; k++) before += wsum[k];
@@ -225,47 +236,66 @@ __global__ __launch_bounds__(256) void k_dense_compress(const uint32_t* __restri
                 atomicAdd(ops, 2ull * kK4Rows * dp * dp * nt);
             }
         }
-        // B's rows: wavefront wv takes rows wv, wv + 4, ...; lane w compresses
-        // word w of row L[i] under the mask and ors it in at bit pre[w]
-        uint32_t* rb = rowbuf[wv];
+        // B's rows, in batches of kCB rpp consecutive rows per wavefront: a row
+        // gets lpr lanes (Wd rounded up to a power of two, at most 64), lane
+        // wl of a row compresses words wl, wl + lpr, ... of row L[i] under the
+        // mask and ors them in at bit pre[w]; the kCB loads of a lane are
+        // issued together.  The batch's rows are then stored contiguously.
+        const int lpr = k4_lanes_per_row(Wd), rpp = 64 / lpr, sub = lane / lpr, wl = lane % lpr;
+        uint32_t* rb = rowbuf + wv * rbw;
         uint32_t* gv = G + slot * vwords;
-        for (uint32_t i = wv; i < d; i += 4) {
-            const int64_t u = L[i];
-            for (int w = lane; w < Wd; w += 64) {
+        const int nb = kCB * rpp;
+        for (uint32_t ib = (uint32_t)(wv * nb); ib < d; ib += (uint32_t)(4 * nb)) {
+            for (int w = wl; w < Wd; w += lpr) {
                 const uint32_t mw = msk[w];
-                if (!mw) continue;
-                uint32_t x = bits[u * Wd + w] & mw;
+                uint32_t x[kCB];
 #pragma unroll
-                for (int s = 0; s < 5; s++) {
-                    const uint32_t t = x & mv[s][w];
-                    x = (x ^ t) | (t >> (1 << s));
+                for (int b2 = 0; b2 < kCB; b2++) {
+                    const uint32_t i = ib + (uint32_t)(b2 * rpp + sub);
+                    x[b2] = (mw && i < d) ? bits[(int64_t)L[i] * Wd + w] : 0u;
                 }
+                if (!mw) continue;
                 const uint32_t off = pre[w], sh = off & 31;
-                atomicOr(&rb[off >> 5], x << sh);
-                if (sh && sh + (uint32_t)__popc(mw) > 32) atomicOr(&rb[(off >> 5) + 1], x >> (32 - sh));
+                const bool spill = sh && sh + (uint32_t)__popc(mw) > 32;
+#pragma unroll
+                for (int b2 = 0; b2 < kCB; b2++) {
+                    uint32_t y = x[b2] & mw;
+#pragma unroll
+                    for (int s2 = 0; s2 < 5; s2++) {
+                        const uint32_t t = y & mv[s2][w];
+                        y = (y ^ t) | (t >> (1 << s2));
+                    }
+                    uint32_t* row = rb + (b2 * rpp + sub) * stride;
+                    if (y) {
+                        atomicOr(&row[off >> 5], y << sh);
+                        if (spill) atomicOr(&row[(off >> 5) + 1], y >> (32 - sh));
+                    }
+                }
             }
             __builtin_amdgcn_wave_barrier();
-            for (int q = lane; q < stride; q += 64) {
-                gv[(int64_t)i * stride + q] = rb[q];
-                rb[q] = 0u;
+            const int nw = (int)min<int64_t>((int64_t)nb * stride, ((int64_t)d - ib) * stride);
+            for (int f = lane; f < nb * stride; f += 64) {
+                if (f < nw) gv[(int64_t)ib * stride + f] = rb[f];
+                rb[f] = 0u;
             }
             __builtin_amdgcn_wave_barrier();
         }
     }
 }
 
-// 32 bits as 32 bytes of 0/1 (bit k -> byte k): each nibble spread by a 24-bit
-// multiply, whose four shifted copies of the nibble do not overlap
+// 32 bits as 32 bytes of 0/1, in the order byte 4s + q = bit 8q + s (two
+// operations per four bytes).  The products sum over the depth, so any order
+// used alike for both operands gives the same sums.
 __device__ __forceinline__ void bits_to_bytes(int8_t* __restrict__ dst, uint32_t x) {
     uint4 lo, hi;
-    lo.x = __umul24(x & 0xfu, 0x204081u) & 0x01010101u;
-    lo.y = __umul24((x >> 4) & 0xfu, 0x204081u) & 0x01010101u;
-    lo.z = __umul24((x >> 8) & 0xfu, 0x204081u) & 0x01010101u;
-    lo.w = __umul24((x >> 12) & 0xfu, 0x204081u) & 0x01010101u;
-    hi.x = __umul24((x >> 16) & 0xfu, 0x204081u) & 0x01010101u;
-    hi.y = __umul24((x >> 20) & 0xfu, 0x204081u) & 0x01010101u;
-    hi.z = __umul24((x >> 24) & 0xfu, 0x204081u) & 0x01010101u;
-    hi.w = __umul24(x >> 28, 0x204081u) & 0x01010101u;
+    lo.x = x & 0x01010101u;
+    lo.y = (x >> 1) & 0x01010101u;
+    lo.z = (x >> 2) & 0x01010101u;
+    lo.w = (x >> 3) & 0x01010101u;
+    hi.x = (x >> 4) & 0x01010101u;
+    hi.y = (x >> 5) & 0x01010101u;
+    hi.z = (x >> 6) & 0x01010101u;
+    hi.w = (x >> 7) & 0x01010101u;
     *(uint4*)dst = lo;
     *(uint4*)(dst + 16) = hi;
 }
@@ -273,22 +303,30 @@ __device__ __forceinline__ void bits_to_bytes(int8_t* __restrict__ dst, uint32_t
 // One block per row tile of one vertex's local graph (tiles from the list
 // k_dense_compress built): for 128 rows i of B (from i0 + 128 t) and every
 // 128-column tile of B, the int8 product (B.B) over the depth dp (d rounded up
-// to 128), staged 64 deep in LDS as k_gemm_i8 stages it but expanded from the
-// bit rows; the epilogue adds the products where B[i][j] = 1 to per-row sums.
-// Twice K4(v, L[i]) goes to K4M[v N + L[i]].
+// to 128), one 128-deep step at a time: both operands' bit rows are expanded
+// into LDS (rows of 144 bytes; the lanes of a store take consecutive rows, so
+// neither the stores nor the fragment reads meet bank conflicts), double-
+// buffered with one barrier per step.  MFMA operands and accumulators as in
+// k_gemm_i8.  The epilogue adds the products where B[i][j] = 1 to per-row
+// sums, reading those bits from row j (B is symmetric: a lane's column j holds
+// the bits of its rows in one 32-bit window of row j).  Rows and columns past
+// d have zero products, so their bits are never needed.  Twice K4(v, L[i])
+// goes to K4M[v N + L[i]].
+constexpr int kK4Step = 128;    // depth per staged step
+constexpr int kK4Stride = 144;  // LDS bytes per staged row
 __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ G, const uint16_t* __restrict__ Lg,
                                                    const K4Info* __restrict__ info, const uint32_t* __restrict__ tiles,
                                                    const uint32_t* __restrict__ count, int64_t stride,
                                                    int64_t vwords, int64_t dpad, int64_t N,
                                                    int32_t* __restrict__ K4M) {
-    __shared__ __align__(16) int8_t As[2][kTile * kGStride];
-    __shared__ __align__(16) int8_t Xs[2][kTile * kGStride];
-    __shared__ uint32_t msk[kK4Rows * 4];
+    __shared__ __align__(16) int8_t As[2][kK4Rows * kK4Stride];
+    __shared__ __align__(16) int8_t Xs[2][kK4Rows * kK4Stride];
     __shared__ int32_t rs[kK4Rows];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wm = w >> 1, wn = w & 1;
-    const int r0 = tid >> 1, h = tid & 1;  // staging: 32 bits of row r0 (word h of each 64-deep step)
-    const int fo = (lane & 31) * kGStride + 16 * (lane >> 5);
+    const int r0 = tid & 127, h = tid >> 7;  // staging: 64 bits of row r0 (words 2h, 2h + 1 of each step)
+    const int fo = (lane & 31) * kK4Stride + 16 * (lane >> 5);
     const uint32_t ntiles = *count;
+    int buf = 0;
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t tl = tiles[t];
         const int64_t slot = tl >> 6;
@@ -297,7 +335,7 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
         const int dp = (d + kK4Rows - 1) / kK4Rows * kK4Rows;
         const uint32_t* gv = G + slot * vwords;
         const int ia = rbase + r0;
-        const uint32_t* pa = gv + (int64_t)min(ia, d - 1) * stride + h;
+        const uint2* pa = (const uint2*)(gv + (int64_t)min(ia, d - 1) * stride) + h;
         if (tid < kK4Rows) rs[tid] = 0;
         int32_t part[2][16];
 #pragma unroll
@@ -306,29 +344,33 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
             for (int r = 0; r < 16; r++) part[a][r] = 0;
         for (int ct = 0; ct < dp; ct += kK4Rows) {
             const int jx = ct + r0;
-            const uint32_t* px = gv + (int64_t)min(jx, d - 1) * stride + h;
-            uint32_t wa = ia < d ? pa[0] : 0u, wx = jx < d ? px[0] : 0u;
+            const uint2* px = (const uint2*)(gv + (int64_t)min(jx, d - 1) * stride) + h;
+            const uint2 zero2 = make_uint2(0u, 0u);
+            uint2 wa = ia < d ? pa[0] : zero2, wx = jx < d ? px[0] : zero2;
             v16i acc[2][2];
 #pragma unroll
             for (int a = 0; a < 2; a++)
 #pragma unroll
                 for (int b = 0; b < 2; b++) acc[a][b] = v16i{};
-            int buf = 0;
-            for (int ks = 0; ks < dp; ks += kGK) {
-                bits_to_bytes(As[buf] + r0 * kGStride + 32 * h, wa);
-                bits_to_bytes(Xs[buf] + r0 * kGStride + 32 * h, wx);
+            for (int ks = 0; ks < dp; ks += kK4Step) {
+                int8_t* sa = As[buf] + r0 * kK4Stride + 64 * h;
+                int8_t* sx = Xs[buf] + r0 * kK4Stride + 64 * h;
+                bits_to_bytes(sa, wa.x);
+                bits_to_bytes(sa + 32, wa.y);
+                bits_to_bytes(sx, wx.x);
+                bits_to_bytes(sx + 32, wx.y);
                 __syncthreads();
-                if (ks + kGK < dp) {
-                    const int nw = (ks + kGK) >> 5;
-                    wa = ia < d ? pa[nw] : 0u;
-                    wx = jx < d ? px[nw] : 0u;
+                if (ks + kK4Step < dp) {
+                    const int q = (ks + kK4Step) / 64;  // uint2 index of the next step's words
+                    wa = ia < d ? pa[q] : zero2;
+                    wx = jx < d ? px[q] : zero2;
                 }
-                const int8_t* as = As[buf] + (wm * 64) * kGStride + fo;
-                const int8_t* xs = Xs[buf] + (wn * 64) * kGStride + fo;
+                const int8_t* as = As[buf] + (wm * 64) * kK4Stride + fo;
+                const int8_t* xs = Xs[buf] + (wn * 64) * kK4Stride + fo;
 #pragma unroll
-                for (int kk = 0; kk < kGK; kk += 32) {
-                    const v4i a0 = *(const v4i*)(as + kk), a1 = *(const v4i*)(as + 32 * kGStride + kk);
-                    const v4i x0 = *(const v4i*)(xs + kk), x1 = *(const v4i*)(xs + 32 * kGStride + kk);
+                for (int kk = 0; kk < kK4Step; kk += 32) {
+                    const v4i a0 = *(const v4i*)(as + kk), a1 = *(const v4i*)(as + 32 * kK4Stride + kk);
+                    const v4i x0 = *(const v4i*)(xs + kk), x1 = *(const v4i*)(xs + 32 * kK4Stride + kk);
                     acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, x0, acc[0][0], 0, 0, 0);
                     acc[0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, x1, acc[0][1], 0, 0, 0);
                     acc[1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, x0, acc[1][0], 0, 0, 0);
@@ -336,27 +378,25 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
                 }
                 buf ^= 1;
             }
-            // B's bits of this tile (rows rbase.., words ct / 32 .. + 4)
-            {
-                const int rr = tid >> 1, q = 2 * (tid & 1), i = rbase + rr;
-                uint2 mw = make_uint2(0u, 0u);
-                if (i < d) mw = *(const uint2*)(gv + (int64_t)i * stride + (ct >> 5) + q);
-                msk[rr * 4 + q] = mw.x;
-                msk[rr * 4 + q + 1] = mw.y;
-            }
-            __syncthreads();
+            // accumulator r of (a, b) is row wm 64 + a 32 + k with
+            // k = (r mod 4) + 8 (r / 4) + 4 (lane / 32), column cl: its bit is
+            // bit k of row cl's 32-bit window at rbase + wm 64 + a 32
 #pragma unroll
-            for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                const int cl = ct + wn * 64 + b * 32 + (lane & 31);
+                const uint32_t* gr = gv + (int64_t)min(cl, dp - 1) * stride;
 #pragma unroll
-                for (int b = 0; b < 2; b++)
+                for (int a = 0; a < 2; a++) {
+                    const int o = rbase + wm * 64 + a * 32;
+                    const int q0 = min(o >> 5, (int)stride - 1), q1 = min((o >> 5) + 1, (int)stride - 1);
+                    const uint32_t win = __builtin_amdgcn_alignbit(gr[q1], gr[q0], (uint32_t)(o & 31));
 #pragma unroll
                     for (int r = 0; r < 16; r++) {
-                        const int rl = wm * 64 + a * 32 + (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
-                        const int cl = wn * 64 + b * 32 + (lane & 31);
-                        const uint32_t bit = (msk[rl * 4 + (cl >> 5)] >> (cl & 31)) & 1u;
-                        part[a][r] += bit ? acc[a][b][r] : 0;
+                        const int k = (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
+                        part[a][r] += ((win >> k) & 1u) ? acc[a][b][r] : 0;
                     }
-            __syncthreads();  // msk and the staging buffers are rewritten by the next column tile
+                }
+            }
         }
         // each row's sum over the 32 lanes of its half wavefront, then over the
         // two wavefronts that hold its two column halves
@@ -505,6 +545,7 @@ void dense_launch(pgd_ctx* ctx, const GraphView& g, const DenseBufs& d, EdgeCoun
         unsigned long long* ops = (unsigned long long*)(d.W + kp.oCount + 8);
         hipLaunchKernelGGL(d_fill_u32, dim3(1), dim3(64), 0, s, count, (int64_t)4);
         PGD_HIP(hipGetLastError());
+        const int rbw = k4_rowbuf_words(Wd, kp.stride);  // <= kCB kK4MaxW: 32 KiB per block at most
         int per_cu = 0;
         PGD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dense_k4m, 256, 0));
         per_cu = std::max(per_cu, 1);
@@ -512,8 +553,9 @@ void dense_launch(pgd_ctx* ctx, const GraphView& g, const DenseBufs& d, EdgeCoun
             const int64_t v1 = std::min<int64_t>(n, v0 + kp.chunk);
             if (v0) hipLaunchKernelGGL(d_fill_u32, dim3(1), dim3(64), 0, s, count, (int64_t)1);
             if (timer) timer->begin("k_dense_compress");
-            hipLaunchKernelGGL(k_dense_compress, dim3(grid_cap(ctx, v1 - v0, 8)), dim3(256), 0, s, d.bits, Wd, v0,
-                               v1, kp.stride, kp.vwords, kp.dpad, G, Lg, info, tiles, count, ops);
+            hipLaunchKernelGGL(k_dense_compress, dim3(grid_cap(ctx, v1 - v0, 8)), dim3(256),
+                               (size_t)4 * rbw * sizeof(uint32_t), s, d.bits, Wd, v0, v1, kp.stride, kp.vwords,
+                               kp.dpad, G, Lg, info, tiles, count, ops, rbw);
             PGD_HIP(hipGetLastError());
             if (timer) timer->end();
             if (timer) timer->begin("k_dense_k4m");
