@@ -178,7 +178,8 @@ __global__ __launch_bounds__(256) void k_dense_compress(const uint32_t* __restri
                                                         uint32_t* __restrict__ G, uint16_t* __restrict__ Lg,
                                                         K4Info* __restrict__ info, uint32_t* __restrict__ tiles,
                                                         uint32_t* __restrict__ count,
-                                                        unsigned long long* __restrict__ ops, int rbw) {
+                                                        unsigned long long* __restrict__ ops, int rbw, int64_t N,
+                                                        int32_t* __restrict__ K4M) {
     __shared__ uint32_t msk[kK4MaxW], pre[kK4MaxW], mv[5][kK4MaxW];
     extern __shared__ uint32_t rowbuf[];  // per wavefront: kCB rpp rows of `stride` words (k4_rowbuf_words)
     __shared__ uint32_t wsum[4];
@@ -225,15 +226,21 @@ __global__ __launch_bounds__(256) void k_dense_compress(const uint32_t* __restri
         __syncthreads();
         const int vw = (int)(v >> 5);
         const uint32_t i0 = pre[vw] + (uint32_t)__popc(msk[vw] & ((1u << (v & 31)) - 1u));
-        for (uint32_t i = tid; i < d; i += 256) Lg[slot * dpad + i] = L[i];
+        for (uint32_t i = tid; i < d; i += 256) {
+            Lg[slot * dpad + i] = L[i];
+            if (i >= i0) K4M[v * N + L[i]] = 0;  // the products add into these
+        }
         if (tid == 0) {
             info[slot] = K4Info{(int32_t)v, (int32_t)d, (int32_t)i0, 0};
             const uint32_t nt = (d - i0 + kK4Rows - 1) / kK4Rows;
             if (nt) {
                 const uint32_t base = atomicAdd(count, nt);
                 for (uint32_t t = 0; t < nt; t++) tiles[base + t] = (uint32_t)slot << 6 | t;
+                // row tile t multiplies the column tiles below i0 and those from t on
                 const unsigned long long dp = (d + kK4Rows - 1) / kK4Rows * kK4Rows;
-                atomicAdd(ops, 2ull * kK4Rows * dp * dp * nt);
+                const unsigned long long nlow = (i0 + kK4Rows - 1) / kK4Rows;
+                const unsigned long long ncols = nt * nlow + (unsigned long long)nt * (nt + 1) / 2;
+                atomicAdd(ops, 2ull * kK4Rows * kK4Rows * dp * ncols);
             }
         }
         // B's rows, in batches of kCB rpp consecutive rows per wavefront: a row
@@ -301,17 +308,22 @@ __device__ __forceinline__ void bits_to_bytes(int8_t* __restrict__ dst, uint32_t
 }
 
 // One block per row tile of one vertex's local graph (tiles from the list
-// k_dense_compress built): for 128 rows i of B (from i0 + 128 t) and every
-// 128-column tile of B, the int8 product (B.B) over the depth dp (d rounded up
-// to 128), one 128-deep step at a time: both operands' bit rows are expanded
+// k_dense_compress built): for 128 rows i of B (from i0 + 128 t), the int8
+// product (B.B) with 128-column tiles of B over the depth dp (d rounded up to
+// 128), one 128-deep step at a time: both operands' bit rows are expanded
 // into LDS (rows of 144 bytes; the lanes of a store take consecutive rows, so
 // neither the stores nor the fragment reads meet bank conflicts), double-
 // buffered with one barrier per step.  MFMA operands and accumulators as in
-// k_gemm_i8.  The epilogue adds the products where B[i][j] = 1 to per-row
-// sums, reading those bits from row j (B is symmetric: a lane's column j holds
-// the bits of its rows in one 32-bit window of row j).  Rows and columns past
-// d have zero products, so their bits are never needed.  Twice K4(v, L[i])
-// goes to K4M[v N + L[i]].
+// k_gemm_i8.  The epilogue keeps the products where B[i][j] = 1, reading those
+// bits from row j (B is symmetric: a lane's column j holds the bits of its
+// rows in one 32-bit window of row j); rows and columns past d have zero
+// products, so their bits are never needed.
+// Column tiles are cut at i0 + 128 c like the row tiles.  (B.B) o B is
+// symmetric, so among the rows and columns from i0 on, row tile t takes the
+// column tiles c >= t only: the kept products add to the sums of their rows,
+// and, for c > t, to the sums of their columns (which are rows of tile c);
+// the column tiles below i0 add to the rows only.  Twice K4(v, L[i]) is the
+// total, added into K4M[v N + L[i]] (cleared by k_dense_compress).
 constexpr int kK4Step = 128;    // depth per staged step
 constexpr int kK4Stride = 144;  // LDS bytes per staged row
 __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ G, const uint16_t* __restrict__ Lg,
@@ -342,11 +354,16 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
         for (int a = 0; a < 2; a++)
 #pragma unroll
             for (int r = 0; r < 16; r++) part[a][r] = 0;
-        for (int ct = 0; ct < dp; ct += kK4Rows) {
+        const int rt = (int)(tl & 63), nlow = (inf.i0 + kK4Rows - 1) / kK4Rows;
+        const int ntile = (d - inf.i0 + kK4Rows - 1) / kK4Rows;
+        for (int c = -nlow; c < ntile; c++) {
+            if (c >= 0 && c < rt) continue;  // met from row tile c
+            const int ct = inf.i0 + c * kK4Rows;  // first column (negative below 0)
             const int jx = ct + r0;
-            const uint2* px = (const uint2*)(gv + (int64_t)min(jx, d - 1) * stride) + h;
+            const bool vx = jx >= 0 && jx < d;
+            const uint2* px = (const uint2*)(gv + (int64_t)min(max(jx, 0), d - 1) * stride) + h;
             const uint2 zero2 = make_uint2(0u, 0u);
-            uint2 wa = ia < d ? pa[0] : zero2, wx = jx < d ? px[0] : zero2;
+            uint2 wa = ia < d ? pa[0] : zero2, wx = vx ? px[0] : zero2;
             v16i acc[2][2];
 #pragma unroll
             for (int a = 0; a < 2; a++)
@@ -363,7 +380,7 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
                 if (ks + kK4Step < dp) {
                     const int q = (ks + kK4Step) / 64;  // uint2 index of the next step's words
                     wa = ia < d ? pa[q] : zero2;
-                    wx = jx < d ? px[q] : zero2;
+                    wx = vx ? px[q] : zero2;
                 }
                 const int8_t* as = As[buf] + (wm * 64) * kK4Stride + fo;
                 const int8_t* xs = Xs[buf] + (wn * 64) * kK4Stride + fo;
@@ -384,7 +401,8 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
 #pragma unroll
             for (int b = 0; b < 2; b++) {
                 const int cl = ct + wn * 64 + b * 32 + (lane & 31);
-                const uint32_t* gr = gv + (int64_t)min(cl, dp - 1) * stride;
+                const uint32_t* gr = gv + (int64_t)min(max(cl, 0), dp - 1) * stride;
+                int32_t col = 0;
 #pragma unroll
                 for (int a = 0; a < 2; a++) {
                     const int o = rbase + wm * 64 + a * 32;
@@ -393,8 +411,14 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
 #pragma unroll
                     for (int r = 0; r < 16; r++) {
                         const int k = (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
-                        part[a][r] += ((win >> k) & 1u) ? acc[a][b][r] : 0;
+                        const int32_t kept = ((win >> k) & 1u) ? acc[a][b][r] : 0;
+                        part[a][r] += kept;
+                        col += kept;
                     }
+                }
+                if (c > rt) {  // the column sums of a tile above the diagonal
+                    col += __shfl_xor(col, 32, 64);
+                    if (lane < 32 && cl < d) atomicAdd(&K4M[(int64_t)inf.v * N + Lg[slot * dpad + cl]], col);
                 }
             }
         }
@@ -412,7 +436,7 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
         __syncthreads();
         if (tid < kK4Rows) {
             const int i = rbase + tid;
-            if (i < d) K4M[(int64_t)inf.v * N + Lg[slot * dpad + i]] = rs[tid];
+            if (i < d) atomicAdd(&K4M[(int64_t)inf.v * N + Lg[slot * dpad + i]], rs[tid]);
         }
         __syncthreads();  // rs is cleared for the next tile
     }
@@ -555,7 +579,7 @@ void dense_launch(pgd_ctx* ctx, const GraphView& g, const DenseBufs& d, EdgeCoun
             if (timer) timer->begin("k_dense_compress");
             hipLaunchKernelGGL(k_dense_compress, dim3(grid_cap(ctx, v1 - v0, 8)), dim3(256),
                                (size_t)4 * rbw * sizeof(uint32_t), s, d.bits, Wd, v0, v1, kp.stride, kp.vwords,
-                               kp.dpad, G, Lg, info, tiles, count, ops, rbw);
+                               kp.dpad, G, Lg, info, tiles, count, ops, rbw, N, K4M);
             PGD_HIP(hipGetLastError());
             if (timer) timer->end();
             if (timer) timer->begin("k_dense_k4m");
