@@ -417,7 +417,7 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
                                           int log2r, const WedgeSeg* __restrict__ seg,
                                           const int32_t* __restrict__ col, uint32_t* cnt,
                                           uint64_t* __restrict__ PS, uint32_t* __restrict__ PS32,
-                                          int64_t* gbase, int32_t* gslot, unsigned long long* gpsum,
+                                          int64_t* gbase, int2* gbnd, int32_t* gslot, unsigned long long* gpsum,
                                           uint32_t* wflag, int* ovf, int* rep, int lane, int w,
                                           unsigned long long* done) {
     constexpr int NW = NT / 64;
@@ -442,10 +442,21 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
             sl = g.sab;
         }
         const uint64_t wmask = __ballot(wide);  // segments of wide lists
-        const int32_t x = (int32_t)wave_incl_sum((uint32_t)len);
+        // pass 1 lays each segment out on the 16-slot (64-byte) lines of its
+        // slots: its positions start at a multiple of 16, with st mod 16
+        // unused positions before its first slot and padding after its last
+        // up to the next multiple of 16.  The 16 lanes of an add instruction
+        // that share a line then take that line's slots only, and each line
+        // of a segment is one atomic request (unaligned, a 64-lane run of
+        // slots spans five lines and shares one with the next run).
+        const int32_t front = pass ? (int32_t)(st & 15) : 0;
+        const int32_t plen = pass && lane < kc ? (front + len + 15) & ~15 : len;
+        const int32_t x = (int32_t)wave_incl_sum((uint32_t)plen);
         if (w == 0) {
-            // slot of flattened position q in segment i: gbase[i] + q
-            gbase[lane] = st - (int64_t)(x - len);
+            // slot of flattened position q in segment i: gbase[i] + q, for q in
+            // [gbnd[i].x, gbnd[i].y)
+            gbase[lane] = st - front - (int64_t)(x - plen);
+            gbnd[lane] = make_int2(x - plen + front, x - plen + front + len);
             gslot[lane] = sl;
             gpsum[lane] = 0;
         }
@@ -453,13 +464,16 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
         const int32_t total = __shfl(x, 63, 64);
         // wedges counted (once per tile) and wedges accumulated (tiles with
         // a repeated endpoint; an overflowing tile once)
-        if (w == 0 && lane == 0 && (pass ? mode < 2 : mode == 0)) done[pass] += (unsigned long long)total;
+        if (pass ? mode < 2 : mode == 0) {
+            const int32_t nw = pass ? __shfl((int32_t)wave_incl_sum((uint32_t)len), 63, 64) : total;
+            if (w == 0 && lane == 0) done[pass] += (unsigned long long)nw;
+        }
         // (the owner of position q, the segment holding it, is the last segment
         // starting at or before q: the segment starts inside a round are
         // flagged in LDS and read back as one 64-bit ballot per step, so the
         // owner is a popcount, not a search -- segments are non-empty, so their
         // starts are distinct)
-        const int32_t es = lane < kc ? x - len : INT32_MAX;  // exclusive start of segment `lane`
+        const int32_t es = lane < kc ? x - plen : INT32_MAX;  // exclusive start of segment `lane`
         const uint64_t upto = (2ULL << lane) - 1;             // lanes 0..lane
         for (int32_t rb0 = w * 64 * kU; rb0 < total; rb0 += NW * 64 * kU) {
             int o[kU];
@@ -482,11 +496,17 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
             for (int u = 0; u < kU; u++) {
                 starts[u] = __ballot(fl[u] != 0);
                 const int32_t q = rb0 + 64 * u + lane;
-                ok[u] = q < total;
                 // past the end: the owner is the last segment; clamp to its last position
                 o[u] = cum + __popcll(starts[u] & upto);
                 cum += __popcll(starts[u]);
-                slot[u] = gbase[o[u]] + (ok[u] ? q : total - 1);
+                if (pass) {  // padding positions: clamped to the segment's slots, no update
+                    const int2 bd = gbnd[o[u]];
+                    ok[u] = q >= bd.x && q < bd.y;
+                    slot[u] = gbase[o[u]] + min(max(q, bd.x), bd.y - 1);
+                } else {
+                    ok[u] = q < total;
+                    slot[u] = gbase[o[u]] + (ok[u] ? q : total - 1);
+                }
             }
 #pragma unroll
             for (int u = 0; u < kU; u++) c[u] = col[slot[u]] - c0;
@@ -705,6 +725,7 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     __shared__ uint32_t wflag[NW][64];  // segment-start flags of one round of a wavefront (4 bytes per lane)
     __shared__ unsigned long long wpsum[NW][64];
     __shared__ int64_t gbase[64];  // the block passes' current group of segments
+    __shared__ int2 gbnd[64];
     __shared__ int32_t gslot[64];
     __shared__ unsigned long long gpsum[64];
     __shared__ int64_t item_sh;
@@ -787,13 +808,13 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
                 rep = 0;
             }
             __syncthreads();
-            tile_pass<NT>(0, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gslot, gpsum, wflag[w], &ovf, &rep,
+            tile_pass<NT>(0, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gbnd, gslot, gpsum, wflag[w], &ovf, &rep,
                           lane, w, done);
             __syncthreads();
             if (!ovf) {
                 // every count 1: no 4-cycle closes in this tile
                 if (rep) {
-                    tile_pass<NT>(1, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gslot, gpsum,
+                    tile_pass<NT>(1, 0, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gbnd, gslot, gpsum,
                                   wflag[w], &ovf, &rep, lane, w, done);
                     __syncthreads();
                 }
@@ -801,10 +822,10 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
                 for (int mode = 1; mode <= 2; mode++) {
                     clear_lds<NT>(cnt, R / 2);
                     __syncthreads();
-                    tile_pass<NT>(0, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gslot, gpsum,
+                    tile_pass<NT>(0, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gbnd, gslot, gpsum,
                                   wflag[w], &ovf, &rep, lane, w, done);
                     __syncthreads();
-                    tile_pass<NT>(1, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gslot, gpsum,
+                    tile_pass<NT>(1, mode, chk, sb, se, c0, log2r, seg, col, cnt, PS, PS32, gbase, gbnd, gslot, gpsum,
                                   wflag[w], &ovf, &rep, lane, w, done);
                     __syncthreads();
                 }
