@@ -1406,11 +1406,12 @@ pgd_status pgd_decompose(pgd_graph* g, uint32_t flags, uint64_t* sums) {
         if (flags & PGD_ENGINE_ORIENT) engine = 2;
         else if (flags & PGD_ENGINE_DENSE) engine = 3;
         else engine = engine_dense_ok(v) ? 3 : 2;
+        DenseRunPtr drun;  // the dense tier's timers, read after the final synchronisation
         if (g->m > 0) {
             if (engine == 2) engine_orient(ctx, v, c, &t, &g->orient, g->prm);
             else {
                 PGD_CHECK(engine_dense_fits(v), PGD_ERR_UNSUPPORTED, "graph too large for the dense engine");
-                engine_dense(ctx, v, c, &t);
+                drun = engine_dense(ctx, v, c);
             }
         }
         epilogue(ctx, v, c, g->sums.get(), micro ? g->micro.get() : nullptr);
@@ -1418,6 +1419,7 @@ pgd_status pgd_decompose(pgd_graph* g, uint32_t flags, uint64_t* sums) {
         PGD_HIP(hipMemcpyAsync(sums, g->sums.get(), PGD_NSUMS * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                ctx->stream));
         sync(ctx);
+        if (drun) dense_timing(*drun, &t);
         float ms = 0;
         PGD_HIP(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);

@@ -608,33 +608,45 @@ bool engine_dense_ok(const GraphView& g) {
     return g.n >= 64 && g.n <= kDenseMaxN && 10 * g.m > 4 * g.n * (g.n - 1);
 }
 
-void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm) {
-    PGD_CHECK(g.n <= kDenseMaxN, PGD_ERR_UNSUPPORTED, "graph too large for the dense engine");
-    const int64_t n = g.n, m = g.m;
-    const int64_t N = (n + kTile - 1) / kTile * kTile, NN = N * N;
+struct DenseRun {
+    explicit DenseRun(pgd_ctx* c) : timer{c} {}
+    Timer timer;
     DBuf<unsigned char> buf, work;
-    buf.alloc(ctx, dense_bytes(N));
-    DenseBufs d = carve(buf.get(), N);
-    if (m) {
-        d.wbytes = k4_work_bytes(n, g.max_deg, N);
-        work.alloc(ctx, d.wbytes);
-        d.W = work.get();
+    const unsigned long long* d_ops = nullptr;  // the 4-clique products' operation count, on the device
+    int64_t n = 0, m = 0, N = 0;
+};
+void DenseRunDeleter::operator()(DenseRun* r) const { delete r; }
+
+DenseRunPtr engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out) {
+    PGD_CHECK(g.n <= kDenseMaxN, PGD_ERR_UNSUPPORTED, "graph too large for the dense engine");
+    DenseRunPtr r(new DenseRun(ctx));
+    r->n = g.n;
+    r->m = g.m;
+    r->N = (g.n + kTile - 1) / kTile * kTile;
+    r->buf.alloc(ctx, dense_bytes(r->N));
+    DenseBufs d = carve(r->buf.get(), r->N);
+    if (g.m) {
+        d.wbytes = k4_work_bytes(g.n, g.max_deg, r->N);
+        r->work.alloc(ctx, d.wbytes);
+        d.W = r->work.get();
     }
-    Timer timer{ctx};
-    dense_launch(ctx, g, d, out, &timer);
-    unsigned long long k4_ops = 0;
-    if (m) {
-        const K4Plan kp = k4_plan(g.max_deg, d.wbytes);
-        PGD_HIP(hipMemcpyAsync(&k4_ops, d.W + kp.oCount + 8, sizeof(k4_ops), hipMemcpyDeviceToHost, ctx->stream));
-    }
-    PGD_HIP(hipStreamSynchronize(ctx->stream));
+    dense_launch(ctx, g, d, out, &r->timer);
+    if (g.m) r->d_ops = (const unsigned long long*)(d.W + k4_plan(g.max_deg, d.wbytes).oCount + 8);
+    return r;
+}
+
+void dense_timing(DenseRun& r, Timing* tm) {
     // work models: the MFMA products are 2 N^3 (S), 8 N^3 ([A3 | AT] over
-    // K = 2N) and, for the 4-clique products, 2 x 128 x dp^2 per row tile of
-    // a vertex (dp its degree rounded up to 128; counted on the device) int8
-    // operations; the adjacency pass writes 2 N^2 + N^2/8 bytes; the edge pass
-    // reads four int32 products and writes T, K4 and C4 per edge.  A pass run
-    // once per chunk of vertices is one phase (times summed).
-    const double n3 = (double)N * N * N;
+    // K = 2N) and, for the 4-clique products, 2 x 128 x 128 x dp per column
+    // tile a row tile multiplies (dp: the vertex's degree rounded up to 128;
+    // counted on the device) int8 operations; the adjacency pass writes
+    // 2 N^2 + N^2/8 bytes; the edge pass reads four int32 products and writes
+    // T, K4 and C4 per edge.  A pass run once per chunk of vertices is one
+    // phase (times summed).
+    const double N = (double)r.N, NN = N * N, n3 = NN * N;
+    unsigned long long k4_ops = 0;
+    if (r.d_ops) PGD_HIP(hipMemcpy(&k4_ops, r.d_ops, sizeof(k4_ops), hipMemcpyDeviceToHost));
+    Timer& timer = r.timer;
     tm->phases.clear();
     for (size_t i = 0; i < timer.ev.size(); i++) {
         float ms = 0;
@@ -648,13 +660,13 @@ void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* tm) 
             continue;
         }
         double bytes = 0, units = 0;  // for the MFMA passes, bytes holds operations
-        if (name == "k_dense_adj") { bytes = 2.0 * NN + NN / 8.0; units = (double)NN; }
+        if (name == "k_dense_adj") { bytes = 2.0 * NN + NN / 8.0; units = NN; }
         if (name == "k_gemm_i8") { bytes = 2.0 * n3; units = 1; }
-        if (name == "k_dense_digits") { bytes = 9.0 * NN; units = (double)NN; }  // S and A read, four digit bytes
+        if (name == "k_dense_digits") { bytes = 9.0 * NN; units = NN; }  // S and A read, four digit bytes
         if (name == "k_gemm_i8_pq") { bytes = 8.0 * n3; units = 1; }
-        if (name == "k_dense_compress") { units = (double)n; }
+        if (name == "k_dense_compress") { units = (double)r.n; }
         if (name == "k_dense_k4m") { bytes = (double)k4_ops; units = 1; }
-        if (name == "k_dense_finish") { bytes = 36.0 * m; units = (double)m; }
+        if (name == "k_dense_finish") { bytes = 36.0 * r.m; units = (double)r.m; }
         tm->phases.push_back(Phase{name, ms, bytes, units});
     }
     double best = 0;

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <map>
 #include <mutex>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -176,7 +177,17 @@ const char* tuning_problem(const pgd_tuning& t);
 void orient_state_free(pgd_ctx* ctx, OrientState* s);
 bool engine_dense_ok(const GraphView& g);    // chosen automatically (above 80% density, small enough)
 bool engine_dense_fits(const GraphView& g);  // may be requested (small enough)
-void engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out, Timing* t);
+// the dense tier launches its passes without a host synchronisation; the run
+// keeps its buffers and timers until dense_timing reads them, once the
+// context's stream has finished the run (freeing it earlier is safe: its
+// memory returns to the stream-ordered cache)
+struct DenseRun;
+struct DenseRunDeleter {
+    void operator()(DenseRun* r) const;
+};
+using DenseRunPtr = std::unique_ptr<DenseRun, DenseRunDeleter>;
+DenseRunPtr engine_dense(pgd_ctx* ctx, const GraphView& g, EdgeCounts out);
+void dense_timing(DenseRun& r, Timing* t);
 // allocation-free path (pgd_decompose_noalloc): the dense engine and the
 // epilogue over the static scratch array of this GPU
 void noalloc_scratch_init(int device);
