@@ -601,11 +601,17 @@ void dense_launch(pgd_ctx* ctx, const GraphView& g, const DenseBufs& d, EdgeCoun
 bool engine_dense_fits(const GraphView& g) { return g.n <= kDenseMaxN; }
 
 bool engine_dense_ok(const GraphView& g) {
-    // the reference's own dense criterion (create_adj_mat,
-    // graphlet_core.cpp:602-633): more than 80% of all pairs are edges; the
-    // N x N products stay modest.  Below that density the oriented engine is
-    // faster (G(2000, 0.5): 10 ms against 17 ms for this tier).
-    return g.n >= 64 && g.n <= kDenseMaxN && 10 * g.m > 4 * g.n * (g.n - 1);
+    // where this tier is the faster engine.  Measured on G(n, p), one
+    // decomposition each way: at n <= 4000 it wins at every density from 0.01
+    // to 0.5 (the oriented engine's launches and host synchronisations cost
+    // about 0.6 ms even for a few hundred edges; G(4000, 0.5): 19 against 167
+    // ms); at n = 8000 the oriented engine wins up to density 0.05 (1.6
+    // against 5.0 ms at 0.01, 4.6 against 6.4 ms at 0.05) and this tier from
+    // 0.1 on (10 against 15 ms).  So: every graph up to 4096 vertices, and up
+    // to 8192 vertices those with at least 8% of all pairs as edges (at most
+    // 18 N^2 bytes plus a 512 MiB work area).
+    if (g.n < 2 || g.n > kDenseMaxN) return false;
+    return g.n <= 4096 || 25 * g.m >= g.n * (g.n - 1);
 }
 
 struct DenseRun {

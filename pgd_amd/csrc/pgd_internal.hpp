@@ -175,7 +175,7 @@ inline pgd_tuning tuning_from(const OrientParams& p) {
 // empty string when t is usable, else what is wrong
 const char* tuning_problem(const pgd_tuning& t);
 void orient_state_free(pgd_ctx* ctx, OrientState* s);
-bool engine_dense_ok(const GraphView& g);    // chosen automatically (above 80% density, small enough)
+bool engine_dense_ok(const GraphView& g);    // chosen automatically (n <= 4096, or n <= 8192 and density >= 0.08)
 bool engine_dense_fits(const GraphView& g);  // may be requested (small enough)
 // the dense tier launches its passes without a host synchronisation; the run
 // keeps its buffers and timers until dense_timing reads them, once the

@@ -43,10 +43,10 @@ def test_dense_configurations(n, p, seed, ctx):
     s, d = pgd_amd.gen_gnp(n, p, seed)
     g, out = both_engines(ctx, s, d, n)
     assert_same(out)
-    # left to itself the library picks the dense tier above the reference's
-    # 80% density (create_adj_mat), the oriented engine below
+    # left to itself the library picks the faster engine: the dense tier for
+    # every graph up to 4096 vertices
     g.decompose(micro=False)
-    assert g.last_engine() == ("dense" if p > 0.8 else "orient")
+    assert g.last_engine() == "dense"
     # the oracle on a sample of edges
     T, K4, C4 = out["dense"][1]
     og = OracleGraph.from_edges(s, d, n)
@@ -90,6 +90,21 @@ def test_dense_matches_sparse(case, ctx):
         assert np.all(T == n - 2)
         assert np.all(K4 == (n - 2) * (n - 3) // 2)
         assert np.all(C4 == 0)
+    g.free()
+
+
+@pytest.mark.parametrize("p,engine", [(0.02, "orient"), (0.1, "dense")])
+def test_automatic_engine_above_4096_vertices(p, engine, ctx):
+    # above 4096 vertices the dense tier is picked from 8% of all pairs on
+    # (where it is the faster engine), and both engines agree
+    say(f"[gpu] automatic engine G(5000,{p})")
+    s, d = pgd_amd.gen_gnp(5000, p, 3)
+    g = pgd_amd.GraphletCore(s, d, n=5000, ctx=ctx)
+    g.sort_edges("deg")
+    auto = g.decompose(micro=False)
+    assert g.last_engine() == engine
+    other = "dense" if engine == "orient" else "orient"
+    assert g.decompose(micro=False, engine=other) == auto
     g.free()
 
 
