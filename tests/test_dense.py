@@ -65,10 +65,11 @@ def complete(n):
     return i.astype(np.int32), j.astype(np.int32)
 
 
-# padded sizes N = 128, 256, 896, 4096, 6528 and 8064: the 4-clique pass holds
-# slabs of 16, 16, 16, 4, 2 and 1 words of every bit row in LDS
+# padded sizes N = 128 to 8064; the fan (a hub joined to every vertex of a
+# path, n = 4096) gives one vertex a local graph of 4095 vertices, so the
+# 4-clique work area holds 256 vertices at a time (16 chunks)
 @pytest.mark.parametrize("case", ["k64", "k100", "gnp65", "gnp130_0.7", "gnp777_0.3", "gnp4000_0.3", "gnp6500_0.05",
-                                  "gnp8000_0.02", "sparse_ba"])
+                                  "gnp8000_0.02", "sparse_ba", "fan4096"])
 def test_dense_matches_sparse(case, ctx):
     say(f"[gpu] dense vs sparse {case}")
     if case == "k64":
@@ -79,6 +80,11 @@ def test_dense_matches_sparse(case, ctx):
         n, (s, d) = 65, pgd_amd.gen_gnp(65, 0.5, 3)
     elif case == "sparse_ba":
         n, (s, d) = 3000, pgd_amd.gen_ba(3000, 3, 8)    # requested explicitly on a sparse graph
+    elif case == "fan4096":
+        n = 4096
+        leaves = np.arange(1, n, dtype=np.int32)
+        s = np.concatenate([np.zeros(n - 1, np.int32), leaves[:-1]])
+        d = np.concatenate([leaves, leaves[1:]])
     else:
         _, nn, pp = case.split("_")[0][:3], int(case.split("_")[0][3:]), float(case.split("_")[1])
         n, (s, d) = nn, pgd_amd.gen_gnp(nn, pp, 5)
@@ -90,6 +96,15 @@ def test_dense_matches_sparse(case, ctx):
         assert np.all(T == n - 2)
         assert np.all(K4 == (n - 2) * (n - 3) // 2)
         assert np.all(C4 == 0)
+    if case == "fan4096":
+        # triangles: hub-leaf edges carry the leaf's path degree, path edges
+        # the hub; the hub's chords leave no 4-clique and no induced 4-cycle
+        T, K4, C4 = out["dense"][1]
+        ev, eu = g.edge_arrays()
+        lo, hi = np.minimum(ev, eu), np.maximum(ev, eu)
+        want = np.where(lo == 0, np.where((hi == 1) | (hi == n - 1), 1, 2), 1)
+        assert np.array_equal(T, want.astype(T.dtype))
+        assert not K4.any() and not C4.any()
     g.free()
 
 
