@@ -290,53 +290,50 @@ __global__ __launch_bounds__(256) void k_dense_compress(const uint32_t* __restri
     }
 }
 
-// 32 bits as 32 bytes of 0/1, in the order byte 4s + q = bit 8q + s (two
-// operations per four bytes).  The products sum over the depth, so any order
-// used alike for both operands gives the same sums.
-__device__ __forceinline__ void bits_to_bytes(int8_t* __restrict__ dst, uint32_t x) {
-    uint4 lo, hi;
-    lo.x = x & 0x01010101u;
-    lo.y = (x >> 1) & 0x01010101u;
-    lo.z = (x >> 2) & 0x01010101u;
-    lo.w = (x >> 3) & 0x01010101u;
-    hi.x = (x >> 4) & 0x01010101u;
-    hi.y = (x >> 5) & 0x01010101u;
-    hi.z = (x >> 6) & 0x01010101u;
-    hi.w = (x >> 7) & 0x01010101u;
-    *(uint4*)dst = lo;
-    *(uint4*)(dst + 16) = hi;
+// One 32-deep slice of an MFMA operand from a 32-bit word of a bit row: the
+// lane's 16 bytes (half hi of the slice) are 0/1 with byte 4s + q = bit
+// 8q + 4 hi + s (two operations per four bytes).  The products sum over the
+// depth, so this order, used alike for both operands, gives the same sums.
+__device__ __forceinline__ v4i slice_bytes(uint32_t x, int hi) {
+    const uint32_t y = x >> (4 * hi);
+    v4i f;
+    f.x = (int)(y & 0x01010101u);
+    f.y = (int)((y >> 1) & 0x01010101u);
+    f.z = (int)((y >> 2) & 0x01010101u);
+    f.w = (int)((y >> 3) & 0x01010101u);
+    return f;
 }
 
 // One block per row tile of one vertex's local graph (tiles from the list
 // k_dense_compress built): for 128 rows i of B (from i0 + 128 t), the int8
 // product (B.B) with 128-column tiles of B over the depth dp (d rounded up to
-// 128), one 128-deep step at a time: both operands' bit rows are expanded
-// into LDS (rows of 144 bytes; the lanes of a store take consecutive rows, so
-// neither the stores nor the fragment reads meet bank conflicts), double-
-// buffered with one barrier per step.  MFMA operands and accumulators as in
-// k_gemm_i8.  The epilogue keeps the products where B[i][j] = 1, reading those
-// bits from row j (B is symmetric: a lane's column j holds the bits of its
-// rows in one 32-bit window of row j); rows and columns past d have zero
-// products, so their bits are never needed.
+// 128), one 128-deep step at a time: both operands' bit rows are staged in
+// LDS as bits (16 bytes per row and step), double-buffered with one barrier
+// per step, and each wavefront builds its MFMA operand slices from them in
+// registers (one 32-bit LDS read per slice serves both halves of the
+// wavefront).  MFMA operands and accumulators as in k_gemm_i8.  The epilogue
+// keeps the products where B[i][j] = 1, reading those bits from row j (B is
+// symmetric: a lane's column j holds the bits of its rows in one 32-bit
+// window of row j); rows and columns past d have zero products, so their
+// bits are never needed.
 // Column tiles are cut at i0 + 128 c like the row tiles.  (B.B) o B is
 // symmetric, so among the rows and columns from i0 on, row tile t takes the
 // column tiles c >= t only: the kept products add to the sums of their rows,
 // and, for c > t, to the sums of their columns (which are rows of tile c);
 // the column tiles below i0 add to the rows only.  Twice K4(v, L[i]) is the
 // total, added into K4M[v N + L[i]] (cleared by k_dense_compress).
-constexpr int kK4Step = 128;    // depth per staged step
-constexpr int kK4Stride = 144;  // LDS bytes per staged row
+constexpr int kK4Step = 128;  // depth per staged step (4 words per row)
 __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ G, const uint16_t* __restrict__ Lg,
                                                    const K4Info* __restrict__ info, const uint32_t* __restrict__ tiles,
                                                    const uint32_t* __restrict__ count, int64_t stride,
                                                    int64_t vwords, int64_t dpad, int64_t N,
                                                    int32_t* __restrict__ K4M) {
-    __shared__ __align__(16) int8_t As[2][kK4Rows * kK4Stride];
-    __shared__ __align__(16) int8_t Xs[2][kK4Rows * kK4Stride];
+    __shared__ __align__(16) uint32_t As[2][kK4Rows * 4];
+    __shared__ __align__(16) uint32_t Xs[2][kK4Rows * 4];
     __shared__ int32_t rs[kK4Rows];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wm = w >> 1, wn = w & 1;
-    const int r0 = tid & 127, h = tid >> 7;  // staging: 64 bits of row r0 (words 2h, 2h + 1 of each step)
-    const int fo = (lane & 31) * kK4Stride + 16 * (lane >> 5);
+    const int r0 = tid & 127, h = tid >> 7;  // staging: words 2h, 2h + 1 of row r0 in each step
+    const int hi = lane >> 5, lr = lane & 31;
     const uint32_t ntiles = *count;
     int buf = 0;
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -370,24 +367,20 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
 #pragma unroll
                 for (int b = 0; b < 2; b++) acc[a][b] = v16i{};
             for (int ks = 0; ks < dp; ks += kK4Step) {
-                int8_t* sa = As[buf] + r0 * kK4Stride + 64 * h;
-                int8_t* sx = Xs[buf] + r0 * kK4Stride + 64 * h;
-                bits_to_bytes(sa, wa.x);
-                bits_to_bytes(sa + 32, wa.y);
-                bits_to_bytes(sx, wx.x);
-                bits_to_bytes(sx + 32, wx.y);
+                *(uint2*)(As[buf] + r0 * 4 + 2 * h) = wa;
+                *(uint2*)(Xs[buf] + r0 * 4 + 2 * h) = wx;
                 __syncthreads();
                 if (ks + kK4Step < dp) {
                     const int q = (ks + kK4Step) / 64;  // uint2 index of the next step's words
                     wa = ia < d ? pa[q] : zero2;
                     wx = vx ? px[q] : zero2;
                 }
-                const int8_t* as = As[buf] + (wm * 64) * kK4Stride + fo;
-                const int8_t* xs = Xs[buf] + (wn * 64) * kK4Stride + fo;
+                const uint32_t* as = As[buf] + (wm * 64 + lr) * 4;
+                const uint32_t* xs = Xs[buf] + (wn * 64 + lr) * 4;
 #pragma unroll
-                for (int kk = 0; kk < kK4Step; kk += 32) {
-                    const v4i a0 = *(const v4i*)(as + kk), a1 = *(const v4i*)(as + 32 * kK4Stride + kk);
-                    const v4i x0 = *(const v4i*)(xs + kk), x1 = *(const v4i*)(xs + 32 * kK4Stride + kk);
+                for (int sl = 0; sl < 4; sl++) {
+                    const v4i a0 = slice_bytes(as[sl], hi), a1 = slice_bytes(as[32 * 4 + sl], hi);
+                    const v4i x0 = slice_bytes(xs[sl], hi), x1 = slice_bytes(xs[32 * 4 + sl], hi);
                     acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, x0, acc[0][0], 0, 0, 0);
                     acc[0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, x1, acc[0][1], 0, 0, 0);
                     acc[1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, x0, acc[1][0], 0, 0, 0);
