@@ -7,7 +7,9 @@
 //   A3   = A.S                  P(e) = A3_vu - d_v - d_u + 1 (4-cycles through e)
 //   AT   = A.T, T = S o A       Q(e) = AT_vu + AT_uv  (sum over triangles {v,u,w}
 //                               of T(v,w) + T(u,w))
-//   K4(e) = edges among the common neighbours of v and u (bit rows, popcounts)
+//   K4(e) = edges among the common neighbours of v and u: for e = (v, u),
+//           v < u, half of rowsum((B.B) o B) at u's row of B = A[N(v), N(v)],
+//           v's local graph (int8 products of compressed bit rows, below)
 //   C4(e) = P - Q + 2T + 2K4    (the oriented engine's identity)
 //
 // Every product is an int8 x int8 -> int32 MFMA (v_mfma_i32_32x32x32_i8).
