@@ -1405,7 +1405,12 @@ pgd_status pgd_decompose(pgd_graph* g, uint32_t flags, uint64_t* sums) {
         int engine;
         if (flags & PGD_ENGINE_ORIENT) engine = 2;
         else if (flags & PGD_ENGINE_DENSE) engine = 3;
-        else engine = engine_dense_ok(v) ? 3 : 2;
+        else {
+            // the faster engine, unless the dense tier would pass the context's memory cap
+            const bool room = ctx->mem_cap <= 0 ||
+                              ctx->mem_used.load() + (int64_t)engine_dense_bytes(v) <= ctx->mem_cap;
+            engine = engine_dense_ok(v) && room ? 3 : 2;
+        }
         DenseRunPtr drun;  // the dense tier's timers, read after the final synchronisation
         if (g->m > 0) {
             if (engine == 2) engine_orient(ctx, v, c, &t, &g->orient, g->prm);

@@ -595,6 +595,11 @@ void dense_launch(pgd_ctx* ctx, const GraphView& g, const DenseBufs& d, EdgeCoun
 
 bool engine_dense_fits(const GraphView& g) { return g.n <= kDenseMaxN; }
 
+size_t engine_dense_bytes(const GraphView& g) {
+    const int64_t N = (g.n + kTile - 1) / kTile * kTile;
+    return dense_bytes(N) + (g.m ? k4_work_bytes(g.n, g.max_deg, N) : 0);
+}
+
 bool engine_dense_ok(const GraphView& g) {
     // where this tier is the faster engine.  Measured on G(n, p), one
     // decomposition each way: at n <= 4000 it wins at every density from 0.01

@@ -177,6 +177,7 @@ const char* tuning_problem(const pgd_tuning& t);
 void orient_state_free(pgd_ctx* ctx, OrientState* s);
 bool engine_dense_ok(const GraphView& g);    // chosen automatically (n <= 4096, or n <= 8192 and density >= 0.08)
 bool engine_dense_fits(const GraphView& g);  // may be requested (small enough)
+size_t engine_dense_bytes(const GraphView& g);  // the tier's device memory for the graph
 // the dense tier launches its passes without a host synchronisation; the run
 // keeps its buffers and timers until dense_timing reads them, once the
 // context's stream has finished the run (freeing it earlier is safe: its

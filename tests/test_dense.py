@@ -123,6 +123,22 @@ def test_automatic_engine_above_4096_vertices(p, engine, ctx):
     g.free()
 
 
+def test_automatic_engine_respects_memory_cap():
+    # under a cap the dense tier's arrays would pass, the automatic choice
+    # takes the oriented engine (same counts)
+    s, d = pgd_amd.gen_gnp(3000, 0.01, 4)
+    with pgd_amd.Context(0) as c:
+        g = pgd_amd.GraphletCore(s, d, n=3000, ctx=c)
+        g.sort_edges("deg")
+        free = g.decompose(micro=False)
+        assert g.last_engine() == "dense"
+        c.set_mem_cap(c.mem_in_use() + (64 << 20))    # 64 MiB more: below the tier's 18 N^2 bytes at N = 3072
+        capped = g.decompose(micro=False)
+        assert g.last_engine() == "orient"
+        assert capped == free
+        g.free()
+
+
 def test_dense_rejects_large(ctx):
     s, d = pgd_amd.gen_ba(9000, 2, 1)
     g = pgd_amd.GraphletCore(s, d, ctx=ctx)
