@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0   # dense int8 MFMA, MI355X
+FP4_PEAK_TFLOPS = 10100.0  # dense fp4/fp6 MFMA peak of one MI355X
 
 
 def log(*a):
@@ -258,7 +259,8 @@ def bench_dense(pgd_amd, ctx) -> dict:
     MFMA engine and of the sparse oriented engine on the same graph, the
     product launches' achieved int8 rate (S = A.A, 2 N^3 operations, and the
     fused [A3 | AT], 8 N^3; the digit kernel between them is timed apart), the
-    rate of the 4-clique products of the local graphs, and the tier's phases."""
+    rate of the 4-clique products of the local graphs (fp4 MFMA, against the
+    fp4 peak), and the tier's phases."""
     out = {}
     for n, p in ((500, 0.9), (2000, 0.5)):
         s, d = pgd_amd.gen_gnp(n, p, 1)
@@ -266,10 +268,10 @@ def bench_dense(pgd_amd, ctx) -> dict:
         g.sort_edges("deg")
         res = {}
 
-        def mfma(ms_ops):
+        def mfma(ms_ops, peak=INT8_PEAK_TOPS, unit="TOP/s"):
             gms, ops = sorted(ms_ops)[len(ms_ops) // 2]
-            return {"bound": "mfma", "achieved": ops / (gms / 1e3) / 1e12, "peak": INT8_PEAK_TOPS,
-                    "unit": "TOP/s", "frac": ops / (gms / 1e3) / 1e12 / INT8_PEAK_TOPS, "ops": ops, "ms": gms}
+            return {"bound": "mfma", "achieved": ops / (gms / 1e3) / 1e12, "peak": peak,
+                    "unit": unit, "frac": ops / (gms / 1e3) / 1e12 / peak, "ops": ops, "ms": gms}
 
         for eng in ("dense", "orient"):
             g.decompose(micro=True, engine=eng)  # warm-up (allocations, plan)
@@ -283,13 +285,13 @@ def bench_dense(pgd_amd, ctx) -> dict:
                 if mm:  # the MFMA launches (S, then [A3 | AT]); bytes holds their operations
                     gemm.append((sum(q["ms"] for q in mm), sum(q["bytes"] for q in mm)))
                 kk = [q for q in ph if q["name"] == "k_dense_k4m"]
-                if kk:  # the 4-clique products of the local graphs (operations counted on the device)
+                if kk:  # the 4-clique products of the local graphs (fp4 MFMA; operations counted on the device)
                     k4m.append((kk[0]["ms"], kk[0]["bytes"]))
             res[eng] = float(np.median(ms))
             if gemm:
                 res["mfma"] = mfma(gemm)
-            if k4m:
-                res["mfma_k4"] = mfma(k4m)
+            if k4m:  # fp4 (e2m1) operands, f32 accumulation
+                res["mfma_k4"] = mfma(k4m, FP4_PEAK_TFLOPS, "TFLOP/s")
             if eng == "dense":
                 res["phases_ms"] = {k: float(np.median([p_[k] for p_ in phases])) for k in phases[0]}
         out[f"gnp_{n}_{p}_seed1"] = {"m": g.m, "dense_ms": res["dense"], "orient_ms": res["orient"],

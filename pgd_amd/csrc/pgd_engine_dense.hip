@@ -292,18 +292,28 @@ __global__ __launch_bounds__(256) void k_dense_compress(const uint32_t* __restri
     }
 }
 
-// One 32-deep slice of an MFMA operand from a 32-bit word of a bit row: the
-// lane's 16 bytes (half hi of the slice) are 0/1 with byte 4s + q = bit
-// 8q + 4 hi + s (two operations per four bytes).  The products sum over the
-// depth, so this order, used alike for both operands, gives the same sums.
-__device__ __forceinline__ v4i slice_bytes(uint32_t x, int hi) {
-    const uint32_t y = x >> (4 * hi);
-    v4i f;
-    f.x = (int)(y & 0x01010101u);
-    f.y = (int)((y >> 1) & 0x01010101u);
-    f.z = (int)((y >> 2) & 0x01010101u);
-    f.w = (int)((y >> 3) & 0x01010101u);
+// One 64-deep slice of an fp4 MFMA operand (v_mfma_scale_f32_32x32x64_f8f6f4
+// with e2m1 operands: 32 values per lane, the lane's half of the slice, in 4
+// registers) from a 32-bit word of a bit row: a set bit becomes the nibble 1
+// (0.5 in e2m1), with register q holding bits q, q + 4, ..., q + 28 (two
+// operations per eight values).  With a block scale of 2 for each operand
+// (E8M0 128) every product of two set bits is 1 and the f32 sums are the
+// exact counts (at most 8192).  The products sum over the depth, so this
+// order, used alike for both operands, gives the same sums.
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+constexpr int kFp4 = 4;         // operand format code of e2m1
+constexpr int kScaleTwo = 128;  // E8M0 2^(128 - 127)
+__device__ __forceinline__ v8i slice_fp4(uint32_t x) {
+    v8i f = v8i{};
+    f[0] = (int)(x & 0x11111111u);
+    f[1] = (int)((x >> 1) & 0x11111111u);
+    f[2] = (int)((x >> 2) & 0x11111111u);
+    f[3] = (int)((x >> 3) & 0x11111111u);
     return f;
+}
+__device__ __forceinline__ v16f mfma_fp4(v8i a, v8i b, v16f c) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, kFp4, kFp4, 0, kScaleTwo, 0, kScaleTwo);
 }
 
 // One block per row tile of one vertex's local graph (tiles from the list
@@ -311,9 +321,10 @@ __device__ __forceinline__ v4i slice_bytes(uint32_t x, int hi) {
 // product (B.B) with 128-column tiles of B over the depth dp (d rounded up to
 // 128), one 128-deep step at a time: both operands' bit rows are staged in
 // LDS as bits (16 bytes per row and step), double-buffered with one barrier
-// per step, and each wavefront builds its MFMA operand slices from them in
-// registers (one 32-bit LDS read per slice serves both halves of the
-// wavefront).  MFMA operands and accumulators as in k_gemm_i8.  The epilogue
+// per step, and each wavefront builds its fp4 MFMA operand slices from them
+// in registers (one 32-bit LDS read per lane and 64-deep slice; lane l feeds
+// row l mod 32 and half l / 32 of the slice, as for the int8 products).
+// Accumulators as in k_gemm_i8 (f32, exact integers).  The epilogue
 // keeps the products where B[i][j] = 1, reading those bits from row j (B is
 // symmetric: a lane's column j holds the bits of its rows in one 32-bit
 // window of row j); rows and columns past d have zero products, so their
@@ -363,11 +374,11 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
             const uint2* px = (const uint2*)(gv + (int64_t)min(max(jx, 0), d - 1) * stride) + h;
             const uint2 zero2 = make_uint2(0u, 0u);
             uint2 wa = ia < d ? pa[0] : zero2, wx = vx ? px[0] : zero2;
-            v16i acc[2][2];
+            v16f acc[2][2];
 #pragma unroll
             for (int a = 0; a < 2; a++)
 #pragma unroll
-                for (int b = 0; b < 2; b++) acc[a][b] = v16i{};
+                for (int b = 0; b < 2; b++) acc[a][b] = v16f{};
             for (int ks = 0; ks < dp; ks += kK4Step) {
                 *(uint2*)(As[buf] + r0 * 4 + 2 * h) = wa;
                 *(uint2*)(Xs[buf] + r0 * 4 + 2 * h) = wx;
@@ -380,13 +391,14 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
                 const uint32_t* as = As[buf] + (wm * 64 + lr) * 4;
                 const uint32_t* xs = Xs[buf] + (wn * 64 + lr) * 4;
 #pragma unroll
-                for (int sl = 0; sl < 4; sl++) {
-                    const v4i a0 = slice_bytes(as[sl], hi), a1 = slice_bytes(as[32 * 4 + sl], hi);
-                    const v4i x0 = slice_bytes(xs[sl], hi), x1 = slice_bytes(xs[32 * 4 + sl], hi);
-                    acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, x0, acc[0][0], 0, 0, 0);
-                    acc[0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, x1, acc[0][1], 0, 0, 0);
-                    acc[1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, x0, acc[1][0], 0, 0, 0);
-                    acc[1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, x1, acc[1][1], 0, 0, 0);
+                for (int sl = 0; sl < 2; sl++) {
+                    const int wd = 2 * sl + hi;  // this lane's word of the slice
+                    const v8i a0 = slice_fp4(as[wd]), a1 = slice_fp4(as[32 * 4 + wd]);
+                    const v8i x0 = slice_fp4(xs[wd]), x1 = slice_fp4(xs[32 * 4 + wd]);
+                    acc[0][0] = mfma_fp4(a0, x0, acc[0][0]);
+                    acc[0][1] = mfma_fp4(a0, x1, acc[0][1]);
+                    acc[1][0] = mfma_fp4(a1, x0, acc[1][0]);
+                    acc[1][1] = mfma_fp4(a1, x1, acc[1][1]);
                 }
                 buf ^= 1;
             }
@@ -406,7 +418,7 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
 #pragma unroll
                     for (int r = 0; r < 16; r++) {
                         const int k = (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
-                        const int32_t kept = ((win >> k) & 1u) ? acc[a][b][r] : 0;
+                        const int32_t kept = ((win >> k) & 1u) ? (int32_t)acc[a][b][r] : 0;
                         part[a][r] += kept;
                         col += kept;
                     }
