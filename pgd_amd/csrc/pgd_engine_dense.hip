@@ -317,15 +317,16 @@ __device__ __forceinline__ v16f mfma_fp4(v8i a, v8i b, v16f c) {
 }
 
 // One block per row tile of one vertex's local graph (tiles from the list
-// k_dense_compress built): for 128 rows i of B (from i0 + 128 t), the int8
+// k_dense_compress built): for 128 rows i of B (from i0 + 128 t), the
 // product (B.B) with 128-column tiles of B over the depth dp (d rounded up to
 // 128), one 128-deep step at a time: both operands' bit rows are staged in
-// LDS as bits (16 bytes per row and step), double-buffered with one barrier
-// per step, and each wavefront builds its fp4 MFMA operand slices from them
-// in registers (one 32-bit LDS read per lane and 64-deep slice; lane l feeds
-// row l mod 32 and half l / 32 of the slice, as for the int8 products).
-// Accumulators as in k_gemm_i8 (f32, exact integers).  The epilogue
-// keeps the products where B[i][j] = 1, reading those bits from row j (B is
+// LDS as bits (16 bytes per row and step, rows 24 bytes apart), double-
+// buffered with one barrier per step, and each wavefront builds its fp4 MFMA
+// operand slices from them in registers (one 32-bit LDS read per lane and
+// 64-deep slice; lane l feeds row l mod 32 and half l / 32 of the slice, as
+// for the int8 products).  Accumulators as in k_gemm_i8 (f32, exact
+// integers).  The epilogue keeps the products where B[i][j] = 1, reading
+// those bits from row j (B is
 // symmetric: a lane's column j holds the bits of its rows in one 32-bit
 // window of row j); rows and columns past d have zero products, so their
 // bits are never needed.
@@ -336,13 +337,17 @@ __device__ __forceinline__ v16f mfma_fp4(v8i a, v8i b, v16f c) {
 // the column tiles below i0 add to the rows only.  Twice K4(v, L[i]) is the
 // total, added into K4M[v N + L[i]] (cleared by k_dense_compress).
 constexpr int kK4Step = 128;  // depth per staged step (4 words per row)
+// LDS words per staged row: a slice read takes word 2s + hi of 32
+// consecutive rows, and 6 r mod 64 is distinct and even for 32 rows, so the
+// 64 addresses fall on 64 banks
 __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ G, const uint16_t* __restrict__ Lg,
                                                    const K4Info* __restrict__ info, const uint32_t* __restrict__ tiles,
                                                    const uint32_t* __restrict__ count, int64_t stride,
                                                    int64_t vwords, int64_t dpad, int64_t N,
                                                    int32_t* __restrict__ K4M) {
-    __shared__ __align__(16) uint32_t As[2][kK4Rows * 4];
-    __shared__ __align__(16) uint32_t Xs[2][kK4Rows * 4];
+    constexpr int kRowW = 6;
+    __shared__ __align__(16) uint32_t As[2][kK4Rows * kRowW];
+    __shared__ __align__(16) uint32_t Xs[2][kK4Rows * kRowW];
     __shared__ int32_t rs[kK4Rows];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wm = w >> 1, wn = w & 1;
     const int r0 = tid & 127, h = tid >> 7;  // staging: words 2h, 2h + 1 of row r0 in each step
@@ -380,21 +385,21 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
 #pragma unroll
                 for (int b = 0; b < 2; b++) acc[a][b] = v16f{};
             for (int ks = 0; ks < dp; ks += kK4Step) {
-                *(uint2*)(As[buf] + r0 * 4 + 2 * h) = wa;
-                *(uint2*)(Xs[buf] + r0 * 4 + 2 * h) = wx;
+                *(uint2*)(As[buf] + r0 * kRowW + 2 * h) = wa;
+                *(uint2*)(Xs[buf] + r0 * kRowW + 2 * h) = wx;
                 __syncthreads();
                 if (ks + kK4Step < dp) {
                     const int q = (ks + kK4Step) / 64;  // uint2 index of the next step's words
                     wa = ia < d ? pa[q] : zero2;
                     wx = vx ? px[q] : zero2;
                 }
-                const uint32_t* as = As[buf] + (wm * 64 + lr) * 4;
-                const uint32_t* xs = Xs[buf] + (wn * 64 + lr) * 4;
+                const uint32_t* as = As[buf] + (wm * 64 + lr) * kRowW;
+                const uint32_t* xs = Xs[buf] + (wn * 64 + lr) * kRowW;
 #pragma unroll
                 for (int sl = 0; sl < 2; sl++) {
                     const int wd = 2 * sl + hi;  // this lane's word of the slice
-                    const v8i a0 = slice_fp4(as[wd]), a1 = slice_fp4(as[32 * 4 + wd]);
-                    const v8i x0 = slice_fp4(xs[wd]), x1 = slice_fp4(xs[32 * 4 + wd]);
+                    const v8i a0 = slice_fp4(as[wd]), a1 = slice_fp4(as[32 * kRowW + wd]);
+                    const v8i x0 = slice_fp4(xs[wd]), x1 = slice_fp4(xs[32 * kRowW + wd]);
                     acc[0][0] = mfma_fp4(a0, x0, acc[0][0]);
                     acc[0][1] = mfma_fp4(a0, x1, acc[0][1]);
                     acc[1][0] = mfma_fp4(a1, x0, acc[1][0]);
