@@ -61,7 +61,7 @@ typedef struct pgd_graph pgd_graph;
 #define PGD_MICRO        1u   /* also produce the per-edge columns */
 /* 2u is not a flag: any bit other than those below is PGD_ERR_INVALID */
 #define PGD_ENGINE_ORIENT 4u  /* force the degree-oriented engine */
-#define PGD_ENGINE_DENSE 8u   /* force the dense (int8 MFMA) engine */
+#define PGD_ENGINE_DENSE 8u   /* force the dense (int8 and fp4 MFMA) engine */
 
 /* ---- context -------------------------------------------------------- */
 pgd_status  pgd_open(int32_t device, pgd_ctx **out);
@@ -174,7 +174,7 @@ pgd_status pgd_decompose(pgd_graph *g, uint32_t flags, uint64_t *sums);
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * The same result as pgd_decompose(g, flags | PGD_ENGINE_DENSE, ...) -- the
- * dense int8-MFMA engine and the epilogue -- with every intermediate array
+ * dense matrix-core engine and the epilogue -- with every intermediate array
  * (adjacency, products, per-edge counts) in one static device array of
  * PGD_NOALLOC_SCRATCH_BYTES per GPU, whose address is taken once per device
  * at pgd_open.  It calls no allocating routine.
