@@ -54,17 +54,18 @@ __global__ void k_dense_adj(const int64_t* __restrict__ row, const int32_t* __re
 
 // C = A . X^T over K: C[i][j] = sum_{k < K} A[i lda + k] X[j ldx + k] for int8
 // row-major A (M x K used) and X (cols x K used), int32 C (row stride ldc);
-// M, cols multiples of 128, K of 64.  One block per 128 x 128 output tile,
-// four wavefronts of 64 x 64 (2 x 2 MFMA tiles of 32 x 32).  Each 64-deep
-// step of both operands is staged in LDS (rows padded to 80 bytes, so the
-// 16-byte fragment reads of 32 consecutive rows spread over the banks),
-// double-buffered: the next step's global loads are in flight during the
-// current step's products, and one barrier per step separates the buffers.
-// MFMA operands: lane l feeds row (l mod 32) and bytes 16 (l / 32) .. + 16 of
+// M, cols and K multiples of 128.  One block per 128 x 128 output tile, four
+// wavefronts of 64 x 64 (2 x 2 MFMA tiles of 32 x 32).  Each 128-deep step
+// of both operands is staged in LDS, double-buffered: the next step's global
+// loads are in flight during the current step's products, and one barrier
+// per step separates the buffers.  LDS rows are 144 bytes: the 16-byte
+// fragment reads of 16 consecutive rows then fall on distinct banks, and the
+// staging stores, eight lanes per row (a row's 128 bytes), too.  MFMA
+// operands: lane l feeds row (l mod 32) and bytes 16 (l / 32) .. + 16 of
 // the 32-deep k slice, for both operands; accumulator r of lane l holds row
 // (r mod 4) + 8 (r / 4) + 4 (l / 32), column l mod 32.
-constexpr int kGK = 64;       // k per staged step
-constexpr int kGStride = 80;  // LDS bytes per staged row
+constexpr int kGK = 128;       // k per staged step
+constexpr int kGStride = 144;  // LDS bytes per staged row
 __global__ __launch_bounds__(256) void k_gemm_i8(const int8_t* __restrict__ A, int64_t lda,
                                                  const int8_t* __restrict__ X, int64_t ldx,
                                                  int32_t* __restrict__ C, int64_t ldc, int64_t K) {
@@ -72,15 +73,17 @@ __global__ __launch_bounds__(256) void k_gemm_i8(const int8_t* __restrict__ A, i
     __shared__ __align__(16) int8_t Xs[2][kTile * kGStride];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t bi = (int64_t)blockIdx.y * kTile, bj = (int64_t)blockIdx.x * kTile;
-    // staging: thread t moves 16-byte chunks t and t + 256 of each 128 x 64 tile
-    // (chunk c: row c / 4, bytes 16 (c mod 4) .. + 16)
-    const int r0 = tid >> 2, c0 = (tid & 3) * 16;
-    const int8_t* ga0 = A + (bi + r0) * lda + c0;
-    const int8_t* ga1 = A + (bi + r0 + 64) * lda + c0;
-    const int8_t* gx0 = X + (bj + r0) * ldx + c0;
-    const int8_t* gx1 = X + (bj + r0 + 64) * ldx + c0;
-    const int so0 = r0 * kGStride + c0, so1 = (r0 + 64) * kGStride + c0;
-    v4i ra0 = *(const v4i*)ga0, ra1 = *(const v4i*)ga1, rx0 = *(const v4i*)gx0, rx1 = *(const v4i*)gx1;
+    // staging: thread t moves 16-byte chunk t mod 8 of rows t / 8 + 32 j
+    // (j = 0..3) of each 128 x 128 tile
+    const int r0 = tid >> 3, c0 = (tid & 7) * 16;
+    const int8_t* ga = A + (bi + r0) * lda + c0;
+    const int8_t* gx = X + (bj + r0) * ldx + c0;
+    v4i ra[4], rx[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        ra[j] = *(const v4i*)(ga + 32 * j * lda);
+        rx[j] = *(const v4i*)(gx + 32 * j * ldx);
+    }
     const int wm = w >> 1, wn = w & 1;
     const int fo = (lane & 31) * kGStride + 16 * (lane >> 5);  // this lane's fragment offset in a 32-row slice
     v16i acc[2][2];
@@ -90,16 +93,18 @@ __global__ __launch_bounds__(256) void k_gemm_i8(const int8_t* __restrict__ A, i
         for (int b = 0; b < 2; b++) acc[a][b] = v16i{};
     int buf = 0;
     for (int64_t k0 = 0; k0 < K; k0 += kGK) {
-        *(v4i*)(As[buf] + so0) = ra0;
-        *(v4i*)(As[buf] + so1) = ra1;
-        *(v4i*)(Xs[buf] + so0) = rx0;
-        *(v4i*)(Xs[buf] + so1) = rx1;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            *(v4i*)(As[buf] + (r0 + 32 * j) * kGStride + c0) = ra[j];
+            *(v4i*)(Xs[buf] + (r0 + 32 * j) * kGStride + c0) = rx[j];
+        }
         __syncthreads();
         if (k0 + kGK < K) {
-            ra0 = *(const v4i*)(ga0 + k0 + kGK);
-            ra1 = *(const v4i*)(ga1 + k0 + kGK);
-            rx0 = *(const v4i*)(gx0 + k0 + kGK);
-            rx1 = *(const v4i*)(gx1 + k0 + kGK);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                ra[j] = *(const v4i*)(ga + 32 * j * lda + k0 + kGK);
+                rx[j] = *(const v4i*)(gx + 32 * j * ldx + k0 + kGK);
+            }
         }
         const int8_t* as = As[buf] + (wm * 64) * kGStride + fo;
         const int8_t* xs = Xs[buf] + (wn * 64) * kGStride + fo;
