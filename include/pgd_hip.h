@@ -62,6 +62,7 @@ typedef struct pgd_graph pgd_graph;
 /* 2u is not a flag: any bit other than those below is PGD_ERR_INVALID */
 #define PGD_ENGINE_ORIENT 4u  /* force the degree-oriented engine */
 #define PGD_ENGINE_DENSE 8u   /* force the dense (int8 and fp4 MFMA) engine */
+#define PGD_DENSE_MAX_N 8192  /* the dense engine's largest vertex count */
 
 /* ---- context -------------------------------------------------------- */
 pgd_status  pgd_open(int32_t device, pgd_ctx **out);

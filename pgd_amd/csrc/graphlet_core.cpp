@@ -144,8 +144,17 @@ void graphlet_core::read_graph(const std::string& filename) {
     basic_stats();
 }
 
-void graphlet_core::preprocess(std::string neigh, bool s2l, int /*adj_limit*/, std::string /*rep*/) {
+// The representation (-r) as the reference applies it (optimize_graph_ops and
+// set_rep_manually, graphlet_core.cpp:602-611, 663-684), mapped onto the two
+// engines, which give the same counts: csc -> the degree-oriented engine;
+// adj and hybrid, where the reference builds its adjacency matrix -> the
+// dense tier (the oriented engine above PGD_DENSE_MAX_N vertices, which the
+// tier does not hold); auto (or none) -> the library's choice, by speed.
+void graphlet_core::preprocess(std::string neigh, bool s2l, int /*adj_limit*/, std::string rep) {
     check(pgd_preprocess(g_, neigh.c_str(), s2l ? 1 : 0), "preprocess");
+    if (rep == "csc") engine_ = PGD_ENGINE_ORIENT;
+    else if (rep == "adj" || rep == "hybrid") engine_ = n_ <= PGD_DENSE_MAX_N ? PGD_ENGINE_DENSE : PGD_ENGINE_ORIENT;
+    else engine_ = 0;
     fetch_edges();
 }
 
@@ -213,7 +222,7 @@ void graphlet_core::decompose(bool micro) {
     if (E_ordered.empty()) sort_edges();  // graphlet_core.cpp:1376-1378
     check(pgd_set_e_ordered(g_, E_ordered.data(), (int64_t)E_ordered.size()), "E_ordered");
     uint64_t sums[PGD_NSUMS], tot[PGD_NTOTALS];
-    check(pgd_decompose(g_, micro ? PGD_MICRO : 0u, sums), "graphlet_decomposition");
+    check(pgd_decompose(g_, (micro ? PGD_MICRO : 0u) | engine_, sums), "graphlet_decomposition");
     check(pgd_derive(n_, m_, sums, tot), "pgd_derive");
     set_totals(tot);
     double total = 0;

@@ -99,6 +99,7 @@ private:
     pgd_graph* g_ = nullptr;
     void append_order();
     int64_t n_ = 0, m_ = 0;
+    uint32_t engine_ = 0;  // PGD_ENGINE_* from the representation (-r), 0: the library's choice
 };
 
 // pgd_ctx shared by every graphlet_core of the process (device 0)

@@ -27,7 +27,7 @@
 namespace pgd {
 namespace {
 
-constexpr int kDenseMaxN = 8192;  // int32 products of N^2 entries, bit rows of N/32 words
+constexpr int kDenseMaxN = PGD_DENSE_MAX_N;  // int32 products of N^2 entries, bit rows of N/32 words
 constexpr int kTile = 128;        // output tile of one GEMM block (2 x 2 waves of 64 x 64); N is a multiple
 
 typedef int v16i __attribute__((ext_vector_type(16)));

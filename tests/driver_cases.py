@@ -10,6 +10,10 @@ CASES = [
     ("DD21.mtx", ["--s2l", "-n", "kcore"]), ("sample_graph.csv", []), ("rd_csv_header.csv", []),
     ("rd_weighted.txt", []), ("rd_triple_space.txt", []), ("rd_gap_ids.mtx", []), ("rd_tab.edges", []),
     ("C250-9.mtx", []), ("gnp_2000_0.01", ["--s2l_neigh"]), ("ba_3000_4", ["-o", "natural"]),
+    # the representation: csc -> the degree-oriented engine, adj and hybrid ->
+    # the dense tier, both with the reference's counts and files
+    ("DD21.mtx", ["-r", "csc"]), ("C250-9.mtx", ["-r", "adj"]), ("gnp_2000_0.01", ["-r", "hybrid"]),
+    ("ba_3000_4", ["-r", "csc"]),
 ]
 
 
