@@ -415,11 +415,18 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
             // accumulator r of (a, b) is row wm 64 + a 32 + k with
             // k = (r mod 4) + 8 (r / 4) + 4 (lane / 32), column cl: its bit is
             // bit k of row cl's 32-bit window at rbase + wm 64 + a 32
+            // (sums within one column tile stay below 128 x 8192 < 2^24, so
+            // they are exact in f32; each is converted once)
+            float tile[2][16];
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) tile[a][r] = 0.f;
 #pragma unroll
             for (int b = 0; b < 2; b++) {
                 const int cl = ct + wn * 64 + b * 32 + (lane & 31);
                 const uint32_t* gr = gv + (int64_t)min(max(cl, 0), dp - 1) * stride;
-                int32_t col = 0;
+                float col = 0.f;
 #pragma unroll
                 for (int a = 0; a < 2; a++) {
                     const int o = rbase + wm * 64 + a * 32;
@@ -428,16 +435,21 @@ __global__ __launch_bounds__(256) void k_dense_k4m(const uint32_t* __restrict__ 
 #pragma unroll
                     for (int r = 0; r < 16; r++) {
                         const int k = (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5);
-                        const int32_t kept = ((win >> k) & 1u) ? (int32_t)acc[a][b][r] : 0;
-                        part[a][r] += kept;
+                        const float kept = ((win >> k) & 1u) ? acc[a][b][r] : 0.f;
+                        tile[a][r] += kept;
                         col += kept;
                     }
                 }
                 if (c > rt) {  // the column sums of a tile above the diagonal
-                    col += __shfl_xor(col, 32, 64);
-                    if (lane < 32 && cl < d) atomicAdd(&K4M[(int64_t)inf.v * N + Lg[slot * dpad + cl]], col);
+                    int32_t ci = (int32_t)col;
+                    ci += __shfl_xor(ci, 32, 64);
+                    if (lane < 32 && cl < d) atomicAdd(&K4M[(int64_t)inf.v * N + Lg[slot * dpad + cl]], ci);
                 }
             }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) part[a][r] += (int32_t)tile[a][r];
         }
         // each row's sum over the 32 lanes of its half wavefront, then over the
         // two wavefronts that hold its two column halves
