@@ -475,6 +475,55 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
         // starts are distinct)
         const int32_t es = lane < kc ? x - plen : INT32_MAX;  // exclusive start of segment `lane`
         const uint64_t upto = (2ULL << lane) - 1;             // lanes 0..lane
+        if (pass == 0 && mode == 0) {
+            // the count pass takes two rounds per step: the owners of both
+            // are looked up first, so each lane has 2 kU endpoint loads in
+            // flight before the counter adds that need them
+            constexpr int kB = 2 * kU;
+            for (int32_t rb0 = w * 64 * kB; rb0 < total; rb0 += NW * 64 * kB) {
+                bool ok[kB];
+                int64_t slot[kB];
+                int32_t c[kB];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int32_t r0 = rb0 + h * 64 * kU;
+                    wflag[lane] = 0;
+                    __builtin_amdgcn_wave_barrier();
+                    if (es >= r0 && es < r0 + 64 * kU) ((uint8_t*)wflag)[((es - r0) & 63) * 4 + ((es - r0) >> 6)] = 1;
+                    __builtin_amdgcn_wave_barrier();
+                    const uint32_t f4 = wflag[lane];
+                    int cum = __popcll(__ballot(es < r0)) - 1;
+#pragma unroll
+                    for (int u = 0; u < kU; u++) {
+                        const uint64_t st4 = __ballot(((f4 >> (8 * u)) & 0xffu) != 0);
+                        const int32_t q = r0 + 64 * u + lane;
+                        const int o = cum + __popcll(st4 & upto);
+                        cum += __popcll(st4);
+                        ok[h * kU + u] = q < total;
+                        slot[h * kU + u] = gbase[o] + (q < total ? q : total - 1);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+#pragma unroll
+                for (int u = 0; u < kB; u++) c[u] = col[slot[u]] - c0;
+                // the counts before the add tell whether an endpoint repeats
+                // (a tile without repeats closes no 4-cycle: its accumulate
+                // pass is skipped) and, with chk, whether a count would pass
+                // 65535
+                uint32_t old[kB];
+#pragma unroll
+                for (int u = 0; u < kB; u++)
+                    old[u] = ok[u] ? atomicAdd(&cnt[c[u] & hm], 1u << ((c[u] >> hb) * 16)) : 0u;
+                bool hit = false;
+#pragma unroll
+                for (int u = 0; u < kB; u++) {
+                    const uint32_t h = (old[u] >> ((c[u] >> hb) * 16)) & 0xffffu;
+                    hit |= h == 0xffffu;
+                    again |= h != 0;
+                }
+                if (chk && hit) *ovf = 1;
+            }
+        } else
         for (int32_t rb0 = w * 64 * kU; rb0 < total; rb0 += NW * 64 * kU) {
             int o[kU];
             bool ok[kU];
@@ -487,18 +536,21 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
             if (es >= rb0 && es < rb0 + 64 * kU) ((uint8_t*)wflag)[((es - rb0) & 63) * 4 + ((es - rb0) >> 6)] = 1;
             __builtin_amdgcn_wave_barrier();
             const uint32_t f4 = wflag[lane];
-            uint32_t fl[kU];
-#pragma unroll
-            for (int u = 0; u < kU; u++) fl[u] = (f4 >> (8 * u)) & 0xffu;
-            int cum = __popcll(__ballot(es < rb0)) - 1;  // last segment starting before this round
+            const int cum0 = __popcll(__ballot(es < rb0)) - 1;  // last segment starting before this round
+            int cum = cum0;
             uint64_t starts[kU];
+            int cend[kU];  // the owner of the slice's last lane
 #pragma unroll
             for (int u = 0; u < kU; u++) {
-                starts[u] = __ballot(fl[u] != 0);
+                const uint32_t own = ((f4 >> (8 * u)) & 0xffu) != 0;
+                starts[u] = __ballot(own);
                 const int32_t q = rb0 + 64 * u + lane;
-                // past the end: the owner is the last segment; clamp to its last position
-                o[u] = cum + __popcll(starts[u] & upto);
+                // past the end: the owner is the last segment; clamp to its
+                // last position.  Owner = starts at lanes below, plus this one.
+                o[u] = cum + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(starts[u] >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)starts[u], own));
                 cum += __popcll(starts[u]);
+                cend[u] = cum;
                 if (pass) {  // padding positions: clamped to the segment's slots, no update
                     const int2 bd = gbnd[o[u]];
                     ok[u] = q >= bd.x && q < bd.y;
@@ -510,47 +562,60 @@ __device__ __forceinline__ void tile_pass(int pass, int mode, bool chk, uint32_t
             }
 #pragma unroll
             for (int u = 0; u < kU; u++) c[u] = col[slot[u]] - c0;
-            if (mode == 0) {
-                if (pass == 0) {
-                    // the counts before the add tell whether an endpoint repeats
-                    // (a tile without repeats closes no 4-cycle: its
-                    // accumulate pass is skipped) and, with chk, whether a
-                    // count would pass 65535
-                    uint32_t old[kU];
-#pragma unroll
-                    for (int u = 0; u < kU; u++)
-                        old[u] = ok[u] ? atomicAdd(&cnt[c[u] & hm], 1u << ((c[u] >> hb) * 16)) : 0u;
-                    bool hit = false;
-#pragma unroll
-                    for (int u = 0; u < kU; u++) {
-                        const uint32_t h = (old[u] >> ((c[u] >> hb) * 16)) & 0xffffu;
-                        hit |= h == 0xffffu;
-                        again |= h != 0;
-                    }
-                    if (chk && hit) *ovf = 1;
-                    continue;
-                }
+            if (mode == 0) {  // (the count pass of mode 0 is the two-round loop above)
                 uint32_t v[kU];
 #pragma unroll
-                for (int u = 0; u < kU; u++) v[u] = ((cnt[c[u] & hm] >> ((c[u] >> hb) * 16)) & 0xffffu) - 1;
+                for (int u = 0; u < kU; u++) {
+                    v[u] = ((cnt[c[u] & hm] >> ((c[u] >> hb) * 16)) & 0xffffu) - 1;
+                    v[u] = ok[u] ? v[u] : 0u;
+                }
+                if (wmask == 0) {  // no wide list in the group: 32-bit sums only
 #pragma unroll
-                for (int u = 0; u < kU; u++)
-                    if (ok[u] && v[u]) slot_add(PS, PS32, wmask, o[u], slot[u], v[u]);
-                // the edge (c, b) of each segment: the lanes of one segment are
-                // a run (delimited by segment starts), summed by a wavefront
-                // scan and added once by the run's last lane (v < 2^16, so a
-                // step's sums fit 32 bits)
+                    for (int u = 0; u < kU; u++)
+                        if (v[u]) atomicAdd(&PS32[slot[u]], v[u]);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < kU; u++)
+                        if (v[u]) slot_add(PS, PS32, wmask, o[u], slot[u], v[u]);
+                }
+                // the edge (c, b) of each segment.  The lanes of one segment
+                // form runs delimited by segment starts.  A slice without a
+                // start continues the open run (its owner oo); a slice with
+                // starts closes it, adds the runs between its first and last
+                // start (a wavefront scan; the last lane of each run adds),
+                // and opens the run of its last start.  v < 2^16, so the sums
+                // of a round fit 32 bits.
+                uint32_t open = 0;
+                int oo = cum0;
+                auto flush = [&](uint32_t x, int owner) {
+                    const uint32_t t = __builtin_amdgcn_readlane(wave_incl_sum(x), 63);
+                    if (lane == 0 && t) atomicAdd(&gpsum[owner], (unsigned long long)t);
+                };
 #pragma unroll
                 for (int u = 0; u < kU; u++) {
-                    const uint32_t val = ok[u] ? v[u] : 0u;
-                    const uint32_t P = wave_incl_sum(val);
-                    const uint64_t sm = starts[u] & upto;
-                    const int rs = sm ? 63 - __clzll((long long)sm) : 0;  // run start (lane)
-                    const uint32_t before = __shfl(P, rs > 0 ? rs - 1 : 0, 64);
-                    const uint32_t run = P - (rs > 0 ? before : 0u);
-                    const bool last = lane == 63 || ((starts[u] >> (lane + 1)) & 1);
-                    if (last && run) atomicAdd(&gpsum[o[u]], (unsigned long long)run);
+                    const uint64_t sm = starts[u];
+                    if (sm == 0) {
+                        open += v[u];
+                        continue;
+                    }
+                    const int f = (int)__builtin_ctzll(sm);       // first start
+                    const int l = 63 - (int)__builtin_clzll(sm);  // last start
+                    if (lane < f) open += v[u];
+                    if (oo >= 0) flush(open, oo);
+                    if (l > f) {
+                        const uint32_t val = lane >= f && lane < l ? v[u] : 0u;
+                        const uint32_t P = wave_incl_sum(val);
+                        const uint64_t below = sm & upto;
+                        const int rs = 63 - (int)__builtin_clzll(below | 1ull);  // run start (lane; >= f here)
+                        const uint32_t before = __shfl(P, rs > 0 ? rs - 1 : 0, 64);
+                        const uint32_t run = P - (rs > 0 ? before : 0u);
+                        const bool last = lane < l && ((sm >> (lane + 1)) & 1);
+                        if (last && run) atomicAdd(&gpsum[o[u]], (unsigned long long)run);
+                    }
+                    open = lane >= l ? v[u] : 0u;
+                    oo = cend[u];
                 }
+                if (oo >= 0) flush(open, oo);
             } else {
                 bool in[kU];
                 int32_t ci[kU];
@@ -720,10 +785,16 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     extern __shared__ __align__(16) uint32_t cnt[];
     __shared__ uint32_t base[kTmax + 1];
     __shared__ uint32_t cur[kTmax];
-    __shared__ int64_t wbase[NW][64];
-    __shared__ int32_t wslot[NW][64];
     __shared__ uint32_t wflag[NW][64];  // segment-start flags of one round of a wavefront (4 bytes per lane)
-    __shared__ unsigned long long wpsum[NW][64];
+    // a wavefront's small-tile segment bases, slots and sums.  With 1024
+    // threads they live in the counter array beside the small-tile tables
+    // (the block passes clear it before use), so that two blocks of 16
+    // wavefronts fit a CU's LDS with 2^15-rank tiles.
+    constexpr bool kMetaInCnt = NT == 1024;
+    constexpr int kMetaWords = 64 * 2 + 64 + 64 * 2;
+    __shared__ int64_t wbase_s[kMetaInCnt ? 1 : NW][64];
+    __shared__ int32_t wslot_s[kMetaInCnt ? 1 : NW][64];
+    __shared__ unsigned long long wpsum_s[kMetaInCnt ? 1 : NW][64];
     __shared__ int64_t gbase[64];  // the block passes' current group of segments
     __shared__ int2 gbnd[64];
     __shared__ int32_t gslot[64];
@@ -733,8 +804,24 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int R = 1 << log2r;
     // a wavefront's small-tile table: 2^tbits words of the counter array's R/2
+    // (followed by the wavefronts' small-tile segment data when it is there)
+    const int meta = kMetaInCnt ? kMetaWords : 0;
     int tbits = 9;
-    while (tbits > 2 && (NW << tbits) > R / 2) tbits--;
+    while (tbits > 2 && NW * ((1 << tbits) + meta) > R / 2) tbits--;
+    const bool small_fits = NW * ((1 << tbits) + meta) <= R / 2;
+    int64_t* wbase;
+    int32_t* wslot;
+    unsigned long long* wpsum;
+    if constexpr (kMetaInCnt) {
+        uint32_t* m = cnt + (NW << tbits) + w * kMetaWords;  // 8-byte aligned: both terms are even
+        wbase = (int64_t*)m;
+        wslot = (int32_t*)(m + 128);
+        wpsum = (unsigned long long*)(m + 192);
+    } else {
+        wbase = wbase_s[w];
+        wslot = wslot_s[w];
+        wpsum = wpsum_s[w];
+    }
     unsigned long long done[2] = {0, 0};  // wedges counted, wedges accumulated
     WedgeSeg* seg = segs + (size_t)blockIdx.x * (size_t)segcap;
     for (;;) {
@@ -791,10 +878,10 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
         // the counter array; cur[t] then marks the tiles left for the block
         for (int t = w; t < T; t += NW) {
             const uint32_t sb = base[t], se = base[t + 1];
-            uint32_t left = se - sb > 64u;
+            uint32_t left = se - sb > 64u || !small_fits;
             if (!left && sb < se)
                 left = !small_tile<4>(sb, se, (W.t0 + t) << log2r, seg, col, cnt + w * (1 << tbits), tbits, PS,
-                                      PS32, wbase[w], wslot[w], wpsum[w], wflag[w], lane, done);
+                                      PS32, wbase, wslot, wpsum, wflag[w], lane, done);
             if (lane == 0) cur[t] = left;
         }
         __syncthreads();
