@@ -280,6 +280,70 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
     }
 }
 
+// the segments of every (anchor, tile) cell, written from the cell's offset
+// in the plan's segment array (the walk of k_plan_cells again; within a cell
+// the order is that of the adds, on which no sum depends).  A segment is the
+// part of N(b)'s suffix in one tile: its first CSR position, its length (top
+// bit: N(b) is a wide list) and the slot of the pair (anchor, b).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_plan_fill(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
+                                                  const int32_t* __restrict__ heavy, int64_t nheavy,
+                                                  const int32_t* __restrict__ sfx, const uint64_t* __restrict__ tt,
+                                                  const int64_t* __restrict__ ttrow, const int32_t* __restrict__ hbase,
+                                                  const int64_t* __restrict__ htoff, const int64_t* __restrict__ cseg,
+                                                  uint32_t* __restrict__ gcur, int32_t wide_deg,
+                                                  WedgeSeg* __restrict__ gseg) {
+    constexpr int NW = NT / 64;
+    __shared__ uint32_t lcur[kPlanLds];
+    __shared__ int64_t pj[NW][64], pe[NW][64];
+    __shared__ uint32_t pw[NW][64], pd[NW][64];
+    __shared__ uint32_t flag[NW][64];
+    __shared__ int64_t prow[NW][64];   // row[b] of the wavefront's pairs, by rank
+    __shared__ int32_t pslot[NW][64];  // their slots in the anchor's list
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
+        const int32_t c = heavy[h];
+        const int64_t c0 = htoff[h], T = htoff[h + 1] - c0;
+        const uint32_t base = (uint32_t)hbase[h];
+        const bool in_lds = T <= kPlanLds;
+        if (in_lds)
+            for (int i = threadIdx.x; i < T; i += NT) lcur[i] = 0;
+        __syncthreads();
+        const int64_t ib = row[c], ie = row[c + 1];
+        for (int64_t g0 = ib + (int64_t)wv * 64; g0 < ie; g0 += NT) {
+            const int64_t s = g0 + lane;
+            int64_t jb = 0, je = 0, tend = 0, rb = 0;
+            uint32_t q = 0, deg = 0;
+            if (s < ie) {
+                const int32_t b = col[s];
+                q = (uint32_t)sfx[s];
+                rb = row[b];
+                deg = (uint32_t)(row[b + 1] - rb);
+                if (q < deg) {
+                    tend = ttrow[b + 1];
+                    jb = tt_seek_off(tt, ttrow[b], tend, q + 1) - 1;  // the entry holding q
+                    je = tend;
+                }
+            }
+            // the pairs' ranks as walk_entries numbers them: lanes with entries
+            const uint64_t nz = __ballot(je > jb);
+            if (je > jb) {
+                const int r = __popcll(nz & ((1ULL << lane) - 1));
+                prow[wv][r] = rb;
+                pslot[wv][r] = (int32_t)s;
+            }
+            walk_entries(jb, je, tend, deg, q, pj[wv], pe[wv], pw[wv], pd[wv], flag[wv], tt, lane,
+                         [&](int pr, uint32_t tile, uint32_t off, uint32_t len) {
+                             const uint32_t t = tile - base;
+                             const uint32_t k = in_lds ? atomicAdd(&lcur[t], 1u) : atomicAdd(&gcur[c0 + t], 1u);
+                             const uint32_t lw = len | (pd[wv][pr] >= (uint32_t)wide_deg ? 0x80000000u : 0u);
+                             gseg[cseg[c0 + t] + k] = WedgeSeg{(uint32_t)(prow[wv][pr] + off), (int32_t)lw, pslot[wv][pr]};
+                         });
+        }
+        __syncthreads();
+    }
+}
+
 // cut each anchor's tiles into items: at most segcap segments, kTmax tiles
 // and (unless a single tile exceeds it) wcap wedges; empty items are dropped.
 // With out == nullptr only the number of items per anchor is written, and
@@ -775,11 +839,10 @@ __device__ __forceinline__ void clear_lds(uint32_t* p, int n) {
 template <int NT>
 __global__ __launch_bounds__(NT) void k_wedge_tiled(
     const int64_t* __restrict__ row, const int32_t* __restrict__ col,
-    const int32_t* __restrict__ heavy, const int32_t* __restrict__ sfx, const uint64_t* __restrict__ tt,
-    const int64_t* __restrict__ ttrow, const int32_t* __restrict__ hbase, const int64_t* __restrict__ htoff,
-    const uint32_t* __restrict__ htseg, const WedgeItem* __restrict__ items, int64_t nitems, int log2r,
-    WedgeSeg* __restrict__ segs, int64_t segcap, uint32_t* __restrict__ qctr, uint64_t* __restrict__ PS,
-    uint32_t* __restrict__ PS32, int32_t wide_deg, unsigned long long* __restrict__ work) {
+    const int32_t* __restrict__ heavy, const int32_t* __restrict__ hbase, const int64_t* __restrict__ htoff,
+    const int64_t* __restrict__ cseg, const WedgeSeg* __restrict__ gseg, const WedgeItem* __restrict__ items,
+    int64_t nitems, int log2r, uint32_t* __restrict__ qctr, uint64_t* __restrict__ PS,
+    uint32_t* __restrict__ PS32, unsigned long long* __restrict__ work) {
     constexpr int NW = NT / 64;
     // 2^log2r 16-bit endpoint counters in 2^(log2r-1) words (c and c + 2^(log2r-1) share one)
     extern __shared__ __align__(16) uint32_t cnt[];
@@ -823,7 +886,6 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
         wpsum = wpsum_s[w];
     }
     unsigned long long done[2] = {0, 0};  // wedges counted, wedges accumulated
-    WedgeSeg* seg = segs + (size_t)blockIdx.x * (size_t)segcap;
     for (;;) {
         if (tid == 0) item_sh = (int64_t)atomicAdd(qctr, 1u);
         __syncthreads();
@@ -834,38 +896,12 @@ __global__ __launch_bounds__(NT) void k_wedge_tiled(
         const int32_t a = heavy[W.hub];  // the anchor
         const int64_t ib = row[a], ie = row[a + 1];
         const int T = W.t1 - W.t0;
-        // bucket the anchor's segments in [t0, t1) by tile: offsets from the plan's
-        // per-cell counts, then one walk scatters
-        if (tid == 0) {
-            const uint32_t* cells = htseg + htoff[W.hub] + (W.t0 - hbase[W.hub]);
-            uint32_t acc = 0;
-            for (int i = 0; i < T; i++) {
-                base[i] = acc;
-                cur[i] = acc;
-                acc += cells[i];
-            }
-            base[T] = acc;
-        }
-        __syncthreads();
-        for (int64_t s = ib + tid; s < ie; s += NT) {
-            const int32_t b = col[s];
-            const uint32_t q = (uint32_t)sfx[s];  // the suffix of N(b) above max(b, a)
-            const uint32_t deg = (uint32_t)(row[b + 1] - row[b]);
-            if (q >= deg) continue;
-            const int64_t j1 = ttrow[b + 1];
-            int64_t j = tt_seek_off(tt, ttrow[b], j1, q + 1) - 1;  // the entry holding q
-            if ((int32_t)(tt[j] >> 32) < W.t0) j = tt_seek(tt, j, j1, (uint32_t)W.t0);
-            for (; j < j1; j++) {
-                const uint64_t e = tt[j];
-                const uint32_t off = max((uint32_t)e, q);
-                const int t = (int)(e >> 32) - W.t0;
-                if (t >= T) break;
-                const uint32_t nxt = j + 1 < j1 ? (uint32_t)tt[j + 1] : deg;
-                const uint32_t p = atomicAdd(&cur[t], 1u);
-                const uint32_t lw = (nxt - off) | (deg >= (uint32_t)wide_deg ? 0x80000000u : 0u);
-                seg[p] = WedgeSeg{(uint32_t)(row[b] + off), (int32_t)lw, (int32_t)s};
-            }
-        }
+        // the item's segments: cells [first, first + T) of the plan's segment
+        // array, in tile order
+        const int64_t first = htoff[W.hub] + (W.t0 - hbase[W.hub]);
+        const int64_t s0 = cseg[first];
+        for (int i = tid; i <= T; i += NT) base[i] = (uint32_t)(cseg[first + i] - s0);
+        const WedgeSeg* seg = gseg + s0;
         __syncthreads();
         // tile by tile: count the endpoints, then accumulate cnt-1 per wedge.
         // Counters are 16-bit, two per word (mode 0); a count that would pass
@@ -932,7 +968,7 @@ __global__ void k_ps_fold(const uint64_t* __restrict__ PS, const uint32_t* __res
     }
 }
 
-// Exclusive prefix sums of n int64 values without LDS, for the plan's scans
+// Exclusive prefix sums of n integer values (into int64) without LDS, for the plan's scans
 // that run beside the local passes (whose blocks hold the LDS, so an
 // LDS-using scan waits for them): one wavefront per chunk of kScanChunk
 // values sums it, one wavefront scans the chunk sums, and one wavefront per
@@ -946,7 +982,8 @@ __device__ __forceinline__ long long wave_incl_sum64(long long x, int lane) {
     }
     return x;
 }
-__global__ void k_scan_parts(const int64_t* __restrict__ in, int64_t n, int64_t* __restrict__ part) {
+template <class T>
+__global__ void k_scan_parts(const T* __restrict__ in, int64_t n, int64_t* __restrict__ part) {
     const int lane = threadIdx.x;  // one wavefront per block, block b = chunk b
     const int64_t c0 = (int64_t)blockIdx.x * kScanChunk, c1 = min(n, c0 + kScanChunk);
     long long x = 0;
@@ -966,7 +1003,8 @@ __global__ void k_scan_tops(int64_t* __restrict__ part, int64_t np) {  // one wa
         carry += __shfl(x, 63, 64);
     }
 }
-__global__ void k_scan_apply(const int64_t* __restrict__ in, int64_t n, const int64_t* __restrict__ part,
+template <class T>
+__global__ void k_scan_apply(const T* __restrict__ in, int64_t n, const int64_t* __restrict__ part,
                              int64_t* __restrict__ out) {
     const int lane = threadIdx.x;
     const int64_t c0 = (int64_t)blockIdx.x * kScanChunk, c1 = min(n, c0 + kScanChunk);
@@ -979,16 +1017,17 @@ __global__ void k_scan_apply(const int64_t* __restrict__ in, int64_t n, const in
         carry += __shfl(x, 63, 64);
     }
 }
-void excl_sum_nolds(pgd_ctx* ctx, const int64_t* in, int64_t* out, int64_t n) {
+template <class T>
+void excl_sum_nolds(pgd_ctx* ctx, const T* in, int64_t* out, int64_t n) {
     if (n <= 0) return;
     hipStream_t s = ctx->stream;
     const int64_t np = (n + kScanChunk - 1) / kScanChunk;
     PGD_CHECK(np <= INT32_MAX, PGD_ERR_UNSUPPORTED, "scan too large");
     DBuf<int64_t> part;
     part.alloc(ctx, np);
-    hipLaunchKernelGGL(k_scan_parts, dim3((unsigned)np), dim3(64), 0, s, in, n, part.get());
+    hipLaunchKernelGGL(k_scan_parts<T>, dim3((unsigned)np), dim3(64), 0, s, in, n, part.get());
     hipLaunchKernelGGL(k_scan_tops, dim3(1), dim3(64), 0, s, part.get(), np);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)np), dim3(64), 0, s, in, n, part.get(), out);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)np), dim3(64), 0, s, in, n, part.get(), out);
     PGD_HIP(hipGetLastError());
 }
 
@@ -997,9 +1036,8 @@ void launch_tiled(pgd_ctx* ctx, OrientState& st, unsigned long long* work) {
     WedgePlan& wp = st.wp;
     const size_t dyn = (size_t)2 << wp.log2r;
     hipLaunchKernelGGL(k_wedge_tiled<NT>, dim3(wp.grid), dim3(NT), dyn, ctx->stream, st.rrow.get(), st.rcol.get(),
-                       st.b_heavy.get(), wp.sfx.get(), wp.tt.get(), wp.ttrow.get(), wp.hbase.get(),
-                       wp.htoff.get(), wp.htseg.get(), wp.items.get(), wp.nitems, wp.log2r, wp.segs.get(), wp.segcap,
-                       wp.qctr.get(), wp.ps.get(), wp.ps32.get(), wp.wide_deg, work);
+                       st.b_heavy.get(), wp.hbase.get(), wp.htoff.get(), wp.cseg.get(), wp.seg.get(),
+                       wp.items.get(), wp.nitems, wp.log2r, wp.qctr.get(), wp.ps.get(), wp.ps32.get(), work);
 }
 
 template <int NT>
@@ -1075,11 +1113,13 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         PGD_HIP(hipMemcpyAsync(&ncell, wp.htoff.get() + nh, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         PGD_HIP(hipStreamSynchronize(s));
     }
-    ensure(wp.htseg, ctx, ncell);
+    // (one count past the last cell stays 0: the scan of the counts then ends
+    // with their total)
+    ensure(wp.htseg, ctx, ncell + 1);
     ensure(wp.htw, ctx, ncell);
     {
         const int g = grid_cap(ctx, (ncell + 255) / 256, 16);
-        hipLaunchKernelGGL(w_fill_u32, dim3(g), dim3(256), 0, s, wp.htseg.get(), ncell);
+        hipLaunchKernelGGL(w_fill_u32, dim3(g), dim3(256), 0, s, wp.htseg.get(), ncell + 1);
         hipLaunchKernelGGL(w_fill_u64, dim3(g), dim3(256), 0, s, wp.htw.get(), ncell);
         ensure(wp.pacc, ctx, 3);
         hipLaunchKernelGGL(w_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)wp.pacc.get(), (int64_t)3);
@@ -1092,6 +1132,25 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     PGD_HIP(hipStreamSynchronize(s));
     const uint64_t segtot = ((volatile unsigned long long*)ctx->pin)[0];
     wp.wedges = (int64_t)((volatile unsigned long long*)ctx->pin)[1];
+    // smallest degree d with d * (max degree) >= 2^32
+    wp.wide_deg = st.max_deg > 0
+                      ? (int32_t)std::min<int64_t>(INT32_MAX, (((int64_t)1 << 32) + st.max_deg - 1) / st.max_deg)
+                      : INT32_MAX;
+    // every cell's segments, in cell order: offsets from a scan of the
+    // counts, then the walk again to write them
+    PGD_CHECK(segtot < ((uint64_t)1 << 40), PGD_ERR_UNSUPPORTED, "too many 4-cycle segments");
+    ensure(wp.cseg, ctx, ncell + 1);
+    excl_sum_nolds(ctx, wp.htseg.get(), wp.cseg.get(), ncell + 1);
+    ensure(wp.seg, ctx, std::max<uint64_t>(segtot, 1));
+    ensure(wp.gcur, ctx, std::max<int64_t>(ncell, 1));
+    {
+        const int g = grid_cap(ctx, (ncell + 255) / 256, 16);
+        hipLaunchKernelGGL(w_fill_u32, dim3(g), dim3(256), 0, s, wp.gcur.get(), ncell);
+        hipLaunchKernelGGL(k_plan_fill<256>, dim3(grid_cap(ctx, nh, 8)), dim3(256), 0, s, st.rrow.get(),
+                           st.rcol.get(), st.b_heavy.get(), nh, wp.sfx.get(), wp.tt.get(), wp.ttrow.get(),
+                           wp.hbase.get(), wp.htoff.get(), wp.cseg.get(), wp.gcur.get(), wp.wide_deg, wp.seg.get());
+        PGD_HIP(hipGetLastError());
+    }
     // a cell holds at most one segment per neighbour of its anchor: the
     // maximum degree bounds it (read from the cells when that bound is large)
     const uint32_t maxcell = st.max_deg <= (1 << 20) ? (uint32_t)st.max_deg
@@ -1103,8 +1162,8 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     else per_cu = tiled_blocks_per_cu<1024>(ctx, wp.log2r);
     PGD_CHECK(per_cu > 0, PGD_ERR_INVALID, "wedge tile and block size exceed the LDS of a CU");
     wp.grid = ctx->num_cus * per_cu;
-    // per-block segment buffer: the largest cell, or an even share of all
-    // segments (items are cut to fit it), at most 2^18
+    // segments per item: the largest cell, or twice an even share of all
+    // segments, at most 2^18 (items are cut to fit it)
     wp.segcap = (int64_t)std::max<uint64_t>(
         maxcell, std::min<uint64_t>((uint64_t)1 << 18, 2 * (segtot + (uint64_t)wp.grid - 1) / (uint64_t)wp.grid));
     wp.segcap = std::max<int64_t>(wp.segcap, 1);
@@ -1146,14 +1205,9 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
             PGD_HIP(hipGetLastError());
         }
     }
-    ensure(wp.segs, ctx, (size_t)wp.grid * (size_t)wp.segcap);
     ensure(wp.ps, ctx, nnz);
     ensure(wp.ps32, ctx, nnz);
     ensure(wp.qctr, ctx, 1);
-    // smallest degree d with d * (max degree) >= 2^32
-    wp.wide_deg = st.max_deg > 0
-                      ? (int32_t)std::min<int64_t>(INT32_MAX, (((int64_t)1 << 32) + st.max_deg - 1) / st.max_deg)
-                      : INT32_MAX;
 }
 
 // the heavy 4-cycle pass (capturable: fills, the persistent kernel, the fold)

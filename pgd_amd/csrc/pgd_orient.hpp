@@ -262,17 +262,19 @@ struct WedgeItem {
 struct WedgePlan {
     int log2r = 15;                 // endpoint tile = 2^log2r consecutive ranks (LDS counters)
     int tmax = 128;                 // tiles per item
-    int64_t nitems = 0, segcap = 0, wedges = 0;
+    int64_t nitems = 0, segcap = 0, wedges = 0;  // segcap: segments per item (unless one cell has more)
     int grid = 0;
     DBuf<int32_t> sfx;              // per slot (c, b) of a heavy anchor: offset in N(b) of the first endpoint above max(b, c)
     DBuf<uint64_t> tt;              // tile table entries: (tile << 32) | offset in N(b)
     DBuf<int64_t> ttrow;            // first tile-table entry of each vertex (n+1)
     DBuf<int32_t> hbase;            // first endpoint tile of each heavy anchor
     DBuf<int64_t> htoff;            // first (anchor, tile) cell of each heavy anchor (+1)
-    DBuf<uint32_t> htseg;           // segments per (anchor, tile) cell
+    DBuf<uint32_t> htseg;           // segments per (anchor, tile) cell (+1, zero)
+    DBuf<int64_t> cseg;             // first segment of each cell in seg (+1: the total)
+    DBuf<WedgeSeg> seg;             // every cell's segments, in cell order
+    DBuf<uint32_t> gcur;            // per-cell cursors of the plan's segment walk (cells outside LDS)
     DBuf<uint64_t> htw;             // wedges per (anchor, tile) cell
     DBuf<WedgeItem> items;          // work queue, largest first
-    DBuf<WedgeSeg> segs;            // per-block segment buffers, segcap each
     DBuf<uint64_t> ps;              // per-slot P accumulator (ascending CSR slots), 64-bit sums
     DBuf<uint32_t> ps32;            // per-slot P accumulator of the lists that are not wide
     int32_t wide_deg = 0;           // lists of at least this degree are wide
