@@ -214,7 +214,7 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
                                                    const int64_t* __restrict__ ttrow, const int32_t* __restrict__ hbase,
                                                    const int64_t* __restrict__ htoff, uint32_t* __restrict__ htseg,
                                                    uint64_t* __restrict__ htw, unsigned long long* __restrict__ acc,
-                                                   unsigned long long* hout) {
+                                                   unsigned long long* hout, int32_t* __restrict__ pjb) {
     constexpr int NW = NT / 64;
     unsigned long long mseg = 0, mw = 0;  // this thread's segments and wedges
     __shared__ uint32_t lseg[kPlanLds];
@@ -244,6 +244,7 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
                     tend = ttrow[b + 1];
                     jb = tt_seek_off(tt, ttrow[b], tend, q + 1) - 1;  // the entry holding q
                     je = tend;
+                    pjb[s] = (int32_t)jb;  // for the segment walk (k_plan_fill)
                 }
             }
             walk_entries(jb, je, tend, deg, q, pj[wv], pe[wv], pw[wv], pd[wv], flag[wv], tt, lane,
@@ -281,15 +282,17 @@ __global__ __launch_bounds__(NT) void k_plan_cells(const int64_t* __restrict__ r
 }
 
 // the segments of every (anchor, tile) cell, written from the cell's offset
-// in the plan's segment array (the walk of k_plan_cells again; within a cell
-// the order is that of the adds, on which no sum depends).  A segment is the
-// part of N(b)'s suffix in one tile: its first CSR position, its length (top
-// bit: N(b) is a wide list) and the slot of the pair (anchor, b).
+// in the plan's segment array (the walk of k_plan_cells again, from the tile
+// entries it found; within a cell the order is that of the adds, on which no
+// sum depends).  A segment is the part of N(b)'s suffix in one tile: its
+// first CSR position, its length (top bit: N(b) is a wide list) and the slot
+// of the pair (anchor, b).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_plan_fill(const int64_t* __restrict__ row, const int32_t* __restrict__ col,
                                                   const int32_t* __restrict__ heavy, int64_t nheavy,
                                                   const int32_t* __restrict__ sfx, const uint64_t* __restrict__ tt,
-                                                  const int64_t* __restrict__ ttrow, const int32_t* __restrict__ hbase,
+                                                  const int64_t* __restrict__ ttrow, const int32_t* __restrict__ pjb,
+                                                  const int32_t* __restrict__ hbase,
                                                   const int64_t* __restrict__ htoff, const int64_t* __restrict__ cseg,
                                                   uint32_t* __restrict__ gcur, int32_t wide_deg,
                                                   WedgeSeg* __restrict__ gseg) {
@@ -321,7 +324,7 @@ __global__ __launch_bounds__(NT) void k_plan_fill(const int64_t* __restrict__ ro
                 deg = (uint32_t)(row[b + 1] - rb);
                 if (q < deg) {
                     tend = ttrow[b + 1];
-                    jb = tt_seek_off(tt, ttrow[b], tend, q + 1) - 1;  // the entry holding q
+                    jb = pjb[s];  // the entry holding q, found by k_plan_cells
                     je = tend;
                 }
             }
@@ -1095,7 +1098,7 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         const int gw = grid_cap(ctx, (n * 64 + 255) / 256, 16);
         hipLaunchKernelGGL(k_tt_flags, dim3(gw), dim3(256), 0, s, st.rrow.get(), st.rcol.get(), n, wp.log2r, flag);
         PGD_HIP(hipGetLastError());
-        excl_sum(ctx, flag, pos, (size_t)nnz);
+        excl_sum_nolds(ctx, flag, pos, nnz);
         // at most one entry per slot: sized without reading the total back
         ensure(wp.tt, ctx, nnz);
         ensure(wp.ttrow, ctx, n + 1);
@@ -1109,7 +1112,7 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     {
         hipLaunchKernelGGL(w_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)nt.get() + nh, (int64_t)1);
         PGD_HIP(hipGetLastError());
-        excl_sum(ctx, nt.get(), wp.htoff.get(), (size_t)(nh + 1));
+        excl_sum_nolds(ctx, nt.get(), wp.htoff.get(), nh + 1);
         PGD_HIP(hipMemcpyAsync(&ncell, wp.htoff.get() + nh, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         PGD_HIP(hipStreamSynchronize(s));
     }
@@ -1123,12 +1126,18 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
         hipLaunchKernelGGL(w_fill_u64, dim3(g), dim3(256), 0, s, wp.htw.get(), ncell);
         ensure(wp.pacc, ctx, 3);
         hipLaunchKernelGGL(w_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)wp.pacc.get(), (int64_t)3);
+        ensure(wp.gcur, ctx, std::max<int64_t>(ncell, 1));  // cursors of the segment walk
+        hipLaunchKernelGGL(w_fill_u32, dim3(g), dim3(256), 0, s, wp.gcur.get(), ncell);
         hipLaunchKernelGGL(k_plan_cells<256>, dim3(grid_cap(ctx, nh, 8)), dim3(256), 0, s, st.rrow.get(),
                            st.rcol.get(), st.b_heavy.get(), nh, wp.sfx.get(), wp.tt.get(),
                            wp.ttrow.get(), wp.hbase.get(), wp.htoff.get(), wp.htseg.get(), wp.htw.get(),
-                           wp.pacc.get(), pinned_words(ctx));
+                           wp.pacc.get(), pinned_words(ctx), st.v0.get());
         PGD_HIP(hipGetLastError());
     }
+    // the cells' segment offsets: a scan of the counts, queued behind the
+    // count walk (so it runs on the CU slots that walk leaves)
+    ensure(wp.cseg, ctx, ncell + 1);
+    excl_sum_nolds(ctx, wp.htseg.get(), wp.cseg.get(), ncell + 1);
     PGD_HIP(hipStreamSynchronize(s));
     const uint64_t segtot = ((volatile unsigned long long*)ctx->pin)[0];
     wp.wedges = (int64_t)((volatile unsigned long long*)ctx->pin)[1];
@@ -1136,19 +1145,13 @@ void wedge_plan_build(pgd_ctx* ctx, OrientState& st) {
     wp.wide_deg = st.max_deg > 0
                       ? (int32_t)std::min<int64_t>(INT32_MAX, (((int64_t)1 << 32) + st.max_deg - 1) / st.max_deg)
                       : INT32_MAX;
-    // every cell's segments, in cell order: offsets from a scan of the
-    // counts, then the walk again to write them
-    PGD_CHECK(segtot < ((uint64_t)1 << 40), PGD_ERR_UNSUPPORTED, "too many 4-cycle segments");
-    ensure(wp.cseg, ctx, ncell + 1);
-    excl_sum_nolds(ctx, wp.htseg.get(), wp.cseg.get(), ncell + 1);
+    // every cell's segments, in cell order (the walk again)
     ensure(wp.seg, ctx, std::max<uint64_t>(segtot, 1));
-    ensure(wp.gcur, ctx, std::max<int64_t>(ncell, 1));
     {
-        const int g = grid_cap(ctx, (ncell + 255) / 256, 16);
-        hipLaunchKernelGGL(w_fill_u32, dim3(g), dim3(256), 0, s, wp.gcur.get(), ncell);
         hipLaunchKernelGGL(k_plan_fill<256>, dim3(grid_cap(ctx, nh, 8)), dim3(256), 0, s, st.rrow.get(),
                            st.rcol.get(), st.b_heavy.get(), nh, wp.sfx.get(), wp.tt.get(), wp.ttrow.get(),
-                           wp.hbase.get(), wp.htoff.get(), wp.cseg.get(), wp.gcur.get(), wp.wide_deg, wp.seg.get());
+                           st.v0.get(), wp.hbase.get(), wp.htoff.get(), wp.cseg.get(), wp.gcur.get(), wp.wide_deg,
+                           wp.seg.get());
         PGD_HIP(hipGetLastError());
     }
     // a cell holds at most one segment per neighbour of its anchor: the
