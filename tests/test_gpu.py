@@ -180,6 +180,25 @@ def test_tuning_never_changes_counts(ti, ctx):
     g.free()
 
 
+def test_plan_cells_outside_lds(ctx):
+    """256-rank endpoint tiles on R-MAT-20 (4096 tiles): an anchor whose tile
+    range passes 2048 cells has its cells counted and its segments placed
+    with global cursors instead of LDS ones; the counts equal those of the
+    default 2^15-rank tiles, edge for edge."""
+    say("[gpu] R-MAT-20 with 256-rank tiles")
+    s, d = pgd_amd.gen_rmat(20, 16, seed=3)
+    n = int(max(s.max(), d.max())) + 1
+    g = pgd_amd.GraphletCore(s, d, n=n, ctx=ctx)
+    g.sort_edges("deg")
+    base_sums = g.decompose(engine="orient")
+    base = g.edge_counts()
+    g.set_tuning(wedge_log2r=8)
+    assert g.decompose(engine="orient") == base_sums
+    for a, b in zip(base, g.edge_counts()):
+        assert np.array_equal(a, b)
+    g.free()
+
+
 def test_wedge_counter_overflow(ctx):
     """A hub with 70,000 in-neighbours that all reach one other vertex: the
     16-bit endpoint counter of that vertex overflows and its tile is
