@@ -96,7 +96,7 @@ __global__ void k_order_group(const int32_t* __restrict__ order, const int32_t* 
 // segmented epilogue: per-graph sums and micro columns, tiles from a queue
 __global__ __launch_bounds__(256) void k_epilogue_batch(
     const EpiItem* __restrict__ items, int64_t nitems, uint32_t* __restrict__ qctr, const int32_t* __restrict__ order,
-    const int32_t* __restrict__ ev, const int32_t* __restrict__ eu, const int64_t* __restrict__ row,
+    const int32_t* __restrict__ ev, const int32_t* __restrict__ eu, const int32_t* __restrict__ deg,
     const uint32_t* __restrict__ Tc, const uint64_t* __restrict__ K4c, const uint64_t* __restrict__ C4c,
     const int64_t* __restrict__ gn, const int64_t* __restrict__ gm, uint64_t* __restrict__ sums,
     uint64_t* __restrict__ micro, int64_t mstride) {
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_epilogue_batch(
         for (int64_t p = I.p0 + threadIdx.x; p < I.p1; p += blockDim.x) {
             const int32_t id = order[p];
             const int32_t v = ev[id], u = eu[id];
-            edge_terms(row[v + 1] - row[v], row[u + 1] - row[u], Tc[id], K4c[id], C4c[id], n, m, acc, micro,
+            edge_terms(deg[v], deg[u], Tc[id], K4c[id], C4c[id], n, m, acc, micro,
                        mstride, p);
         }
         block_add_sums(acc, sums + (size_t)I.graph * PGD_NSUMS);
@@ -196,7 +196,7 @@ void batch_count(pgd_batch* b) {
     PGD_HIP(hipGetLastError());
     if (b->nitems) {
         hipLaunchKernelGGL(k_epilogue_batch, dim3(grid_cap(ctx, b->nitems, 8)), dim3(256), 0, s, b->items.get(),
-                           b->nitems, b->ctl.get(), U->order.get(), U->ev.get(), U->eu.get(), U->row.get(),
+                           b->nitems, b->ctl.get(), U->order.get(), U->ev.get(), U->eu.get(), U->deg.get(),
                            U->T.get(), U->K4.get(), U->C4.get(), b->gn.get(), b->gm.get(), b->sums.get(),
                            (b->flags & PGD_MICRO) ? U->micro.get() : nullptr, b->M);
         PGD_HIP(hipGetLastError());

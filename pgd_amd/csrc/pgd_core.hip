@@ -535,7 +535,7 @@ __global__ void k_order_from_keys(const uint64_t* __restrict__ key, int64_t m, i
 // of the 15 sums (1429-1448) in uint64 wrap-around arithmetic.
 // ===========================================================================
 __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ order, const int32_t* __restrict__ ev,
-                                                  const int32_t* __restrict__ eu, const int64_t* __restrict__ row,
+                                                  const int32_t* __restrict__ eu, const int32_t* __restrict__ deg,
                                                   const uint32_t* __restrict__ Tc, const uint64_t* __restrict__ K4c,
                                                   const uint64_t* __restrict__ C4c, int64_t n, int64_t m,
                                                   int64_t nord, uint64_t* __restrict__ sums,
@@ -557,7 +557,7 @@ __global__ __launch_bounds__(256) void k_epilogue(const int32_t* __restrict__ or
         }
         int32_t id = order[e];
         int32_t v = ev[id], u = eu[id];
-        edge_terms(row[v + 1] - row[v], row[u + 1] - row[u], Tc[id], K4c[id], C4c[id], n, m, acc, micro, m, e);
+        edge_terms(deg[v], deg[u], Tc[id], K4c[id], C4c[id], n, m, acc, micro, m, e);
     }
     block_add_sums(acc, sums);
 }
@@ -576,7 +576,7 @@ void epilogue(pgd_ctx* ctx, const GraphView& g, EdgeCounts c, uint64_t* d_sums, 
     PGD_HIP(hipGetLastError());
     if (g.m == 0 && g.nord == 0) return;
     int grid = std::min(grid_for(std::max(g.m, g.nord)), ctx->num_cus * 8);
-    hipLaunchKernelGGL(k_epilogue, dim3(grid), dim3(256), 0, ctx->stream, g.order, g.ev, g.eu, g.row, c.T, c.K4,
+    hipLaunchKernelGGL(k_epilogue, dim3(grid), dim3(256), 0, ctx->stream, g.order, g.ev, g.eu, g.deg, c.T, c.K4,
                        c.C4, g.n, g.m, g.nord, d_sums, d_micro);
     PGD_HIP(hipGetLastError());
 }
